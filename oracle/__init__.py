@@ -1,0 +1,371 @@
+"""ORACLE -- test infrastructure only.
+
+Python side of the CPU restatement of DUMMEST's search path (oracle.cpp).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker.
+
+* ``parse_hmm`` restates ``readProfiles`` (dummest-profile.hh:235-316): the
+  HMMER3/f text state machine and the "-ln p" conversion of
+  ``probFromText`` (dummest-profile.hh:14-19).
+* everything else binds ``_build/liboracle.so`` (built by ``make`` here).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+WIDTH = 29
+ALPHABET = "ACDEFGHIKLMNPQRSTVWYUO?*"
+# dummest-core.hh:66-71 defaults: insert1 insert2 delete1 delete2 stop bg-stop
+DEFAULT_OPTS = (0.0171, 0.0018, 0.0328, 0.0083, 0.0005, 0.046875)
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _p(t):
+    return ctypes.POINTER(t)
+
+
+def _declare(L):
+    f, d, i, u8, c = ctypes.c_float, ctypes.c_double, ctypes.c_int, ctypes.c_uint8, ctypes.c_char
+    L.or_finalize_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), i]
+    L.or_finalize_f64.argtypes = [i, _p(d), _p(d), _p(d), _p(d), i]
+    L.or_decode.argtypes = [ctypes.c_char_p, i, _p(u8)]
+    for suf, t in (("f32", f), ("f64", d)):
+        getattr(L, "or_null_" + suf).argtypes = [_p(t), _p(u8), i, _p(d), _p(t), _p(t), _p(t),
+                                                 _p(f), _p(i), _p(f), _p(i)]
+        getattr(L, "or_filter_" + suf).argtypes = [i, _p(t), _p(t), _p(t), _p(d), _p(u8), i]
+        getattr(L, "or_filter_" + suf).restype = d
+        getattr(L, "or_search_" + suf).argtypes = [i, _p(t), _p(t), _p(t), _p(d), _p(u8), i, d, i, i, i, d,
+                                                   _p(d), _p(i), i, i, _p(f), _p(f), _p(f)]
+        getattr(L, "or_search_" + suf).restype = i
+    L.or_filter_batch_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), _p(ctypes.c_int64), _p(i), i,
+                                      _p(f), i]
+    L.or_fwdbwd_batch_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), _p(ctypes.c_int64), _p(i), i,
+                                      _p(f), _p(f), i]
+    L.or_gumbel.argtypes = [_p(d), i, d, _p(d)]
+    L.or_calib_sequence_f32.argtypes = [_p(f), _p(d), i, i, i, ctypes.c_char_p]
+    for name in ("add", "badd", "mul", "bmul"):
+        getattr(L, "or_es_" + name).argtypes = [f, i, f, i, _p(f), _p(i)]
+    L.or_es_normalize.argtypes = [d, i, _p(f), _p(i)]
+    L.or_es_scale.argtypes = [f, i, d, _p(f), _p(i)]
+    L.or_es_bscale.argtypes = [f, i, d, _p(f), _p(i)]
+    L.or_es_promote.argtypes = [f, i, _p(f), _p(i)]
+    L.or_es_log.argtypes = [f, i]
+    L.or_es_log.restype = d
+    L.or_es_to_rowscale.argtypes = [f, i, i, _p(f)]
+    L.or_es_from_rowscale.argtypes = [f, i, _p(f), _p(i)]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _ct(dtype):
+    return ctypes.c_float if np.dtype(dtype) == np.float32 else ctypes.c_double
+
+
+# ---------------------------------------------------------------------------
+# HMMER3/f reader, dummest-profile.hh:235-316
+# ---------------------------------------------------------------------------
+@dataclass
+class RawProfile:
+    name: str
+    length: int
+    raw: np.ndarray            # (M+1, 29) float64, values as read
+    consensus: list = field(default_factory=list)
+
+
+def _prob(word: str) -> float:
+    # probFromText, dummest-profile.hh:14-19
+    if word.startswith("*"):
+        return 0.0
+    try:
+        d = float(word)
+    except ValueError:
+        d = _strtod_prefix(word)
+    return math.exp(-d)
+
+
+def _strtod_prefix(word):
+    best = 0.0
+    for k in range(len(word), 0, -1):
+        try:
+            return float(word[:k])
+        except ValueError:
+            continue
+    return best
+
+
+def parse_hmm(text: str) -> list[RawProfile]:
+    profiles = []
+    state = 0
+    name = None
+    rows: list[list[float]] = []
+    cons: list[int] = []
+    length = 0
+    for line in text.splitlines():
+        words = line.split()
+        word = words[0] if words else ""
+        if state == 0:
+            if word == "NAME":
+                name = words[1] if len(words) > 1 else ""
+            elif word == "HMM":
+                state = 1
+        elif state == 1:
+            state = 2
+        elif state == 2:
+            if word != "COMPO":
+                state = 3
+        elif state == 3:
+            if len(words) < 7:
+                raise ValueError("bad transition line")
+            mtoi = _prob(words[1])
+            mtod = _prob(words[2])
+            itoi = _prob(words[4])
+            dtod = _prob(words[6])
+            if mtoi > 1 or mtod > 1 or itoi > 1 or dtod > 1:
+                raise ValueError("bad probability")
+            rows.append([mtoi, itoi, mtod, dtod])
+            state = 4
+        elif state == 4:
+            if word == "//":
+                if length < 2:
+                    raise ValueError("profile too short")
+                rows[-1].extend([0.0] * (WIDTH - 4))
+                raw = np.array(rows, dtype=np.float64)
+                profiles.append(RawProfile(name, length, raw, cons))
+                rows, cons, length, name = [], [], 0, None
+                state = 0
+            else:
+                probs = []
+                for w in words[1:]:
+                    if "." not in w:
+                        break
+                    p = _prob(w)
+                    if p > 1:
+                        raise ValueError("bad probability")
+                    probs.append(p)
+                if len(probs) != 20:
+                    raise ValueError("only 20-letter protein profiles are supported")
+                rows[-1].extend(probs + [0.0] * 5)
+                length += 1
+                letter = probs.index(max(probs))
+                cons.append(letter)
+                state = 2
+    if state != 0:
+        raise ValueError("truncated profile file")
+    return profiles
+
+
+@dataclass
+class Profile:
+    name: str
+    M: int
+    values: np.ndarray   # (M+1, 29) Float, finalized
+    params: np.ndarray   # (M+2, 11) Float
+    bg: np.ndarray       # (29,) Float
+    consensus: list
+
+
+def finalize(rp: RawProfile, dtype=np.float32, opts=DEFAULT_OPTS, bg_type="G") -> Profile:
+    """finalizeProfile (dummest-profile.hh:105-233) with -m 0."""
+    dt = np.dtype(dtype)
+    vals = np.ascontiguousarray(rp.raw.astype(dt))
+    params = np.zeros((rp.length + 2, 11), dtype=dt)
+    bg = np.zeros(WIDTH, dtype=dt)
+    o = np.array(opts, dtype=np.float64)
+    fn = lib().or_finalize_f32 if dt == np.float32 else lib().or_finalize_f64
+    t = _ct(dt)
+    ok = fn(rp.length, _ptr(vals, t), _ptr(params, t), _ptr(bg, t), _ptr(o, ctypes.c_double), ord(bg_type))
+    if not ok:
+        raise ValueError("finalizeProfile rejected the profile")
+    return Profile(rp.name, rp.length, vals, params, bg, list(rp.consensus))
+
+
+def decode(dna: str | bytes) -> np.ndarray:
+    b = dna.encode() if isinstance(dna, str) else dna
+    out = np.zeros(len(b), dtype=np.uint8)
+    lib().or_decode(b, len(b), _ptr(out, ctypes.c_uint8))
+    return out
+
+
+def null_model(bg: np.ndarray, codes: np.ndarray, opts=DEFAULT_OPTS):
+    dt = bg.dtype
+    t = _ct(dt)
+    n = len(codes)
+    ps = np.zeros(n, dt)
+    ss = np.zeros(n, dt)
+    sc = np.zeros(3, dt)
+    pm = np.zeros(n + 4, np.float32)
+    pe = np.zeros(n + 4, np.int32)
+    sm = np.zeros(n + 4, np.float32)
+    se = np.zeros(n + 4, np.int32)
+    o = np.array(opts, np.float64)
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    fn = lib().or_null_f32 if dt == np.float32 else lib().or_null_f64
+    fn(_ptr(bg, t), _ptr(c, ctypes.c_uint8), n, _ptr(o, ctypes.c_double), _ptr(ps, t), _ptr(ss, t), _ptr(sc, t),
+       _ptr(pm, ctypes.c_float), _ptr(pe, ctypes.c_int), _ptr(sm, ctypes.c_float), _ptr(se, ctypes.c_int))
+    return dict(prefix_seed=ps, suffix_seed=ss, scales=sc, prefix=(pm, pe), suffix=(sm, se))
+
+
+def filter_score(p: Profile, codes: np.ndarray, opts=DEFAULT_OPTS) -> float:
+    dt = p.values.dtype
+    t = _ct(dt)
+    o = np.array(opts, np.float64)
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    fn = lib().or_filter_f32 if dt == np.float32 else lib().or_filter_f64
+    return fn(p.M, _ptr(p.values, t), _ptr(p.params, t), _ptr(p.bg, t), _ptr(o, ctypes.c_double),
+              _ptr(c, ctypes.c_uint8), len(c))
+
+
+@dataclass
+class SearchResult:
+    hits: list          # (lnratio, m, e, anchor1, anchor2, segments[(s1,s2,len)])
+    global_best: tuple  # (m, e)
+    bwd_best: tuple     # (m, e)
+    col_best: np.ndarray  # (n, 3): m, e, row
+
+
+def es_log(m, e):
+    return -math.inf if m == 0 else math.log(m) + e * 0.6931471805599453
+
+
+def search(p: Profile, codes: np.ndarray, min_prob_ratio: float, need_align=True, want_bwd=True,
+           opts=DEFAULT_OPTS, rescale_bits=64, thresh=None, hit_cap=4096, seg_cap=200000) -> SearchResult:
+    dt = p.values.dtype
+    t = _ct(dt)
+    if thresh is None:
+        thresh = 1e30 if dt == np.float32 else 1e150
+    o = np.array(opts, np.float64)
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    n = len(c)
+    hits = np.zeros((hit_cap, 6), np.float64)
+    segs = np.zeros((seg_cap, 3), np.int32)
+    gb = np.zeros(2, np.float32)
+    bb = np.zeros(2, np.float32)
+    cb = np.zeros((n, 3), np.float32)
+    fn = lib().or_search_f32 if dt == np.float32 else lib().or_search_f64
+    nh = fn(p.M, _ptr(p.values, t), _ptr(p.params, t), _ptr(p.bg, t), _ptr(o, ctypes.c_double),
+            _ptr(c, ctypes.c_uint8), n, float(min_prob_ratio), int(need_align), int(want_bwd), rescale_bits,
+            float(thresh), _ptr(hits, ctypes.c_double), _ptr(segs, ctypes.c_int), hit_cap, seg_cap,
+            _ptr(gb, ctypes.c_float), _ptr(bb, ctypes.c_float), _ptr(cb, ctypes.c_float))
+    if nh < 0:
+        raise RuntimeError("oracle output capacity exceeded")
+    out = []
+    k = 0
+    for h in hits[:nh]:
+        ns = int(h[5])
+        out.append((h[0], float(h[1]), int(h[2]), int(h[3]), int(h[4]),
+                    [tuple(int(x) for x in s) for s in segs[k:k + ns]]))
+        k += ns
+    return SearchResult(out, (float(gb[0]), int(gb[1])), (float(bb[0]), int(bb[1])), cb)
+
+
+def filter_batch(p: Profile, seqs: list, opts=DEFAULT_OPTS, threads=None) -> np.ndarray:
+    codes, offs, lens = pack(seqs)
+    out = np.zeros(len(seqs), np.float32)
+    o = np.array(opts, np.float64)
+    f = ctypes.c_float
+    lib().or_filter_batch_f32(p.M, _ptr(p.values, f), _ptr(p.params, f), _ptr(p.bg, f), _ptr(o, ctypes.c_double),
+                              _ptr(codes, ctypes.c_uint8), _ptr(offs, ctypes.c_int64), _ptr(lens, ctypes.c_int),
+                              len(seqs), _ptr(out, f), threads or os.cpu_count())
+    return out
+
+
+def fwdbwd_batch(p: Profile, seqs: list, opts=DEFAULT_OPTS, threads=None):
+    codes, offs, lens = pack(seqs)
+    mid = np.zeros((len(seqs), 2), np.float32)
+    bwd = np.zeros((len(seqs), 2), np.float32)
+    o = np.array(opts, np.float64)
+    f = ctypes.c_float
+    lib().or_fwdbwd_batch_f32(p.M, _ptr(p.values, f), _ptr(p.params, f), _ptr(p.bg, f), _ptr(o, ctypes.c_double),
+                              _ptr(codes, ctypes.c_uint8), _ptr(offs, ctypes.c_int64), _ptr(lens, ctypes.c_int),
+                              len(seqs), _ptr(mid, f), _ptr(bwd, f), threads or os.cpu_count())
+    return mid, bwd
+
+
+def pack(seqs):
+    lens = np.array([len(s) for s in seqs], np.int32)
+    offs = np.zeros(len(seqs), np.int64)
+    if len(seqs):
+        offs[1:] = np.cumsum(lens[:-1])
+    codes = np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if seqs else np.zeros(0, np.uint8)
+    return np.ascontiguousarray(codes), offs, lens
+
+
+def gumbel(scores, seq_len):
+    s = np.array(scores, np.float64)
+    out = np.zeros(8, np.float64)
+    lib().or_gumbel(_ptr(s, ctypes.c_double), len(s), float(seq_len), _ptr(out, ctypes.c_double))
+    keys = ("mmL", "mmK", "mmKsimple", "mlL", "mlK", "mlKsimple", "lmL", "lmK")
+    return dict(zip(keys, out.tolist()))
+
+
+def calib_sequence(bg: np.ndarray, length: int, border: int, trial: int, opts=DEFAULT_OPTS) -> bytes:
+    """Random frameshifting codon DNA of one calibration trial (letters)."""
+    freqs = np.ascontiguousarray(bg[4:24].astype(np.float32))
+    o = np.array(opts, np.float64)
+    buf = ctypes.create_string_buffer(length + border)
+    lib().or_calib_sequence_f32(_ptr(freqs, ctypes.c_float), _ptr(o, ctypes.c_double), length, border, trial, buf)
+    return buf.raw[:length + border]
+
+
+# ExpScore primitive wrappers (contract tests)
+def _es2(fn, *args):
+    om, oe = ctypes.c_float(), ctypes.c_int()
+    fn(*args, ctypes.byref(om), ctypes.byref(oe))
+    return om.value, oe.value
+
+
+def es_normalize(m, e):
+    return _es2(lib().or_es_normalize, m, e)
+
+
+def es_add(a, b):
+    return _es2(lib().or_es_add, a[0], a[1], b[0], b[1])
+
+
+def es_batch_add(a, b):
+    return _es2(lib().or_es_badd, a[0], a[1], b[0], b[1])
+
+
+def es_mul(a, b):
+    return _es2(lib().or_es_mul, a[0], a[1], b[0], b[1])
+
+
+def es_batch_mul(a, b):
+    return _es2(lib().or_es_bmul, a[0], a[1], b[0], b[1])
+
+
+def es_scale(a, f):
+    return _es2(lib().or_es_scale, a[0], a[1], f)
+
+
+def es_batch_scale(a, f):
+    return _es2(lib().or_es_bscale, a[0], a[1], f)
+
+
+def es_promote(v, cum):
+    return _es2(lib().or_es_promote, v, cum)
