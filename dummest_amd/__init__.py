@@ -1,0 +1,288 @@
+"""dummest_amd -- MI355X (gfx950) implementation of DUMMEST's frameshift-aware
+DNA-vs-profile-HMM forward/backward DP.
+
+Python binding over the C ABI of ``libdummest_amd.so`` (declared in
+``include/dummest_amd.h``). Names follow the reference's operator interface:
+
+* ``Engine.find_similarities_backward_only`` -- score-only filter DP
+  (dummest-dp-backward.hh:300-438)
+* ``Engine.fwdbwd_scores`` -- forward+backward best anchored score and
+  backward start score (findSimilarities with needAlign=false,
+  dummest-search.hh:9-17, as used by estimateK)
+* ``Engine.find_similarities`` -- local hits with alignments of one pair
+  (findSimilarities with needAlign=true + traceback)
+
+There is no CPU fallback: loading fails loudly when the HIP library is absent.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdummest_amd.so")
+WIDTH = 29
+ALPHABET = "ACDEFGHIKLMNPQRSTVWYUO?*"
+DEFAULT_RATES = (0.0171, 0.0018, 0.0328, 0.0083, 0.0005, 0.046875)
+
+_lib = None
+
+
+class DummestError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libdummest_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DummestError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "or `make -C dummest_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _P(t):
+    return ctypes.POINTER(t)
+
+
+def _declare(L):
+    i32, i64, f, d, u8, vp = (ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double, ctypes.c_uint8,
+                             ctypes.c_void_p)
+    L.dm_engine_create.argtypes = [ctypes.c_int, _P(vp)]
+    L.dm_engine_destroy.argtypes = [vp]
+    L.dm_engine_destroy.restype = None
+    L.dm_engine_error.argtypes = [vp]
+    L.dm_engine_error.restype = ctypes.c_char_p
+    L.dm_set_rates.argtypes = [vp, _P(d)]
+    L.dm_read_hmm.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _P(i32), _P(i32)]
+    L.dm_parse_hmm_text.argtypes = [ctypes.c_char_p, ctypes.c_int, _P(d), i32, _P(i32), _P(f), _P(f), _P(f), _P(i32)]
+    L.dm_add_profile.argtypes = [vp, i32, _P(f), _P(f), _P(f), _P(i32)]
+    L.dm_get_profile.argtypes = [vp, i32, _P(i32), _P(f), _P(f), _P(f)]
+    L.dm_profile_count.argtypes = [vp]
+    L.dm_decode.argtypes = [ctypes.c_char_p, i64, _P(u8)]
+    L.dm_add_sequences.argtypes = [vp, i32, _P(u8), _P(i64), _P(i32), _P(i32)]
+    L.dm_clear_sequences.argtypes = [vp]
+    L.dm_filter_scores.argtypes = [vp, i64, _P(i32), _P(i32), _P(f)]
+    L.dm_fwdbwd_scores.argtypes = [vp, i64, _P(i32), _P(i32), ctypes.c_int, _P(f), _P(i32), _P(f), _P(i32)]
+    L.dm_pair_matrices.argtypes = [vp, i32, i32, _P(f), _P(i32), _P(f), _P(i32)]
+    L.dm_pair_null.argtypes = [vp, i32, i32, _P(f), _P(f), _P(f)]
+    L.dm_pair_hits.argtypes = [vp, i32, i32, d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
+    L.dm_set_option.argtypes = [vp, ctypes.c_char_p, i64]
+    L.dm_last_dp_ms.argtypes = [vp]
+    L.dm_last_dp_ms.restype = d
+    L.dm_last_cells.argtypes = [vp]
+    L.dm_last_cells.restype = i64
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def decode(dna) -> np.ndarray:
+    """Codon codes of a DNA string (decodeSequence, dummest-sequence.hh:181-204)."""
+    b = dna.encode() if isinstance(dna, str) else bytes(dna)
+    out = np.zeros(len(b), np.uint8)
+    rc = lib().dm_decode(b, len(b), _ptr(out, ctypes.c_uint8))
+    if rc:
+        raise DummestError(f"dm_decode failed ({rc})")
+    return out
+
+
+def parse_hmm_text(text: str, index: int = 0, bg_type: str = "G", rates=DEFAULT_RATES):
+    """Host-side readProfiles + finalizeProfile of one profile of an HMMER3/f text
+    (no device needed): returns (M, values, params, bg, count)."""
+    r = np.array(rates, np.float64)
+    M, cnt = ctypes.c_int32(), ctypes.c_int32()
+    t = text.encode()
+    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), index, ctypes.byref(M), None, None,
+                                 None, ctypes.byref(cnt))
+    if rc:
+        raise DummestError(f"dm_parse_hmm_text failed ({rc})")
+    m = M.value
+    v = np.zeros((m + 1, WIDTH), np.float32)
+    p = np.zeros((m + 2, 11), np.float32)
+    b = np.zeros(WIDTH, np.float32)
+    f = ctypes.c_float
+    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), index, ctypes.byref(M), _ptr(v, f),
+                                 _ptr(p, f), _ptr(b, f), ctypes.byref(cnt))
+    if rc:
+        raise DummestError(f"dm_parse_hmm_text failed ({rc})")
+    return m, v, p, b, cnt.value
+
+
+def es_log(m, e):
+    """Natural log of an ExpScore (scoreToLog, dummest-score.hh:202-207)."""
+    m = np.asarray(m, np.float64)
+    e = np.asarray(e, np.float64)
+    with np.errstate(divide="ignore"):
+        return np.where(m == 0, -np.inf, np.log(np.where(m == 0, 1.0, m)) + e * math.log(2.0))
+
+
+class Engine:
+    """One HIP stream on one device plus its uploaded profiles and sequences."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().dm_engine_create(device, ctypes.byref(h))
+        if rc:
+            raise DummestError(f"dm_engine_create failed ({rc}): no usable HIP device {device}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dm_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().dm_engine_error(self._h).decode(errors="replace")
+            raise DummestError(f"{what} failed ({rc}): {msg}")
+
+    # -- configuration ------------------------------------------------------
+    def set_rates(self, rates=DEFAULT_RATES):
+        r = np.array(rates, np.float64)
+        self._check(lib().dm_set_rates(self._h, _ptr(r, ctypes.c_double)), "dm_set_rates")
+
+    def set_option(self, name: str, value: int):
+        self._check(lib().dm_set_option(self._h, name.encode(), int(value)), "dm_set_option")
+
+    # -- profiles -----------------------------------------------------------
+    def read_hmm(self, path: str, bg_type: str = "G") -> list:
+        first, count = ctypes.c_int32(), ctypes.c_int32()
+        self._check(lib().dm_read_hmm(self._h, path.encode(), ord(bg_type), ctypes.byref(first), ctypes.byref(count)),
+                    "dm_read_hmm")
+        return list(range(first.value, first.value + count.value))
+
+    def add_profile(self, M, values, params, bg) -> int:
+        v = np.ascontiguousarray(values, np.float32)
+        p = np.ascontiguousarray(params, np.float32)
+        b = np.ascontiguousarray(bg, np.float32)
+        assert v.size == (M + 1) * WIDTH and p.size == (M + 2) * 11 and b.size == WIDTH
+        out = ctypes.c_int32()
+        f = ctypes.c_float
+        self._check(lib().dm_add_profile(self._h, int(M), _ptr(v, f), _ptr(p, f), _ptr(b, f), ctypes.byref(out)),
+                    "dm_add_profile")
+        return out.value
+
+    def get_profile(self, pid: int):
+        M = ctypes.c_int32()
+        self._check(lib().dm_get_profile(self._h, pid, ctypes.byref(M), None, None, None), "dm_get_profile")
+        m = M.value
+        v = np.zeros((m + 1, WIDTH), np.float32)
+        p = np.zeros((m + 2, 11), np.float32)
+        b = np.zeros(WIDTH, np.float32)
+        f = ctypes.c_float
+        self._check(lib().dm_get_profile(self._h, pid, ctypes.byref(M), _ptr(v, f), _ptr(p, f), _ptr(b, f)),
+                    "dm_get_profile")
+        return m, v, p, b
+
+    def profile_length(self, pid: int) -> int:
+        M = ctypes.c_int32()
+        self._check(lib().dm_get_profile(self._h, pid, ctypes.byref(M), None, None, None), "dm_get_profile")
+        return M.value
+
+    # -- sequences ----------------------------------------------------------
+    def add_sequences(self, seqs) -> list:
+        """Append decoded sequences (arrays of codon codes); returns their ids."""
+        lens = np.array([len(s) for s in seqs], np.int32)
+        offs = np.zeros(len(seqs), np.int64)
+        if len(seqs) > 1:
+            offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+        codes = (np.concatenate([np.asarray(s, np.uint8) for s in seqs]) if len(seqs)
+                 else np.zeros(1, np.uint8))
+        codes = np.ascontiguousarray(codes)
+        first = ctypes.c_int32()
+        self._check(lib().dm_add_sequences(self._h, len(seqs), _ptr(codes, ctypes.c_uint8),
+                                           _ptr(offs, ctypes.c_int64), _ptr(lens, ctypes.c_int32),
+                                           ctypes.byref(first)), "dm_add_sequences")
+        return list(range(first.value, first.value + len(seqs)))
+
+    def clear_sequences(self):
+        self._check(lib().dm_clear_sequences(self._h), "dm_clear_sequences")
+
+    # -- scoring ------------------------------------------------------------
+    @staticmethod
+    def _pairs(prof, seq):
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(prof, np.int32), np.shape(seq)), np.int32)
+        s = np.ascontiguousarray(seq, np.int32)
+        return p.ravel(), s.ravel()
+
+    def find_similarities_backward_only(self, prof, seq) -> np.ndarray:
+        """Filter score per pair: max over cells of W1 * prefixSeed, FP32, no rescale."""
+        p, s = self._pairs(prof, seq)
+        out = np.zeros(len(p), np.float32)
+        i32 = ctypes.c_int32
+        self._check(lib().dm_filter_scores(self._h, len(p), _ptr(p, i32), _ptr(s, i32), _ptr(out, ctypes.c_float)),
+                    "dm_filter_scores")
+        return out
+
+    def fwdbwd_scores(self, prof, seq, want_bwd=True):
+        """(mid_m, mid_e, bwd_m, bwd_e): best anchored score and backward start score."""
+        p, s = self._pairs(prof, seq)
+        n = len(p)
+        mm = np.zeros(n, np.float32)
+        me = np.zeros(n, np.int32)
+        bm = np.zeros(n, np.float32)
+        be = np.zeros(n, np.int32)
+        i32, f = ctypes.c_int32, ctypes.c_float
+        self._check(lib().dm_fwdbwd_scores(self._h, n, _ptr(p, i32), _ptr(s, i32), int(want_bwd), _ptr(mm, f),
+                                           _ptr(me, i32), _ptr(bm, f), _ptr(be, i32)), "dm_fwdbwd_scores")
+        return mm, me, bm, be
+
+    def pair_matrices(self, prof: int, seq: int, n: int):
+        M = self.profile_length(prof)
+        w0m = np.zeros((M + 1, n), np.float32)
+        w0e = np.zeros((M + 1, n), np.int32)
+        W1m = np.zeros((M + 1, n), np.float32)
+        W1e = np.zeros((M + 1, n), np.int32)
+        i32, f = ctypes.c_int32, ctypes.c_float
+        self._check(lib().dm_pair_matrices(self._h, prof, seq, _ptr(w0m, f), _ptr(w0e, i32), _ptr(W1m, f),
+                                           _ptr(W1e, i32)), "dm_pair_matrices")
+        return (w0m, w0e), (W1m, W1e)
+
+    def pair_null(self, prof: int, seq: int, n: int):
+        ps = np.zeros(n, np.float32)
+        ss = np.zeros(n, np.float32)
+        sc = np.zeros(3, np.float32)
+        f = ctypes.c_float
+        self._check(lib().dm_pair_null(self._h, prof, seq, _ptr(ps, f), _ptr(ss, f), _ptr(sc, f)), "dm_pair_null")
+        return ps, ss, sc
+
+    def find_similarities(self, prof: int, seq: int, min_prob_ratio: float, hit_cap=4096, seg_cap=1 << 20):
+        """Hits of one pair: list of (lnratio, m, e, anchor1, anchor2, [(start1, start2, length)])."""
+        hits = np.zeros((hit_cap, 6), np.float64)
+        segs = np.zeros((seg_cap, 3), np.int32)
+        nh, ns = ctypes.c_int32(), ctypes.c_int32()
+        i32 = ctypes.c_int32
+        self._check(lib().dm_pair_hits(self._h, prof, seq, float(min_prob_ratio), _ptr(hits, ctypes.c_double),
+                                       hit_cap, _ptr(segs, i32), seg_cap, ctypes.byref(nh), ctypes.byref(ns)),
+                    "dm_pair_hits")
+        out = []
+        k = 0
+        for h in hits[:nh.value]:
+            c = int(h[5])
+            out.append((float(h[0]), float(h[1]), int(h[2]), int(h[3]), int(h[4]),
+                        [tuple(int(x) for x in r) for r in segs[k:k + c]]))
+            k += c
+        return out
+
+    @property
+    def last_dp_ms(self) -> float:
+        return lib().dm_last_dp_ms(self._h)
+
+    @property
+    def last_cells(self) -> int:
+        return lib().dm_last_cells(self._h)

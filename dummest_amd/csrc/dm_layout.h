@@ -1,0 +1,121 @@
+// Data layout shared by the host library and the gfx950 kernels.
+//
+// HBM layout (all arrays device-resident, uploaded once per call):
+//   profiles : per profile p, ProfInfo{M, parOff, emOff}
+//              params [parOff + 11*i + f], i = 0..M+1   (alpha'[3] beta'[3] delta'[3] eps' enter,
+//                                                        dummest-core.hh:233-239)
+//              emis   [emOff + 32*i + code], i = 0..M   (row i holds node i+1's codon emissions,
+//                                                        codes 0..24, dummest-profile.hh:206-224)
+//              bg     [p*32 + code]                     (background per codon code, bg_probs+4)
+//   sequences: decoded codes, uint8, concatenated (dummest-sequence.hh:181-204)
+//   pairs    : PairInfo per (profile, sequence) pair, in work-queue order
+//   records  : per pair, columns j in [-RPAD, n+RPAD): BRec (backward) and FRec (forward)
+//   units    : work units = up to 64/G pairs of one profile with similar lengths
+#pragma once
+#include <stdint.h>
+
+namespace dm {
+
+constexpr int kWidth = 29;     // values per profile row (20 letters + 9)
+constexpr int kNCode = 25;     // decoded codes 0..24; 24 is the zero-probability pad (zero_idx)
+constexpr int kZeroIdx = 24;
+constexpr int kEmStride = 32;  // floats per emission row in HBM
+constexpr int kRPad = 64;      // record padding columns on each side of a pair's sequence
+
+// Backward record for column j: g[j+1], suffixSeed[j], prefixSeed[j], a[j+1].
+struct BRec {
+    float g;
+    float ss;
+    float ps;
+    int code;
+};
+// Forward record for column j: g[j-2], prefixSeed[j], a[j-2].
+struct FRec {
+    float g;
+    float ps;
+    int code;
+    int pad;
+};
+
+struct ProfInfo {
+    int M;
+    int pad;
+    int64_t parOff;
+    int64_t emOff;
+};
+
+struct PairInfo {
+    int prof;
+    int seq;
+    int n;
+    int status;      // 0 ok; nonzero: null-model failure (sequence shorter than 3)
+    int64_t seqOff;  // offset of the sequence codes
+    int64_t recOff;  // index of column 0 in the record arrays
+    int64_t bufOff;  // float offset of this pair's W1 / w0 buffers (full and dump modes)
+    float s1, s2, sC;  // per-pair background scales, packed floats (dummest-dp-backward.hh:244-246)
+    float pad2;
+};
+
+struct Unit {
+    int prof;
+    int first;  // first pair index
+    int count;  // pairs in this unit (<= 64/G)
+    int maxN;   // longest sequence among them
+};
+
+// Output of the full (forward+backward) score pass per pair, as ExpScore pairs.
+struct PairScore {
+    float midM;  // best anchored score wMid, mantissa in [0.5,1) or 0
+    int midE;
+    float bwdM;  // best backward start score (bestOut, dummest-dp-backward.hh:579-609)
+    int bwdE;
+};
+
+// ExpScore: value m * 2^e with m in [0.5,1) or m == 0 (dummest-score.hh:29-33).
+struct ES {
+    float m;
+    int e;
+};
+
+enum Mode { kFilter = 0, kScore = 1, kDump = 2 };
+
+struct NullArgs {
+    const uint8_t *codes;
+    const float *bg;   // [prof*32 + code]
+    PairInfo *pairs;
+    BRec *brec;
+    FRec *frec;
+    ES *scratchPre;    // per pair, n + 4 entries starting at recOff
+    ES *scratchSuf;
+    int npairs;
+    float bgfs1, bgfs2;  // BACKGROUND_FRAMESHIFT_RATE(_2), as Float sums
+};
+
+struct DPArgs {
+    const ProfInfo *profs;
+    const float *params;
+    const float *emis;
+    const PairInfo *pairs;
+    const BRec *brec;
+    const FRec *frec;
+    const Unit *units;
+    int nUnits;
+    int *unitCounter;
+    float *wbuf;          // W1 per step: per-wave scratch (score mode) or per pair (dump mode)
+    float *w0buf;         // dump mode: forward w0 per step, per pair
+    int *ebuf;            // per-step backward exponents (same indexing as wbuf / ROWS)
+    int *fbuf;            // dump mode: per-step forward exponents
+    int64_t waveFloats;   // score mode: scratch floats per wave
+    float *filterOut;     // filter mode: best W1*prefixSeed per pair
+    PairScore *scoreOut;  // score mode
+    int wantBwd;
+};
+
+#ifdef __HIPCC__
+// Implemented in kernels.hip.
+int dp_configs(int *gs, int *rs, int cap);
+hipError_t launch_null(const NullArgs &a, hipStream_t s);
+hipError_t launch_dp(int G, int R, int mode, const DPArgs &a, int grid, hipStream_t s);
+#endif
+
+}  // namespace dm

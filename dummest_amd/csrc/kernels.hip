@@ -1,0 +1,646 @@
+// gfx950 kernels for DUMMEST's frameshift-aware DNA-vs-profile DP.
+//
+// Kernel design (one wavefront, 64 lanes):
+//   * A wave takes a work unit: up to P = 64/G pairs that share one profile.
+//     Each pair owns a group of G lanes (G = 16: four pairs per wave, G = 64:
+//     one pair per wave). Lane k of a group holds R consecutive profile rows
+//     kR .. kR+R-1, so a group covers G*R rows (M+1 <= G*R).
+//   * Lanes are skewed by one DNA column: at step t lane k works on the
+//     anti-diagonal D = t (forward) or D = T-1-t (backward), column c = D - k.
+//     Every dependency on the neighbouring row block (W1/w0 of the row below or
+//     above, the Y chain of the D state) is then a value the neighbour lane
+//     produced in the previous steps, read with one DPP row/wave shift. Inside
+//     a lane the D chain runs through the R rows in order.
+//   * Recurrences: backward dummest-dp-backward.hh:523-548, forward
+//     dummest-dp-forward.hh:112-223, score-only filter dummest-dp-backward.hh:389-427.
+//   * Scaling: one power-of-two exponent per pair per anti-diagonal step,
+//     raised by 64 whenever a value passes 2^100 (the reference's per-row
+//     2^-64 halving at 1e30, dummest-score.hh:83-94, moved onto the step the
+//     GPU computes at once). Power-of-two scaling is exact in FP32 apart from
+//     flushing values more than ~2^226 below the step maximum.
+//   * The codon emission table of the unit's profile sits in LDS as
+//     [code][row]; per-column inputs (background codon probability, seeds,
+//     codon code) are 16-byte records read once per step per lane.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+#include <type_traits>
+
+#include "dm_layout.h"
+
+namespace dm {
+
+// ---------------------------------------------------------------------------
+// ExpScore arithmetic (m in [0.5,1) or 0, value m*2^e), dummest-score.hh and
+// dummest-batch.hh, evaluated in double exactly as the reference does.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ ES es_zero() { return ES{0.0f, 0}; }
+
+__device__ __forceinline__ ES es_normalize(double m, int e) {  // dummest-score.hh:62-67
+    if (m == 0.0) return es_zero();
+    int ex = 0;
+    m = frexp(m, &ex);
+    return ES{(float)m, e + ex};
+}
+
+__device__ __forceinline__ bool es_less(const ES &a, const ES &b) {  // dummest-score.hh:70-76
+    if (a.m == 0.0f) return b.m != 0.0f;
+    if (b.m == 0.0f) return false;
+    if (a.e != b.e) return a.e < b.e;
+    return a.m < b.m;
+}
+
+__device__ __forceinline__ ES es_scale(const ES &a, double f) {  // dummest-score.hh:155-160
+    if (a.m == 0.0f) return es_zero();
+    if (!(f > 0.0) || !isfinite(f)) return es_zero();
+    return es_normalize((double)a.m * f, a.e);
+}
+
+__device__ __forceinline__ ES es_add(ES a, ES b) {  // dummest-score.hh:169-181
+    if (a.m == 0.0f) return b;
+    if (b.m == 0.0f) return a;
+    if (a.e >= b.e) {
+        int d = a.e - b.e;
+        if (d > 60) return a;
+        return es_normalize((double)a.m + ldexp((double)b.m, -d), a.e);
+    }
+    int d = b.e - a.e;
+    if (d > 60) return b;
+    return es_normalize((double)b.m + ldexp((double)a.m, -d), b.e);
+}
+
+__device__ __forceinline__ ES es_mul(const ES &a, const ES &b) {  // dummest-score.hh:184-188
+    if (a.m == 0.0f || b.m == 0.0f) return es_zero();
+    return es_normalize((double)a.m * (double)b.m, a.e + b.e);
+}
+
+__device__ __forceinline__ ES es_pow2(int e2) { return ES{0.5f, e2 + 1}; }
+
+__device__ __forceinline__ void split_double(double x, float &mOut, int &eAdd) {  // dummest-batch.hh:42-53
+    if (x == 0.0) { mOut = 0.0f; eAdd = 0; return; }
+    uint64_t u = (uint64_t)__double_as_longlong(x);
+    int ef = (int)((u >> 52) & 0x7FFULL);
+    if (ef == 0x7FF) { mOut = (float)x; eAdd = 0; return; }
+    if (ef == 0) { mOut = 0.0f; eAdd = 0; return; }
+    eAdd = ef - 1022;
+    mOut = (float)__longlong_as_double((long long)((u & 0xFFFFFFFFFFFFFULL) | (1022ULL << 52)));
+}
+
+__device__ __forceinline__ ES es_bmul(const ES &A, const ES &B) {  // dummest-batch.hh:58-72
+    const bool nz = (A.m != 0.0f) & (B.m != 0.0f);
+    const double p = (double)A.m * (double)B.m;
+    const int e = A.e + B.e;
+    const bool lo = (p < 0.5);
+    const double ps = lo ? p * 2.0 : p;
+    const int es = lo ? e - 1 : e;
+    return nz ? ES{(float)ps, es} : es_zero();
+}
+
+__device__ __forceinline__ ES es_badd(const ES &A, const ES &B) {  // dummest-batch.hh:78-108
+    const bool a0 = (A.m == 0.0f), b0 = (B.m == 0.0f);
+    const bool agebe = (A.e >= B.e);
+    const float mh = agebe ? A.m : B.m;
+    const float ml = agebe ? B.m : A.m;
+    const int eh = agebe ? A.e : B.e;
+    const int el = agebe ? B.e : A.e;
+    int d = eh - el;
+    d = d > 60 ? 60 : d;
+    double s = (double)mh + (double)ml * (1.0 / (double)(1ULL << (unsigned)d));
+    int e = eh;
+    const bool hi = (s >= 1.0);
+    s = hi ? s * 0.5 : s;
+    e = hi ? e + 1 : e;
+    const ES arith{(float)s, e};
+    return a0 ? B : (b0 ? A : arith);
+}
+
+__device__ __forceinline__ ES es_bscale(const ES &A, double fv) {  // dummest-batch.hh:114-130
+    const bool a0 = (A.m == 0.0f);
+    const bool fok = (fv > 0.0) && isfinite(fv);
+    float m;
+    int ex;
+    split_double(a0 ? 0.0 : (double)A.m * (fok ? fv : 1.0), m, ex);
+    return (a0 || !fok) ? es_zero() : ES{m, A.e + ex};
+}
+
+__device__ __forceinline__ float es_to_float_clamped(const ES &a) {  // dummest-score.hh:224-230 (FP32: 120)
+    if (a.m == 0.0f) return 0.0f;
+    int e = a.e;
+    if (e > 120) e = 120;
+    return ldexpf(a.m, e);
+}
+
+// ---------------------------------------------------------------------------
+// Null model, one thread per pair: dummest-dp-backward.hh:6-188 for one lane
+// (the reference's lane arithmetic is independent of the batch). Writes the
+// pair's record arrays and scales.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= a.npairs) return;
+    PairInfo P = a.pairs[pi];
+    const int n = P.n;
+    BRec *br = a.brec + P.recOff;
+    FRec *fr = a.frec + P.recOff;
+    if (n < 3) {
+        // The reference asserts n >= 3 (dummest-dp.hh:84). Zero records make the
+        // DP produce nothing; status reports the rejection.
+        for (int j = -kRPad; j < n + kRPad; j++) {
+            br[j] = BRec{0.f, 0.f, 0.f, kZeroIdx};
+            fr[j] = FRec{0.f, 0.f, kZeroIdx, 0};
+        }
+        a.pairs[pi].status = 1;
+        a.pairs[pi].s1 = a.pairs[pi].s2 = a.pairs[pi].sC = 0.f;
+        return;
+    }
+    const uint8_t *cd = a.codes + P.seqOff;
+    const float *bgp = a.bg + (int64_t)P.prof * 32;
+    auto code = [&](int j) -> int { return (j >= 0 && j < n) ? (int)cd[j] : kZeroIdx; };
+    ES *pre = a.scratchPre + P.recOff;  // indices 0 .. n+3
+    ES *suf = a.scratchSuf + P.recOff;
+    const double single = (double)a.bgfs1, dbl = (double)a.bgfs2;
+    const double codonKeep = 1 - single - dbl;
+    const double singleEmit = single * 0.25;
+    const double doubleEmit = dbl * 0.0625;
+    const double frame = 1.0 / 3.0;
+    const int W = n;
+    for (int j = 0; j < n + 4; j++) { pre[j] = es_zero(); suf[j] = es_zero(); }
+    // buildNullSuffix, dummest-dp-backward.hh:6-55
+    {
+        float tm;
+        int te;
+        split_double(0.5 * codonKeep, tm, te);
+        suf[n] = es_pow2(0);
+        for (int i = W - 1; i >= 0; i--) {
+            const float bgc = bgp[code(i)];
+            const double codonScale = (double)bgc * codonKeep;
+            ES singleIn = suf[i + 1];
+            ES doubleIn = (i + 1 < n) ? suf[i + 2] : es_pow2(2);
+            ES codonRaw = es_bscale(suf[i + 3], codonScale);
+            ES sb = es_bscale(singleIn, singleEmit);
+            ES db = es_bscale(doubleIn, doubleEmit);
+            ES cb = (i + 2 < n) ? codonRaw : ES{tm, te - 2 * (n - i) + 1};
+            suf[i] = es_badd(cb, es_badd(sb, db));
+        }
+    }
+    // buildNullPrefixFirstCodon, dummest-dp-backward.hh:58-100
+    for (int i = 0; i < W; i++) {
+        const float bgc = bgp[code(i - 2)];
+        const double codonScale = (double)bgc * codonKeep;
+        ES ps = (i > 0) ? pre[i - 1] : es_zero();
+        ES pd = (i >= 2) ? pre[i - 2] : es_zero();
+        ES cb;
+        if (i >= 3) cb = es_bscale(pre[i - 3], codonScale);
+        else if (i == 2) cb = es_scale(es_normalize((double)bgc, 0), frame);
+        else if (i == 1) cb = es_scale(es_pow2(-4), frame);
+        else cb = es_scale(es_pow2(-2), frame);
+        pre[i] = es_badd(cb, es_badd(es_bscale(ps, singleEmit), es_bscale(pd, doubleEmit)));
+    }
+    // buildNullTotalAndScales, dummest-dp-backward.hh:103-129
+    ES perBase, perBaseInv, s1, s2, sC;
+    {
+        const double keepSingle = 1.0 - single;
+        const double keepBoth = keepSingle - dbl;
+        ES ts = es_scale(pre[n - 2], keepSingle * 0.25);
+        ES td = es_scale(pre[n - 3], keepBoth * 0.0625);
+        ES total = es_add(es_add(pre[n - 1], ts), td);
+        const float nullLog = (total.m == 0.0f) ? -INFINITY : (float)(log2((double)total.m) + (double)total.e);
+        double perBaseLog = -(double)nullLog / (double)(float)n;
+        if (!(perBaseLog < 120.0)) perBaseLog = 120.0;
+        const double raw = exp2(perBaseLog);
+        perBase = es_normalize(raw, 0);
+        perBaseInv = es_normalize(1.0 / raw, 0);
+        s1 = es_mul(perBase, es_normalize(0.25, 0));
+        s2 = es_mul(es_mul(perBase, perBase), es_normalize(0.0625, 0));
+        sC = es_mul(es_mul(perBase, perBase), perBase);
+    }
+    // buildNullSeedModels, dummest-dp-backward.hh:133-165, and the records.
+    {
+        ES running = es_pow2(0);
+        for (int t = 0; t < n - 1; t++) running = es_bmul(running, perBase);
+        for (int j = 0; j < W; j++) {
+            const float ssd = es_to_float_clamped(es_bmul(suf[j + 1], running));
+            running = es_bmul(running, perBaseInv);
+            br[j].ss = ssd;  // suffix seed (prefix seed filled below)
+        }
+        running = perBase;
+        for (int j = 0; j < W; j++) {
+            const float psd = es_to_float_clamped(es_bmul(pre[j], running));
+            running = es_bmul(running, perBase);
+            br[j].ps = psd;
+        }
+    }
+    for (int j = -kRPad; j < n + kRPad; j++) {
+        const bool in = (j >= 0 && j < n);
+        const float ss = in ? br[j].ss : 0.f;
+        const float ps = in ? br[j].ps : 0.f;
+        const int cb = code(j + 1), cf = code(j - 2);
+        br[j] = BRec{bgp[cb], ss, ps, cb};
+        fr[j] = FRec{bgp[cf], ps, cf, 0};
+    }
+    a.pairs[pi].s1 = es_to_float_clamped(s1);
+    a.pairs[pi].s2 = es_to_float_clamped(s2);
+    a.pairs[pi].sC = es_to_float_clamped(sC);
+    a.pairs[pi].status = 0;
+}
+
+// ---------------------------------------------------------------------------
+// DPP neighbour reads. Backward reads the lane below (k+1), forward the lane
+// above (k-1); the group's edge lane reads 0 (bound_ctrl).
+// row_shl/row_shr act inside 16-lane rows (G = 16), wave_shl/wave_shr across
+// the whole wave (G = 64).
+// ---------------------------------------------------------------------------
+//
+// G = 32 uses the whole-wave shifts, so the edge lanes of the two groups read
+// the other group's values. Those reads are harmless: the group's edge rows are
+// row M or past it, and row M has zero delete/enter coefficients
+// (dummest-profile.hh:166-191 leaves them unset for i == length), so a finite
+// neighbour value is multiplied by zero. The score-only filter never rescales
+// and can hold +inf, so there the edge lane is zeroed explicitly.
+template <int G> __device__ __forceinline__ float from_below(float v) {
+    constexpr int ctrl = (G == 16) ? 0x101 : 0x130;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, true));
+}
+template <int G> __device__ __forceinline__ float from_above(float v) {
+    constexpr int ctrl = (G == 16) ? 0x111 : 0x138;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, true));
+}
+
+// Exact running maximum of float values whose power-of-two offset changes
+// rarely: cur holds the max at offset off; (sm, se) the exact max of all
+// values retired at earlier offsets.
+struct ExpMax {
+    float cur;
+    int off;
+    float sm;
+    int se;
+    __device__ __forceinline__ void init(int o) { cur = 0.f; off = o; sm = 0.f; se = 0; }
+    __device__ __forceinline__ void retire() {
+        if (cur > 0.f) {
+            int ex;
+            float m = frexpf(cur, &ex);
+            ES c{m, ex + off};
+            if (es_less(ES{sm, se}, c)) { sm = c.m; se = c.e; }
+        }
+    }
+    __device__ __forceinline__ void rebase(int newOff) {
+        retire();
+        off = newOff;
+        if (sm == 0.f) { cur = 0.f; return; }
+        const int d = se - newOff;
+        cur = d > 127 ? FLT_MAX : (d < -150 ? 0.f : ldexpf(sm, d));
+        if (!(cur <= FLT_MAX)) cur = FLT_MAX;
+    }
+};
+
+// Exact ExpScore max over the G lanes of a group; result valid in every lane.
+template <int G> __device__ __forceinline__ ES group_max(ES v) {
+#pragma unroll
+    for (int s = 1; s < G; s <<= 1) {
+        ES o;
+        o.m = __shfl_xor(v.m, s, 64);
+        o.e = __shfl_xor(v.e, s, 64);
+        if (es_less(v, o)) v = o;
+    }
+    return v;
+}
+
+constexpr float kBig = 1.2676506e30f;  // 2^100: rescale trigger
+constexpr float kStep = 5.421011e-20f;  // 2^-64
+constexpr int kPre = 100;               // bulk-mode prescale of stored W1 (2^-100)
+
+template <int G, int R, int MODE>
+__global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
+    constexpr int P = 64 / G;
+    constexpr int ROWS = G * R;
+    extern __shared__ float etab[];  // [kNCode][ROWS]
+    const int lane = threadIdx.x;
+    const int grp = lane / G;
+    const int k = lane - grp * G;
+    (void)P;
+    for (;;) {
+        int got = 0;
+        if (lane == 0) got = atomicAdd(a.unitCounter, 1);
+        const int u = __builtin_amdgcn_readfirstlane(got);
+        if (u >= a.nUnits) break;
+        const Unit U = a.units[u];
+        const ProfInfo PI = a.profs[U.prof];
+        const int M = PI.M;
+        __syncthreads();
+        for (int idx = lane; idx < kNCode * ROWS; idx += 64) {
+            const int c = idx / ROWS, r = idx - c * ROWS;
+            etab[idx] = (r <= M) ? a.emis[PI.emOff + (int64_t)r * kEmStride + c] : 0.f;
+        }
+        __syncthreads();
+        const bool active = grp < U.count;
+        const int pi = U.first + (active ? grp : 0);
+        const PairInfo PR = a.pairs[pi];
+        const int n = active ? PR.n : 0;
+        const BRec *br = a.brec + PR.recOff;
+        const FRec *fr = a.frec + PR.recOff;
+        const float s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
+        const int T = U.maxN + G - 1;
+        const int Tmine = n + G - 1;
+
+        // Per-row coefficients (dummest-dp-backward.hh:481-491, dummest-dp-forward.hh:72-80).
+        float enterC[R], d0[R], d1s2[R], d2s1[R], a0[R], a1s1[R], a2s2[R], b0[R], eps[R], valid[R];
+        float a0sC[R], b0sC[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const int r = k * R + q;
+            const bool ok = active && r <= M;
+            const float *pp = a.params + PI.parOff + (int64_t)(ok ? r : 0) * 11;
+            enterC[q] = ok ? pp[10] * sC : 0.f;
+            d0[q] = ok ? pp[6] : 0.f;
+            d1s2[q] = ok ? pp[7] * s2 : 0.f;
+            d2s1[q] = ok ? pp[8] * s1 : 0.f;
+            a0[q] = ok ? pp[0] : 0.f;
+            a1s1[q] = ok ? pp[1] * s1 : 0.f;
+            a2s2[q] = ok ? pp[2] * s2 : 0.f;
+            b0[q] = ok ? pp[3] : 0.f;
+            eps[q] = ok ? pp[9] : 0.f;
+            valid[q] = ok ? 1.f : 0.f;
+            a0sC[q] = a0[q] * sC;  // backward alpha0Coeff/beta0Coeff carry sC
+            b0sC[q] = b0[q] * sC;
+        }
+        const float *etk = etab + k * R;
+        float *wsc;
+        int *esc;
+        if (MODE == kDump) {
+            wsc = a.wbuf + PR.bufOff;
+            esc = a.ebuf + PR.bufOff / ROWS;
+        } else {
+            wsc = a.wbuf + (int64_t)blockIdx.x * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G));
+            esc = a.ebuf + ((int64_t)blockIdx.x * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G))) / ROWS;
+        }
+
+        // ------------------------------------------------------------- backward
+        float H[4][R], Zr[4][R], Yc[R];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int q = 0; q < R; q++) { H[s][q] = 0.f; Zr[s][q] = 0.f; }
+#pragma unroll
+        for (int q = 0; q < R; q++) Yc[q] = 0.f;
+        int Eb = 0;             // backward exponent of the current step
+        float rs = 1.f;         // 2^-Eb, the seed's row scale
+        float fbest = 0.f;      // filter mode: max W1 * prefixSeed
+        ExpMax bwdMax;
+        bwdMax.init(kPre);
+
+        auto bstep = [&](auto PHc, int t) {
+            constexpr int PH = decltype(PHc)::value;
+            constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
+            const int c = Tmine - 1 - t - k;  // column: D = Tmine-1-t
+            const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
+            const BRec rc = br[cc];
+            // lane below (k+1): its row 0 at columns c+1, c+2, c+3 and its Y at c
+            float bw1 = from_below<G>(H[S2][0]);
+            float bw2 = from_below<G>(H[S3][0]);
+            float bw3 = from_below<G>(H[S4][0]);
+            float bY = from_below<G>(Yc[0]);
+            if (G == 32 && MODE == kFilter && k == G - 1) { bw1 = 0.f; bw2 = 0.f; bw3 = 0.f; bY = 0.f; }
+            const float ssr = rc.ss * rs;
+            const float psp = rc.ps;
+            const float *et = etk + rc.code * ROWS;
+            float smax = 0.f;
+#pragma unroll
+            for (int q = R - 1; q >= 0; q--) {
+                const float e = et[q];
+                const float eE = e * enterC[q];
+                const float zg = Zr[S3][q] * rc.g;
+                float w = fmaf(bw3, eE, ssr * valid[q]);
+                w = fmaf(bw2, d1s2[q], w);
+                w = fmaf(bw1, d2s1[q], w);
+                w = fmaf(zg, a0sC[q], w);
+                w = fmaf(H[S2][q], a2s2[q], w);
+                w = fmaf(H[S1][q], a1s1[q], w);
+                w = fmaf(bY, d0[q], w);
+                const float y = fmaf(eps[q], bY, w);
+                const float z = fmaf(b0sC[q], zg, w);
+                if (MODE == kFilter) {
+                    fbest = fmaxf(fbest, w * psp);
+                } else {
+                    smax = fmaxf(smax, fmaxf(w, y));
+                    if (a.wantBwd) bwdMax.cur = fmaxf(bwdMax.cur, w * (psp * 7.888609052e-31f));
+                }
+                // row q at columns c+1, c+2, c+3 for row q-1 (same lane, same column)
+                bw1 = H[S1][q];
+                bw2 = H[S2][q];
+                bw3 = H[S3][q];
+                bY = y;
+                H[S4][q] = w;
+                Zr[S4][q] = z;
+                Yc[q] = y;
+            }
+            if (MODE != kFilter) {
+                const int D = Tmine - 1 - t;
+                if (D >= 0 && active) {
+                    float *dst = wsc + (int64_t)D * ROWS + k * R;
+#pragma unroll
+                    for (int q = 0; q < R; q++) dst[q] = (MODE == kScore) ? H[S4][q] * 7.888609052e-31f : H[S4][q];
+                    if (k == 0) esc[D] = Eb;
+                }
+                // rescale: one exponent per pair per step
+                const bool big = smax > kBig;
+                if (__builtin_expect(__any(big), 0)) {
+                    // group-wide OR of big
+                    bool gbig = big;
+#pragma unroll
+                    for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
+                    if (gbig) {
+#pragma unroll
+                        for (int s = 0; s < 4; s++)
+#pragma unroll
+                            for (int q = 0; q < R; q++) { H[s][q] *= kStep; Zr[s][q] *= kStep; }
+#pragma unroll
+                        for (int q = 0; q < R; q++) Yc[q] *= kStep;
+                        Eb += 64;
+                        rs = ldexpf(1.f, -Eb);
+                        if (a.wantBwd) bwdMax.rebase(Eb + kPre);
+                    }
+                }
+            }
+        };
+        {
+            int t = 0;
+            for (; t + 4 <= T; t += 4) {
+                bstep(std::integral_constant<int, 0>{}, t);
+                bstep(std::integral_constant<int, 1>{}, t + 1);
+                bstep(std::integral_constant<int, 2>{}, t + 2);
+                bstep(std::integral_constant<int, 3>{}, t + 3);
+            }
+            if (t < T) bstep(std::integral_constant<int, 0>{}, t++);
+            if (t < T) bstep(std::integral_constant<int, 1>{}, t++);
+            if (t < T) bstep(std::integral_constant<int, 2>{}, t++);
+        }
+        if (MODE == kFilter) {
+            // group max of the plain float best (no rescale in the filter)
+            float v = fbest;
+#pragma unroll
+            for (int s = 1; s < G; s <<= 1) v = fmaxf(v, __shfl_xor(v, s, 64));
+            if (active && k == 0) a.filterOut[pi] = v;
+            continue;
+        }
+        __syncthreads();  // W1 steps and exponents stored above are read below
+        ES bwdBest = es_zero();
+        if (a.wantBwd) {
+            bwdMax.retire();
+            bwdBest = group_max<G>(ES{bwdMax.sm, bwdMax.se});
+        }
+
+        // -------------------------------------------------------------- forward
+        float Wf[4][R], Zf[4][R];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int q = 0; q < R; q++) { Wf[s][q] = 0.f; Zf[s][q] = 0.f; }
+        float lastN = 0.f, lastY = 0.f;  // row R-1 outputs of the previous step
+        int Ef = 0;
+        float rf = 1.f;
+        int EbPrev = 0;
+        ExpMax midMax;
+        midMax.init(kPre);
+        float *w0d = (MODE == kDump) ? a.w0buf + PR.bufOff : nullptr;
+        int *efd = (MODE == kDump) ? a.fbuf + PR.bufOff / ROWS : nullptr;
+
+        auto fstep = [&](auto PHc, int t) {
+            constexpr int PH = decltype(PHc)::value;
+            constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
+            const int D = t;
+            const int c = D - k;
+            const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
+            const FRec rc = fr[cc];
+            float Nin = from_above<G>(lastN);
+            float Ya = from_above<G>(lastY);
+            const bool inD = D < Tmine;
+            const int EbD = inD ? esc[D] : EbPrev;
+            if (MODE == kScore && __builtin_expect(__any(EbD != EbPrev), 0)) {
+                if (EbD != EbPrev) midMax.rebase(Ef + EbD + kPre);
+            }
+            EbPrev = EbD;
+            float W1v[R];
+#pragma unroll
+            for (int q = 0; q < R; q++) W1v[q] = (MODE == kScore && inD) ? wsc[(int64_t)D * ROWS + k * R + q] : 0.f;
+            const float gsC = rc.g * sC;
+            const float psr = rc.ps * rf;
+            const float *et = etk + rc.code * ROWS;
+            float smax = 0.f;
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                const float w1 = Wf[S1][q], w2 = Wf[S2][q], w3 = Wf[S3][q];
+                const float e = et[q];
+                const float X = enterC[q] * e * w3;
+                const float Z0 = gsC * fmaf(b0[q], Zf[S3][q], a0[q] * w3);
+                float w0 = fmaf(a1s1[q], w1, Z0);
+                w0 = fmaf(a2s2[q], w2, w0);
+                w0 = Nin + (w0 + psr);
+                const float y = fmaf(d0[q], w0, eps[q] * Ya);
+                float nn = X + y;
+                nn = fmaf(d1s2[q], w2, nn);
+                nn = fmaf(d2s1[q], w1, nn);
+                if (MODE == kScore) midMax.cur = fmaxf(midMax.cur, w0 * W1v[q]);
+                if (MODE == kDump && inD && active) w0d[(int64_t)D * ROWS + k * R + q] = w0;
+                smax = fmaxf(smax, fmaxf(w0, y));
+                Wf[S4][q] = w0;
+                Zf[S4][q] = Z0;
+                Nin = nn;
+                Ya = y;
+            }
+            lastN = Nin;
+            lastY = Ya;
+            if (MODE == kDump && inD && active && k == 0) efd[D] = Ef;
+            const bool big = smax > kBig;
+            if (__builtin_expect(__any(big), 0)) {
+                bool gbig = big;
+#pragma unroll
+                for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
+                if (gbig) {
+#pragma unroll
+                    for (int s = 0; s < 4; s++)
+#pragma unroll
+                        for (int q = 0; q < R; q++) { Wf[s][q] *= kStep; Zf[s][q] *= kStep; }
+                    lastN *= kStep;
+                    lastY *= kStep;
+                    Ef += 64;
+                    rf = ldexpf(1.f, -Ef);
+                    if (MODE == kScore) midMax.rebase(Ef + EbPrev + kPre);
+                }
+            }
+        };
+        {
+            int t = 0;
+            for (; t + 4 <= T; t += 4) {
+                fstep(std::integral_constant<int, 0>{}, t);
+                fstep(std::integral_constant<int, 1>{}, t + 1);
+                fstep(std::integral_constant<int, 2>{}, t + 2);
+                fstep(std::integral_constant<int, 3>{}, t + 3);
+            }
+            if (t < T) fstep(std::integral_constant<int, 0>{}, t++);
+            if (t < T) fstep(std::integral_constant<int, 1>{}, t++);
+            if (t < T) fstep(std::integral_constant<int, 2>{}, t++);
+        }
+        if (MODE == kScore) {
+            midMax.retire();
+            const ES mid = group_max<G>(ES{midMax.sm, midMax.se});
+            if (active && k == 0) a.scoreOut[pi] = PairScore{mid.m, mid.e, bwdBest.m, bwdBest.e};
+        }
+    }
+}
+
+}  // namespace dm
+
+// ---------------------------------------------------------------------------
+// Launch table: (G, R) configurations compiled for gfx950.
+// ---------------------------------------------------------------------------
+namespace dm {
+
+#define DM_CFGS(X)                                                                                 \
+    X(16, 2) X(16, 3) X(16, 4) X(16, 5) X(16, 6) X(16, 7) X(16, 8)                               \
+    X(32, 2) X(32, 3) X(32, 4) X(32, 5) X(32, 6) X(32, 7) X(32, 8)                               \
+    X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 6) X(64, 8) X(64, 12) X(64, 16)
+
+int dp_configs(int *gs, int *rs, int cap) {
+    int n = 0;
+#define DM_LIST(g, r)        \
+    if (n < cap) {           \
+        gs[n] = g;           \
+        rs[n] = r;           \
+    }                        \
+    n++;
+    DM_CFGS(DM_LIST)
+#undef DM_LIST
+    return n;
+}
+
+hipError_t launch_null(const NullArgs &a, hipStream_t s) {
+    const int blocks = (a.npairs + 63) / 64;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(null_kernel, dim3(blocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// Tables above 64 KiB of LDS need the per-kernel dynamic-LDS limit raised.
+template <class K> static hipError_t launch_one(K kernel, size_t smem, int grid, hipStream_t s, const DPArgs &a) {
+    if (smem > 65536) {
+        hipError_t r = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (r != hipSuccess) return r;
+    }
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(64), smem, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_dp(int G, int R, int mode, const DPArgs &a, int grid, hipStream_t s) {
+    const size_t smem = sizeof(float) * kNCode * G * R;
+#define DM_LAUNCH(g, r)                                                                             \
+    if (G == g && R == r) {                                                                         \
+        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter>, smem, grid, s, a);         \
+        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore>, smem, grid, s, a);           \
+        return launch_one(dp_kernel<g, r, kDump>, smem, grid, s, a);                                \
+    }
+    DM_CFGS(DM_LAUNCH)
+#undef DM_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+}  // namespace dm

@@ -60,6 +60,8 @@ def _declare(L):
         getattr(L, "or_search_" + suf).argtypes = [i, _p(t), _p(t), _p(t), _p(d), _p(u8), i, d, i, i, i, d,
                                                    _p(d), _p(i), i, i, _p(f), _p(f), _p(f)]
         getattr(L, "or_search_" + suf).restype = i
+    for suf, t in (("f32", f), ("f64", d)):
+        getattr(L, "or_totals_" + suf).argtypes = [i, _p(t), _p(t), _p(t), _p(d), _p(u8), i, i, d, _p(d)]
     L.or_filter_batch_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), _p(ctypes.c_int64), _p(i), i,
                                       _p(f), i]
     L.or_fwdbwd_batch_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), _p(ctypes.c_int64), _p(i), i,
@@ -281,6 +283,21 @@ def search(p: Profile, codes: np.ndarray, min_prob_ratio: float, need_align=True
                     [tuple(int(x) for x in s) for s in segs[k:k + ns]]))
         k += ns
     return SearchResult(out, (float(gb[0]), int(gb[1])), (float(bb[0]), int(bb[1])), cb)
+
+
+def totals(p: Profile, codes: np.ndarray, opts=DEFAULT_OPTS, rescale_bits=64, thresh=None):
+    """(ln backward total, ln forward total) of all local paths."""
+    dt = p.values.dtype
+    t = _ct(dt)
+    if thresh is None:
+        thresh = 1e30 if dt == np.float32 else 1e150
+    o = np.array(opts, np.float64)
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    out = np.zeros(2, np.float64)
+    fn = lib().or_totals_f32 if dt == np.float32 else lib().or_totals_f64
+    fn(p.M, _ptr(p.values, t), _ptr(p.params, t), _ptr(p.bg, t), _ptr(o, ctypes.c_double), _ptr(c, ctypes.c_uint8),
+       len(c), rescale_bits, float(thresh), _ptr(out, ctypes.c_double))
+    return float(out[0]), float(out[1])
 
 
 def filter_batch(p: Profile, seqs: list, opts=DEFAULT_OPTS, threads=None) -> np.ndarray:

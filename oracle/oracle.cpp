@@ -55,11 +55,16 @@ struct ES {
     int32_t e = 0;
 };
 
-static inline void ftz_on() {
-    // dummest-core.hh:143-146: FTZ + DAZ on every worker thread.
-    _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
-    _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
-}
+// dummest-core.hh:143-146: the reference runs every worker thread with FTZ +
+// DAZ. Entry points set it for their duration and restore the caller's mode.
+struct FtzScope {
+    unsigned old;
+    FtzScope() : old(_mm_getcsr()) {
+        _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_ON);
+        _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_ON);
+    }
+    ~FtzScope() { _mm_setcsr(old); }
+};
 
 // dummest-score.hh:62-67
 static inline ES normalizeScore(double m, int e) {
@@ -1205,7 +1210,7 @@ static int searchImpl(int M, const F *values, const F *params, const F *bg, cons
                       int n, double minProbRatio, int needAlign, int wantBwd, int rescaleBits, double thresh,
                       double *hitScalars /*cap x 6: lnratio, m, e, a1, a2, nseg*/, int *segs /*segCap x 3*/,
                       int hitCap, int segCap, float *gbest /*m,e*/, float *bwdBest, float *colBest /*n x (m,e,row)*/) {
-    ftz_on();
+    FtzScope ftz_;
     Rates<F> R;
     R.init(opts);
     Prof<F> P = mkProf<F>(M, values, params, bg);
@@ -1242,6 +1247,49 @@ static int searchImpl(int M, const F *values, const F *params, const F *bg, cons
     return nh;
 }
 
+// Total path mass computed both ways (ln): forward sum_{i,j} w0(i,j)*suffixSeed(j)
+// and backward sum_{i,j} prefixSeed(j)*W1(i,j). Equal for a consistent pair of
+// recurrences; used to pin the DP restatement.
+template <class F>
+static void totalsImpl(int M, const F *values, const F *params, const F *bg, const double *opts, const uint8_t *codes,
+                       int n, int rescaleBits, double thresh, double *out) {
+    FtzScope ftz_;
+    Rates<F> R;
+    R.init(opts);
+    Prof<F> P = mkProf<F>(M, values, params, bg);
+    Work<F> Wk;
+    std::vector<Sim> sims;
+    search<F>(Wk, sims, P, codes, n, (F)-2.0, true, false, R, (F)thresh, rescaleBits);
+    // backward total: W1 stored at row scale bcnt[i]
+    double bmax = -INFINITY, fmax = -INFINITY;
+    std::vector<double> bl, fl;
+    for (int i = 0; i <= M; i++)
+        for (int j = 0; j < n; j++) {
+            double w = (double)*Wk.w1(i, j) * (double)Wk.N.prefixSeed[j];
+            if (w > 0) {
+                double l = std::log(w) + (double)rescaleBits * Wk.bcnt[i] * 0.6931471805599453;
+                bl.push_back(l);
+                bmax = std::max(bmax, l);
+            }
+        }
+    double bs = 0;
+    for (double l : bl) bs += std::exp(l - bmax);
+    out[0] = bmax + std::log(bs);
+    // forward total from the per-row forward values is not kept by search();
+    // use the column totals forwardSide (sum over rows of w0 at scale fcnt[M]).
+    for (int j = 0; j < n; j++) {
+        double w = (double)Wk.fwdSide[j] * (double)Wk.N.suffixSeed[j];
+        if (w > 0) {
+            double l = std::log(w) + (double)rescaleBits * Wk.fcnt[M] * 0.6931471805599453;
+            fl.push_back(l);
+            fmax = std::max(fmax, l);
+        }
+    }
+    double fs = 0;
+    for (double l : fl) fs += std::exp(l - fmax);
+    out[1] = fmax + std::log(fs);
+}
+
 extern "C" {
 
 int or_finalize_f32(int M, float *values, float *params, float *bg, const double *opts, int bgType) {
@@ -1262,7 +1310,7 @@ void or_decode(const char *dna, int n, uint8_t *codes) { decodeDna(dna, n, codes
 #define OR_NULL(F, SUF)                                                                                          \
     void or_null_##SUF(const F *bg29, const uint8_t *codes, int n, const double *opts, F *prefixSeed,          \
                        F *suffixSeed, F *scales, float *prefixM, int *prefixE, float *suffixM, int *suffixE) {  \
-        ftz_on();                                                                                                \
+        FtzScope ftz_;                                                                                                \
         Rates<F> R;                                                                                              \
         R.init(opts);                                                                                            \
         Null<F> N;                                                                                               \
@@ -1280,14 +1328,14 @@ OR_NULL(double, f64)
 
 double or_filter_f32(int M, const float *values, const float *params, const float *bg, const double *opts,
                      const uint8_t *codes, int n) {
-    ftz_on();
+    FtzScope ftz_;
     Rates<float> R;
     R.init(opts);
     return (double)filterScore<float>(mkProf<float>(M, values, params, bg), codes, n, R);
 }
 double or_filter_f64(int M, const double *values, const double *params, const double *bg, const double *opts,
                      const uint8_t *codes, int n) {
-    ftz_on();
+    FtzScope ftz_;
     Rates<double> R;
     R.init(opts);
     return filterScore<double>(mkProf<double>(M, values, params, bg), codes, n, R);
@@ -1300,7 +1348,7 @@ void or_filter_batch_f32(int M, const float *values, const float *params, const 
                          int nthreads) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (int s = 0; s < nseq; s++) {
-        ftz_on();
+        FtzScope ftz_;
         Rates<float> R;
         R.init(opts);
         out[s] = filterScore<float>(mkProf<float>(M, values, params, bg), codes + offsets[s], lens[s], R);
@@ -1322,6 +1370,15 @@ int or_search_f64(int M, const double *values, const double *params, const doubl
                               thresh, hits, segs, hitCap, segCap, gbest, bwdBest, colBest);
 }
 
+void or_totals_f32(int M, const float *values, const float *params, const float *bg, const double *opts,
+                   const uint8_t *codes, int n, int rescaleBits, double thresh, double *out) {
+    totalsImpl<float>(M, values, params, bg, opts, codes, n, rescaleBits, thresh, out);
+}
+void or_totals_f64(int M, const double *values, const double *params, const double *bg, const double *opts,
+                   const uint8_t *codes, int n, int rescaleBits, double thresh, double *out) {
+    totalsImpl<double>(M, values, params, bg, opts, codes, n, rescaleBits, thresh, out);
+}
+
 // Batched full search (scores only): per pair (global best mid, bwd best), threaded.
 // The CPU baseline's fwd+bwd leg.
 void or_fwdbwd_batch_f32(int M, const float *values, const float *params, const float *bg, const double *opts,
@@ -1329,7 +1386,7 @@ void or_fwdbwd_batch_f32(int M, const float *values, const float *params, const 
                          float *bwdOut, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (int s = 0; s < nseq; s++) {
-        ftz_on();
+        FtzScope ftz_;
         Rates<float> R;
         R.init(opts);
         Work<float> Wk;
@@ -1371,7 +1428,7 @@ void or_es_mul(float am, int ae, float bm, int be, float *om, int *oe) { ES r = 
 void or_es_bmul(float am, int ae, float bm, int be, float *om, int *oe) { ES r = batchMul({am, ae}, {bm, be}, 0); *om = r.m; *oe = r.e; }
 void or_es_scale(float am, int ae, double f, float *om, int *oe) { ES r = scaleScore({am, ae}, f); *om = r.m; *oe = r.e; }
 void or_es_bscale(float am, int ae, double f, float *om, int *oe) { ES r = batchScale({am, ae}, f); *om = r.m; *oe = r.e; }
-void or_es_promote(float v, int cum, float *om, int *oe) { ftz_on(); ES r = batchPromote<float>(v, cum); *om = r.m; *oe = r.e; }
+void or_es_promote(float v, int cum, float *om, int *oe) { FtzScope ftz_; ES r = batchPromote<float>(v, cum); *om = r.m; *oe = r.e; }
 double or_es_log(float m, int e) { return scoreToLog({m, e}); }
 void or_es_to_rowscale(float m, int e, int E, float *out) { *out = scoreToRowScale(m, e, E); }
 void or_es_from_rowscale(float s, int E, float *om, int *oe) { ES r = scoreFromRowScale(s, E); *om = r.m; *oe = r.e; }

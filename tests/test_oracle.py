@@ -1,0 +1,287 @@
+"""Pins the CPU oracle independently of its own formulas.
+
+* ExpScore primitives: the contracts of the reference's tests/test_batch_pairs.cc
+  (batch kernels equal the scalar ops bit for bit; promote edge cases;
+  row-scale round trip), restated against frexp/ldexp.
+* Decoding: the standard genetic code (known answers) and the reference's
+  quirk that the last two codes are 0.
+* Profile transform: probability identities every finalized row satisfies.
+* Null model: brute-force enumeration of all 1/2/3-nt tilings on short DNA.
+* DP: forward and backward total path mass agree (the two recurrences are
+  coded independently); power-of-two rescaling changes nothing; single-lane
+  results do not depend on padding.
+* Gumbel fits: recover known parameters from synthetic Gumbel samples.
+* Golden fixtures: the medium fixture's alignment spans as fingerprinted by
+  the reference's tests/test_overflow.sh:68 (the spans do not depend on
+  tantan masking; the score and E-value there do and are not checked).
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.common import GOLDEN, golden_hmm, read_fasta, synthetic_case
+
+import os
+
+F32 = np.float32
+
+
+# --------------------------------------------------------------------------- ExpScore
+def _rand_es(rng):
+    e = int(rng.integers(-300, 300))
+    m = float(np.float32(0.5 + 0.5 * rng.random()))
+    if m >= 1.0:
+        m = 0.5
+    return (0.0, 0) if rng.random() < 0.05 else (m, e)
+
+
+def test_batch_ops_equal_scalar_ops():
+    # tests/test_batch_pairs.cc:68-112
+    rng = np.random.default_rng(12345)
+    for _ in range(3000):
+        a, b = _rand_es(rng), _rand_es(rng)
+        assert O.es_batch_add(a, b) == O.es_add(a, b)
+        assert O.es_batch_mul(a, b) == O.es_mul(a, b)
+        f = float(10 ** rng.uniform(-30, 30))
+        assert O.es_batch_scale(a, f) == O.es_scale(a, f)
+
+
+def test_normalize_is_frexp():
+    # tests/test_batch_pairs.cc:171-192
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        x = float(10 ** rng.uniform(-200, 200))
+        m, e = O.es_normalize(x, 3)
+        fm, fe = math.frexp(x)
+        assert (m, e) == (float(np.float32(fm)), fe + 3)
+    assert O.es_normalize(0.0, 5) == (0.0, 0)
+
+
+def test_promote_edge_cases():
+    # tests/test_batch_pairs.cc:116-168: inf saturates, <=0/NaN -> 0, subnormal -> 0, counters clamp
+    assert O.es_promote(float("inf"), 2) == (float(np.nextafter(F32(1), F32(0))), 1023 + 128)
+    assert O.es_promote(-1.0, 0) == (0.0, 0)
+    assert O.es_promote(float("nan"), 0) == (0.0, 0)
+    assert O.es_promote(1e-40, 0) == (0.0, 0)
+    m, e = O.es_promote(3.0, 5000000)
+    assert (m, e) == (0.75, 2 + 64 * 1000000)
+    m, e = O.es_promote(3.0, 1)
+    assert (m, e) == (0.75, 2 + 64)
+
+
+def test_rowscale_round_trip():
+    # tests/test_batch_pairs.cc:195-206
+    rng = np.random.default_rng(3)
+    lib = O.lib()
+    import ctypes
+    for _ in range(2000):
+        m, e = _rand_es(rng)
+        if m == 0:
+            continue
+        E = e + int(rng.integers(1, 60))
+        out = ctypes.c_float()
+        lib.or_es_to_rowscale(m, e, E, ctypes.byref(out))
+        back = O._es2(lib.or_es_from_rowscale, out.value, E)
+        assert back == (m, e)
+        assert out.value == math.ldexp(m, e - E)
+
+
+# --------------------------------------------------------------------------- decode
+def test_decode_genetic_code():
+    table = {"ATG": "M", "TGG": "W", "TAA": "*", "TAG": "*", "TGA": "*", "TTT": "F", "GGG": "G", "CAT": "H",
+             "AAA": "K", "GAT": "D", "TCA": "S", "AGA": "R", "CTG": "L", "ATA": "I", "GTC": "V", "CCC": "P",
+             "ACG": "T", "GCA": "A", "TAT": "Y", "CAG": "Q", "AAC": "N", "GAG": "E", "TGC": "C", "AGT": "S"}
+    for codon, aa in table.items():
+        codes = O.decode(codon + "AA")
+        assert O.ALPHABET[codes[0]] == aa, codon
+    c = O.decode("ATGNTG")
+    assert [O.ALPHABET[x] for x in c[:4]] == ["M", "?", "?", "?"]
+    c = O.decode("ATGGCC")
+    assert list(c[-2:]) == [0, 0]  # dummest-sequence.hh:188 leaves the last two at 0
+
+
+# --------------------------------------------------------------------------- profile
+@pytest.mark.parametrize("fixture", ["herpes.hmm.gz", "MET-test.hmm.gz"])
+def test_profile_identities(fixture):
+    for rp in O.parse_hmm(golden_hmm(fixture)):
+        p = O.finalize(rp, np.float64)
+        codons = np.array([O.ALPHABET[k] for k in range(20)])
+        ncod = np.array([{"A": 4, "G": 4, "P": 4, "T": 4, "V": 4, "R": 6, "L": 6, "S": 6, "I": 3, "M": 1,
+                          "W": 1}.get(a, 2) for a in codons], float)
+        # background codon distribution sums to 1 over the 64 codons
+        assert abs((p.bg[4:24] * ncod).sum() + 3 * p.bg[27] - 1.0) < 1e-12
+        # each node's codon emissions sum to the HMMER emission mass (1) incl. stop
+        for i in range(p.M):
+            em = p.values[i, 4:24]
+            assert abs((em * ncod).sum() + 3 * p.values[i, 27] - 1.0) < 2e-5
+        # match-state exits sum to 1 (enter + insert + delete branches)
+        opts = O.DEFAULT_OPTS
+        for i in range(p.M):
+            pr = p.params[i]
+            alpha = pr[0] / (1 - pr[3]) if pr[3] < 1 else 0
+            delta = p.values[i, 2]
+            total = pr[10] + alpha + pr[1] + pr[2] + delta + pr[7] + pr[8]
+            assert abs(total - 1.0) < 1e-12
+        # last row has no delete/enter transitions (dummest-profile.hh:190-191)
+        assert np.all(p.params[p.M, 6:11] == 0)
+
+
+def test_background_geometric_mean_closed_form():
+    # identical emission rows: the geometric mean is that row
+    rng = np.random.default_rng(1)
+    text, _, _ = synthetic_case(1, 20, [], 0)
+    rp = O.parse_hmm(text)[0]
+    row = rp.raw[0, 4:24].copy()
+    rp.raw[:-1, 4:24] = row
+    p = O.finalize(rp, np.float64)
+    expect = row / row.sum()
+    ncod = np.array([{"A": 4, "G": 4, "P": 4, "T": 4, "V": 4, "R": 6, "L": 6, "S": 6, "I": 3, "M": 1,
+                      "W": 1}.get(O.ALPHABET[k], 2) for k in range(20)], float)
+    np.testing.assert_allclose(p.bg[4:24], expect * (1 - O.DEFAULT_OPTS[5]) / ncod, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------- null model
+def _brute_null(bg, codes, n, opts):
+    """All tilings of the DNA into 1/2/3-nt pieces (dummest-dp-backward.hh:6-129 semantics)."""
+    s1 = F32(opts[0]) + F32(opts[3])
+    s2 = F32(opts[1]) + F32(opts[2])
+    s1, s2 = float(s1), float(s2)
+    keep = 1 - s1 - s2
+    bgc = lambda j: float(bg[4 + codes[j]]) if 0 <= j < n else 0.0
+
+    def prefix(i):  # bases 0..i, first piece a partial codon (1/3 frame choice)
+        tot = 0.0
+        def rec(pos, w):
+            nonlocal tot
+            if pos == i + 1:
+                tot += w
+                return
+            if pos > i + 1:
+                return
+            rec(pos + 1, w * s1 * 0.25)
+            rec(pos + 2, w * s2 * 0.0625)
+            rec(pos + 3, w * keep * bgc(pos))
+        for first, wf in ((1, 0.25 / 3), (2, 0.0625 / 3), (3, bgc(0) / 3)):
+            if first <= i + 1:
+                rec(first, wf)
+        return tot
+
+    def suffix(i):  # bases i..n-1, the last piece may be cut short
+        tot = 0.0
+        def rec(pos, w):
+            nonlocal tot
+            if pos == n:
+                tot += w
+                return
+            left = n - pos
+            rec(pos + 1, w * s1 * 0.25)
+            if left >= 2:
+                rec(pos + 2, w * s2 * 0.0625)
+            else:
+                tot += w * s2 * 0.25
+            if left >= 3:
+                rec(pos + 3, w * keep * bgc(pos))
+            else:
+                tot += w * keep * 0.25 ** left
+        rec(i, 1.0)
+        return tot
+
+    return prefix, suffix
+
+
+@pytest.mark.parametrize("n", [3, 5, 8, 11])
+def test_null_model_brute_force(n):
+    text, prof, _ = synthetic_case(2, 20, [], 0)
+    rng = np.random.default_rng(n)
+    dna = "".join(rng.choice(list("ACGT"), size=n))
+    codes = O.decode(dna)
+    p64 = O.finalize(O.parse_hmm(text)[0], np.float64)
+    res = O.null_model(p64.bg, codes)
+    pre, suf = _brute_null(p64.bg, codes, n, O.DEFAULT_OPTS)
+    pm, pe = res["prefix"]
+    sm, se = res["suffix"]
+    for i in range(n):
+        assert math.isclose(math.ldexp(pm[i], int(pe[i])), pre(i), rel_tol=1e-6)
+        assert math.isclose(math.ldexp(sm[i], int(se[i])), suf(i), rel_tol=1e-6)
+    # scale^n * total == 1 by construction (dummest-dp-backward.hh:116-127)
+    keep1 = 1 - float(F32(O.DEFAULT_OPTS[0]) + F32(O.DEFAULT_OPTS[3]))
+    keepb = keep1 - float(F32(O.DEFAULT_OPTS[1]) + F32(O.DEFAULT_OPTS[2]))
+    total = pre(n - 1) + keep1 * 0.25 * pre(n - 2) + keepb * 0.0625 * pre(n - 3)
+    s1, s2, sC = res["scales"]
+    per_base = float(sC) ** (1 / 3)
+    assert math.isclose(per_base ** n * total, 1.0, rel_tol=1e-5)
+
+
+# --------------------------------------------------------------------------- DP
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("L,n", [(3, 9), (12, 40), (60, 250)])
+def test_forward_backward_totals_agree(dtype, L, n):
+    text, _, dnas = synthetic_case(40 + L, L, [n, n + 7], 1)
+    p = O.finalize(O.parse_hmm(text)[0], dtype)
+    tol = 1e-11 if dtype == np.float64 else 2e-5
+    for d in dnas:
+        c = O.decode(d)
+        b, f = O.totals(p, c)
+        assert abs(b - f) < tol * max(1.0, abs(b))
+
+
+def test_rescale_invariance():
+    # AUDIT_FORCE_RESCALE (dummest-score.hh:80-94): tiny thresholds and 8-bit steps
+    text, _, dnas = synthetic_case(77, 40, [300, 800], 1)
+    p = O.finalize(O.parse_hmm(text)[0], np.float32)
+    for d in dnas:
+        c = O.decode(d)
+        a = O.search(p, c, 1.0)
+        b = O.search(p, c, 1.0, rescale_bits=8, thresh=0.5)
+        assert a.global_best == b.global_best
+        assert [(h[3], h[4], h[5]) for h in a.hits] == [(h[3], h[4], h[5]) for h in b.hits]
+
+
+def test_float_and_double_agree_on_scores():
+    text, _, dnas = synthetic_case(5, 80, [600], 1)
+    rp = O.parse_hmm(text)[0]
+    p32, p64 = O.finalize(rp, np.float32), O.finalize(rp, np.float64)
+    c = O.decode(dnas[0])
+    a, b = O.search(p32, c, -2.0, need_align=False), O.search(p64, c, -2.0, need_align=False)
+    assert abs(O.es_log(*a.global_best) - O.es_log(*b.global_best)) < 1e-3
+    assert abs(O.es_log(*a.bwd_best) - math.log(O.filter_score(p64, c))) < 1e-3
+
+
+# --------------------------------------------------------------------------- Gumbel
+def test_gumbel_fits_recover_parameters():
+    rng = np.random.default_rng(9)
+    lam, mu, L = 0.7, 8.0, 3000
+    x = mu + rng.gumbel(0, 1 / lam, size=200000)
+    g = O.gumbel(x, L)
+    assert abs(g["mmL"] - lam) / lam < 0.01
+    assert abs(g["mlL"] - lam) / lam < 0.01
+    assert abs(g["lmL"] - lam) / lam < 0.02
+    # K = exp(lambda * mu_location) / L for the Gumbel location parameter
+    assert abs(math.log(g["mmK"] * L) - lam * mu) < 0.05
+
+
+# --------------------------------------------------------------------------- golden fixture
+def test_medium_fixture_spans():
+    # tests/test_overflow.sh:68 spans TRAIN.Herpes_MCP,428,504,+;medium0,1994,1510,+
+    rp = O.parse_hmm(golden_hmm("herpes.hmm.gz"))[0]
+    p = O.finalize(rp, np.float32)
+    name, dna = read_fasta(os.path.join(GOLDEN, "medium.fa"))[0]
+    r = O.search(p, O.decode(dna), 1.0)
+    top = max((h for h in r.hits if h[5]), key=lambda h: h[0])
+    segs = top[5]
+    start1, start2 = segs[0][0], segs[0][1]
+    span1 = segs[-1][0] + segs[-1][2] - start1
+    span2 = segs[-1][1] + segs[-1][2] - start2
+    assert (start1, span1, start2, span2) == (428, 504, 1994, 1510)
+
+
+def test_calibration_sequences_deterministic():
+    text, p, _ = synthetic_case(3, 50, [], 0)
+    a = O.calib_sequence(p.bg, 500, 100, 7)
+    b = O.calib_sequence(p.bg, 500, 100, 7)
+    c = O.calib_sequence(p.bg, 500, 100, 8)
+    assert a == b and a != c and len(a) == 600
+    assert a[500:] == a[:100]  # border copied from the start (dummest-calibration.hh:479-480)
