@@ -63,6 +63,7 @@ def _declare(L):
     L.dm_set_rates.argtypes = [vp, _P(d)]
     L.dm_read_hmm.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _P(i32), _P(i32)]
     L.dm_parse_hmm_text.argtypes = [ctypes.c_char_p, ctypes.c_int, _P(d), i32, _P(i32), _P(f), _P(f), _P(f), _P(i32)]
+    L.dm_add_raw_profiles.argtypes = [vp, i32, _P(i32), _P(f), ctypes.c_int, _P(i32)]
     L.dm_add_profile.argtypes = [vp, i32, _P(f), _P(f), _P(f), _P(i32)]
     L.dm_get_profile.argtypes = [vp, i32, _P(i32), _P(f), _P(f), _P(f)]
     L.dm_profile_count.argtypes = [vp]
@@ -74,7 +75,18 @@ def _declare(L):
     L.dm_pair_matrices.argtypes = [vp, i32, i32, _P(f), _P(i32), _P(f), _P(i32)]
     L.dm_pair_null.argtypes = [vp, i32, i32, _P(f), _P(f), _P(f)]
     L.dm_pair_hits.argtypes = [vp, i32, i32, d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
+    L.dm_plan_create.argtypes = [vp, ctypes.c_int, i64, _P(i32), _P(i32), ctypes.c_int, _P(vp)]
+    L.dm_plan_execute.argtypes = [vp, vp, _P(f), _P(f), _P(i32), _P(f), _P(i32)]
+    L.dm_plan_capture.argtypes = [vp, vp]
+    L.dm_plan_destroy.argtypes = [vp, vp]
+    L.dm_plan_destroy.restype = None
+    L.dm_kernel_stats.argtypes = [vp, i32, _P(i32), _P(i32), _P(d), _P(i64), _P(i64)]
+    L.dm_post_process.argtypes = [i32, _P(f), _P(f), _P(f), i32, _P(u8), _P(f), _P(i32), _P(f), _P(i32), _P(f),
+                                  _P(f), _P(d), d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
     L.dm_set_option.argtypes = [vp, ctypes.c_char_p, i64]
+    L.dm_calibrate.argtypes = [vp, i32, _P(i32), i32, i32, i32, _P(d), _P(d), _P(d)]
+    L.dm_calib_sequence.argtypes = [_P(f), _P(d), i32, i32, i32, ctypes.c_char_p]
+    L.dm_gumbel_fits.argtypes = [_P(d), i32, d, _P(d)]
     L.dm_last_dp_ms.argtypes = [vp]
     L.dm_last_dp_ms.restype = d
     L.dm_last_cells.argtypes = [vp]
@@ -115,6 +127,64 @@ def parse_hmm_text(text: str, index: int = 0, bg_type: str = "G", rates=DEFAULT_
     if rc:
         raise DummestError(f"dm_parse_hmm_text failed ({rc})")
     return m, v, p, b, cnt.value
+
+
+def post_process(M, values, params, bg, codes, w0, W1, prefix_seed, scales, min_prob_ratio,
+                 rates=DEFAULT_RATES, hit_cap=4096, seg_cap=1 << 20):
+    """Host post-gate processing on given w0/W1 ExpScore matrices (no device)."""
+    f, i32, dd = ctypes.c_float, ctypes.c_int32, ctypes.c_double
+    arr = lambda a, t: np.ascontiguousarray(a, t)
+    v, p, b = arr(values, np.float32), arr(params, np.float32), arr(bg, np.float32)
+    c = arr(codes, np.uint8)
+    w0m, w0e = arr(w0[0], np.float32), arr(w0[1], np.int32)
+    W1m, W1e = arr(W1[0], np.float32), arr(W1[1], np.int32)
+    ps, sc = arr(prefix_seed, np.float32), arr(scales, np.float32)
+    r = np.array(rates, np.float64)
+    hits = np.zeros((hit_cap, 6), np.float64)
+    segs = np.zeros((seg_cap, 3), np.int32)
+    nh, ns = ctypes.c_int32(), ctypes.c_int32()
+    rc = lib().dm_post_process(int(M), _ptr(v, f), _ptr(p, f), _ptr(b, f), len(c), _ptr(c, ctypes.c_uint8),
+                               _ptr(w0m, f), _ptr(w0e, i32), _ptr(W1m, f), _ptr(W1e, i32), _ptr(ps, f), _ptr(sc, f),
+                               _ptr(r, dd), float(min_prob_ratio), _ptr(hits, dd), hit_cap, _ptr(segs, i32), seg_cap,
+                               ctypes.byref(nh), ctypes.byref(ns))
+    if rc:
+        raise DummestError(f"dm_post_process failed ({rc})")
+    return _unpack_hits(hits, segs, nh.value)
+
+
+def _unpack_hits(hits, segs, nh):
+    out = []
+    k = 0
+    for h in hits[:nh]:
+        c = int(h[5])
+        out.append((float(h[0]), float(h[1]), int(h[2]), int(h[3]), int(h[4]),
+                    [tuple(int(x) for x in r) for r in segs[k:k + c]]))
+        k += c
+    return out
+
+
+GUMBEL_KEYS = ("mmL", "mmK", "mmKsimple", "mlL", "mlK", "mlKsimple", "lmL", "lmK")
+
+
+def calib_sequence(bg, length: int, border: int, trial: int, rates=DEFAULT_RATES) -> bytes:
+    """DNA letters of one calibration trial (host only, no device)."""
+    b = np.ascontiguousarray(bg, np.float32)
+    r = np.array(rates, np.float64)
+    buf = ctypes.create_string_buffer(length + border)
+    rc = lib().dm_calib_sequence(_ptr(b, ctypes.c_float), _ptr(r, ctypes.c_double), length, border, trial, buf)
+    if rc:
+        raise DummestError(f"dm_calib_sequence failed ({rc})")
+    return buf.raw[:length + border]
+
+
+def gumbel_fits(scores, seq_len) -> dict:
+    """estimateGumbel of a score sample (host only)."""
+    s = np.ascontiguousarray(scores, np.float64)
+    out = np.zeros(8, np.float64)
+    rc = lib().dm_gumbel_fits(_ptr(s, ctypes.c_double), len(s), float(seq_len), _ptr(out, ctypes.c_double))
+    if rc:
+        raise DummestError(f"dm_gumbel_fits failed ({rc})")
+    return dict(zip(GUMBEL_KEYS, out.tolist()))
 
 
 def es_log(m, e):
@@ -165,6 +235,15 @@ class Engine:
         self._check(lib().dm_read_hmm(self._h, path.encode(), ord(bg_type), ctypes.byref(first), ctypes.byref(count)),
                     "dm_read_hmm")
         return list(range(first.value, first.value + count.value))
+
+    def add_raw_profiles(self, Ms, raws, bg_type: str = "G") -> list:
+        """Profiles from reader-order probabilities ((M+1) x 29 each); returns ids."""
+        Ms = np.ascontiguousarray(Ms, np.int32)
+        raw = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float32).ravel() for r in raws]), np.float32)
+        first = ctypes.c_int32()
+        self._check(lib().dm_add_raw_profiles(self._h, len(Ms), _ptr(Ms, ctypes.c_int32), _ptr(raw, ctypes.c_float),
+                                              ord(bg_type), ctypes.byref(first)), "dm_add_raw_profiles")
+        return list(range(first.value, first.value + len(Ms)))
 
     def add_profile(self, M, values, params, bg) -> int:
         v = np.ascontiguousarray(values, np.float32)
@@ -279,6 +358,47 @@ class Engine:
             k += c
         return out
 
+    def plan(self, prof, seq, mode="score", want_bwd=False):
+        """Build a device-resident work queue for the pairs; returns a Plan."""
+        p, s = self._pairs(prof, seq)
+        h = ctypes.c_void_p()
+        i32 = ctypes.c_int32
+        m = 1 if mode == "score" else 0
+        self._check(lib().dm_plan_create(self._h, m, len(p), _ptr(p, i32), _ptr(s, i32), int(want_bwd),
+                                         ctypes.byref(h)), "dm_plan_create")
+        return Plan(self, h, len(p), m, want_bwd)
+
+    def calibrate(self, prof, seq_len=1000, border=100, trials=1000):
+        """estimateK for each profile id in `prof` (dummest-calibration.hh:338-600).
+
+        Returns (mid_scores, filter_scores, fits): the natural-log scores
+        (nprof x trials) and per profile a dict of the mid-anchored and
+        filter Gumbel fits (lambda, K, ... as estimateGumbel computes them)."""
+        p = np.ascontiguousarray(np.atleast_1d(prof), np.int32)
+        n = len(p)
+        mid = np.zeros((n, trials), np.float64)
+        flt = np.zeros((n, trials), np.float64)
+        fits = np.zeros((n, 16), np.float64)
+        dd = ctypes.c_double
+        self._check(lib().dm_calibrate(self._h, n, _ptr(p, ctypes.c_int32), int(seq_len), int(border), int(trials),
+                                       _ptr(mid, dd), _ptr(flt, dd), _ptr(fits, dd)), "dm_calibrate")
+        out = [{"mid": dict(zip(GUMBEL_KEYS, f[:8].tolist())), "filter": dict(zip(GUMBEL_KEYS, f[8:].tolist()))}
+               for f in fits]
+        return mid, flt, out
+
+    def kernel_stats(self):
+        """[(G, R, ms, cells, launches)] per kernel configuration of the last execution."""
+        cap = 64
+        G = np.zeros(cap, np.int32)
+        R = np.zeros(cap, np.int32)
+        ms = np.zeros(cap, np.float64)
+        cells = np.zeros(cap, np.int64)
+        lc = np.zeros(cap, np.int64)
+        i32, i64 = ctypes.c_int32, ctypes.c_int64
+        n = lib().dm_kernel_stats(self._h, cap, _ptr(G, i32), _ptr(R, i32), _ptr(ms, ctypes.c_double),
+                                  _ptr(cells, i64), _ptr(lc, i64))
+        return [(int(G[k]), int(R[k]), float(ms[k]), int(cells[k]), int(lc[k])) for k in range(min(n, cap))]
+
     @property
     def last_dp_ms(self) -> float:
         return lib().dm_last_dp_ms(self._h)
@@ -286,3 +406,41 @@ class Engine:
     @property
     def last_cells(self) -> int:
         return lib().dm_last_cells(self._h)
+
+
+class Plan:
+    """A device-resident work queue (dm_plan); execute() runs it."""
+
+    def __init__(self, engine, handle, npairs, mode, want_bwd):
+        self.engine, self._h, self.npairs, self.mode, self.want_bwd = engine, handle, npairs, mode, want_bwd
+
+    def execute(self):
+        n = self.npairs
+        f, i32 = ctypes.c_float, ctypes.c_int32
+        e = self.engine
+        if self.mode == 0:
+            best = np.zeros(n, np.float32)
+            e._check(lib().dm_plan_execute(e._h, self._h, _ptr(best, f), None, None, None, None), "dm_plan_execute")
+            return best
+        mm = np.zeros(n, np.float32)
+        me = np.zeros(n, np.int32)
+        bm = np.zeros(n, np.float32)
+        be = np.zeros(n, np.int32)
+        e._check(lib().dm_plan_execute(e._h, self._h, None, _ptr(mm, f), _ptr(me, i32), _ptr(bm, f), _ptr(be, i32)),
+                 "dm_plan_execute")
+        return mm, me, bm, be
+
+    def capture(self):
+        """Record the plan's launches into a hipGraph (single-chunk plans)."""
+        self.engine._check(lib().dm_plan_capture(self.engine._h, self._h), "dm_plan_capture")
+
+    def close(self):
+        if self._h:
+            lib().dm_plan_destroy(self.engine._h, self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -27,6 +27,9 @@ bool readHmmFile(const char *path, int bgType, const double rates[6], std::vecto
 bool readHmmText(const std::string &text, int bgType, const double rates[6], std::vector<HostProfile> &out,
                  std::string &msg);
 
+// finalizeProfile on values as read (probabilities, (M+1) x 29, reader order).
+bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg);
+
 // decodeSequence (dummest-sequence.hh:181-204) on raw DNA letters.
 void decodeDna(const char *dna, int64_t n, uint8_t *codes);
 
@@ -40,19 +43,33 @@ struct Hit {
     std::vector<Seg> segs;
 };
 
-// DP matrices of one pair as produced by the dump-mode kernel.
+// DP matrices of one pair as exact ExpScores, (M+1) x n, row-major.
 struct PostInput {
-    int M, n, G, R;
-    const float *w0, *W1;  // [(n+G-1) * G*R], step-major
-    const int *ef, *eb;    // per-step exponents
-    const uint8_t *codes;
+    int M = 0, n = 0;
+    std::vector<ES> w0, W1;
+    const uint8_t *codes = nullptr;
     std::vector<float> prefixSeed;
-    float s1, s2, sC;
-    float bgfs1, bgfs2;
+    float s1 = 0, s2 = 0, sC = 0;
+    float bgfs1 = 0, bgfs2 = 0;
 };
+
+// A stored FP32 DP value at a power-of-two offset as an ExpScore (0 for
+// non-positive values, +inf saturates as promoteRescaledScore does).
+ES esFromStored(float v, int off);
 
 // forwardPass epilogue for one pair (dummest-dp-forward.hh:252-545): gate,
 // flank totals, MEA tables, anchors, traceback, overlap rejection.
 void postProcess(const HostProfile &P, const PostInput &in, float minProbRatio, std::vector<Hit> &out);
+
+// One calibration trial's random frameshifting codon DNA (L + border letters)
+// from the profile's background letter frequencies (estimateK,
+// dummest-calibration.hh:363-376, 436-482).
+void calibSequence(const float *letterFreqs20, float bgStop, float bgfs1, float bgfs2, int L, int border, int trial,
+                   char *out);
+
+// estimateGumbel (dummest-calibration.hh:7-104) on one score sample:
+// out = MM lambda, MM K, MM K (lambda 1), ML lambda, ML K, ML K (lambda 1),
+// L-moment lambda, L-moment K.
+void gumbelFits(std::vector<double> scores, double seqLen, double out[8]);
 
 }  // namespace dm

@@ -106,6 +106,10 @@ struct DPArgs {
     int *ebuf;            // per-step backward exponents (same indexing as wbuf / ROWS)
     int *fbuf;            // dump mode: per-step forward exponents
     int64_t waveFloats;   // score mode: scratch floats per wave
+    int64_t stripSteps;   // steps per row strip in wbuf/ebuf (>= maxN + G - 1)
+    float *bnd;           // strip kernels: per-wave boundary values (6 x bndCols floats)
+    int *bndE;            // strip kernels: per-wave boundary exponents (3 x bndCols ints)
+    int64_t bndCols;      // >= maxN + 2
     float *filterOut;     // filter mode: best W1*prefixSeed per pair
     PairScore *scoreOut;  // score mode
     int wantBwd;
@@ -115,7 +119,10 @@ struct DPArgs {
 // Implemented in kernels.hip.
 int dp_configs(int *gs, int *rs, int cap);
 hipError_t launch_null(const NullArgs &a, hipStream_t s);
-hipError_t launch_dp(int G, int R, int mode, const DPArgs &a, int grid, hipStream_t s);
+hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s);
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+// Row-strip kernels (profiles longer than one group covers): G = 64 and R = kStripR.
+constexpr int kStripR = 16;
 #endif
 
 }  // namespace dm

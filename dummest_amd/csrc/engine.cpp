@@ -66,12 +66,16 @@ struct dm_engine {
     DevBuf dCodes;
     DevBuf dPairs, dBrec, dFrec, dPre, dSuf, dUnits, dCounter, dWbuf, dW0buf, dEbuf, dFbuf, dOut;
     std::vector<int> cfgG, cfgR;
+    std::vector<char> cfgStrips;  // last entry: the row-strip configuration (G = 64, R = kStripR)
+    DevBuf dBnd, dBndE;
     bool plain = false;
     int gridOpt = 0;
     int numCUs = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double lastMs = 0;
     int64_t lastCells = 0;
+    std::vector<double> cfgMs;
+    std::vector<int64_t> cfgCells, cfgLaunches;
 
     float bgfs1() const { return (float)rates[0] + (float)rates[3]; }  // INSERT1 + DELETE2 (Float)
     float bgfs2() const { return (float)rates[1] + (float)rates[2]; }  // INSERT2 + DELETE1 (Float)
@@ -140,6 +144,7 @@ static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
     double bestCost = 0;
     for (size_t c = 0; c < e->cfgG.size(); c++) {
         const int G = e->cfgG[c], R = e->cfgR[c];
+        if (e->cfgStrips[c]) continue;
         if (G * R < M + 1) continue;
         if (forcePlain && G != 64) continue;
         const double cost = (double)(n + G - 1) * R * G / 64.0;
@@ -148,7 +153,14 @@ static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
             bestCost = cost;
         }
     }
+    if (best < 0)
+        for (size_t c = 0; c < e->cfgG.size(); c++)
+            if (e->cfgStrips[c]) best = (int)c;  // profiles longer than one group: row strips
     return best;
+}
+
+static int stripsOf(const dm_engine *e, int cfg, int M) {
+    return e->cfgStrips[cfg] ? (M + e->cfgG[cfg] * e->cfgR[cfg]) / (e->cfgG[cfg] * e->cfgR[cfg]) : 1;
 }
 
 struct Chunk {
@@ -156,7 +168,7 @@ struct Chunk {
     std::vector<PairInfo> pairs;
     std::vector<Unit> units;
     std::vector<int> unitCfg;    // cfg per unit (units grouped by cfg)
-    std::vector<int> cfgFirstUnit, cfgUnits, cfgMaxN;
+    std::vector<int> cfgFirstUnit, cfgUnits, cfgMaxN, cfgMaxS;
     int64_t records = 0;
 };
 
@@ -168,6 +180,7 @@ static void buildChunk(const dm_engine *E, const std::vector<int64_t> &sorted, c
     C.cfgFirstUnit.assign(ncfg, 0);
     C.cfgUnits.assign(ncfg, 0);
     C.cfgMaxN.assign(ncfg, 0);
+    C.cfgMaxS.assign(ncfg, 1);
     // units per cfg
     std::vector<std::vector<std::vector<int64_t>>> unitsByCfg(ncfg);
     size_t i = b;
@@ -213,6 +226,7 @@ static void buildChunk(const dm_engine *E, const std::vector<int64_t> &sorted, c
                 C.order.push_back(pidx);
             }
             C.cfgMaxN[cfg] = std::max(C.cfgMaxN[cfg], U.maxN);
+            C.cfgMaxS[cfg] = std::max(C.cfgMaxS[cfg], stripsOf(E, cfg, E->profiles[U.prof].M));
             C.units.push_back(U);
             C.unitCfg.push_back(cfg);
         }
@@ -258,9 +272,55 @@ static DPArgs baseArgs(dm_engine *e) {
     return a;
 }
 
-// Run filter or score mode over all caller pairs.
-static int runPairs(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, const int32_t *seq, int wantBwd,
-                    float *filterOut, PairScore *scoreOut) {
+// A plan: the work queue of a pair list, built once and kept on the device.
+// Executing it runs the null-model kernel and the DP kernels chunk by chunk.
+struct Launch {
+    int cfg;
+    bool strips;
+    int grid;
+    DPArgs args;
+    int64_t cells;
+};
+
+struct DevChunk {
+    DevBuf pairs, units;
+    int npairs = 0;
+    int64_t records = 0;
+    std::vector<int64_t> order;
+    std::vector<Launch> launches;
+};
+
+// A plan: the work queue of a pair list, built once; its device arrays and
+// scratch belong to the plan, so an execution only launches kernels (and can
+// be captured into a hipGraph when the plan has one chunk).
+struct dm_plan {
+    int mode = kScore;
+    int wantBwd = 0;
+    int64_t npairs = 0;
+    int64_t cells = 0;
+    int64_t maxChunk = 0;
+    std::vector<DevChunk> chunks;
+    DevBuf out, brec, frec, pre, suf, counter, wbuf, ebuf, bnd, bndE;
+    void *hostOut = nullptr;  // pinned staging for results
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    void release() {
+        for (auto &c : chunks) {
+            c.pairs.release();
+            c.units.release();
+        }
+        for (DevBuf *b : {&out, &brec, &frec, &pre, &suf, &counter, &wbuf, &ebuf, &bnd, &bndE}) b->release();
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (hostOut) (void)hipHostFree(hostOut);
+        graphExec = nullptr;
+        graph = nullptr;
+        hostOut = nullptr;
+    }
+};
+
+static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, const int32_t *seq, int wantBwd,
+                     dm_plan &P) {
     if (npairs < 0 || (npairs > 0 && (!prof || !seq))) return fail(e, DM_EINVAL, "bad pair arrays");
     int rc = syncProfiles(e);
     if (rc) return rc;
@@ -269,15 +329,17 @@ static int runPairs(dm_engine *e, int mode, int64_t npairs, const int32_t *prof,
     const int nprof = (int)e->profiles.size();
     const int nseq = (int)e->seqLen.size();
     std::vector<int> cfgOf(npairs);
-    int64_t cells = 0;
+    P.mode = mode;
+    P.wantBwd = wantBwd;
+    P.npairs = npairs;
+    P.cells = 0;
     for (int64_t i = 0; i < npairs; i++) {
         if (prof[i] < 0 || prof[i] >= nprof || seq[i] < 0 || seq[i] >= nseq)
             return fail(e, DM_EINVAL, "pair references an unknown profile or sequence");
         const int M = e->profiles[prof[i]].M, n = e->seqLen[seq[i]];
         if (n < 3) return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
         cfgOf[i] = chooseConfig(e, M, n, e->plain);
-        if (cfgOf[i] < 0) return fail(e, DM_EUNSUP, "profile longer than 1023 positions is not supported");
-        cells += (int64_t)(M + 1) * n;
+        P.cells += (int64_t)(M + 1) * n;
     }
     std::vector<int64_t> sorted(npairs);
     std::iota(sorted.begin(), sorted.end(), 0);
@@ -286,67 +348,211 @@ static int runPairs(dm_engine *e, int mode, int64_t npairs, const int32_t *prof,
         if (prof[a] != prof[b]) return prof[a] < prof[b];
         return e->seqLen[seq[a]] > e->seqLen[seq[b]];
     });
-    double totalMs = 0;
-    for (size_t b = 0; b < (size_t)npairs;) {
-        const size_t en = std::min((size_t)npairs, b + (e->plain ? 1 : (size_t)kChunkPairs));
+    const size_t step = e->plain ? 1 : (size_t)kChunkPairs;
+    int64_t maxRecords = 1, maxW = 1, maxE = 1, maxB = 1;
+    const size_t ncfg = e->cfgG.size();
+    for (size_t b = 0; b < (size_t)npairs; b += step) {
+        const size_t en = std::min((size_t)npairs, b + step);
         Chunk C;
         buildChunk(e, sorted, cfgOf, prof, seq, b, en, C);
-        rc = uploadChunk(e, C);
-        if (rc) return rc;
-        const size_t np = C.pairs.size();
-        if (mode == kFilter) HIPCHK(e->dOut.ensure(sizeof(float) * np));
-        else HIPCHK(e->dOut.ensure(sizeof(PairScore) * np));
-        HIPCHK(hipMemsetAsync(e->dCounter.p, 0, sizeof(int) * e->cfgG.size(), e->stream));
-        HIPCHK(hipEventRecord(e->ev0, e->stream));
-        for (size_t cfg = 0; cfg < e->cfgG.size(); cfg++) {
+        DevChunk D;
+        D.npairs = (int)C.pairs.size();
+        D.records = C.records;
+        std::vector<int64_t> cfgCells(ncfg, 0);
+        for (size_t u = 0; u < C.units.size(); u++)
+            for (int k = 0; k < C.units[u].count; k++) {
+                const PairInfo &pi = C.pairs[C.units[u].first + k];
+                cfgCells[C.unitCfg[u]] += (int64_t)(e->profiles[pi.prof].M + 1) * pi.n;
+            }
+        D.order = std::move(C.order);
+        HIPCHK(D.pairs.ensure(sizeof(PairInfo) * C.pairs.size()));
+        HIPCHK(D.units.ensure(sizeof(Unit) * std::max<size_t>(1, C.units.size())));
+        HIPCHK(hipMemcpy(D.pairs.p, C.pairs.data(), sizeof(PairInfo) * C.pairs.size(), hipMemcpyHostToDevice));
+        if (!C.units.empty())
+            HIPCHK(hipMemcpy(D.units.p, C.units.data(), sizeof(Unit) * C.units.size(), hipMemcpyHostToDevice));
+        for (size_t cfg = 0; cfg < ncfg; cfg++) {
             const int nu = C.cfgUnits[cfg];
             if (nu == 0) continue;
+            Launch L{};
+            L.cfg = (int)cfg;
+            L.strips = e->cfgStrips[cfg] != 0;
+            L.cells = cfgCells[cfg];
             const int G = e->cfgG[cfg], R = e->cfgR[cfg];
-            const int P = 64 / G, ROWS = G * R;
-            DPArgs a = baseArgs(e);
-            a.units = e->dUnits.as<Unit>() + C.cfgFirstUnit[cfg];
+            const int Pp = 64 / G, ROWS = G * R;
+            DPArgs &a = L.args;
+            a.units = D.units.as<Unit>() + C.cfgFirstUnit[cfg];
             a.nUnits = nu;
-            a.unitCounter = e->dCounter.as<int>() + cfg;
             a.wantBwd = wantBwd;
+            a.stripSteps = C.cfgMaxN[cfg] + G - 1;
             int grid = e->gridOpt > 0 ? e->gridOpt : e->numCUs * 8;
             grid = std::min(grid, nu);
             if (mode == kScore) {
-                const int64_t perPair = (int64_t)(C.cfgMaxN[cfg] + G - 1) * ROWS;
-                a.waveFloats = perPair * P;
+                a.waveFloats = (int64_t)C.cfgMaxS[cfg] * a.stripSteps * ROWS * Pp;
                 const int64_t perWaveBytes = a.waveFloats * (int64_t)sizeof(float);
                 grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, kScratchBudget / perWaveBytes));
-                HIPCHK(e->dWbuf.ensure((size_t)(a.waveFloats * grid) * sizeof(float)));
-                HIPCHK(e->dEbuf.ensure((size_t)(a.waveFloats * grid / ROWS + 1) * sizeof(int)));
-                a.wbuf = e->dWbuf.as<float>();
-                a.ebuf = e->dEbuf.as<int>();
-                a.scoreOut = e->dOut.as<PairScore>();
-            } else {
-                a.filterOut = e->dOut.as<float>();
+                maxW = std::max(maxW, a.waveFloats * grid);
+                maxE = std::max(maxE, a.waveFloats * grid / ROWS + 1);
             }
-            HIPCHK(launch_dp(G, R, mode, a, grid, e->stream));
+            if (L.strips) {
+                a.bndCols = C.cfgMaxN[cfg] + 2;
+                maxB = std::max(maxB, a.bndCols * grid);
+            }
+            L.grid = grid;
+            D.launches.push_back(L);
         }
+        maxRecords = std::max(maxRecords, C.records);
+        P.maxChunk = std::max<int64_t>(P.maxChunk, D.npairs);
+        P.chunks.push_back(std::move(D));
+    }
+    const size_t outBytes = (size_t)std::max<int64_t>(1, P.maxChunk) * sizeof(PairScore);
+    HIPCHK(P.out.ensure(outBytes));
+    HIPCHK(hipHostMalloc(&P.hostOut, outBytes, hipHostMallocDefault));
+    HIPCHK(P.brec.ensure(sizeof(BRec) * maxRecords));
+    HIPCHK(P.frec.ensure(sizeof(FRec) * maxRecords));
+    HIPCHK(P.pre.ensure(sizeof(ES) * maxRecords));
+    HIPCHK(P.suf.ensure(sizeof(ES) * maxRecords));
+    HIPCHK(P.counter.ensure(sizeof(int) * std::max<size_t>(64, ncfg)));
+    if (mode == kScore) {
+        HIPCHK(P.wbuf.ensure(sizeof(float) * maxW));
+        HIPCHK(P.ebuf.ensure(sizeof(int) * maxE));
+    }
+    HIPCHK(P.bnd.ensure(sizeof(float) * 6 * maxB));
+    HIPCHK(P.bndE.ensure(sizeof(int) * 3 * maxB));
+    for (DevChunk &D : P.chunks)
+        for (Launch &L : D.launches) {
+            DPArgs &a = L.args;
+            a.profs = e->dProfs.as<ProfInfo>();
+            a.params = e->dParams.as<float>();
+            a.emis = e->dEmis.as<float>();
+            a.pairs = D.pairs.as<PairInfo>();
+            a.brec = P.brec.as<BRec>();
+            a.frec = P.frec.as<FRec>();
+            a.unitCounter = P.counter.as<int>() + L.cfg;
+            a.wbuf = P.wbuf.as<float>();
+            a.ebuf = P.ebuf.as<int>();
+            a.bnd = P.bnd.as<float>();
+            a.bndE = P.bndE.as<int>();
+            a.scoreOut = P.out.as<PairScore>();
+            a.filterOut = P.out.as<float>();
+        }
+    return DM_OK;
+}
+
+// Enqueue one chunk: null model, counter reset, DP launches, result copy.
+static int enqueueChunk(dm_engine *e, dm_plan &P, DevChunk &D, std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> *evs) {
+    NullArgs na{};
+    na.codes = e->dCodes.as<uint8_t>();
+    na.bg = e->dBg.as<float>();
+    na.pairs = D.pairs.as<PairInfo>();
+    na.brec = P.brec.as<BRec>();
+    na.frec = P.frec.as<FRec>();
+    na.scratchPre = P.pre.as<ES>();
+    na.scratchSuf = P.suf.as<ES>();
+    na.npairs = D.npairs;
+    na.bgfs1 = e->bgfs1();
+    na.bgfs2 = e->bgfs2();
+    HIPCHK(launch_null(na, e->stream));
+    HIPCHK(launch_zero_ints(P.counter.as<int>(), (int)e->cfgG.size(), e->stream));
+    for (Launch &L : D.launches) {
+        hipEvent_t b0 = nullptr, b1 = nullptr;
+        if (evs) {
+            HIPCHK(hipEventCreate(&b0));
+            HIPCHK(hipEventCreate(&b1));
+            HIPCHK(hipEventRecord(b0, e->stream));
+        }
+        HIPCHK(launch_dp(e->cfgG[L.cfg], e->cfgR[L.cfg], P.mode, L.strips, L.args, L.grid, e->stream));
+        if (evs) {
+            HIPCHK(hipEventRecord(b1, e->stream));
+            evs->push_back({L.cfg, {b0, b1}});
+        }
+    }
+    const size_t bytes = (size_t)D.npairs * (P.mode == kFilter ? sizeof(float) : sizeof(PairScore));
+    HIPCHK(hipMemcpyAsync(P.hostOut, P.out.p, bytes, hipMemcpyDeviceToHost, e->stream));
+    return DM_OK;
+}
+
+static void scatterChunk(const dm_plan &P, const DevChunk &D, float *filterOut, PairScore *scoreOut) {
+    for (int k = 0; k < D.npairs; k++) {
+        const int64_t dst = D.order[k];
+        if (P.mode == kFilter) filterOut[dst] = ((const float *)P.hostOut)[k];
+        else scoreOut[dst] = ((const PairScore *)P.hostOut)[k];
+    }
+}
+
+static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *scoreOut) {
+    const size_t ncfg = e->cfgG.size();
+    e->cfgMs.assign(ncfg, 0.0);
+    e->cfgCells.assign(ncfg, 0);
+    e->cfgLaunches.assign(ncfg, 0);
+    double totalMs = 0;
+    if (P.graphExec) {
+        HIPCHK(hipEventRecord(e->ev0, e->stream));
+        HIPCHK(hipGraphLaunch(P.graphExec, e->stream));
         HIPCHK(hipEventRecord(e->ev1, e->stream));
-        std::vector<char> host(np * (mode == kFilter ? sizeof(float) : sizeof(PairScore)));
-        HIPCHK(hipMemcpyAsync(host.data(), e->dOut.p, host.size(), hipMemcpyDeviceToHost, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-        totalMs += ms;
-        for (size_t k = 0; k < np; k++) {
-            const int64_t dst = C.order[k];
-            if (mode == kFilter) filterOut[dst] = ((const float *)host.data())[k];
-            else scoreOut[dst] = ((const PairScore *)host.data())[k];
+        totalMs = ms;
+        scatterChunk(P, P.chunks[0], filterOut, scoreOut);
+        e->lastMs = totalMs;
+        e->lastCells = P.cells;
+        return DM_OK;
+    }
+    for (DevChunk &D : P.chunks) {
+        std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+        int rc = enqueueChunk(e, P, D, &evs);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        for (auto &ev : evs) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev.second.first, ev.second.second));
+            e->cfgMs[ev.first] += ms;
+            totalMs += ms;
+            (void)hipEventDestroy(ev.second.first);
+            (void)hipEventDestroy(ev.second.second);
         }
-        b = en;
+        for (const Launch &L : D.launches) {
+            e->cfgCells[L.cfg] += L.cells;
+            e->cfgLaunches[L.cfg] += 1;
+        }
+        scatterChunk(P, D, filterOut, scoreOut);
     }
     e->lastMs = totalMs;
-    e->lastCells = cells;
+    e->lastCells = P.cells;
     return DM_OK;
+}
+
+// Capture a single-chunk plan's launches into a hipGraph; later executions
+// replay the graph.
+static int capturePlan(dm_engine *e, dm_plan &P) {
+    if (P.chunks.size() != 1) return fail(e, DM_EUNSUP, "only single-chunk plans can be captured");
+    if (P.graphExec) return DM_OK;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueueChunk(e, P, P.chunks[0], nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t r = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (r != hipSuccess) return fail(e, DM_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(r));
+    P.graph = g;
+    HIPCHK(hipGraphInstantiate(&P.graphExec, g, nullptr, nullptr, 0));
+    return DM_OK;
+}
+
+static int runPairs(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, const int32_t *seq, int wantBwd,
+                    float *filterOut, PairScore *scoreOut) {
+    dm_plan P;
+    int rc = buildPlan(e, mode, npairs, prof, seq, wantBwd, P);
+    if (rc == DM_OK) rc = executePlan(e, P, filterOut, scoreOut);
+    P.release();
+    return rc;
 }
 
 // Dump the DP matrices of one pair (dump mode, one unit, one wave).
 struct PairDump {
-    int G, R, M, n;
+    int G, R, M, n, T;
     std::vector<float> w0, W1;      // [(n+G-1) * G*R]
     std::vector<int> ef, eb;        // [(n+G-1)]
     std::vector<BRec> brec;         // columns [-kRPad, n+kRPad)
@@ -363,21 +569,21 @@ static int dumpPair(dm_engine *e, int32_t prof, int32_t seq, PairDump &D) {
     const int M = e->profiles[prof].M, n = e->seqLen[seq];
     if (n < 3) return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
     const int cfg = chooseConfig(e, M, n, false);
-    if (cfg < 0) return fail(e, DM_EUNSUP, "profile longer than 1023 positions is not supported");
     std::vector<int64_t> sorted{0};
     std::vector<int> cfgOf{cfg};
     Chunk C;
     buildChunk(e, sorted, cfgOf, &prof, &seq, 0, 1, C);
     const int G = e->cfgG[cfg], R = e->cfgR[cfg], ROWS = G * R;
+    const int S = stripsOf(e, cfg, M);
     const int64_t T = n + G - 1;
     C.pairs[0].bufOff = 0;
     rc = uploadChunk(e, C);
     if (rc) return rc;
-    HIPCHK(e->dWbuf.ensure(sizeof(float) * T * ROWS));
-    HIPCHK(e->dW0buf.ensure(sizeof(float) * T * ROWS));
-    HIPCHK(e->dEbuf.ensure(sizeof(int) * T));
-    HIPCHK(e->dFbuf.ensure(sizeof(int) * T));
-    HIPCHK(hipMemsetAsync(e->dCounter.p, 0, sizeof(int), e->stream));
+    HIPCHK(e->dWbuf.ensure(sizeof(float) * S * T * ROWS));
+    HIPCHK(e->dW0buf.ensure(sizeof(float) * S * T * ROWS));
+    HIPCHK(e->dEbuf.ensure(sizeof(int) * S * T));
+    HIPCHK(e->dFbuf.ensure(sizeof(int) * S * T));
+    HIPCHK(launch_zero_ints(e->dCounter.as<int>(), 1, e->stream));
     DPArgs a = baseArgs(e);
     a.units = e->dUnits.as<Unit>();
     a.nUnits = 1;
@@ -386,21 +592,31 @@ static int dumpPair(dm_engine *e, int32_t prof, int32_t seq, PairDump &D) {
     a.w0buf = e->dW0buf.as<float>();
     a.ebuf = e->dEbuf.as<int>();
     a.fbuf = e->dFbuf.as<int>();
-    HIPCHK(launch_dp(G, R, kDump, a, 1, e->stream));
+    a.stripSteps = T;
+    const bool strips = e->cfgStrips[cfg] != 0;
+    if (strips) {
+        a.bndCols = n + 2;
+        HIPCHK(e->dBnd.ensure((size_t)(6 * a.bndCols) * sizeof(float)));
+        HIPCHK(e->dBndE.ensure((size_t)(3 * a.bndCols) * sizeof(int)));
+        a.bnd = e->dBnd.as<float>();
+        a.bndE = e->dBndE.as<int>();
+    }
+    HIPCHK(launch_dp(G, R, kDump, strips, a, 1, e->stream));
     D.G = G;
     D.R = R;
     D.M = M;
     D.n = n;
-    D.w0.resize(T * ROWS);
-    D.W1.resize(T * ROWS);
-    D.ef.resize(T);
-    D.eb.resize(T);
+    D.T = (int)T;
+    D.w0.resize(S * T * ROWS);
+    D.W1.resize(S * T * ROWS);
+    D.ef.resize(S * T);
+    D.eb.resize(S * T);
     D.brec.resize(n + 2 * kRPad);
     PairInfo pi{};
-    HIPCHK(hipMemcpyAsync(D.w0.data(), e->dW0buf.p, sizeof(float) * T * ROWS, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(D.W1.data(), e->dWbuf.p, sizeof(float) * T * ROWS, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(D.ef.data(), e->dFbuf.p, sizeof(int) * T, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(D.eb.data(), e->dEbuf.p, sizeof(int) * T, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(D.w0.data(), e->dW0buf.p, sizeof(float) * D.w0.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(D.W1.data(), e->dWbuf.p, sizeof(float) * D.W1.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(D.ef.data(), e->dFbuf.p, sizeof(int) * D.ef.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(D.eb.data(), e->dEbuf.p, sizeof(int) * D.eb.size(), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(D.brec.data(), e->dBrec.as<BRec>(), sizeof(BRec) * D.brec.size(), hipMemcpyDeviceToHost,
                           e->stream));
     HIPCHK(hipMemcpyAsync(&pi, e->dPairs.p, sizeof(PairInfo), hipMemcpyDeviceToHost, e->stream));
@@ -409,6 +625,16 @@ static int dumpPair(dm_engine *e, int32_t prof, int32_t seq, PairDump &D) {
     D.s2 = pi.s2;
     D.sC = pi.sC;
     return DM_OK;
+}
+
+// Cell (i, j) of a dump: strip i / ROWS, lane k = (i % ROWS) / R, step D = j + k.
+static inline size_t dumpIndex(const PairDump &D, int i, int j, size_t *step) {
+    const int ROWS = D.G * D.R;
+    const int s = i / ROWS, ii = i - s * ROWS;
+    const int k = ii / D.R, q = ii - k * D.R;
+    const size_t st = (size_t)s * D.T + (size_t)(j + k);
+    *step = st;
+    return st * ROWS + (size_t)k * D.R + q;
 }
 
 static inline ES esFromFloat(float v, int off) {
@@ -444,7 +670,11 @@ int dm_engine_create(int device, dm_engine **out) {
     for (int i = 0; i < nc && i < 64; i++) {
         e->cfgG.push_back(gs[i]);
         e->cfgR.push_back(rs[i]);
+        e->cfgStrips.push_back(0);
     }
+    e->cfgG.push_back(64);
+    e->cfgR.push_back(kStripR);
+    e->cfgStrips.push_back(1);
     *out = e;
     return DM_OK;
 }
@@ -452,6 +682,8 @@ int dm_engine_create(int device, dm_engine **out) {
 void dm_engine_destroy(dm_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
+    e->dBnd.release();
+    e->dBndE.release();
     for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dBrec, &e->dFrec,
                       &e->dPre, &e->dSuf, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
                       &e->dOut})
@@ -497,6 +729,31 @@ int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], int3
     if (values) std::memcpy(values, p.values.data(), sizeof(float) * p.values.size());
     if (params) std::memcpy(params, p.params.data(), sizeof(float) * p.params.size());
     if (bg) std::memcpy(bg, p.bg.data(), sizeof(float) * p.bg.size());
+    return DM_OK;
+}
+
+int dm_add_raw_profiles(dm_engine *e, int32_t nprof, const int32_t *M, const float *raw, int bg_type,
+                        int32_t *first_id) {
+    if (!e || nprof < 0 || (nprof > 0 && (!M || !raw))) return DM_EINVAL;
+    if (first_id) *first_id = (int32_t)e->profiles.size();
+    size_t off = 0;
+    for (int32_t k = 0; k < nprof; k++) {
+        if (M[k] < 2) return fail(e, DM_EINVAL, "profile shorter than 2 positions");
+        HostProfile p;
+        p.M = M[k];
+        p.name = "profile" + std::to_string(e->profiles.size());
+        p.values.assign(raw + off, raw + off + (size_t)(M[k] + 1) * kWidth);
+        off += (size_t)(M[k] + 1) * kWidth;
+        p.consensus.resize(M[k]);
+        for (int i = 0; i < M[k]; i++) {
+            const float *r = &p.values[(size_t)i * kWidth + 4];
+            p.consensus[i] = (char)(std::max_element(r, r + 20) - r);
+        }
+        std::string msg;
+        if (!finalizeRaw(p, bg_type, e->rates, msg)) return fail(e, DM_EFORMAT, msg);
+        e->profiles.push_back(std::move(p));
+    }
+    e->profDirty = true;
     return DM_OK;
 }
 
@@ -589,12 +846,10 @@ int dm_pair_matrices(dm_engine *e, int32_t prof, int32_t seq, float *w0_m, int32
     PairDump D;
     int rc = dumpPair(e, prof, seq, D);
     if (rc) return rc;
-    const int ROWS = D.G * D.R;
     for (int i = 0; i <= D.M; i++) {
-        const int k = i / D.R, q = i - k * D.R;
         for (int j = 0; j < D.n; j++) {
-            const int d = j + k;
-            const size_t src = (size_t)d * ROWS + (size_t)k * D.R + q;
+            size_t d;
+            const size_t src = dumpIndex(D, i, j, &d);
             const size_t dst = (size_t)i * D.n + j;
             ES a = esFromFloat(D.w0[src], D.ef[d]);
             ES b = esFromFloat(D.W1[src], D.eb[d]);
@@ -623,6 +878,44 @@ int dm_pair_null(dm_engine *e, int32_t prof, int32_t seq, float *prefix_seed, fl
     return DM_OK;
 }
 
+static int writeHits(dm_engine *e, const std::vector<Hit> &out, double *hits, int32_t hit_cap, int32_t *segs,
+                     int32_t seg_cap, int32_t *nhits, int32_t *nsegs);
+
+int dm_post_process(int32_t M, const float *values, const float *params, const float *bg, int32_t n,
+                    const uint8_t *codes, const float *w0_m, const int32_t *w0_e, const float *W1_m,
+                    const int32_t *W1_e, const float *prefix_seed, const float *scales, const double rates[6],
+                    double min_prob_ratio, double *hits, int32_t hit_cap, int32_t *segs, int32_t seg_cap,
+                    int32_t *nhits, int32_t *nsegs) {
+    if (M < 2 || n < 3 || !values || !params || !bg || !codes || !w0_m || !w0_e || !W1_m || !W1_e ||
+        !prefix_seed || !scales || !rates || !hits || !segs || !nhits || !nsegs)
+        return DM_EINVAL;
+    HostProfile P;
+    P.M = M;
+    P.values.assign(values, values + (size_t)(M + 1) * kWidth);
+    P.params.assign(params, params + (size_t)(M + 2) * 11);
+    P.bg.assign(bg, bg + kWidth);
+    PostInput in;
+    in.M = M;
+    in.n = n;
+    const size_t cells = (size_t)(M + 1) * n;
+    in.w0.resize(cells);
+    in.W1.resize(cells);
+    for (size_t c = 0; c < cells; c++) {
+        in.w0[c] = ES{w0_m[c], w0_e[c]};
+        in.W1[c] = ES{W1_m[c], W1_e[c]};
+    }
+    in.codes = codes;
+    in.prefixSeed.assign(prefix_seed, prefix_seed + n);
+    in.s1 = scales[0];
+    in.s2 = scales[1];
+    in.sC = scales[2];
+    in.bgfs1 = (float)rates[0] + (float)rates[3];
+    in.bgfs2 = (float)rates[1] + (float)rates[2];
+    std::vector<Hit> out;
+    postProcess(P, in, (float)min_prob_ratio, out);
+    return writeHits(nullptr, out, hits, hit_cap, segs, seg_cap, nhits, nsegs);
+}
+
 int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio, double *hits, int32_t hit_cap,
                  int32_t *segs, int32_t seg_cap, int32_t *nhits, int32_t *nsegs) {
     if (!e || !hits || !segs || !nhits || !nsegs) return DM_EINVAL;
@@ -634,12 +927,16 @@ int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio,
     PostInput in;
     in.M = D.M;
     in.n = D.n;
-    in.R = D.R;
-    in.G = D.G;
-    in.w0 = D.w0.data();
-    in.W1 = D.W1.data();
-    in.ef = D.ef.data();
-    in.eb = D.eb.data();
+    in.w0.resize((size_t)(D.M + 1) * D.n);
+    in.W1.resize((size_t)(D.M + 1) * D.n);
+    for (int i = 0; i <= D.M; i++) {
+        for (int j = 0; j < D.n; j++) {
+            size_t d;
+            const size_t src = dumpIndex(D, i, j, &d);
+            in.w0[(size_t)i * D.n + j] = esFromStored(D.w0[src], D.ef[d]);
+            in.W1[(size_t)i * D.n + j] = esFromStored(D.W1[src], D.eb[d]);
+        }
+    }
     in.codes = e->codes.data() + e->seqOff[seq];
     in.prefixSeed.resize(D.n);
     for (int j = 0; j < D.n; j++) in.prefixSeed[j] = D.brec[j + kRPad].ps;
@@ -650,6 +947,11 @@ int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio,
     in.bgfs2 = e->bgfs2();
     std::vector<Hit> out;
     postProcess(P, in, (float)min_prob_ratio, out);
+    return writeHits(e, out, hits, hit_cap, segs, seg_cap, nhits, nsegs);
+}
+
+static int writeHits(dm_engine *e, const std::vector<Hit> &out, double *hits, int32_t hit_cap, int32_t *segs,
+                     int32_t seg_cap, int32_t *nhits, int32_t *nsegs) {
     int32_t nh = 0, ns = 0;
     for (const Hit &h : out) {
         if (nh >= hit_cap) return fail(e, DM_EINVAL, "hit capacity exceeded");
@@ -674,11 +976,157 @@ int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio,
     return DM_OK;
 }
 
+int dm_plan_create(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, const int32_t *seq, int want_bwd,
+                   dm_plan **out) {
+    if (!e || !out || (mode != kFilter && mode != kScore)) return DM_EINVAL;
+    HIPCHK(hipSetDevice(e->device));
+    dm_plan *P = new dm_plan();
+    int rc = buildPlan(e, mode, npairs, prof, seq, want_bwd, *P);
+    if (rc) {
+        P->release();
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return DM_OK;
+}
+
+int dm_plan_execute(dm_engine *e, dm_plan *P, float *best, float *mid_m, int32_t *mid_e, float *bwd_m, int32_t *bwd_e) {
+    if (!e || !P) return DM_EINVAL;
+    HIPCHK(hipSetDevice(e->device));
+    int rc = syncProfiles(e);
+    if (rc) return rc;
+    rc = syncSequences(e);
+    if (rc) return rc;
+    if (P->mode == kFilter) {
+        if (P->npairs > 0 && !best) return DM_EINVAL;
+        return executePlan(e, *P, best, nullptr);
+    }
+    if (P->npairs > 0 && (!mid_m || !mid_e)) return DM_EINVAL;
+    std::vector<PairScore> out(P->npairs);
+    rc = executePlan(e, *P, nullptr, out.data());
+    if (rc) return rc;
+    for (int64_t i = 0; i < P->npairs; i++) {
+        mid_m[i] = out[i].midM;
+        mid_e[i] = out[i].midE;
+        if (P->wantBwd && bwd_m && bwd_e) {
+            bwd_m[i] = out[i].bwdM;
+            bwd_e[i] = out[i].bwdE;
+        }
+    }
+    return DM_OK;
+}
+
+int dm_plan_capture(dm_engine *e, dm_plan *P) {
+    if (!e || !P) return DM_EINVAL;
+    HIPCHK(hipSetDevice(e->device));
+    int rc = syncProfiles(e);
+    if (rc) return rc;
+    rc = syncSequences(e);
+    if (rc) return rc;
+    return capturePlan(e, *P);
+}
+
+void dm_plan_destroy(dm_engine *e, dm_plan *P) {
+    if (!P) return;
+    if (e) (void)hipSetDevice(e->device);
+    P->release();
+    delete P;
+}
+
+int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, double *ms, int64_t *cells,
+                    int64_t *launches) {
+    if (!e) return DM_EINVAL;
+    int n = 0;
+    for (size_t c = 0; c < e->cfgMs.size(); c++) {
+        if (e->cfgLaunches[c] == 0) continue;
+        if (n < cap) {
+            if (G) G[n] = e->cfgG[c];
+            if (R) R[n] = e->cfgR[c];
+            if (ms) ms[n] = e->cfgMs[c];
+            if (cells) cells[n] = e->cfgCells[c];
+            if (launches) launches[n] = e->cfgLaunches[c];
+        }
+        n++;
+    }
+    return n;
+}
+
 int dm_set_option(dm_engine *e, const char *name, int64_t value) {
     if (!e || !name) return DM_EINVAL;
     if (!std::strcmp(name, "plain")) e->plain = value != 0;
     else if (!std::strcmp(name, "grid")) e->gridOpt = (int)value;
     else return fail(e, DM_EINVAL, std::string("unknown option ") + name);
+    return DM_OK;
+}
+
+int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border, int32_t trials,
+                 double *mid_scores, double *flt_scores, double *fits) {
+    if (!e || nprof < 1 || !prof || !fits) return DM_EINVAL;
+    if (seq_len < 1 || border < 0 || trials < 2 || (int64_t)seq_len + border < 3 ||
+        (int64_t)seq_len + border > (1 << 30))
+        return fail(e, DM_EINVAL, "calibration needs seq_len >= 1, border >= 0, trials >= 2");
+    for (int32_t k = 0; k < nprof; k++)
+        if (prof[k] < 0 || prof[k] >= (int32_t)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
+    HIPCHK(hipSetDevice(e->device));
+    // The trial sequences go into the engine's store for the duration of the
+    // call and are removed afterwards.
+    const size_t codes0 = e->codes.size(), seqs0 = e->seqLen.size();
+    const int len = seq_len + border;
+    const int64_t npairs = (int64_t)nprof * trials;
+    std::vector<int32_t> pp(npairs), ss(npairs);
+    std::vector<char> dna(len);
+    e->codes.reserve(codes0 + (size_t)npairs * len);
+    for (int32_t k = 0; k < nprof; k++) {
+        const HostProfile &P = e->profiles[prof[k]];
+        for (int32_t t = 0; t < trials; t++) {
+            calibSequence(P.bg.data() + 4, (float)e->rates[5], e->bgfs1(), e->bgfs2(), seq_len, border, t, dna.data());
+            const size_t at = e->codes.size();
+            e->codes.resize(at + len);
+            decodeDna(dna.data(), len, e->codes.data() + at);
+            pp[(size_t)k * trials + t] = prof[k];
+            ss[(size_t)k * trials + t] = (int32_t)e->seqLen.size();
+            e->seqOff.push_back((int64_t)at);
+            e->seqLen.push_back(len);
+        }
+    }
+    e->seqDirty = true;
+    std::vector<PairScore> out(npairs);
+    int rc = runPairs(e, kScore, npairs, pp.data(), ss.data(), 1, nullptr, out.data());
+    e->codes.resize(codes0);
+    e->seqOff.resize(seqs0);
+    e->seqLen.resize(seqs0);
+    e->seqDirty = true;
+    if (rc) return rc;
+    auto lnOf = [](float m, int ex) {  // scoreToLog, dummest-score.hh:201-206
+        if (m == 0.0f) return -INFINITY * 1.0;
+        return (double)std::log(m) + (double)ex * 0.6931471805599453;
+    };
+    std::vector<double> mid(trials), flt(trials);
+    for (int32_t k = 0; k < nprof; k++) {
+        for (int32_t t = 0; t < trials; t++) {
+            const PairScore &s = out[(size_t)k * trials + t];
+            mid[t] = lnOf(s.midM, s.midE);
+            flt[t] = lnOf(s.bwdM, s.bwdE);
+            if (mid_scores) mid_scores[(size_t)k * trials + t] = mid[t];
+            if (flt_scores) flt_scores[(size_t)k * trials + t] = flt[t];
+        }
+        gumbelFits(mid, (double)seq_len, fits + (size_t)k * 16);
+        gumbelFits(flt, (double)seq_len, fits + (size_t)k * 16 + 8);
+    }
+    return DM_OK;
+}
+
+int dm_calib_sequence(const float *bg, const double rates[6], int32_t L, int32_t border, int32_t trial, char *out) {
+    if (!bg || !rates || !out || L < 1 || border < 0) return DM_EINVAL;
+    const float bgfs1 = (float)rates[0] + (float)rates[3], bgfs2 = (float)rates[1] + (float)rates[2];
+    calibSequence(bg + 4, (float)rates[5], bgfs1, bgfs2, L, border, trial, out);
+    return DM_OK;
+}
+
+int dm_gumbel_fits(const double *scores, int32_t n, double seq_len, double out[8]) {
+    if (!scores || !out || n < 2) return DM_EINVAL;
+    gumbelFits(std::vector<double>(scores, scores + n), seq_len, out);
     return DM_OK;
 }
 

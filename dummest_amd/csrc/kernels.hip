@@ -309,15 +309,18 @@ constexpr float kBig = 1.2676506e30f;  // 2^100: rescale trigger
 constexpr float kStep = 5.421011e-20f;  // 2^-64
 constexpr int kPre = 100;               // bulk-mode prescale of stored W1 (2^-100)
 
-template <int G, int R, int MODE>
+// Ghost reads across row strips (STR kernels, G = 64, one pair per wave):
+// boundary buffers hold one value pair per column plus the exponent of the
+// step that produced it; index c+1 for columns c in [-1, n], zero outside [0, n).
+__device__ __forceinline__ int bidx(int c, int n) { return c < -1 ? 0 : (c > n ? n + 1 : c + 1); }
+
+template <int G, int R, int MODE, bool STR>
 __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
-    constexpr int P = 64 / G;
     constexpr int ROWS = G * R;
     extern __shared__ float etab[];  // [kNCode][ROWS]
     const int lane = threadIdx.x;
     const int grp = lane / G;
     const int k = lane - grp * G;
-    (void)P;
     for (;;) {
         int got = 0;
         if (lane == 0) got = atomicAdd(a.unitCounter, 1);
@@ -326,12 +329,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
         const Unit U = a.units[u];
         const ProfInfo PI = a.profs[U.prof];
         const int M = PI.M;
-        __syncthreads();
-        for (int idx = lane; idx < kNCode * ROWS; idx += 64) {
-            const int c = idx / ROWS, r = idx - c * ROWS;
-            etab[idx] = (r <= M) ? a.emis[PI.emOff + (int64_t)r * kEmStride + c] : 0.f;
-        }
-        __syncthreads();
+        const int S = STR ? (M + ROWS) / ROWS : 1;  // row strips of ROWS rows
         const bool active = grp < U.count;
         const int pi = U.first + (active ? grp : 0);
         const PairInfo PR = a.pairs[pi];
@@ -341,128 +339,210 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
         const float s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
         const int T = U.maxN + G - 1;
         const int Tmine = n + G - 1;
-
-        // Per-row coefficients (dummest-dp-backward.hh:481-491, dummest-dp-forward.hh:72-80).
-        float enterC[R], d0[R], d1s2[R], d2s1[R], a0[R], a1s1[R], a2s2[R], b0[R], eps[R], valid[R];
-        float a0sC[R], b0sC[R];
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            const int r = k * R + q;
-            const bool ok = active && r <= M;
-            const float *pp = a.params + PI.parOff + (int64_t)(ok ? r : 0) * 11;
-            enterC[q] = ok ? pp[10] * sC : 0.f;
-            d0[q] = ok ? pp[6] : 0.f;
-            d1s2[q] = ok ? pp[7] * s2 : 0.f;
-            d2s1[q] = ok ? pp[8] * s1 : 0.f;
-            a0[q] = ok ? pp[0] : 0.f;
-            a1s1[q] = ok ? pp[1] * s1 : 0.f;
-            a2s2[q] = ok ? pp[2] * s2 : 0.f;
-            b0[q] = ok ? pp[3] : 0.f;
-            eps[q] = ok ? pp[9] : 0.f;
-            valid[q] = ok ? 1.f : 0.f;
-            a0sC[q] = a0[q] * sC;  // backward alpha0Coeff/beta0Coeff carry sC
-            b0sC[q] = b0[q] * sC;
-        }
-        const float *etk = etab + k * R;
-        float *wsc;
-        int *esc;
+        const int64_t strideS = (int64_t)a.stripSteps * ROWS;  // floats per strip of W1 steps
+        float *wsc0;
+        int *esc0;
         if (MODE == kDump) {
-            wsc = a.wbuf + PR.bufOff;
-            esc = a.ebuf + PR.bufOff / ROWS;
+            wsc0 = a.wbuf + PR.bufOff;
+            esc0 = a.ebuf + PR.bufOff / ROWS;
         } else {
-            wsc = a.wbuf + (int64_t)blockIdx.x * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G));
-            esc = a.ebuf + ((int64_t)blockIdx.x * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G))) / ROWS;
+            const int64_t off = (int64_t)blockIdx.x * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G));
+            wsc0 = a.wbuf + off;
+            esc0 = a.ebuf + off / ROWS;
+        }
+        // strip boundary buffers (per wave): two parities of backward (W, Y, E)
+        // and one of forward (N, Y, E), n + 2 entries each
+        float *bW = nullptr, *bY2 = nullptr, *fN = nullptr, *fY = nullptr;
+        int *bE = nullptr, *fE = nullptr;
+        if (STR) {
+            const int64_t nb = (int64_t)a.bndCols;
+            float *base = a.bnd + (int64_t)blockIdx.x * 6 * nb;
+            bW = base;            // [2][nb]
+            bY2 = base + 2 * nb;  // [2][nb]
+            fN = base + 4 * nb;
+            fY = base + 5 * nb;
+            int *ebase = a.bndE + (int64_t)blockIdx.x * 3 * nb;
+            bE = ebase;           // [2][nb]
+            fE = ebase + 2 * nb;
+            for (int c = lane; c < 2 * nb; c += 64) {
+                bW[c] = 0.f;
+                bY2[c] = 0.f;
+                bE[c] = 0;
+            }
+            for (int c = lane; c < nb; c += 64) {
+                fN[c] = 0.f;
+                fY[c] = 0.f;
+                fE[c] = 0;
+            }
         }
 
-        // ------------------------------------------------------------- backward
-        float H[4][R], Zr[4][R], Yc[R];
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-            for (int q = 0; q < R; q++) { H[s][q] = 0.f; Zr[s][q] = 0.f; }
-#pragma unroll
-        for (int q = 0; q < R; q++) Yc[q] = 0.f;
-        int Eb = 0;             // backward exponent of the current step
-        float rs = 1.f;         // 2^-Eb, the seed's row scale
-        float fbest = 0.f;      // filter mode: max W1 * prefixSeed
+        float enterC[R], d0[R], d1s2[R], d2s1[R], a1s1[R], a2s2[R], eps[R];
+        float fbest = 0.f;  // filter mode: max W1 * prefixSeed
         ExpMax bwdMax;
         bwdMax.init(kPre);
+        const float *etk = etab + k * R;
 
-        auto bstep = [&](auto PHc, int t) {
-            constexpr int PH = decltype(PHc)::value;
-            constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
-            const int c = Tmine - 1 - t - k;  // column: D = Tmine-1-t
-            const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
-            const BRec rc = br[cc];
-            // lane below (k+1): its row 0 at columns c+1, c+2, c+3 and its Y at c
-            float bw1 = from_below<G>(H[S2][0]);
-            float bw2 = from_below<G>(H[S3][0]);
-            float bw3 = from_below<G>(H[S4][0]);
-            float bY = from_below<G>(Yc[0]);
-            if (G == 32 && MODE == kFilter && k == G - 1) { bw1 = 0.f; bw2 = 0.f; bw3 = 0.f; bY = 0.f; }
-            const float ssr = rc.ss * rs;
-            const float psp = rc.ps;
-            const float *et = etk + rc.code * ROWS;
-            float smax = 0.f;
-#pragma unroll
-            for (int q = R - 1; q >= 0; q--) {
-                const float e = et[q];
-                const float eE = e * enterC[q];
-                const float zg = Zr[S3][q] * rc.g;
-                float w = fmaf(bw3, eE, ssr * valid[q]);
-                w = fmaf(bw2, d1s2[q], w);
-                w = fmaf(bw1, d2s1[q], w);
-                w = fmaf(zg, a0sC[q], w);
-                w = fmaf(H[S2][q], a2s2[q], w);
-                w = fmaf(H[S1][q], a1s1[q], w);
-                w = fmaf(bY, d0[q], w);
-                const float y = fmaf(eps[q], bY, w);
-                const float z = fmaf(b0sC[q], zg, w);
-                if (MODE == kFilter) {
-                    fbest = fmaxf(fbest, w * psp);
-                } else {
-                    smax = fmaxf(smax, fmaxf(w, y));
-                    if (a.wantBwd) bwdMax.cur = fmaxf(bwdMax.cur, w * (psp * 7.888609052e-31f));
-                }
-                // row q at columns c+1, c+2, c+3 for row q-1 (same lane, same column)
-                bw1 = H[S1][q];
-                bw2 = H[S2][q];
-                bw3 = H[S3][q];
-                bY = y;
-                H[S4][q] = w;
-                Zr[S4][q] = z;
-                Yc[q] = y;
+        // Per-strip setup: LDS emission rows and per-row coefficients
+        // (dummest-dp-backward.hh:481-491, dummest-dp-forward.hh:72-80).
+        auto loadStrip = [&](int strip, float *a0v, float *b0v, float *valid) {
+            const int rb = strip * ROWS;
+            __syncthreads();
+            for (int idx = lane; idx < kNCode * ROWS; idx += 64) {
+                const int c = idx / ROWS, r = rb + idx - c * ROWS;
+                etab[idx] = (r <= M) ? a.emis[PI.emOff + (int64_t)r * kEmStride + c] : 0.f;
             }
-            if (MODE != kFilter) {
-                const int D = Tmine - 1 - t;
-                if (D >= 0 && active) {
-                    float *dst = wsc + (int64_t)D * ROWS + k * R;
+            __syncthreads();
 #pragma unroll
-                    for (int q = 0; q < R; q++) dst[q] = (MODE == kScore) ? H[S4][q] * 7.888609052e-31f : H[S4][q];
-                    if (k == 0) esc[D] = Eb;
-                }
-                // rescale: one exponent per pair per step
-                const bool big = smax > kBig;
-                if (__builtin_expect(__any(big), 0)) {
-                    // group-wide OR of big
-                    bool gbig = big;
-#pragma unroll
-                    for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
-                    if (gbig) {
-#pragma unroll
-                        for (int s = 0; s < 4; s++)
-#pragma unroll
-                            for (int q = 0; q < R; q++) { H[s][q] *= kStep; Zr[s][q] *= kStep; }
-#pragma unroll
-                        for (int q = 0; q < R; q++) Yc[q] *= kStep;
-                        Eb += 64;
-                        rs = ldexpf(1.f, -Eb);
-                        if (a.wantBwd) bwdMax.rebase(Eb + kPre);
-                    }
-                }
+            for (int q = 0; q < R; q++) {
+                const int r = rb + k * R + q;
+                const bool ok = active && r <= M;
+                const float *pp = a.params + PI.parOff + (int64_t)(ok ? r : 0) * 11;
+                enterC[q] = ok ? pp[10] * sC : 0.f;
+                d0[q] = ok ? pp[6] : 0.f;
+                d1s2[q] = ok ? pp[7] * s2 : 0.f;
+                d2s1[q] = ok ? pp[8] * s1 : 0.f;
+                a0v[q] = ok ? pp[0] : 0.f;
+                a1s1[q] = ok ? pp[1] * s1 : 0.f;
+                a2s2[q] = ok ? pp[2] * s2 : 0.f;
+                b0v[q] = ok ? pp[3] : 0.f;
+                eps[q] = ok ? pp[9] : 0.f;
+                if (valid) valid[q] = ok ? 1.f : 0.f;
             }
         };
-        {
+
+        // ------------------------------------------------------------- backward
+        for (int strip = S - 1; strip >= 0; strip--) {
+            float a0sC[R], b0sC[R], valid[R];
+            loadStrip(strip, a0sC, b0sC, valid);
+#pragma unroll
+            for (int q = 0; q < R; q++) {  // backward alpha0Coeff/beta0Coeff carry sC
+                a0sC[q] *= sC;
+                b0sC[q] *= sC;
+            }
+            float *wsc = wsc0 + (int64_t)strip * strideS;
+            int *esc = esc0 + (int64_t)strip * a.stripSteps;
+            const bool below = STR && strip < S - 1;  // a strip below feeds lane G-1
+            const bool above = STR && strip > 0;      // lane 0 feeds the strip above
+            const int par = strip & 1;
+            float *wOut = STR ? bW + par * a.bndCols : nullptr;
+            float *yOut = STR ? bY2 + par * a.bndCols : nullptr;
+            int *eOut = STR ? bE + par * a.bndCols : nullptr;
+            const float *wIn = STR ? bW + (par ^ 1) * a.bndCols : nullptr;
+            const float *yIn = STR ? bY2 + (par ^ 1) * a.bndCols : nullptr;
+            const int *eIn = STR ? bE + (par ^ 1) * a.bndCols : nullptr;
+
+            float H[4][R], Zr[4][R], Yc[R], GW[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) {
+                GW[s4] = 0.f;
+#pragma unroll
+                for (int q = 0; q < R; q++) { H[s4][q] = 0.f; Zr[s4][q] = 0.f; }
+            }
+#pragma unroll
+            for (int q = 0; q < R; q++) Yc[q] = 0.f;
+            int Eb = 0;      // backward exponent of the current step
+            float rs = 1.f;  // 2^-Eb, the seed's row scale
+
+            auto bstep = [&](auto PHc, int t) {
+                constexpr int PH = decltype(PHc)::value;
+                constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
+                const int c = Tmine - 1 - t - k;  // column: D = Tmine-1-t
+                const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
+                const BRec rc = br[cc];
+                // lane below (k+1): its row 0 at columns c+1, c+2, c+3 and its Y at c
+                float bw1 = from_below<G>(H[S2][0]);
+                float bw2 = from_below<G>(H[S3][0]);
+                float bw3 = from_below<G>(H[S4][0]);
+                float bY = from_below<G>(Yc[0]);
+                if (G == 32 && MODE == kFilter && k == G - 1) { bw1 = 0.f; bw2 = 0.f; bw3 = 0.f; bY = 0.f; }
+                if (STR) {
+                    // last lane: the strip below's top row from the boundary buffer
+                    const int bi = bidx(c, n);
+                    float gw = 0.f, gy = 0.f;
+                    if (below) {
+                        const int sh = eIn[bi] - Eb;
+                        gw = ldexpf(wIn[bi], sh);
+                        gy = ldexpf(yIn[bi], sh);
+                    }
+                    GW[S4] = gw;
+                    if (k == G - 1) {
+                        bw1 = GW[S1];
+                        bw2 = GW[S2];
+                        bw3 = GW[S3];
+                        bY = gy;
+                    }
+                }
+                const float ssr = rc.ss * rs;
+                const float psp = rc.ps;
+                const float *et = etk + rc.code * ROWS;
+                float smax = 0.f;
+#pragma unroll
+                for (int q = R - 1; q >= 0; q--) {
+                    const float e = et[q];
+                    const float eE = e * enterC[q];
+                    const float zg = Zr[S3][q] * rc.g;
+                    float w = fmaf(bw3, eE, ssr * valid[q]);
+                    w = fmaf(bw2, d1s2[q], w);
+                    w = fmaf(bw1, d2s1[q], w);
+                    w = fmaf(zg, a0sC[q], w);
+                    w = fmaf(H[S2][q], a2s2[q], w);
+                    w = fmaf(H[S1][q], a1s1[q], w);
+                    w = fmaf(bY, d0[q], w);
+                    const float y = fmaf(eps[q], bY, w);
+                    const float z = fmaf(b0sC[q], zg, w);
+                    if (MODE == kFilter) {
+                        fbest = fmaxf(fbest, w * psp);
+                    } else {
+                        smax = fmaxf(smax, fmaxf(w, y));
+                        if (a.wantBwd) bwdMax.cur = fmaxf(bwdMax.cur, w * (psp * 7.888609052e-31f));
+                    }
+                    // row q at columns c+1, c+2, c+3 for row q-1 (same lane, same column)
+                    bw1 = H[S1][q];
+                    bw2 = H[S2][q];
+                    bw3 = H[S3][q];
+                    bY = y;
+                    H[S4][q] = w;
+                    Zr[S4][q] = z;
+                    Yc[q] = y;
+                }
+                if (STR && above && k == 0 && c >= 0 && c < n) {
+                    wOut[c + 1] = H[S4][0];
+                    yOut[c + 1] = Yc[0];
+                    eOut[c + 1] = Eb;
+                }
+                if (MODE != kFilter) {
+                    const int D = Tmine - 1 - t;
+                    if (D >= 0 && active) {
+                        float *dst = wsc + (int64_t)D * ROWS + k * R;
+#pragma unroll
+                        for (int q = 0; q < R; q++)
+                            dst[q] = (MODE == kScore) ? H[S4][q] * 7.888609052e-31f : H[S4][q];
+                        if (k == 0) esc[D] = Eb;
+                    }
+                }
+                if (MODE != kFilter) {
+                    // rescale: one exponent per pair per step
+                    const bool big = smax > kBig;
+                    if (__builtin_expect(__any(big), 0)) {
+                        bool gbig = big;
+#pragma unroll
+                        for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
+                        if (gbig) {
+#pragma unroll
+                            for (int s4 = 0; s4 < 4; s4++) {
+                                GW[s4] *= kStep;
+#pragma unroll
+                                for (int q = 0; q < R; q++) { H[s4][q] *= kStep; Zr[s4][q] *= kStep; }
+                            }
+#pragma unroll
+                            for (int q = 0; q < R; q++) Yc[q] *= kStep;
+                            Eb += 64;
+                            rs = ldexpf(1.f, -Eb);
+                            if (a.wantBwd) bwdMax.rebase(Eb + kPre);
+                        }
+                    }
+                }
+            };
             int t = 0;
             for (; t + 4 <= T; t += 4) {
                 bstep(std::integral_constant<int, 0>{}, t);
@@ -473,6 +553,8 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             if (t < T) bstep(std::integral_constant<int, 0>{}, t++);
             if (t < T) bstep(std::integral_constant<int, 1>{}, t++);
             if (t < T) bstep(std::integral_constant<int, 2>{}, t++);
+            if (STR) __syncthreads();  // boundary written above is read by the next strip
+            if (a.wantBwd && MODE != kFilter) bwdMax.rebase(kPre);  // next strip starts at exponent 0
         }
         if (MODE == kFilter) {
             // group max of the plain float best (no rescale in the filter)
@@ -490,85 +572,108 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
         }
 
         // -------------------------------------------------------------- forward
-        float Wf[4][R], Zf[4][R];
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-            for (int q = 0; q < R; q++) { Wf[s][q] = 0.f; Zf[s][q] = 0.f; }
-        float lastN = 0.f, lastY = 0.f;  // row R-1 outputs of the previous step
-        int Ef = 0;
-        float rf = 1.f;
-        int EbPrev = 0;
         ExpMax midMax;
         midMax.init(kPre);
-        float *w0d = (MODE == kDump) ? a.w0buf + PR.bufOff : nullptr;
-        int *efd = (MODE == kDump) ? a.fbuf + PR.bufOff / ROWS : nullptr;
+        for (int strip = 0; strip < S; strip++) {
+            float a0[R], b0[R];
+            loadStrip(strip, a0, b0, nullptr);
+            const float *wsc = wsc0 + (int64_t)strip * strideS;
+            const int *esc = esc0 + (int64_t)strip * a.stripSteps;
+            const bool above = STR && strip > 0;      // lane 0 reads the strip above
+            const bool below = STR && strip < S - 1;  // last lane feeds the strip below
+            float Wf[4][R], Zf[4][R];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++)
+#pragma unroll
+                for (int q = 0; q < R; q++) { Wf[s4][q] = 0.f; Zf[s4][q] = 0.f; }
+            float lastN = 0.f, lastY = 0.f;  // row R-1 outputs of the previous step
+            int Ef = 0;
+            float rf = 1.f;
+            int EbPrev = 0;
+            if (MODE == kScore) midMax.rebase(Ef + EbPrev + kPre);
+            float *w0d = (MODE == kDump) ? a.w0buf + PR.bufOff + (int64_t)strip * strideS : nullptr;
+            int *efd = (MODE == kDump) ? a.fbuf + PR.bufOff / ROWS + (int64_t)strip * a.stripSteps : nullptr;
 
-        auto fstep = [&](auto PHc, int t) {
-            constexpr int PH = decltype(PHc)::value;
-            constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
-            const int D = t;
-            const int c = D - k;
-            const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
-            const FRec rc = fr[cc];
-            float Nin = from_above<G>(lastN);
-            float Ya = from_above<G>(lastY);
-            const bool inD = D < Tmine;
-            const int EbD = inD ? esc[D] : EbPrev;
-            if (MODE == kScore && __builtin_expect(__any(EbD != EbPrev), 0)) {
-                if (EbD != EbPrev) midMax.rebase(Ef + EbD + kPre);
-            }
-            EbPrev = EbD;
-            float W1v[R];
-#pragma unroll
-            for (int q = 0; q < R; q++) W1v[q] = (MODE == kScore && inD) ? wsc[(int64_t)D * ROWS + k * R + q] : 0.f;
-            const float gsC = rc.g * sC;
-            const float psr = rc.ps * rf;
-            const float *et = etk + rc.code * ROWS;
-            float smax = 0.f;
-#pragma unroll
-            for (int q = 0; q < R; q++) {
-                const float w1 = Wf[S1][q], w2 = Wf[S2][q], w3 = Wf[S3][q];
-                const float e = et[q];
-                const float X = enterC[q] * e * w3;
-                const float Z0 = gsC * fmaf(b0[q], Zf[S3][q], a0[q] * w3);
-                float w0 = fmaf(a1s1[q], w1, Z0);
-                w0 = fmaf(a2s2[q], w2, w0);
-                w0 = Nin + (w0 + psr);
-                const float y = fmaf(d0[q], w0, eps[q] * Ya);
-                float nn = X + y;
-                nn = fmaf(d1s2[q], w2, nn);
-                nn = fmaf(d2s1[q], w1, nn);
-                if (MODE == kScore) midMax.cur = fmaxf(midMax.cur, w0 * W1v[q]);
-                if (MODE == kDump && inD && active) w0d[(int64_t)D * ROWS + k * R + q] = w0;
-                smax = fmaxf(smax, fmaxf(w0, y));
-                Wf[S4][q] = w0;
-                Zf[S4][q] = Z0;
-                Nin = nn;
-                Ya = y;
-            }
-            lastN = Nin;
-            lastY = Ya;
-            if (MODE == kDump && inD && active && k == 0) efd[D] = Ef;
-            const bool big = smax > kBig;
-            if (__builtin_expect(__any(big), 0)) {
-                bool gbig = big;
-#pragma unroll
-                for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
-                if (gbig) {
-#pragma unroll
-                    for (int s = 0; s < 4; s++)
-#pragma unroll
-                        for (int q = 0; q < R; q++) { Wf[s][q] *= kStep; Zf[s][q] *= kStep; }
-                    lastN *= kStep;
-                    lastY *= kStep;
-                    Ef += 64;
-                    rf = ldexpf(1.f, -Ef);
-                    if (MODE == kScore) midMax.rebase(Ef + EbPrev + kPre);
+            auto fstep = [&](auto PHc, int t) {
+                constexpr int PH = decltype(PHc)::value;
+                constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
+                const int D = t;
+                const int c = D - k;
+                const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
+                const FRec rc = fr[cc];
+                float Nin = from_above<G>(lastN);
+                float Ya = from_above<G>(lastY);
+                if (STR && above) {
+                    // lane 0: the strip above's last row from the boundary buffer
+                    const int bi = bidx(c, n);
+                    const int sh = fE[bi] - Ef;
+                    const float gn = ldexpf(fN[bi], sh), gy = ldexpf(fY[bi], sh);
+                    if (k == 0) {
+                        Nin = gn;
+                        Ya = gy;
+                    }
                 }
-            }
-        };
-        {
+                const bool inD = D < Tmine;
+                const int EbD = inD ? esc[D] : EbPrev;
+                if (MODE == kScore && __builtin_expect(__any(EbD != EbPrev), 0)) {
+                    if (EbD != EbPrev) midMax.rebase(Ef + EbD + kPre);
+                }
+                EbPrev = EbD;
+                float W1v[R];
+#pragma unroll
+                for (int q = 0; q < R; q++)
+                    W1v[q] = (MODE == kScore && inD) ? wsc[(int64_t)D * ROWS + k * R + q] : 0.f;
+                const float gsC = rc.g * sC;
+                const float psr = rc.ps * rf;
+                const float *et = etk + rc.code * ROWS;
+                float smax = 0.f;
+#pragma unroll
+                for (int q = 0; q < R; q++) {
+                    const float w1 = Wf[S1][q], w2 = Wf[S2][q], w3 = Wf[S3][q];
+                    const float e = et[q];
+                    const float X = enterC[q] * e * w3;
+                    const float Z0 = gsC * fmaf(b0[q], Zf[S3][q], a0[q] * w3);
+                    float w0 = fmaf(a1s1[q], w1, Z0);
+                    w0 = fmaf(a2s2[q], w2, w0);
+                    w0 = Nin + (w0 + psr);
+                    const float y = fmaf(d0[q], w0, eps[q] * Ya);
+                    float nn = X + y;
+                    nn = fmaf(d1s2[q], w2, nn);
+                    nn = fmaf(d2s1[q], w1, nn);
+                    if (MODE == kScore) midMax.cur = fmaxf(midMax.cur, w0 * W1v[q]);
+                    if (MODE == kDump && inD && active) w0d[(int64_t)D * ROWS + k * R + q] = w0;
+                    smax = fmaxf(smax, fmaxf(w0, y));
+                    Wf[S4][q] = w0;
+                    Zf[S4][q] = Z0;
+                    Nin = nn;
+                    Ya = y;
+                }
+                lastN = Nin;
+                lastY = Ya;
+                if (STR && below && k == G - 1 && c >= 0 && c < n) {
+                    fN[c + 1] = lastN;
+                    fY[c + 1] = lastY;
+                    fE[c + 1] = Ef;
+                }
+                if (MODE == kDump && inD && active && k == 0) efd[D] = Ef;
+                const bool big = smax > kBig;
+                if (__builtin_expect(__any(big), 0)) {
+                    bool gbig = big;
+#pragma unroll
+                    for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
+                    if (gbig) {
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; s4++)
+#pragma unroll
+                            for (int q = 0; q < R; q++) { Wf[s4][q] *= kStep; Zf[s4][q] *= kStep; }
+                        lastN *= kStep;
+                        lastY *= kStep;
+                        Ef += 64;
+                        rf = ldexpf(1.f, -Ef);
+                        if (MODE == kScore) midMax.rebase(Ef + EbPrev + kPre);
+                    }
+                }
+            };
             int t = 0;
             for (; t + 4 <= T; t += 4) {
                 fstep(std::integral_constant<int, 0>{}, t);
@@ -579,6 +684,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             if (t < T) fstep(std::integral_constant<int, 0>{}, t++);
             if (t < T) fstep(std::integral_constant<int, 1>{}, t++);
             if (t < T) fstep(std::integral_constant<int, 2>{}, t++);
+            if (STR) __syncthreads();  // boundary written above is read by the next strip
         }
         if (MODE == kScore) {
             midMax.retire();
@@ -630,17 +736,42 @@ template <class K> static hipError_t launch_one(K kernel, size_t smem, int grid,
     return hipGetLastError();
 }
 
-hipError_t launch_dp(int G, int R, int mode, const DPArgs &a, int grid, hipStream_t s) {
+hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s) {
     const size_t smem = sizeof(float) * kNCode * G * R;
 #define DM_LAUNCH(g, r)                                                                             \
-    if (G == g && R == r) {                                                                         \
-        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter>, smem, grid, s, a);         \
-        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore>, smem, grid, s, a);           \
-        return launch_one(dp_kernel<g, r, kDump>, smem, grid, s, a);                                \
+    if (G == g && R == r && !strips) {                                                              \
+        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter, false>, smem, grid, s, a);  \
+        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore, false>, smem, grid, s, a);    \
+        return launch_one(dp_kernel<g, r, kDump, false>, smem, grid, s, a);                         \
+    }
+#define DM_LAUNCH_STRIPS(g, r)                                                                      \
+    if (G == g && R == r && strips) {                                                               \
+        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter, true>, smem, grid, s, a);   \
+        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore, true>, smem, grid, s, a);     \
+        return launch_one(dp_kernel<g, r, kDump, true>, smem, grid, s, a);                          \
     }
     DM_CFGS(DM_LAUNCH)
+    DM_LAUNCH_STRIPS(64, 16)
 #undef DM_LAUNCH
+#undef DM_LAUNCH_STRIPS
     return hipErrorInvalidValue;
+}
+
+}  // namespace dm
+
+namespace dm {
+
+// Clears the work-unit counters (a kernel, not a memset node, so that plans
+// captured into a hipGraph replay correctly).
+__global__ void zero_ints_kernel(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+    return hipGetLastError();
 }
 
 }  // namespace dm

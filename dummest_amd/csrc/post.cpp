@@ -1,6 +1,6 @@
 // Post-gate processing of one hit pair on the host, over the w0/W1 matrices
-// the dump-mode kernel wrote (step-major, one power-of-two exponent per
-// anti-diagonal step).
+// of the pair as exact ExpScores (the dump-mode kernel's step-major values and
+// per-step exponents, converted by the engine).
 //
 // Follows forwardPass's epilogue: anchor products and per-column best
 // (dummest-dp-forward.hh:174-206), the posterior match matrix
@@ -62,13 +62,19 @@ inline ES bmul(const ES &A, const ES &B) {  // dummest-batch.hh:216-235
     }
     return ES{(float)p, e};
 }
-inline ES fromFloat(float v, int off) {  // promote a stored value at a power-of-two offset
+}  // namespace
+
+ES esFromStored(float v, int off) {  // promote a stored value at a power-of-two offset
     if (!(v > 0.f)) return ES{0.f, 0};
     if (!(v < INFINITY)) return ES{std::nextafterf(1.0f, 0.0f), 1023 + off};
     int ex;
     const float m = std::frexp(v, &ex);
     return ES{m, ex + off};
 }
+namespace {
+
+inline ES fromFloat(float v, int off) { return esFromStored(v, off); }
+
 inline double lnOf(const ES &a) { return a.m == 0.0f ? -INFINITY : std::log(a.m) + (double)a.e * 0.6931471805599453; }
 
 inline float toRow(const ES &s, int E) {  // scoreToRowScale, dummest-scalerow.hh:26-38
@@ -111,17 +117,15 @@ struct Mat {  // ExpScore matrix with a front pad, zero outside the written rang
 }  // namespace
 
 void postProcess(const HostProfile &P, const PostInput &in, float minProbRatio, std::vector<Hit> &out) {
-    const int M = in.M, n = in.n, R = in.R, ROWS = in.G * in.R;
+    const int M = in.M, n = in.n;
     auto code = [&](int j) -> int { return (j >= 0 && j < n) ? (int)in.codes[j] : kZeroIdx; };
     auto w0at = [&](int i, int j) -> ES {
         if (j < 0 || j >= n || i < 0 || i > M) return ES{0.f, 0};
-        const int k = i / R, q = i - k * R, d = j + k;
-        return fromFloat(in.w0[(size_t)d * ROWS + k * R + q], in.ef[d]);
+        return in.w0[(size_t)i * n + j];
     };
     auto W1at = [&](int i, int j) -> ES {
         if (j < 0 || j >= n || i < 0 || i > M) return ES{0.f, 0};
-        const int k = i / R, q = i - k * R, d = j + k;
-        return fromFloat(in.W1[(size_t)d * ROWS + k * R + q], in.eb[d]);
+        return in.W1[(size_t)i * n + j];
     };
     // anchors, X, column totals
     std::vector<ES> colBest(n, ES{0.f, 0});

@@ -52,7 +52,7 @@ double backgroundMean(const float *v, int length, int step, int type) {  // myMe
     return med[length / 2];
 }
 
-bool finalize(HostProfile &p, int bgType, const double rates[6], std::string &msg) {
+bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string &msg) {
     const float ins1 = (float)rates[0], ins2 = (float)rates[1], del1 = (float)rates[2], del2 = (float)rates[3];
     const float stop = (float)rates[4], bgStop = (float)rates[5];
     const int M = p.M;
@@ -152,7 +152,7 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
                 if (cur.M < 2) return bad("profile shorter than 2 positions");
                 vals.insert(vals.end(), kWidth - 4, 0.f);
                 cur.values = vals;
-                if (!finalize(cur, bgType, rates, msg)) return false;
+                if (!finalizeImpl(cur, bgType, rates, msg)) return false;
                 out.push_back(std::move(cur));
                 cur = HostProfile();
                 vals.clear();
@@ -181,6 +181,10 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
     }
     if (state != 0) return bad("truncated profile data");
     return true;
+}
+
+bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg) {
+    return finalizeImpl(p, bgType, rates, msg);
 }
 
 bool readHmmFile(const char *path, int bgType, const double rates[6], std::vector<HostProfile> &out, std::string &msg) {

@@ -135,3 +135,139 @@ def embed(rng, host: str, insert: str) -> str:
 
 def lognormal_lengths(rng, n, median=150, sigma=0.8, lo=30, hi=2000):
     return np.clip(np.round(np.exp(np.log(median) + sigma * rng.standard_normal(n))), lo, hi).astype(int)
+
+
+# ---------------------------------------------------------------------------
+# Vectorized generators for the large workloads.
+# ---------------------------------------------------------------------------
+def raw_profile(rng, L: int, ndesc=50, sub=0.30, indel=0.02):
+    """Reader-order probabilities ((L+1) x 29) of a profile built from `ndesc`
+    simulated descendants of a random ancestor, and the ancestor.
+
+    Same model as make_profile, vectorized over descendants: substitutions are
+    uniform off-diagonal, deletions start at rate indel/2 per position with
+    geometric lengths, insertions at rate indel/2 (they only count towards
+    m->i)."""
+    anc = rng.integers(0, 20, size=L)
+    d = np.broadcast_to(anc, (ndesc, L)).copy()
+    s = rng.random((ndesc, L)) < sub
+    d[s] = (d[s] + rng.integers(1, 20, size=int(s.sum()))) % 20
+    deleted = np.zeros((ndesc, L), bool)
+    starts = np.argwhere(rng.random((ndesc, L)) < indel / 2)
+    lens = rng.geometric(0.5, size=len(starts))
+    del_ev = np.zeros(L + 1)
+    for (r, c), k in zip(starts, lens):
+        deleted[r, c:c + k] = True
+        del_ev[c] += 1
+    ins_ev = np.bincount(np.nonzero(rng.random((ndesc, L + 1)) < indel / 2)[1], minlength=L + 1).astype(float)
+    counts = np.ones((L, 20))
+    rows, cols = np.nonzero(~deleted)
+    np.add.at(counts, (cols, d[rows, cols]), 1)
+    em = counts / counts.sum(1, keepdims=True)
+    raw = np.zeros((L + 1, 29), np.float64)
+    mi = (ins_ev + 0.5) / (ndesc + 1.5)
+    md = (del_ev + 0.5) / (ndesc + 1.5)
+    md[L] = 0.0
+    raw[:, 0] = mi       # m->i
+    raw[:, 1] = 0.5      # i->i
+    raw[:, 2] = md       # m->d
+    raw[:, 3] = 0.5      # d->d
+    raw[L, 3] = 0.0
+    raw[:L, 4:24] = em   # row r holds node r+1's emissions
+    return raw.astype(np.float32), anc
+
+
+_CODON_TABLE = np.array([[ord(c) for c in CODONS[a][k % len(CODONS[a])]] for a in AA for k in range(6)],
+                        np.uint8).reshape(20, 6, 3)
+_NCOD = np.array([len(CODONS[a]) for a in AA])
+
+
+def reverse_translate_np(rng, prot: np.ndarray) -> np.ndarray:
+    """DNA bytes of a protein with uniform synonymous codons."""
+    k = (rng.random(len(prot)) * _NCOD[prot]).astype(int)
+    return _CODON_TABLE[prot, k].reshape(-1)
+
+
+def random_dna_np(rng, n: int, gc=0.5) -> np.ndarray:
+    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
+    return np.frombuffer(b"ACGT", np.uint8)[rng.choice(4, size=n, p=p)]
+
+
+def mutate_np(rng, dna: np.ndarray, indel=0.01, sub=0.0) -> np.ndarray:
+    """1-nt deletions and insertions at rate indel (half each) and substitutions."""
+    bases = np.frombuffer(b"ACGT", np.uint8)
+    r = rng.random(len(dna))
+    keep = r >= indel / 2
+    ins = (r >= indel / 2) & (r < indel)
+    out = dna.copy()
+    if sub:
+        s = rng.random(len(dna)) < sub
+        out[s] = bases[rng.integers(0, 4, size=int(s.sum()))]
+    reps = keep.astype(int) + ins.astype(int)
+    res = np.repeat(out, reps)
+    # the inserted copy becomes a random base
+    pos = np.cumsum(reps)[ins] - 1
+    res[pos] = bases[rng.integers(0, 4, size=len(pos))]
+    return res
+
+
+def workload_c4(seed=1004, n_profiles=5000, n_reads=100000, pair_rate=0.004, max_len=1023):
+    """Uneven reads against many profiles (the fwd+bwd-at-scale configuration).
+
+    profiles: lengths lognormal (median 150, sigma 0.8, clipped to [30, 2000]
+    and here also to `max_len`); reads: uniform 150-300 bp, half from a
+    profile's ancestor (reverse-translated, then 1% 1-nt indels and 0.5%
+    substitutions), half random (GC 0.5); pairs: each (read, profile) kept
+    with probability `pair_rate`.
+    Returns dict(Ms, raws, reads (uint8 DNA arrays), prof, seq)."""
+    rng = np.random.default_rng(seed)
+    Ms = np.minimum(lognormal_lengths(rng, n_profiles), max_len)
+    raws, ancs = [], []
+    for M in Ms:
+        r, a = raw_profile(rng, int(M))
+        raws.append(r)
+        ancs.append(a)
+    lens = rng.integers(150, 301, size=n_reads)
+    homolog = rng.random(n_reads) < 0.5
+    src = rng.integers(0, n_profiles, size=n_reads)
+    reads = []
+    for k in range(n_reads):
+        n = int(lens[k])
+        if homolog[k]:
+            anc = ancs[src[k]]
+            naa = min(len(anc), n // 3 + 2)
+            st = int(rng.integers(0, len(anc) - naa + 1))
+            dna = mutate_np(rng, reverse_translate_np(rng, anc[st:st + naa]), indel=0.01, sub=0.005)
+            if len(dna) < n:
+                dna = np.concatenate([dna, random_dna_np(rng, n - len(dna))])
+            reads.append(dna[:n])
+        else:
+            reads.append(random_dna_np(rng, n))
+    per = rng.binomial(n_profiles, pair_rate, size=n_reads)
+    seq = np.repeat(np.arange(n_reads, dtype=np.int32), per)
+    prof = rng.integers(0, n_profiles, size=len(seq)).astype(np.int32)
+    return dict(Ms=Ms, raws=raws, reads=reads, prof=prof, seq=seq, homolog=homolog, src=src)
+
+
+def workload_c1(seed=1001, n_profiles=1000, n_windows=10000, homolog_frac=0.2, max_len=1023):
+    """Profiles x windows, all pairs (the forward-only score configuration).
+    windows: uniform 1-10 kb, GC uniform 0.35-0.65, 20% carrying an embedded
+    frameshifted homolog of a random profile's ancestor."""
+    rng = np.random.default_rng(seed)
+    Ms = np.minimum(lognormal_lengths(rng, n_profiles), max_len)
+    raws, ancs = [], []
+    for M in Ms:
+        r, a = raw_profile(rng, int(M))
+        raws.append(r)
+        ancs.append(a)
+    reads = []
+    for k in range(n_windows):
+        n = int(rng.integers(1000, 10001))
+        dna = random_dna_np(rng, n, float(rng.uniform(0.35, 0.65)))
+        if rng.random() < homolog_frac:
+            anc = ancs[int(rng.integers(n_profiles))]
+            ins = mutate_np(rng, reverse_translate_np(rng, anc), indel=0.01)[: n // 2]
+            at = int(rng.integers(0, n - len(ins) + 1))
+            dna[at:at + len(ins)] = ins
+        reads.append(dna)
+    return dict(Ms=Ms, raws=raws, reads=reads)

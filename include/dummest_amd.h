@@ -50,6 +50,12 @@ int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, 
 int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], int32_t index, int32_t *M,
                       float *values, float *params, float *bg, int32_t *count);
 
+/* Add profiles from values as the reader stores them (probabilities, (M+1)x29
+ * per profile, concatenated: transitions m->i i->i m->d d->d then node i+1's
+ * 20 emissions, dummest-profile.hh:264-314) and finalize them. */
+int dm_add_raw_profiles(dm_engine *e, int32_t nprof, const int32_t *M, const float *raw, int bg_type,
+                        int32_t *first_id);
+
 /* Add one finalized profile: values (M+1)x29, params (M+2)x11 (alpha'[3]
  * beta'[3] delta'[3] eps' enter, Params of dummest-core.hh:233-239), bg 29
  * (bg_probs). Replaces Profile construction in readProfiles. */
@@ -100,6 +106,60 @@ int dm_pair_null(dm_engine *e, int32_t prof, int32_t seq, float *prefix_seed, fl
  * (start1, start2, length); *nhits / *nsegs receive the counts. */
 int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio, double *hits, int32_t hit_cap,
                  int32_t *segs, int32_t seg_cap, int32_t *nhits, int32_t *nsegs);
+
+/* Plans: the work queue of a pair list (configuration choice, length sort,
+ * units, device-resident pair descriptors) built once, then executed any
+ * number of times with only kernel launches on the device. mode 0 = filter
+ * scores, 1 = forward+backward scores. Replaces the per-call job list of
+ * findFinalSimilaritiesBatched (dummest-search.hh:76-285). */
+typedef struct dm_plan dm_plan;
+int dm_plan_create(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, const int32_t *seq, int want_bwd,
+                   dm_plan **out);
+/* mode 0 fills best[npairs]; mode 1 fills mid_m/mid_e (and bwd_m/bwd_e if want_bwd). */
+int dm_plan_execute(dm_engine *e, dm_plan *plan, float *best, float *mid_m, int32_t *mid_e, float *bwd_m,
+                    int32_t *bwd_e);
+void dm_plan_destroy(dm_engine *e, dm_plan *plan);
+/* Capture a single-chunk plan (up to 2^18 pairs) into a hipGraph; later
+ * dm_plan_execute calls replay the graph. The engine's profiles and sequences
+ * must not change while a captured plan is in use. */
+int dm_plan_capture(dm_engine *e, dm_plan *plan);
+
+/* Per kernel configuration of the last execution: G (lanes per pair), R (rows
+ * per lane), device milliseconds (HIP events around each launch), DP cells,
+ * launches. Returns the number of configurations used (fills up to cap). */
+int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, double *ms, int64_t *cells,
+                    int64_t *launches);
+
+/* Post-gate processing on caller-supplied DP matrices (no device): the same
+ * host code dm_pair_hits runs after its dump, fed with w0/W1 ExpScore matrices
+ * ((M+1) x n), the pair's prefixSeed[n] and scales s1 s2 sC, and the rates. */
+int dm_post_process(int32_t M, const float *values, const float *params, const float *bg, int32_t n,
+                    const uint8_t *codes, const float *w0_m, const int32_t *w0_e, const float *W1_m,
+                    const int32_t *W1_e, const float *prefix_seed, const float *scales, const double rates[6],
+                    double min_prob_ratio, double *hits, int32_t hit_cap, int32_t *segs, int32_t seg_cap,
+                    int32_t *nhits, int32_t *nsegs);
+
+/* Calibration of the E-value parameters of nprof profiles (estimateK,
+ * dummest-calibration.hh:338-600, called per profile from dummest.cc:384-408):
+ * per profile, `trials` random frameshifting codon sequences of seq_len +
+ * border letters drawn from the profile's background (trial t seeded with
+ * 5489 + t), all scored in one batched forward+backward pass. Writes the
+ * natural-log scores (mid-anchored and filter, nprof x trials each; either
+ * may be NULL) and per profile 16 fits: estimateGumbel of the mid scores
+ * (MM lambda, MM K, MM K lambda=1, ML lambda, ML K, ML K lambda=1, LM lambda,
+ * LM K) then the same 8 of the filter scores. The reference keeps lambda =
+ * fits[0], gumbelKmidAnchored = fits[1], lambda_filter = fits[8],
+ * gumbel_k_filter = fits[9]. Its end- and start-anchored samples are all
+ * -inf in this build (dummest-dp-forward.hh:536-543) and are not produced. */
+int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border, int32_t trials,
+                 double *mid_scores, double *flt_scores, double *fits);
+
+/* Host-only pieces of dm_calibrate: the DNA letters (L + border) of trial
+ * `trial` for a profile with background bg[29] (bg_probs, letters at 4..23)
+ * under the given rates (dummest-calibration.hh:436-482), and estimateGumbel
+ * of n scores (dummest-calibration.hh:7-104; out as in dm_calibrate's 8 fits). */
+int dm_calib_sequence(const float *bg, const double rates[6], int32_t L, int32_t border, int32_t trial, char *out);
+int dm_gumbel_fits(const double *scores, int32_t n, double seq_len, double out[8]);
 
 /* Options: "plain" = 1 launches the DP once per pair (the unbatched form);
  * "grid" = persistent grid size (0 = automatic). */

@@ -62,6 +62,7 @@ def _declare(L):
         getattr(L, "or_search_" + suf).restype = i
     for suf, t in (("f32", f), ("f64", d)):
         getattr(L, "or_totals_" + suf).argtypes = [i, _p(t), _p(t), _p(t), _p(d), _p(u8), i, i, d, _p(d)]
+    L.or_matrices_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), i, _p(f), _p(i), _p(f), _p(i), _p(f), _p(f)]
     L.or_filter_batch_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), _p(ctypes.c_int64), _p(i), i,
                                       _p(f), i]
     L.or_fwdbwd_batch_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), _p(ctypes.c_int64), _p(i), i,
@@ -298,6 +299,24 @@ def totals(p: Profile, codes: np.ndarray, opts=DEFAULT_OPTS, rescale_bits=64, th
     fn(p.M, _ptr(p.values, t), _ptr(p.params, t), _ptr(p.bg, t), _ptr(o, ctypes.c_double), _ptr(c, ctypes.c_uint8),
        len(c), rescale_bits, float(thresh), _ptr(out, ctypes.c_double))
     return float(out[0]), float(out[1])
+
+
+def matrices(p: Profile, codes: np.ndarray, opts=DEFAULT_OPTS):
+    """w0 and W1 of one pair as ExpScore matrices ((M+1) x n each), prefixSeed, scales."""
+    assert p.values.dtype == np.float32
+    n = len(codes)
+    shape = (p.M + 1, n)
+    w0m, W1m = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+    w0e, W1e = np.zeros(shape, np.int32), np.zeros(shape, np.int32)
+    ps = np.zeros(n, np.float32)
+    sc = np.zeros(3, np.float32)
+    o = np.array(opts, np.float64)
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    f, i = ctypes.c_float, ctypes.c_int
+    lib().or_matrices_f32(p.M, _ptr(p.values, f), _ptr(p.params, f), _ptr(p.bg, f), _ptr(o, ctypes.c_double),
+                          _ptr(c, ctypes.c_uint8), n, _ptr(w0m, f), _ptr(w0e, i), _ptr(W1m, f), _ptr(W1e, i),
+                          _ptr(ps, f), _ptr(sc, f))
+    return (w0m, w0e), (W1m, W1e), ps, sc
 
 
 def filter_batch(p: Profile, seqs: list, opts=DEFAULT_OPTS, threads=None) -> np.ndarray:

@@ -604,6 +604,8 @@ template <class F> struct Work {
     std::vector<int> colRow;
     ES globalBest, bwdBest;
     int W = 0;
+    bool capture = false;
+    std::vector<ES> w0es, W1es;  // (M+1) x n when capture is set
     F *w1(int i, int j) { return &W1[(size_t)i * (W + 4) + j]; }
 };
 
@@ -856,6 +858,10 @@ static void search(Work<F> &Wk, std::vector<Sim> &out, const Prof<F> &P, const u
         Wk.bwdBest = bm;
     }
     // ---- forward, dummest-dp-forward.hh:29-250
+    if (Wk.capture) {
+        Wk.w0es.assign((size_t)(M + 1) * n, ES{});
+        Wk.W1es.assign((size_t)(M + 1) * n, ES{});
+    }
     std::fill(Ynext.begin(), Ynext.end(), (F)0);
     std::vector<F> W0c(W + 4, (F)0), W0n(W + 4, (F)0);
     Wk.colBest.assign(W + 4, ES{});
@@ -900,6 +906,10 @@ static void search(Work<F> &Wk, std::vector<Sim> &out, const Prof<F> &P, const u
             F act = (F)(j < n ? 1 : 0);
             ES fm = promote(w0, Wk.fcnt[i]);
             ES bm = promote(w1i, Wk.bcnt[i]);
+            if (Wk.capture && j < n) {
+                Wk.w0es[(size_t)i * n + j] = fm;
+                Wk.W1es[(size_t)i * n + j] = bm;
+            }
             if (j >= n) { fm.m = 0.0f; bm.m = 0.0f; }
             ES wMid = batchMul(fm, bm, 0);
             if (!(j < n)) wMid = ES{};
@@ -1368,6 +1378,32 @@ int or_search_f64(int M, const double *values, const double *params, const doubl
                   float *colBest) {
     return searchImpl<double>(M, values, params, bg, opts, codes, n, minProbRatio, needAlign, wantBwd, rescaleBits,
                               thresh, hits, segs, hitCap, segCap, gbest, bwdBest, colBest);
+}
+
+// DP matrices of one pair as exact ExpScores (w0 promoted at the forward row
+// count, W1 at the backward row count), plus prefixSeed and the scales: the
+// inputs of the post-gate processing.
+void or_matrices_f32(int M, const float *values, const float *params, const float *bg, const double *opts,
+                     const uint8_t *codes, int n, float *w0m, int *w0e, float *W1m, int *W1e, float *prefixSeed,
+                     float *scales) {
+    FtzScope ftz_;
+    Rates<float> R;
+    R.init(opts);
+    Work<float> Wk;
+    Wk.capture = true;
+    std::vector<Sim> sims;
+    search<float>(Wk, sims, mkProf<float>(M, values, params, bg), codes, n, -2.0f, false, false, R,
+                  rescaleThresh<float>(), RESCALE_BITS);
+    for (size_t c = 0; c < Wk.w0es.size(); c++) {
+        w0m[c] = Wk.w0es[c].m;
+        w0e[c] = Wk.w0es[c].e;
+        W1m[c] = Wk.W1es[c].m;
+        W1e[c] = Wk.W1es[c].e;
+    }
+    for (int j = 0; j < n; j++) prefixSeed[j] = Wk.N.prefixSeed[j];
+    scales[0] = Wk.N.s1f;
+    scales[1] = Wk.N.s2f;
+    scales[2] = Wk.N.sCf;
 }
 
 void or_totals_f32(int M, const float *values, const float *params, const float *bg, const double *opts,

@@ -65,3 +65,21 @@ def test_decode_matches_oracle():
 def test_malformed_profile_is_rejected():
     with pytest.raises(D.DummestError):
         D.parse_hmm_text("NAME x\nHMM A C\n m->m\n  0.1 0.2\n")
+
+
+def test_calibration_sequences_match_oracle():
+    text, p, _ = synthetic_case(3, 50, [], 0)
+    bg = np.asarray(p.bg, np.float32)
+    for L, border, trial in [(500, 100, 0), (1000, 0, 7), (31, 5, 123)]:
+        assert D.calib_sequence(bg, L, border, trial) == O.calib_sequence(bg, L, border, trial)
+    rates = (0.03, 0.01, 0.02, 0.004, 0.0005, 0.02)
+    assert D.calib_sequence(bg, 800, 50, 3, rates) == O.calib_sequence(bg, 800, 50, 3, rates)
+
+
+def test_gumbel_fits_match_oracle():
+    rng = np.random.default_rng(4)
+    x = 5.0 + rng.gumbel(0, 1 / 0.8, size=3000)
+    a = D.gumbel_fits(x, 1000)
+    b = O.gumbel(x, 1000)
+    for k in D.GUMBEL_KEYS:
+        assert a[k] == pytest.approx(b[k], rel=1e-12), k

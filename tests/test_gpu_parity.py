@@ -71,14 +71,110 @@ def test_fwdbwd_scores(engine, L, lengths):
         assert abs(ln_es(bm[k], be[k]) - ln_es(*r.bwd_best)) < TOL_NATS
 
 
-def test_hits_match_oracle(engine):
-    print("hits L=150", flush=True)
+@pytest.mark.parametrize("L,n", [(40, 300), (150, 1500), (300, 700)])
+def test_pair_matrices(engine, L, n):
+    """Dumped w0/W1 against the oracle's, cell by cell, in log space."""
+    print(f"matrices L={L} n={n}", flush=True)
+    pid, prof, codes, sids = load_case(engine, 400 + L, L, [n], homolog_every=1)
+    c, s = codes[0], sids[0]
+    (gw0m, gw0e), (gW1m, gW1e) = engine.pair_matrices(pid, s, len(c))
+    (ow0m, ow0e), (oW1m, oW1e), _, _ = O.matrices(prof, c)
+    for gm, ge, om, oe in ((gw0m, gw0e, ow0m, ow0e), (gW1m, gW1e, oW1m, oW1e)):
+        gl = D.es_log(gm, ge)
+        ol = D.es_log(om, oe)
+        top = np.max(ol)
+        live = ol > top - 60.0  # cells within e^-60 of the largest
+        assert live.sum() > 0
+        err = np.abs(gl[live] - ol[live])
+        assert err.max() < TOL_NATS, (err.max(), np.unravel_index(np.argmax(np.where(live, np.abs(gl - ol), 0)), gl.shape))
+
+
+@pytest.mark.parametrize("min_ln", [8.0, 20.0])
+def test_hits_match_oracle(engine, min_ln):
+    """Hits above a realistic threshold (E-value gate): identical anchors and
+    alignments, scores within 1e-4 nats."""
+    print(f"hits min_ln={min_ln}", flush=True)
     pid, prof, codes, sids = load_case(engine, 300, 150, [1500, 2500, 900], homolog_every=1)
+    thr = float(np.exp(min_ln))
     for c, s in zip(codes, sids):
-        ref = O.search(prof, c, 1.0)
-        got = engine.find_similarities(pid, s, 1.0)
+        ref = O.search(prof, c, thr)
+        got = engine.find_similarities(pid, s, thr)
         assert len(got) == len(ref.hits)
         for g, r in zip(got, ref.hits):
             assert abs(g[0] - r[0]) < TOL_NATS
             assert (g[3], g[4]) == (r[3], r[4])
             assert g[5] == r[5]
+
+
+@pytest.mark.parametrize("L,lengths", [(1100, [400, 1300]), (1500, [700])])
+def test_strip_scores(engine, L, lengths):
+    """Profiles longer than one lane group (row strips with boundary buffers)."""
+    print(f"strips L={L} lengths={lengths}", flush=True)
+    pid, prof, codes, sids = load_case(engine, 500 + L, L, lengths)
+    mm, me, bm, be = engine.fwdbwd_scores(pid, sids, want_bwd=True)
+    best = engine.find_similarities_backward_only(pid, sids)
+    for k, c in enumerate(codes):
+        r = O.search(prof, c, -2.0, need_align=False, want_bwd=True)
+        assert abs(ln_es(mm[k], me[k]) - ln_es(*r.global_best)) < TOL_NATS
+        assert abs(ln_es(bm[k], be[k]) - ln_es(*r.bwd_best)) < TOL_NATS
+        ref = O.filter_score(prof, c)
+        if np.isfinite(ref):
+            assert abs(np.log(best[k]) - np.log(ref)) < TOL_NATS
+        else:
+            assert not np.isfinite(best[k])
+
+
+def test_herpes_medium_fixture(engine):
+    """The reference's overflow fixture (M=1359, strong hit, many rescales):
+    scores and the hit list match the oracle; the top hit's spans are the ones
+    tests/test_overflow.sh:68 fingerprints."""
+    import gzip
+    from tests.common import GOLDEN, read_fasta
+    print("herpes x medium", flush=True)
+    with tempfile.NamedTemporaryFile("wb", suffix=".hmm", delete=False) as f:
+        f.write(gzip.open(os.path.join(GOLDEN, "herpes.hmm.gz")).read())
+        path = f.name
+    try:
+        pid = engine.read_hmm(path)[0]
+    finally:
+        os.unlink(path)
+    M, v, p, b = engine.get_profile(pid)
+    prof = O.Profile("herpes", M, v, p, b, [])
+    dna = read_fasta(os.path.join(GOLDEN, "medium.fa"))[0][1]
+    c = D.decode(dna)
+    sid = engine.add_sequences([c])[0]
+    mm, me, bm, be = engine.fwdbwd_scores([pid], [sid], want_bwd=True)
+    r = O.search(prof, c, float(np.exp(20.0)))
+    assert abs(ln_es(mm[0], me[0]) - ln_es(*r.global_best)) < TOL_NATS
+    assert abs(ln_es(bm[0], be[0]) - ln_es(*r.bwd_best)) < TOL_NATS
+    got = engine.find_similarities(pid, sid, float(np.exp(20.0)))
+    assert len(got) == len(r.hits)
+    for g, h in zip(got, r.hits):
+        assert abs(g[0] - h[0]) < TOL_NATS
+        assert (g[3], g[4], g[5]) == (h[3], h[4], h[5])
+    top = max((h for h in got if h[5]), key=lambda h: h[0])
+    segs = top[5]
+    assert (segs[0][0], segs[-1][0] + segs[-1][2] - segs[0][0], segs[0][1],
+            segs[-1][1] + segs[-1][2] - segs[0][1]) == (428, 504, 1994, 1510)
+
+
+def test_calibration_matches_oracle(engine):
+    """dm_calibrate: the trial sequences scored in one batch against the
+    oracle on the same sequences; scores within the DP tolerance, fits
+    (smooth functions of the scores) within 1e-3 relative."""
+    print("calibration", flush=True)
+    pa, prof_a, _, _ = load_case(engine, 501, 60, [], 0)
+    pb, prof_b, _, _ = load_case(engine, 502, 150, [], 0)
+    seq_len, border, trials = 300, 30, 48
+    mid, flt, fits = engine.calibrate([pa, pb], seq_len, border, trials)
+    for k, p in enumerate([prof_a, prof_b]):
+        seqs = [O.decode(O.calib_sequence(p.bg, seq_len, border, t)) for t in range(trials)]
+        rm, rb = O.fwdbwd_batch(p, seqs)
+        ref_mid = np.array([ln_es(m, e) for m, e in rm])
+        ref_flt = np.array([ln_es(m, e) for m, e in rb])
+        np.testing.assert_allclose(mid[k], ref_mid, atol=TOL_NATS, rtol=0)
+        np.testing.assert_allclose(flt[k], ref_flt, atol=TOL_NATS, rtol=0)
+        for key, ref in O.gumbel(ref_mid, seq_len).items():
+            assert fits[k]["mid"][key] == pytest.approx(ref, rel=1e-3), key
+        for key, ref in O.gumbel(ref_flt, seq_len).items():
+            assert fits[k]["filter"][key] == pytest.approx(ref, rel=1e-3), key

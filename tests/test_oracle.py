@@ -17,6 +17,7 @@
 """
 import itertools
 import math
+import re
 
 import numpy as np
 import pytest
@@ -285,3 +286,69 @@ def test_calibration_sequences_deterministic():
     c = O.calib_sequence(p.bg, 500, 100, 8)
     assert a == b and a != c and len(a) == 600
     assert a[500:] == a[:100]  # border copied from the start (dummest-calibration.hh:479-480)
+
+
+def test_calibration_codon_usage_matches_weights():
+    # With the background frameshift rates at zero every codon slot after the
+    # 0-2 base offset is one draw: amino acid x with weight freq[x] * (number
+    # of its codons), stop with weight bg_stop (dummest-calibration.hh:363-376),
+    # then a uniform synonymous codon. Check the in-frame codon counts against
+    # those weights (a wrong weight, codon list or frame offset fails this).
+    from dummest_amd.synth import AA, CODONS
+    text, p, _ = synthetic_case(3, 50, [], 0)
+    bg = p.bg.astype(np.float32).copy()
+    freqs = np.linspace(1.0, 3.0, 20)
+    bg[4:24] = (freqs / freqs.sum()).astype(np.float32)
+    opts = (0.0, 0.0, 0.0, 0.0, 0.0005, 0.05)  # insert1 insert2 delete1 delete2 stop bg_stop
+    L = 30000
+    stops = ["TAA", "TAG", "TGA"]
+    w = {a: float(bg[4 + k]) * len(CODONS[a]) for k, a in enumerate(AA)}
+    w["*"] = 0.05
+    tot = sum(w.values())
+    codon_p = {c: w[a] / tot / len(cods) for a, cods in list(CODONS.items()) + [("*", stops)] for c in cods}
+    counts = {}
+    nslots = 0
+    for trial in range(6):
+        s = O.calib_sequence(bg, L, 0, trial, opts).decode()
+        # the 0-2 offset bases come first: take the frame whose codons are
+        # most likely under the expected codon distribution
+        off = max(range(3), key=lambda o: sum(math.log(codon_p.get(s[j:j + 3], 1e-9))
+                                               for j in range(o, L - 2, 3)))
+        for j in range(off, L - 2, 3):
+            counts[s[j:j + 3]] = counts.get(s[j:j + 3], 0) + 1
+            nslots += 1
+    for a, cods in list(CODONS.items()) + [("*", stops)]:
+        exp_aa = w[a] / tot * nslots
+        got_aa = sum(counts.get(c, 0) for c in cods)
+        assert abs(got_aa - exp_aa) < 5 * math.sqrt(exp_aa) + 5, (a, got_aa, exp_aa)
+        for c in cods:  # uniform among synonymous codons
+            e = got_aa / len(cods)
+            assert abs(counts.get(c, 0) - e) < 5 * math.sqrt(e) + 5, (c, counts.get(c, 0), e)
+
+
+def test_calibration_frameshift_rate():
+    # A 1-base (2-base) frameshift event happens per codon slot with
+    # probability bgfs1 = insert1 + delete2 (bgfs2 = insert2 + delete1) and
+    # consumes 1 (2) letters instead of 3; over L letters the expected number
+    # of slots is L / (3 - 2*f1 - f2) (renewal), so with frameshifts on the
+    # run of letters outside in-frame codons grows at that rate. Here: all
+    # amino-acid weight on Trp (TGG) makes in-frame text 'TGGTGG...', so the
+    # count of letters not inside a TGG codon measures the frameshift letters.
+    text, p, _ = synthetic_case(3, 50, [], 0)
+    bg = p.bg.astype(np.float32).copy()
+    bg[4:24] = 0.0
+    bg[4 + "ACDEFGHIKLMNPQRSTVWY".index("W")] = 1.0
+    f1, f2 = 0.02, 0.01
+    opts = (f1, f2 / 2, f2 / 2, 0.0, 0.0005, 0.0)
+    L = 60000
+    frames_letters = 0
+    for trial in range(4):
+        s = O.calib_sequence(bg, L, 0, trial, opts).decode()
+        covered = np.zeros(L + 3, bool)
+        for m in re.finditer("(?=TGG)", s):
+            covered[m.start():m.start() + 3] = True
+        frames_letters += int((~covered[:L]).sum())
+    slots = 4 * L / (3 - 2 * f1 - f2)
+    expect = slots * (f1 * 1 + f2 * 2)
+    # random frameshift letters occasionally form or join a TGG, so allow 15%
+    assert abs(frames_letters - expect) / expect < 0.15, (frames_letters, expect)
