@@ -97,6 +97,10 @@ def main():
     cells = int(((M + 1) * n).sum())
 
     t = time.perf_counter()
+    table = e.autotune(300)
+    log(f"autotune: {time.perf_counter() - t:.2f} s; ns per wave-step: " +
+        " ".join(f"G{G}R{R}{'s' if st else ''}={ns:.0f}" for G, R, st, ns in table))
+    t = time.perf_counter()
     plan = e.plan(prof, seq, mode="score", want_bwd=False)
     log(f"plan: {time.perf_counter() - t:.1f} s, {cells:.4g} cells per step")
     for _ in range(args.warmup):
@@ -120,6 +124,9 @@ def main():
             a[0] += ms
             a[1] += c
             a[2] += launches
+    for (g, r), (ms_, c_, l_) in sorted(agg.items(), key=lambda kv: -kv[1][0]):
+        log(f"  G={g:2d} R={r:2d}: {ms_ / args.steps:8.2f} ms/step {c_ / args.steps:.3g} cells/step "
+            f"{c_ / (ms_ * 1e-3):.3g} cells/s")
     (G, R), (ms, c, launches) = max(agg.items(), key=lambda kv: kv[1][0])
     avg_ms = ms / launches
     flops_per_launch = FLOPS_PER_CELL * c / launches

@@ -85,6 +85,8 @@ def _declare(L):
                                   _P(f), _P(d), d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
     L.dm_set_option.argtypes = [vp, ctypes.c_char_p, i64]
     L.dm_calibrate.argtypes = [vp, i32, _P(i32), i32, i32, i32, _P(d), _P(d), _P(d)]
+    L.dm_autotune.argtypes = [vp, i32, _P(d), i32]
+    L.dm_config_table.argtypes = [vp, i32, _P(i32), _P(i32), _P(i32), _P(d)]
     L.dm_calib_sequence.argtypes = [_P(f), _P(d), i32, i32, i32, ctypes.c_char_p]
     L.dm_gumbel_fits.argtypes = [_P(d), i32, d, _P(d)]
     L.dm_last_dp_ms.argtypes = [vp]
@@ -385,6 +387,23 @@ class Engine:
         out = [{"mid": dict(zip(GUMBEL_KEYS, f[:8].tolist())), "filter": dict(zip(GUMBEL_KEYS, f[8:].tolist()))}
                for f in fits]
         return mid, flt, out
+
+    def autotune(self, seq_len=300):
+        """Time every kernel configuration on the device; later plans pick the
+        configuration with the least measured cost per pair. Returns the table."""
+        ns = np.zeros(64, np.float64)
+        n = lib().dm_autotune(self._h, int(seq_len), _ptr(ns, ctypes.c_double), 64)
+        self._check(n if n < 0 else 0, "dm_autotune")
+        return self.config_table()
+
+    def config_table(self):
+        """[(G, R, strips, ns_per_wave_step)] in configuration order."""
+        cap = 64
+        G, R, S = (np.zeros(cap, np.int32) for _ in range(3))
+        ns = np.zeros(cap, np.float64)
+        i32 = ctypes.c_int32
+        n = lib().dm_config_table(self._h, cap, _ptr(G, i32), _ptr(R, i32), _ptr(S, i32), _ptr(ns, ctypes.c_double))
+        return [(int(G[k]), int(R[k]), bool(S[k]), float(ns[k])) for k in range(min(n, cap))]
 
     def kernel_stats(self):
         """[(G, R, ms, cells, launches)] per kernel configuration of the last execution."""

@@ -1,0 +1,738 @@
+// dummest-amd: the engine executable, a drop-in for the reference's
+// dummestl_core (dummest.cc:18-660) with the DP on the GPU.
+//
+// Same argv, stdin/FASTA input (pipeline headers included) and output records:
+//   1. read and finalize the profiles (host, dummest-profile.hh:235-329);
+//   2. calibrate every profile in one batched GPU pass (dm_calibrate;
+//      estimateK, dummest-calibration.hh:338-600) and print the same
+//      per-profile header lines (dummest.cc:384-413, dummest-calibration.hh:603-649);
+//   3. read the sequences in contigs (readContig, dummest-sequence.hh:6-89),
+//      total length first (dummest.cc:508-528), then in chunks of --batch;
+//   4. per chunk: score-only filter wave on the GPU and the E-value test of
+//      the filter (dummest-search.hh:146-179), then the forward+backward score
+//      wave for the survivors and the anchor gate (dummest-dp-forward.hh:252-274),
+//      then the matrices of each gated pair and the host traceback
+//      (dm_pair_hits), and the records (dummest-search.hh:89-105,
+//      dummest-output.hh:54-107).
+//
+// Not available here (the command fails with a message rather than giving
+// different results): tantan masking (-m 1/2/3; -m 0 is required) and the
+// Gaussian profile filter (-d/-D). There is no calibration cache: every run
+// calibrates (on the GPU).
+#include <getopt.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dm_host.h"
+#include "../../include/dummest_amd.h"
+
+using namespace dm;
+
+namespace {
+
+const char *kAlphabet = "ACDEFGHIKLMNPQRSTVWYUO?*";  // getAlphabet(20), dummest-core.hh:350-355
+
+const double kDefaults[6] = {0.0171, 0.0018, 0.0328, 0.0083, 0.0005, 0.046875};
+
+int badOpt() {
+    std::cerr << "dummest-amd: bad option value\n";
+    return 1;
+}
+
+int intFromText(const char *s) {
+    char *end = nullptr;
+    long v = std::strtol(s, &end, 10);
+    if (!end || *end || v < INT32_MIN || v > INT32_MAX) return -1;
+    return (int)v;
+}
+
+// Decimal log-score text: fixed, 6 decimals, trailing zeros trimmed
+// (scoreStr, dummest-output.hh:48-66).
+std::string scoreStr(double v) {
+    if (v == 0.0) v = 0.0;
+    std::ostringstream os;
+    if (!std::isfinite(v)) {
+        os << v;
+        return os.str();
+    }
+    os << std::fixed << std::setprecision(6) << v;
+    std::string s = os.str();
+    if (s.find('.') != std::string::npos) {
+        s.erase(s.find_last_not_of('0') + 1);
+        if (s.back() == '.') s.pop_back();
+    }
+    return s;
+}
+
+double natsToLog2(double ln) { return ln / 0.6931471805599453; }
+
+int numOfDigits(int x) {
+    int n = 0;
+    do ++n;
+    while (x /= 10);
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// Sequence reading (readContig, dummest-sequence.hh:6-89, PIPELINE_MODE on).
+// The record state persists from one record to the next as the reference's
+// reused Sequence object does; is_plus starts true.
+// ---------------------------------------------------------------------------
+struct SeqState {
+    std::string name;
+    int length = 0;  // letters of the record read so far (contigs and other symbols)
+    size_t wStart = 0, wEnd = 0, trueLength = 0;
+    std::string targetProfile;
+    bool isPlus = true;
+    bool hasPipelineFields = false;
+};
+
+struct Contig {
+    int start = 0;
+    int length = 0;
+};
+
+class ContigReader {
+public:
+    explicit ContigReader(std::istream &in) : in_(in) {
+        std::memset(map_, 127, sizeof map_);
+        std::memset(map_, 125, ' ' + 1);
+        map_[(unsigned char)'>'] = 126;
+        for (int i = 0; kAlphabet[i]; i++) {
+            map_[(unsigned char)std::toupper(kAlphabet[i])] = (char)i;
+            map_[(unsigned char)std::tolower(kAlphabet[i])] = (char)i;
+        }
+        for (const char *c = "RYSWKMBDHVN"; *c; c++) {  // IUPAC ambiguity (dummest.cc:445-456)
+            map_[(unsigned char)*c] = 22;
+            map_[(unsigned char)std::tolower(*c)] = 22;
+        }
+    }
+    // Next contig into `letters` (alphabet indices); false at the end or on a
+    // format error (err set).
+    bool next(SeqState &s, Contig &c, std::vector<char> &letters, std::string &err) {
+        letters.clear();
+        std::streambuf *buf = in_.rdbuf();
+        if (c.length == 0) {
+            char x;
+            if (!(in_ >> x)) return false;
+            if (x != '>') {
+                err = "bad sequence data: no '>'";
+                return false;
+            }
+            std::string line, word;
+            std::getline(in_, line);
+            std::istringstream iss(line);
+            if (!(iss >> word)) {
+                err = "bad sequence data: no name";
+                return false;
+            }
+            const size_t slash = word.find('/');
+            if (slash != std::string::npos) {
+                const std::string chr = word.substr(0, slash), range = word.substr(slash + 1);
+                const size_t dash = range.find('-');
+                if (dash != std::string::npos) {
+                    s.wStart = std::stoul(range.substr(0, dash));
+                    s.wEnd = std::stoul(range.substr(dash + 1));
+                }
+                std::string tok, length, profile, strand;
+                bool haveStrand = false;
+                while (iss >> tok) {
+                    if (tok.rfind("length=", 0) == 0) length = tok;
+                    else if (tok.rfind("profile=", 0) == 0) profile = tok;
+                    else if (tok.rfind("bitscore=", 0) == 0) continue;
+                    else if (!haveStrand) {
+                        strand = tok;
+                        haveStrand = true;
+                    }
+                }
+                if (!length.empty() && !profile.empty() && haveStrand) {
+                    s.trueLength = std::stoull(length.substr(length.find('=') + 1));
+                    s.targetProfile = profile.substr(profile.find('=') + 1);
+                    s.isPlus = strand.find("plus_strand") != std::string::npos;
+                }
+                s.hasPipelineFields = true;
+                word = chr;
+            }
+            s.name = word;
+            s.length = 0;
+        }
+        int ch = buf->sgetc();
+        while (ch != EOF && ch != '>') {
+            if (map_[ch & 255] < 125) break;
+            if (ch > ' ') ++s.length;
+            ch = buf->snextc();
+        }
+        while (ch != EOF && map_[ch & 255] < 126) {
+            if (ch > ' ') letters.push_back(map_[ch & 255]);
+            ch = buf->snextc();
+        }
+        if (letters.size() > (size_t)INT32_MAX - 64) {
+            err = "sequence is too long!";
+            return false;
+        }
+        c.start = s.length;
+        c.length = (int)letters.size();
+        s.length += c.length;
+        return true;
+    }
+
+private:
+    std::istream &in_;
+    char map_[256];
+};
+
+// One contig searched as one strand (strandNum even: PIPELINE_MODE fixes the
+// strand option to 1, dummest.cc:460-462, so only s = 0 runs).
+struct Target {
+    SeqState rec;   // record state when the contig was read
+    Contig contig;
+    std::string letters;  // alphabet letters (uppercase) of the contig
+    std::vector<uint8_t> codes;
+};
+
+struct Options {
+    double evalue = 10;
+    long long totOverride = -1;
+    int mask = 3;
+    int trials = 1000, length = 5000, border = 100;
+    int batch = 0;
+    int bgType = 'G';
+    const char *scoresFile = nullptr;
+    bool warmup = false;
+    bool maxMode = false;
+    double filterEvalue = -1;
+    bool filterExplicit = false;
+    int threads = 1;
+    int verbosity = 0;
+    double rates[6];
+    double tantan = 0.5;
+    int device = 0;
+};
+
+int die(const std::string &msg) {
+    std::cerr << "dummest-amd: " << msg << "\n";
+    return 1;
+}
+
+int engineFail(dm_engine *e, const char *what, int rc) {
+    std::cerr << "dummest-amd: " << what << " failed (" << rc << "): " << dm_engine_error(e) << "\n";
+    return 1;
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+    Options o;
+    std::copy(kDefaults, kDefaults + 6, o.rates);
+    {
+        const unsigned hc = std::thread::hardware_concurrency();
+        o.threads = hc ? (int)hc : 1;
+    }
+    double filterStdDev = 0;
+    const char help[] =
+        "usage: dummest-amd profiles.hmm [sequences.fa]\n"
+        "\n"
+        "Find similarities between DNA sequences and protein profiles, allowing\n"
+        "frameshifts; the dynamic programming runs on the GPU.\n"
+        "\n"
+        "Options:\n"
+        "  -h, --help        show this help message and exit\n"
+        "  -V, --version     show version and exit\n"
+        "  -v, --verbose     show progress messages\n"
+        "  -T N, --threads N host threads (default: CPU cores)\n"
+        "  -e E, --evalue E  find similarities with E-value <= this (default: 10)\n"
+        "  -N N, --total-length N  override total search space size (default: sum of input lengths)\n"
+        "  -s S, --strand S  accepted for compatibility; each record is searched as given\n"
+        "  -m M, --mask M    simple-region masking: only 0 (none) is available here\n"
+        "  -t T, --trials T  random sequences for calibration (default: 1000)\n"
+        "  -l L, --length L  length of each random sequence (default: 5000)\n"
+        "  -b B, --border B  border added to each random sequence (default: 100)\n"
+        "  --batch N         stream sequences in chunks of N\n"
+        "  -S F, --scores-file F  write bit scores of the random sequences to F and exit\n"
+        "  --warmup          calibrate and exit\n"
+        "  --barithmetic / --bgeometric (default) / --bmedian  background probabilities\n"
+        "  --insert1 F --insert2 F --delete1 F --delete2 F  frameshift rates per base\n"
+        "  --stop-codon-prob F  --bg-stop-codon-prob F  --tantan-threshold F\n"
+        "  --max             skip the filter pass\n"
+        "  -W E, --filter-evalue E  filter-pass E-value threshold (default: value of -e)\n"
+        "  --device D        HIP device (default: 0)\n";
+    enum { OI1 = 256, OI2, OD1, OD2, OST, OBG, OTT, OMAX, OBATCH, OWARM, ODEV };
+    static struct option lopts[] = {{"help", no_argument, 0, 'h'},
+                                    {"version", no_argument, 0, 'V'},
+                                    {"verbose", no_argument, 0, 'v'},
+                                    {"threads", required_argument, 0, 'T'},
+                                    {"evalue", required_argument, 0, 'e'},
+                                    {"total-length", required_argument, 0, 'N'},
+                                    {"strand", required_argument, 0, 's'},
+                                    {"mask", required_argument, 0, 'm'},
+                                    {"dev", required_argument, 0, 'd'},
+                                    {"Dev", required_argument, 0, 'D'},
+                                    {"trials", required_argument, 0, 't'},
+                                    {"length", required_argument, 0, 'l'},
+                                    {"border", required_argument, 0, 'b'},
+                                    {"barithmetic", no_argument, 0, 'A'},
+                                    {"bgeometric", no_argument, 0, 'G'},
+                                    {"bmedian", no_argument, 0, 'M'},
+                                    {"insert1", required_argument, 0, OI1},
+                                    {"insert2", required_argument, 0, OI2},
+                                    {"delete1", required_argument, 0, OD1},
+                                    {"delete2", required_argument, 0, OD2},
+                                    {"stop-codon-prob", required_argument, 0, OST},
+                                    {"bg-stop-codon-prob", required_argument, 0, OBG},
+                                    {"tantan-threshold", required_argument, 0, OTT},
+                                    {"max", no_argument, 0, OMAX},
+                                    {"batch", required_argument, 0, OBATCH},
+                                    {"filter-evalue", required_argument, 0, 'W'},
+                                    {"scores-file", required_argument, 0, 'S'},
+                                    {"warmup", no_argument, 0, OWARM},
+                                    {"device", required_argument, 0, ODEV},
+                                    {0, 0, 0, 0}};
+    auto rate01 = [](const char *s, double &v) {
+        v = std::strtod(s, nullptr);
+        return v >= 0 && v < 1;
+    };
+    int c;
+    while ((c = getopt_long(argc, argv, "hVve:N:s:m:d:D:t:l:b:T:S:W:", lopts, nullptr)) != -1) {
+        switch (c) {
+        case 'h': std::cout << help; return 0;
+        case 'V': std::cout << "DUMMEST-AMD 1\n"; return 0;
+        case 'v': ++o.verbosity; break;
+        case 'T':
+            o.threads = intFromText(optarg);
+            if (o.threads < 1) return badOpt();
+            break;
+        case 'e':
+            o.evalue = std::strtod(optarg, nullptr);
+            if (o.evalue < 0) return badOpt();
+            break;
+        case 'N':
+            o.totOverride = std::atoll(optarg);
+            if (o.totOverride < 0) return badOpt();
+            break;
+        case 's': {
+            const int s = intFromText(optarg);
+            if (s < 0 || s > 2) return badOpt();
+            break;
+        }
+        case 'm':
+            o.mask = intFromText(optarg);
+            if (o.mask < 0 || o.mask > 3) return badOpt();
+            break;
+        case 'd':
+        case 'D':
+            filterStdDev = std::strtod(optarg, nullptr);
+            if (filterStdDev < 2 || filterStdDev > 1000) return badOpt();
+            break;
+        case 't':
+            o.trials = intFromText(optarg);
+            if (o.trials < 1) return badOpt();
+            break;
+        case 'l':
+            o.length = intFromText(optarg);
+            if (o.length < 1 || o.length > INT32_MAX - 64) return badOpt();
+            break;
+        case 'b':
+            o.border = intFromText(optarg);
+            if (o.border < 0) return badOpt();
+            break;
+        case 'S': o.scoresFile = optarg; break;
+        case OWARM: o.warmup = true; break;
+        case 'A': o.bgType = 'A'; break;
+        case 'G': o.bgType = 'G'; break;
+        case 'M': o.bgType = 'M'; break;
+        case OI1: if (!rate01(optarg, o.rates[0])) return badOpt(); break;
+        case OI2: if (!rate01(optarg, o.rates[1])) return badOpt(); break;
+        case OD1: if (!rate01(optarg, o.rates[2])) return badOpt(); break;
+        case OD2: if (!rate01(optarg, o.rates[3])) return badOpt(); break;
+        case OST: if (!rate01(optarg, o.rates[4])) return badOpt(); break;
+        case OBG: if (!rate01(optarg, o.rates[5])) return badOpt(); break;
+        case OTT:
+            o.tantan = std::strtod(optarg, nullptr);
+            if (!(o.tantan >= 0 && o.tantan <= 1)) return badOpt();
+            break;
+        case 'W':
+            o.filterEvalue = std::strtod(optarg, nullptr);
+            if (o.filterEvalue < 0) return badOpt();
+            o.filterExplicit = true;
+            break;
+        case OMAX: o.maxMode = true; break;
+        case OBATCH:
+            o.batch = intFromText(optarg);
+            if (o.batch < 1) return badOpt();
+            break;
+        case ODEV:
+            o.device = intFromText(optarg);
+            if (o.device < 0) return badOpt();
+            break;
+        default: std::cerr << help; return 1;
+        }
+    }
+    const float bgfs1 = (float)o.rates[0] + (float)o.rates[3], bgfs2 = (float)o.rates[1] + (float)o.rates[2];
+    if (!(o.rates[0] + o.rates[1] + o.rates[2] + o.rates[3] < 1)) return badOpt();
+    if (!(bgfs1 + bgfs2 < 1)) return badOpt();
+    if (!o.maxMode && !o.filterExplicit) o.filterEvalue = o.evalue;
+    const int nargs = argc - optind;
+    if (nargs < 1 || nargs > 2) {
+        std::cerr << help;
+        return 1;
+    }
+    if (filterStdDev > 0) return die("the Gaussian profile filter (-d/-D) is not available in this build");
+    if (o.mask != 0)
+        return die("tantan masking (-m 1/2/3; the reference's default is 3) is not available in this build: "
+                   "pass -m 0");
+    if (o.border > INT32_MAX - 64 - o.length) return die("sequence + border is too big");
+
+    // ------------------------------------------------------------- profiles
+    std::vector<HostProfile> profiles;
+    {
+        std::string msg;
+        if (!readHmmFile(argv[optind], o.bgType, o.rates, profiles, msg))
+            return die(msg.empty() ? "can't read the profile data" : msg);
+    }
+    dm_engine *E = nullptr;
+    int rc = dm_engine_create(o.device, &E);
+    if (rc) return die("no usable HIP device");
+    std::unique_ptr<dm_engine, void (*)(dm_engine *)> guard(E, dm_engine_destroy);
+    if ((rc = dm_set_rates(E, o.rates))) return engineFail(E, "dm_set_rates", rc);
+    std::vector<int32_t> pids(profiles.size());
+    for (size_t i = 0; i < profiles.size(); i++) {
+        const HostProfile &p = profiles[i];
+        if ((rc = dm_add_profile(E, p.M, p.values.data(), p.params.data(), p.bg.data(), &pids[i])))
+            return engineFail(E, "dm_add_profile", rc);
+    }
+
+    std::cout << "# DUMMEST-AMD 1\n";
+    std::cout << "# Bytes per floating-point number: 4\n";
+    std::cout << "# Background letter probabilities: "
+              << (o.bgType == 'A' ? "arithmetic mean" : o.bgType == 'G' ? "geometric mean" : "median")
+              << " of foreground probabilities\n";
+    std::cout << "# Random sequences: trials=" << o.trials << " length=" << o.length << " border=" << o.border
+              << "\n";
+    std::cout << "# Frameshift rates: insert1=" << o.rates[0] << " insert2=" << o.rates[1]
+              << " delete1=" << o.rates[2] << " delete2=" << o.rates[3] << "\n";
+    std::cout << "# Stop codon probs: stop=" << o.rates[4] << " bg-stop=" << o.rates[5]
+              << " tantan-threshold=" << o.tantan << "\n";
+    if (o.maxMode) std::cout << "# Max mode: filter pass disabled\n";
+    else if (nargs > 1) std::cout << "# Filter E-value <= " << o.filterEvalue << "\n";
+    if (nargs > 1 && o.evalue > 0) std::cout << "# E-value <= " << o.evalue << "\n";
+
+    // ----------------------------------------------------------- calibration
+    const int np = (int)profiles.size();
+    std::vector<double> fits((size_t)np * 16), midS, fltS;
+    if (o.scoresFile) {
+        midS.resize((size_t)np * o.trials);
+        fltS.resize((size_t)np * o.trials);
+    }
+    if (np > 0) {
+        if (o.trials < 2) return die("calibration needs at least 2 trials");
+        rc = dm_calibrate(E, np, pids.data(), o.length, o.border, o.trials, o.scoresFile ? midS.data() : nullptr,
+                          o.scoresFile ? fltS.data() : nullptr, fits.data());
+        if (rc) return engineFail(E, "dm_calibrate", rc);
+    }
+    std::ofstream scores;
+    if (o.scoresFile) {
+        scores.open(o.scoresFile);
+        if (!scores) return die("can't open scores file");
+        scores << "mode\tprofile_name\ttrial\tanchor_type\tln_score\n";
+    }
+    const int printVerbosity = (nargs < 2) * 2 + (o.evalue <= 0);
+    // The reference's end- and start-anchored samples are all -inf
+    // (dummest-dp-forward.hh:536-543); their fits are those of such a sample.
+    double endFits[8];
+    gumbelFits(std::vector<double>(std::max(2, o.trials), -INFINITY), (double)o.length, endFits);
+    std::vector<double> K(np), lambda(np), Kf(np), lambdaF(np);
+    for (int i = 0; i < np; i++) {
+        const HostProfile &p = profiles[i];
+        const double *f = &fits[(size_t)i * 16];
+        K[i] = f[1];
+        lambda[i] = f[0];
+        Kf[i] = f[9];
+        lambdaF[i] = f[8];
+        std::cout << "\n# Profile name: " << p.name << "\n# Profile length: " << p.M << "\n";
+        std::cout << "# Background letter probabilities:";
+        for (int j = 0; j < 20; j++) std::cout << " " << p.bg[4 + j];
+        std::cout << std::endl;
+        // sum of the amino-acid weights of the trial generator (dummest-calibration.hh:365-376)
+        float sum = 0;
+        static const int ncod[20] = {4, 2, 2, 2, 2, 4, 2, 3, 2, 6, 1, 2, 4, 2, 6, 6, 4, 4, 1, 2};
+        for (int j = 0; j < 20; j++) sum += (float)(p.bg[4 + j] * ncod[j]);
+        sum += (float)o.rates[5];
+        std::cout << "# sum is " << sum << std::endl;
+        // Stats block of one cache entry (dummest-calibration.hh:603-649):
+        // end- and start-anchored fits come from all -inf samples, and the
+        // filter block replaces only the method-of-moments fields.
+        auto printEntry = [&](const double *mm, const double *rest) {
+            const double *x = endFits;
+            if (printVerbosity > 1) {
+                std::cout << "#\tend-\tstart-\tmid-anchored\n";
+                std::cout << "#lamMM\t" << mm[0] << "\t" << mm[0] << "\t" << mm[1] << "\n"
+                          << "#kMM\t" << mm[2] << "\t" << mm[2] << "\t" << mm[3] << "\n"
+                          << "#kMM1\t" << mm[4] << "\t" << mm[4] << "\t" << mm[5] << "\n";
+                std::cout << "#lamML\t" << x[3] << "\t" << x[3] << "\t" << rest[3] << "\n"
+                          << "#kML\t" << x[4] << "\t" << x[4] << "\t" << rest[4] << "\n"
+                          << "#kML1\t" << x[5] << "\t" << x[5] << "\t" << rest[5] << "\n";
+                std::cout << "#lamLM\t" << x[6] << "\t" << x[6] << "\t" << rest[6] << "\n"
+                          << "#kLM\t" << x[7] << "\t" << x[7] << "\t" << rest[7] << "\n";
+            } else if (printVerbosity > 0) {
+                std::cout << "# K: " << mm[4] << " " << mm[4] << " " << mm[5] << "\n";
+            } else {
+                std::cout << "# K: " << mm[5] << "\n";
+            }
+            std::cout << "# Lambda: " << mm[1] << "\n";
+        };
+        // (end/start, mid) pairs of lambda, K, K with lambda 1
+        const double fullMM[6] = {endFits[0], f[0], endFits[1], f[1], endFits[2], f[2]};
+        const double fltMM[6] = {f[8], f[8], f[9], f[9], f[10], f[10]};
+        printEntry(fullMM, f);
+        printEntry(fltMM, f);
+        if (o.scoresFile) {
+            for (int t = 0; t < o.trials; t++) {
+                const size_t k = (size_t)i * o.trials + t;
+                scores << "full\t" << p.name << "\t" << t + 1 << "\tend\t-inf\n";
+                scores << "full\t" << p.name << "\t" << t + 1 << "\tstart\t-inf\n";
+                scores << "full\t" << p.name << "\t" << t + 1 << "\tmid\t" << scoreStr(natsToLog2(midS[k])) << "\n";
+                scores << "filter\t" << p.name << "\t" << t + 1 << "\tall\t" << scoreStr(natsToLog2(fltS[k]))
+                       << "\n";
+            }
+        }
+    }
+    if (o.warmup) std::cout << "# Warmup: " << np << " profiles, 0 cache hits, " << np << " calibrated\n";
+    if (o.scoresFile || o.warmup) return 0;
+    if (nargs < 2 || np < 1) return 0;
+    std::cout << std::endl;
+
+    // -------------------------------------------------------------- sequences
+    const std::string seqPath = argv[optind + 1];
+    std::ifstream file;
+    std::string spool;
+    std::istream *in = nullptr;
+    const bool needTotal = o.totOverride < 0;
+    if (seqPath == "-" && needTotal) {
+        spool = std::string(std::getenv("TMPDIR") ? std::getenv("TMPDIR") : "/tmp") + "/dummest-amd-stdin-" +
+                std::to_string((long)getpid()) + ".fa";
+        {
+            std::ofstream out(spool, std::ios::binary);
+            out << std::cin.rdbuf();
+        }
+        file.open(spool, std::ios::binary);
+        if (!file) return 1;
+        in = &file;
+    } else if (seqPath == "-") {
+        in = &std::cin;
+    } else {
+        file.open(seqPath);
+        if (!file) return die("can't open file: " + seqPath);
+        in = &file;
+    }
+    size_t totLen = needTotal ? 0 : (size_t)o.totOverride;
+    std::string err;
+    if (needTotal) {
+        ContigReader rd(*in);
+        SeqState s;
+        Contig ct;
+        std::vector<char> letters;
+        while (rd.next(s, ct, letters, err))
+            if (ct.length) totLen += ct.length;
+        if (!err.empty()) return die(err);
+        file.clear();
+        file.seekg(0);
+    }
+    std::cout << "# Total sequence length: " << totLen << "\n";
+    std::cout.precision(3);
+
+    const int batch = o.batch > 0 ? o.batch : o.threads * 25000;
+    size_t minProbRatioTotal = needTotal ? 0 : totLen;
+    std::vector<Target> targets;
+    std::vector<std::pair<int, float>> reqProfRatio;  // per request: profile, minProbRatio
+    std::vector<int> reqTarget;
+    int64_t nFilterPairs = 0, nFullPairs = 0, nGated = 0, maxWidth = 0;
+
+    auto flush = [&]() -> int {
+        if (targets.empty()) return 0;
+        std::vector<uint8_t> codes;
+        std::vector<int64_t> offs;
+        std::vector<int32_t> lens;
+        for (const Target &t : targets) {
+            offs.push_back((int64_t)codes.size());
+            lens.push_back((int32_t)t.codes.size());
+            codes.insert(codes.end(), t.codes.begin(), t.codes.end());
+            maxWidth = std::max<int64_t>(maxWidth, (int64_t)t.codes.size());
+        }
+        int r = dm_clear_sequences(E);
+        int32_t first = 0;
+        if (!r) r = dm_add_sequences(E, (int32_t)targets.size(), codes.data(), offs.data(), lens.data(), &first);
+        if (r) return engineFail(E, "dm_add_sequences", r);
+        // requests on sequences shorter than 3 cannot be scored (dummest-dp.hh:84)
+        std::vector<int32_t> pp, ss;
+        std::vector<size_t> rq;
+        for (size_t k = 0; k < reqTarget.size(); k++) {
+            if (targets[reqTarget[k]].codes.size() < 3) continue;
+            pp.push_back(pids[reqProfRatio[k].first]);
+            ss.push_back(first + reqTarget[k]);
+            rq.push_back(k);
+        }
+        if (!o.maxMode && !pp.empty()) {
+            std::vector<float> best(pp.size());
+            r = dm_filter_scores(E, (int64_t)pp.size(), pp.data(), ss.data(), best.data());
+            if (r) return engineFail(E, "dm_filter_scores", r);
+            nFilterPairs += (int64_t)pp.size();
+            std::vector<int32_t> p2, s2;
+            std::vector<size_t> q2;
+            for (size_t k = 0; k < pp.size(); k++) {
+                if (!(best[k] > 0)) continue;
+                const int pi = reqProfRatio[rq[k]].first;
+                const double lp = std::log((double)best[k]);
+                const double pr = std::exp(lp * lambdaF[pi]);
+                const double ev = Kf[pi] * (double)totLen / pr;
+                if (ev <= o.filterEvalue || !std::isfinite(lp)) {
+                    p2.push_back(pp[k]);
+                    s2.push_back(ss[k]);
+                    q2.push_back(rq[k]);
+                }
+            }
+            pp.swap(p2);
+            ss.swap(s2);
+            rq.swap(q2);
+        }
+        std::ostringstream os;
+        if (!pp.empty()) {
+            std::vector<float> mm(pp.size()), bm(pp.size());
+            std::vector<int32_t> me(pp.size()), be(pp.size());
+            r = dm_fwdbwd_scores(E, (int64_t)pp.size(), pp.data(), ss.data(), 0, mm.data(), me.data(), bm.data(),
+                                 be.data());
+            if (r) return engineFail(E, "dm_fwdbwd_scores", r);
+            nFullPairs += (int64_t)pp.size();
+            std::vector<double> hits(6 * 4096);
+            std::vector<int32_t> segs(3 * (1 << 20));
+            for (size_t k = 0; k < pp.size(); k++) {
+                const int pi = reqProfRatio[rq[k]].first;
+                const float ratio = reqProfRatio[rq[k]].second;
+                if (ratio < 0) continue;  // calibration-style lanes report no alignments
+                // gate: best anchored score >= minProbRatio (dummest-dp-forward.hh:263-271)
+                ES minExp{0.f, 0};
+                {
+                    int ex = 0;
+                    const double m = std::frexp((double)ratio, &ex);
+                    if (ratio != 0) minExp = ES{(float)m, ex};
+                }
+                const ES got{mm[k], me[k]};
+                const bool less = got.m == 0.f ? minExp.m != 0.f
+                                               : (minExp.m == 0.f ? false
+                                                  : (got.e != minExp.e ? got.e < minExp.e : got.m < minExp.m));
+                if (less) continue;
+                ++nGated;
+                int32_t nh = 0, ns = 0;
+                r = dm_pair_hits(E, pp[k], ss[k], (double)ratio, hits.data(), 4096, segs.data(), 1 << 20, &nh, &ns);
+                if (r) return engineFail(E, "dm_pair_hits", r);
+                const Target &T = targets[reqTarget[rq[k]]];
+                const HostProfile &P = profiles[pi];
+                const SeqState &S = T.rec;
+                int so = 0;
+                for (int h = 0; h < std::min(nh, 4096); h++) {
+                    const double lnr = hits[6 * h];
+                    const int a1 = (int)hits[6 * h + 3], a2 = (int)hits[6 * h + 4], nseg = (int)hits[6 * h + 5];
+                    const int32_t *sg = &segs[3 * so];
+                    so += nseg;
+                    if (nseg == 0) continue;  // empty alignments are not records (dummest-search.hh:36-41)
+                    // aligned rows (addAlignedProfile / addAlignedSequence, dummest-output.hh:10-46)
+                    std::string row1, row2;
+                    int pos1 = sg[0], pos2 = sg[1];
+                    for (int g = 0; g < nseg; g++) {
+                        const int s1 = sg[3 * g], s2 = sg[3 * g + 1], ln = sg[3 * g + 2];
+                        for (; pos1 < s1; ++pos1) row1.push_back(kAlphabet[P.consensus[pos1] & 31]);
+                        row1.append(s2 - pos2, '-');
+                        for (; pos1 < s1 + ln; ++pos1) row1.push_back(kAlphabet[P.consensus[pos1] & 31]);
+                        pos2 = s2 + ln;
+                    }
+                    pos1 = sg[0];
+                    pos2 = sg[1];
+                    for (int g = 0; g < nseg; g++) {
+                        const int s1 = sg[3 * g], s2 = sg[3 * g + 1], ln = sg[3 * g + 2];
+                        row2.append(s1 - pos1, '-');
+                        for (; pos2 < s2 + ln; ++pos2) row2.push_back(T.letters[pos2]);
+                        pos1 = s1 + ln;
+                    }
+                    const double ev = K[pi] * (double)totLen * std::exp(-lambda[pi] * lnr);
+                    if (o.evalue > 0 && ev > o.evalue) continue;
+                    if (std::isnan(ev)) continue;
+                    // printSimilarity, dummest-output.hh:68-107
+                    const int len = (int)row1.size();
+                    const int span1 = len - (int)std::count(row1.begin(), row1.end(), '-');
+                    const int span2 = len - (int)std::count(row2.begin(), row2.end(), '-');
+                    int start2 = T.contig.start + sg[1];
+                    if (S.hasPipelineFields)
+                        start2 = S.isPlus ? (int)(S.wStart - 1 + start2) : (int)(S.trueLength - S.wEnd + start2);
+                    const int reportLen = S.hasPipelineFields ? (int)S.trueLength : S.length;
+                    const int anchor2 = T.contig.start + a2;
+                    const int start1 = sg[0];
+                    const int w1 = (int)std::max(P.name.size(), S.name.size());
+                    const int w2 = std::max(numOfDigits(start1), numOfDigits(start2));
+                    const int w3 = std::max(numOfDigits(span1), numOfDigits(span2));
+                    const int w4 = std::max(numOfDigits(P.M), numOfDigits(reportLen));
+                    os << "a score=" << scoreStr(natsToLog2(lnr)) << " E=" << ev << " anchor=" << a1 << ","
+                       << anchor2 << "\n";
+                    os << "s " << std::left << std::setw(w1) << P.name << " " << std::right << std::setw(w2)
+                       << start1 << " " << std::setw(w3) << span1 << " +"
+                       << " " << std::setw(w4) << P.M << " " << row1 << "\n";
+                    os << "s " << std::left << std::setw(w1) << S.name << " " << std::right << std::setw(w2)
+                       << start2 << " " << std::setw(w3) << span2 << " " << (S.isPlus ? '+' : '-') << " "
+                       << std::setw(w4) << reportLen << " " << row2 << "\n\n";
+                }
+            }
+        }
+        const std::string out = os.str();
+        std::fwrite(out.data(), 1, out.size(), stdout);
+        targets.clear();
+        reqProfRatio.clear();
+        reqTarget.clear();
+        return 0;
+    };
+
+    {
+        ContigReader rd(*in);
+        SeqState s;
+        Contig ct;
+        std::vector<char> letters;
+        while (rd.next(s, ct, letters, err)) {
+            if (ct.length == 0) continue;
+            if (needTotal) minProbRatioTotal += ct.length;
+            Target T;
+            T.rec = s;
+            T.contig = ct;
+            T.letters.resize(letters.size());
+            for (size_t j = 0; j < letters.size(); j++) T.letters[j] = kAlphabet[(int)letters[j]];
+            T.codes.resize(letters.size());
+            decodeDna(T.letters.data(), (int64_t)T.letters.size(), T.codes.data());
+            const int ti = (int)targets.size();
+            for (int j = 0; j < np; j++) {
+                if (!s.targetProfile.empty() && profiles[j].name != s.targetProfile) continue;
+                const float ratio =
+                    o.evalue > 0 ? (float)std::pow(K[j] * (double)minProbRatioTotal / o.evalue, 1.0 / lambda[j]) : -1.f;
+                reqProfRatio.push_back({j, ratio});
+                reqTarget.push_back(ti);
+            }
+            targets.push_back(std::move(T));
+            if ((int)targets.size() >= batch && (rc = flush())) return rc;
+        }
+        if (!err.empty()) return die(err);
+        if ((rc = flush())) return rc;
+    }
+    std::cout << "# Batches: fast_batches=" << nFullPairs << " calib_batches=" << (int64_t)np * o.trials << "\n";
+    std::cout << "# Filter stats: filter_batches=" << nFilterPairs << " discarded_empty=" << (nFilterPairs - nFullPairs)
+              << " max_dp_width=" << maxWidth << "\n";
+    if (!spool.empty()) std::remove(spool.c_str());
+    return 0;
+}

@@ -67,6 +67,8 @@ struct dm_engine {
     DevBuf dPairs, dBrec, dFrec, dPre, dSuf, dUnits, dCounter, dWbuf, dW0buf, dEbuf, dFbuf, dOut;
     std::vector<int> cfgG, cfgR;
     std::vector<char> cfgStrips;  // last entry: the row-strip configuration (G = 64, R = kStripR)
+    std::vector<double> cfgStepNs;  // autotuned device ns per wave-step (0: not measured)
+    int forceCfg = -1;              // autotuning: every pair runs this configuration
     DevBuf dBnd, dBndE;
     bool plain = false;
     int gridOpt = 0;
@@ -139,16 +141,30 @@ static int syncSequences(dm_engine *e) {
 // ---------------------------------------------------------------------------
 // Estimated wave-steps for one pair: (n + G - 1) steps of R rows, shared by
 // 64/G pairs per wave.
+static int stripsOf(const dm_engine *e, int cfg, int M);
+
+// Cost of one pair under a configuration: wave-steps (n + G - 1 per row
+// strip) shared by the 64/G pairs of a wave, times the autotuned time of one
+// wave-step when it has been measured, else the rows a lane handles (R).
+static double pairCost(const dm_engine *e, int c, int M, int n) {
+    const int G = e->cfgG[c], R = e->cfgR[c];
+    const double steps = (double)(n + G - 1) * stripsOf(e, c, M) * G / 64.0;
+    const double t = (c < (int)e->cfgStepNs.size() && e->cfgStepNs[c] > 0) ? e->cfgStepNs[c] : (double)R;
+    return steps * t;
+}
+
 static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
+    if (e->forceCfg >= 0) return e->forceCfg;
     int best = -1;
     double bestCost = 0;
+    const bool tuned = !e->cfgStepNs.empty();
     for (size_t c = 0; c < e->cfgG.size(); c++) {
         const int G = e->cfgG[c], R = e->cfgR[c];
-        if (e->cfgStrips[c]) continue;
-        if (G * R < M + 1) continue;
+        if (e->cfgStrips[c] && (!tuned || forcePlain)) continue;  // strips: only when measured (or needed)
+        if (!e->cfgStrips[c] && G * R < M + 1) continue;
         if (forcePlain && G != 64) continue;
-        const double cost = (double)(n + G - 1) * R * G / 64.0;
-        if (best < 0 || cost < bestCost - 1e-9) {
+        const double cost = pairCost(e, (int)c, M, n);
+        if (best < 0 || cost < bestCost * (1 - 1e-9)) {
             best = (int)c;
             bestCost = cost;
         }
@@ -1128,6 +1144,104 @@ int dm_gumbel_fits(const double *scores, int32_t n, double seq_len, double out[8
     if (!scores || !out || n < 2) return DM_EINVAL;
     gumbelFits(std::vector<double>(scores, scores + n), seq_len, out);
     return DM_OK;
+}
+
+int dm_autotune(dm_engine *e, int32_t seq_len, double *ns_per_step, int32_t cap) {
+    if (!e || seq_len < 3 || seq_len > (1 << 20)) return DM_EINVAL;
+    HIPCHK(hipSetDevice(e->device));
+    const size_t ncfg = e->cfgG.size();
+    const size_t prof0 = e->profiles.size(), codes0 = e->codes.size(), seqs0 = e->seqLen.size();
+    // Synthetic inputs: per configuration one profile that fills its row
+    // group (two strips for the strip kernel) with flat emissions, and random
+    // DNA of seq_len letters.
+    std::vector<int32_t> cfgProf(ncfg, -1);
+    std::string msg;
+    for (size_t c = 0; c < ncfg; c++) {
+        const int rows = e->cfgG[c] * e->cfgR[c] * (e->cfgStrips[c] ? 2 : 1);
+        HostProfile P;
+        P.name = "autotune";
+        P.M = rows - 1;
+        P.values.assign((size_t)rows * kWidth, 0.f);
+        for (int i = 0; i < rows; i++) {
+            float *v = &P.values[(size_t)i * kWidth];
+            v[0] = 0.02f;
+            v[1] = 0.5f;
+            v[2] = (i < rows - 1) ? 0.02f : 0.f;
+            v[3] = (i < rows - 1) ? 0.5f : 0.f;
+            if (i < rows - 1)
+                for (int a = 0; a < 20; a++) v[4 + a] = 0.03f + 0.004f * (float)((a * 7 + i) % 10);
+        }
+        P.consensus.assign(rows - 1, 0);
+        if (!finalizeRaw(P, 'G', e->rates, msg)) {
+            e->profiles.resize(prof0);
+            return fail(e, DM_EINVAL, "autotune profile: " + msg);
+        }
+        cfgProf[c] = (int32_t)e->profiles.size();
+        e->profiles.push_back(std::move(P));
+    }
+    e->profDirty = true;
+    const int nseq = 64;
+    std::vector<char> dna(seq_len);
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (int k = 0; k < nseq; k++) {
+        for (int j = 0; j < seq_len; j++) {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            dna[j] = "ACGT"[x & 3];
+        }
+        const size_t at = e->codes.size();
+        e->codes.resize(at + seq_len);
+        decodeDna(dna.data(), seq_len, e->codes.data() + at);
+        e->seqOff.push_back((int64_t)at);
+        e->seqLen.push_back(seq_len);
+    }
+    e->seqDirty = true;
+    std::vector<double> tuned(ncfg, 0.0);
+    int rc = DM_OK;
+    for (size_t c = 0; c < ncfg && rc == DM_OK; c++) {
+        const int G = e->cfgG[c];
+        const int S = e->cfgStrips[c] ? 2 : 1;
+        const int units = e->numCUs * (e->cfgStrips[c] ? 2 : 16);
+        const int64_t np = (int64_t)units * (64 / G);
+        std::vector<int32_t> pp(np, cfgProf[c]), ss(np);
+        for (int64_t i = 0; i < np; i++) ss[i] = (int32_t)(seqs0 + (size_t)(i % nseq));
+        e->forceCfg = (int)c;
+        dm_plan P;
+        rc = buildPlan(e, kScore, np, pp.data(), ss.data(), 0, P);
+        std::vector<PairScore> out(np);
+        if (rc == DM_OK) rc = executePlan(e, P, nullptr, out.data());  // warm-up
+        if (rc == DM_OK) rc = executePlan(e, P, nullptr, out.data());
+        if (rc == DM_OK) {
+            const double waveSteps = (double)units * (double)(seq_len + G - 1) * S;
+            tuned[c] = e->cfgMs[c] * 1e6 / waveSteps;
+        }
+        P.release();
+    }
+    e->forceCfg = -1;
+    e->profiles.resize(prof0);
+    e->profDirty = true;
+    e->codes.resize(codes0);
+    e->seqOff.resize(seqs0);
+    e->seqLen.resize(seqs0);
+    e->seqDirty = true;
+    if (rc) return rc;
+    e->cfgStepNs = tuned;
+    if (ns_per_step)
+        for (size_t c = 0; c < ncfg && (int32_t)c < cap; c++) ns_per_step[c] = tuned[c];
+    return (int)ncfg;
+}
+
+int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int32_t *strips, double *ns_per_step) {
+    if (!e) return DM_EINVAL;
+    const size_t ncfg = e->cfgG.size();
+    for (size_t c = 0; c < ncfg && (int32_t)c < cap; c++) {
+        if (G) G[c] = e->cfgG[c];
+        if (R) R[c] = e->cfgR[c];
+        if (strips) strips[c] = e->cfgStrips[c];
+        if (ns_per_step) ns_per_step[c] = c < e->cfgStepNs.size() ? e->cfgStepNs[c] : 0.0;
+    }
+    return (int)ncfg;
 }
 
 double dm_last_dp_ms(const dm_engine *e) { return e ? e->lastMs : 0.0; }

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <float.h>
+#include <limits.h>
 #include <type_traits>
 
 #include "dm_layout.h"
@@ -306,6 +307,17 @@ template <int G> __device__ __forceinline__ ES group_max(ES v) {
 }
 
 constexpr float kBig = 1.2676506e30f;  // 2^100: rescale trigger
+constexpr float kSmall = 6.8719477e10f; // 2^36: the live state below this is raised by 2^64
+constexpr int kGhostTop = 100;          // strip ghost values are kept below 2^100 after conversion
+
+// True (in every lane of the group) when the whole group's live maximum is
+// below kSmall and not all zero.
+template <int G> __device__ __forceinline__ bool group_small(float live, int grp) {
+    const unsigned long long bs = __ballot(live < kSmall);
+    const unsigned long long bn = __ballot(live > 0.f);
+    const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (grp * G));
+    return ((bs & gm) == gm) && ((bn & gm) != 0ull);
+}
 constexpr float kStep = 5.421011e-20f;  // 2^-64
 constexpr int kPre = 100;               // bulk-mode prescale of stored W1 (2^-100)
 
@@ -317,6 +329,7 @@ __device__ __forceinline__ int bidx(int c, int n) { return c < -1 ? 0 : (c > n ?
 template <int G, int R, int MODE, bool STR>
 __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
     constexpr int ROWS = G * R;
+    constexpr int kPF = (R <= 8) ? 4 : 2;  // prefetch distance in steps (divides 4)
     extern __shared__ float etab[];  // [kNCode][ROWS]
     const int lane = threadIdx.x;
     const int grp = lane / G;
@@ -431,10 +444,11 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             const float *yIn = STR ? bY2 + (par ^ 1) * a.bndCols : nullptr;
             const int *eIn = STR ? bE + (par ^ 1) * a.bndCols : nullptr;
 
-            float H[4][R], Zr[4][R], Yc[R], GW[4];
+            float H[4][R], Zr[4][R], Yc[R], GW[4], LV[4];
 #pragma unroll
             for (int s4 = 0; s4 < 4; s4++) {
                 GW[s4] = 0.f;
+                LV[s4] = 0.f;
 #pragma unroll
                 for (int q = 0; q < R; q++) { H[s4][q] = 0.f; Zr[s4][q] = 0.f; }
             }
@@ -443,12 +457,60 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             int Eb = 0;      // backward exponent of the current step
             float rs = 1.f;  // 2^-Eb, the seed's row scale
 
+            // Multiply the pair's whole backward state by 2^s (exact, or a flush
+            // to zero for large negative s) and move the exponent to match.
+            auto scaleB = [&](int s) {
+                const float f = ldexpf(1.f, s);
+#pragma unroll
+                for (int s4 = 0; s4 < 4; s4++) {
+                    GW[s4] *= f;
+                    LV[s4] *= f;
+#pragma unroll
+                    for (int q = 0; q < R; q++) { H[s4][q] *= f; Zr[s4][q] *= f; }
+                }
+#pragma unroll
+                for (int q = 0; q < R; q++) Yc[q] *= f;
+                Eb -= s;
+                rs = ldexpf(1.f, -Eb);
+                if (a.wantBwd) bwdMax.rebase(Eb + kPre);
+            };
+
+            // Column records are loaded kPF steps ahead of their use.
+            auto brecAt = [&](int tt) {
+                const int c = Tmine - 1 - tt - k;
+                return br[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
+            };
+            BRec rq[kPF];
+#pragma unroll
+            for (int f = 0; f < kPF; f++) rq[f] = brecAt(f);
+
             auto bstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
                 constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
                 const int c = Tmine - 1 - t - k;  // column: D = Tmine-1-t
-                const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
-                const BRec rc = br[cc];
+                const BRec rc = rq[PH % kPF];
+                rq[PH % kPF] = brecAt(t + kPF);
+                float gw = 0.f, gy = 0.f;
+                if (STR && below) {
+                    // last lane: the strip below's top row from the boundary buffer.
+                    // When that row sits far above this strip's scale, raise the
+                    // exponent first (before any neighbour value is read).
+                    const int bi = bidx(c, n);
+                    const float rw = wIn[bi], ry = yIn[bi];
+                    const int ge = eIn[bi];
+                    if (MODE != kFilter) {
+                        int need = 0;
+                        if (k == G - 1 && (rw > 0.f || ry > 0.f)) {
+                            int ex;
+                            (void)frexpf(fmaxf(rw, ry), &ex);
+                            need = ex + ge - Eb - kGhostTop;
+                        }
+                        need = __builtin_amdgcn_readlane(need, G - 1);
+                        if (__builtin_expect(need > 0, 0)) scaleB(-((need + 63) & ~63));
+                    }
+                    gw = ldexpf(rw, ge - Eb);
+                    gy = ldexpf(ry, ge - Eb);
+                }
                 // lane below (k+1): its row 0 at columns c+1, c+2, c+3 and its Y at c
                 float bw1 = from_below<G>(H[S2][0]);
                 float bw2 = from_below<G>(H[S3][0]);
@@ -456,14 +518,6 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 float bY = from_below<G>(Yc[0]);
                 if (G == 32 && MODE == kFilter && k == G - 1) { bw1 = 0.f; bw2 = 0.f; bw3 = 0.f; bY = 0.f; }
                 if (STR) {
-                    // last lane: the strip below's top row from the boundary buffer
-                    const int bi = bidx(c, n);
-                    float gw = 0.f, gy = 0.f;
-                    if (below) {
-                        const int sh = eIn[bi] - Eb;
-                        gw = ldexpf(wIn[bi], sh);
-                        gy = ldexpf(yIn[bi], sh);
-                    }
                     GW[S4] = gw;
                     if (k == G - 1) {
                         bw1 = GW[S1];
@@ -493,7 +547,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     if (MODE == kFilter) {
                         fbest = fmaxf(fbest, w * psp);
                     } else {
-                        smax = fmaxf(smax, fmaxf(w, y));
+                        smax = fmaxf(smax, fmaxf(y, z));  // y, z >= w
                         if (a.wantBwd) bwdMax.cur = fmaxf(bwdMax.cur, w * (psp * 7.888609052e-31f));
                     }
                     // row q at columns c+1, c+2, c+3 for row q-1 (same lane, same column)
@@ -521,25 +575,21 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     }
                 }
                 if (MODE != kFilter) {
-                    // rescale: one exponent per pair per step
+                    // One exponent per pair per step: lower the scale by 2^-64 when
+                    // a value passes 2^100; every 4 steps, raise it by 2^64 when the
+                    // whole live state (the last 4 steps) of the pair is below 2^36,
+                    // so decaying regions keep their precision.
+                    LV[S4] = fmaxf(smax, fmaxf(gw, gy));
                     const bool big = smax > kBig;
                     if (__builtin_expect(__any(big), 0)) {
                         bool gbig = big;
 #pragma unroll
                         for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
-                        if (gbig) {
-#pragma unroll
-                            for (int s4 = 0; s4 < 4; s4++) {
-                                GW[s4] *= kStep;
-#pragma unroll
-                                for (int q = 0; q < R; q++) { H[s4][q] *= kStep; Zr[s4][q] *= kStep; }
-                            }
-#pragma unroll
-                            for (int q = 0; q < R; q++) Yc[q] *= kStep;
-                            Eb += 64;
-                            rs = ldexpf(1.f, -Eb);
-                            if (a.wantBwd) bwdMax.rebase(Eb + kPre);
-                        }
+                        if (gbig) scaleB(-64);
+                    }
+                    if (PH == 3) {
+                        const float live = fmaxf(fmaxf(LV[0], LV[1]), fmaxf(LV[2], LV[3]));
+                        if (__builtin_expect(group_small<G>(live, grp), 0)) scaleB(64);
                     }
                 }
             };
@@ -581,11 +631,13 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             const int *esc = esc0 + (int64_t)strip * a.stripSteps;
             const bool above = STR && strip > 0;      // lane 0 reads the strip above
             const bool below = STR && strip < S - 1;  // last lane feeds the strip below
-            float Wf[4][R], Zf[4][R];
+            float Wf[4][R], Zf[4][R], LVf[4];
 #pragma unroll
-            for (int s4 = 0; s4 < 4; s4++)
+            for (int s4 = 0; s4 < 4; s4++) {
+                LVf[s4] = 0.f;
 #pragma unroll
                 for (int q = 0; q < R; q++) { Wf[s4][q] = 0.f; Zf[s4][q] = 0.f; }
+            }
             float lastN = 0.f, lastY = 0.f;  // row R-1 outputs of the previous step
             int Ef = 0;
             float rf = 1.f;
@@ -594,35 +646,87 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             float *w0d = (MODE == kDump) ? a.w0buf + PR.bufOff + (int64_t)strip * strideS : nullptr;
             int *efd = (MODE == kDump) ? a.fbuf + PR.bufOff / ROWS + (int64_t)strip * a.stripSteps : nullptr;
 
+            // Multiply the pair's forward state by 2^s and move the exponent.
+            auto scaleF = [&](int s) {
+                const float f = ldexpf(1.f, s);
+#pragma unroll
+                for (int s4 = 0; s4 < 4; s4++) {
+                    LVf[s4] *= f;
+#pragma unroll
+                    for (int q = 0; q < R; q++) { Wf[s4][q] *= f; Zf[s4][q] *= f; }
+                }
+                lastN *= f;
+                lastY *= f;
+                Ef -= s;
+                rf = ldexpf(1.f, -Ef);
+                if (MODE == kScore) midMax.rebase(Ef + EbPrev + kPre);
+            };
+
+            // Column records and the stored W1 block of a step (with its
+            // exponent) are loaded kPF steps ahead of their use.
+            constexpr int kNoStep = INT_MIN;
+            auto frecAt = [&](int tt) {
+                const int c = tt - k;
+                return fr[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
+            };
+            FRec fq[kPF];
+            float Wq[kPF][R];
+            int Eq[kPF];
+            auto loadW1 = [&](int tt, float *dst, int &e) {
+                const bool in = tt < Tmine;
+                e = in ? esc[tt] : kNoStep;
+#pragma unroll
+                for (int q = 0; q < R; q++) dst[q] = (MODE == kScore && in) ? wsc[(int64_t)tt * ROWS + k * R + q] : 0.f;
+            };
+#pragma unroll
+            for (int f = 0; f < kPF; f++) {
+                fq[f] = frecAt(f);
+                loadW1(f, Wq[f], Eq[f]);
+            }
+
             auto fstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
                 constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
+                constexpr int SL = PH % kPF;
                 const int D = t;
                 const int c = D - k;
-                const int cc = c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c);
-                const FRec rc = fr[cc];
+                const FRec rc = fq[SL];
+                fq[SL] = frecAt(t + kPF);
+                float W1v[R];
+#pragma unroll
+                for (int q = 0; q < R; q++) W1v[q] = Wq[SL][q];
+                const int EbNext = Eq[SL];
+                loadW1(t + kPF, Wq[SL], Eq[SL]);
+                float gn = 0.f, gy = 0.f;
+                if (STR && above) {
+                    // lane 0: the strip above's last row from the boundary buffer;
+                    // raise this strip's exponent first when that row is far above it
+                    const int bi = bidx(c, n);
+                    const float rn = fN[bi], ry = fY[bi];
+                    const int ge = fE[bi];
+                    int need = 0;
+                    if (k == 0 && (rn > 0.f || ry > 0.f)) {
+                        int ex;
+                        (void)frexpf(fmaxf(rn, ry), &ex);
+                        need = ex + ge - Ef - kGhostTop;
+                    }
+                    need = __builtin_amdgcn_readlane(need, 0);
+                    if (__builtin_expect(need > 0, 0)) scaleF(-((need + 63) & ~63));
+                    gn = ldexpf(rn, ge - Ef);
+                    gy = ldexpf(ry, ge - Ef);
+                }
                 float Nin = from_above<G>(lastN);
                 float Ya = from_above<G>(lastY);
-                if (STR && above) {
-                    // lane 0: the strip above's last row from the boundary buffer
-                    const int bi = bidx(c, n);
-                    const int sh = fE[bi] - Ef;
-                    const float gn = ldexpf(fN[bi], sh), gy = ldexpf(fY[bi], sh);
-                    if (k == 0) {
-                        Nin = gn;
-                        Ya = gy;
-                    }
+                if (STR && above && k == 0) {
+                    Nin = gn;
+                    Ya = gy;
                 }
                 const bool inD = D < Tmine;
-                const int EbD = inD ? esc[D] : EbPrev;
+                const int EbD = (EbNext != kNoStep) ? EbNext : EbPrev;
                 if (MODE == kScore && __builtin_expect(__any(EbD != EbPrev), 0)) {
                     if (EbD != EbPrev) midMax.rebase(Ef + EbD + kPre);
                 }
                 EbPrev = EbD;
-                float W1v[R];
-#pragma unroll
-                for (int q = 0; q < R; q++)
-                    W1v[q] = (MODE == kScore && inD) ? wsc[(int64_t)D * ROWS + k * R + q] : 0.f;
                 const float gsC = rc.g * sC;
                 const float psr = rc.ps * rf;
                 const float *et = etk + rc.code * ROWS;
@@ -656,22 +760,18 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     fE[c + 1] = Ef;
                 }
                 if (MODE == kDump && inD && active && k == 0) efd[D] = Ef;
+                // exponent moves as in the backward (up at 2^100, down below 2^36)
+                LVf[S4] = fmaxf(fmaxf(smax, fmaxf(lastN, lastY)), fmaxf(gn, gy));
                 const bool big = smax > kBig;
                 if (__builtin_expect(__any(big), 0)) {
                     bool gbig = big;
 #pragma unroll
                     for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
-                    if (gbig) {
-#pragma unroll
-                        for (int s4 = 0; s4 < 4; s4++)
-#pragma unroll
-                            for (int q = 0; q < R; q++) { Wf[s4][q] *= kStep; Zf[s4][q] *= kStep; }
-                        lastN *= kStep;
-                        lastY *= kStep;
-                        Ef += 64;
-                        rf = ldexpf(1.f, -Ef);
-                        if (MODE == kScore) midMax.rebase(Ef + EbPrev + kPre);
-                    }
+                    if (gbig) scaleF(-64);
+                }
+                if (PH == 3) {
+                    const float live = fmaxf(fmaxf(LVf[0], LVf[1]), fmaxf(LVf[2], LVf[3]));
+                    if (__builtin_expect(group_small<G>(live, grp), 0)) scaleF(64);
                 }
             };
             int t = 0;

@@ -161,6 +161,18 @@ int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_l
 int dm_calib_sequence(const float *bg, const double rates[6], int32_t L, int32_t border, int32_t trial, char *out);
 int dm_gumbel_fits(const double *scores, int32_t n, double seq_len, double out[8]);
 
+/* Autotuner: times every kernel configuration (G lanes per pair, R rows per
+ * lane, and the row-strip kernel) on this device with synthetic pairs of
+ * seq_len letters whose profile fills the configuration's rows, and keeps the
+ * measured device time per wave-step; later plans choose, per pair, the
+ * configuration with the least measured cost (n + G - 1) * strips * G/64 * t.
+ * Writes up to cap times (ns) in configuration order; returns the number of
+ * configurations. Replaces the reference's fixed lane packing
+ * (simdWidth, DUMMEST_MAX_LANES; dummest-search.hh:85-87). */
+int dm_autotune(dm_engine *e, int32_t seq_len, double *ns_per_step, int32_t cap);
+/* The configuration table (G, R, strip kernel or not, tuned ns per wave-step or 0). */
+int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int32_t *strips, double *ns_per_step);
+
 /* Options: "plain" = 1 launches the DP once per pair (the unbatched form);
  * "grid" = persistent grid size (0 = automatic). */
 int dm_set_option(dm_engine *e, const char *name, int64_t value);

@@ -1,0 +1,41 @@
+"""The engine executable (dummest-amd): option handling that needs no GPU.
+The end-to-end record parity against the oracle is in test_gpu_cli.py."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "dummest_amd", "dummest-amd")
+
+
+def run(*args, stdin=None):
+    return subprocess.run([CLI, *args], input=stdin, capture_output=True, text=True, timeout=60)
+
+
+def test_help_lists_reference_options():
+    r = run("-h")
+    assert r.returncode == 0
+    for opt in ("--evalue", "--total-length", "--trials", "--length", "--border", "--batch", "--scores-file",
+                "--warmup", "--barithmetic", "--bgeometric", "--bmedian", "--insert1", "--insert2", "--delete1",
+                "--delete2", "--stop-codon-prob", "--bg-stop-codon-prob", "--max", "--filter-evalue"):
+        assert opt in r.stdout, opt
+
+
+@pytest.mark.parametrize("args", [["-e", "-1"], ["-t", "0"], ["--insert1", "1.5"], ["-m", "7"], ["-s", "3"]])
+def test_bad_option_values_fail(args, tmp_path):
+    r = run(*args, str(tmp_path / "x.hmm"))
+    assert r.returncode == 1
+
+
+def test_masking_is_refused_not_ignored(tmp_path):
+    # tantan masking is not restated: the default -m 3 must fail loudly
+    r = run(str(tmp_path / "x.hmm"))
+    assert r.returncode == 1 and "-m 0" in r.stderr
+    r = run("-d", "5", "-m", "0", str(tmp_path / "x.hmm"))
+    assert r.returncode == 1 and "-d" in r.stderr
+
+
+def test_usage_without_arguments():
+    r = run()
+    assert r.returncode == 1 and "usage" in r.stderr

@@ -1,0 +1,211 @@
+"""End-to-end parity of the engine executable (dummest-amd) against the
+oracle: calibration (trial sequences, scores, Gumbel fits), the filter wave,
+the anchor gate, hits and their alignments, E-values and the record text
+(dummest.cc:473-606, dummest-search.hh:89-179, dummest-output.hh:54-107).
+
+The expected records are built here from oracle results: the contig reader,
+the per-contig running total of the E-value gate and the record layout are
+restated in this file (test code) from the reference lines cited.
+
+Tolerances: scores within 1e-4 nats (printed in bits, 6 decimals); E-values
+within 1e-3 relative (K and lambda come from sample moments of scores that
+agree within 1e-4 nats); everything else (hit set, anchors, starts, spans,
+aligned rows) identical. Hits whose E-value lies within 1% of the threshold
+are left out of the set comparison on both sides.
+"""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dummest_amd import synth
+from tests.common import synthetic_case
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "dummest_amd", "dummest-amd")
+ALPHA = "ACDEFGHIKLMNPQRSTVWYUO?*"
+TRIALS, LENGTH, BORDER = 48, 600, 30
+
+
+def contigs_of(fasta_text):
+    """readContig (dummest-sequence.hh:6-89) with PIPELINE_MODE: yields
+    (name, contig_start, letters, record_len_so_far, pipeline fields)."""
+    cmap = {}
+    for i, a in enumerate(ALPHA):
+        cmap[a] = cmap[a.lower()] = i
+    for a in "RYSWKMBDHVN":
+        cmap[a] = cmap[a.lower()] = 22
+    out = []
+    st = dict(w_start=0, w_end=0, true_length=0, target="", is_plus=True, has_pipe=False)
+    for rec in fasta_text.split(">")[1:]:
+        head, _, body = rec.partition("\n")
+        toks = head.split()
+        word = toks[0]
+        if "/" in word:
+            chrom, rng = word.split("/", 1)
+            if "-" in rng:
+                a, b = rng.split("-", 1)
+                st["w_start"], st["w_end"] = int(a), int(b)
+            length = profile = strand = None
+            for t in toks[1:]:
+                if t.startswith("length="):
+                    length = t
+                elif t.startswith("profile="):
+                    profile = t
+                elif t.startswith("bitscore="):
+                    continue
+                elif strand is None:
+                    strand = t
+            if length and profile and strand:
+                st["true_length"] = int(length.split("=", 1)[1])
+                st["target"] = profile.split("=", 1)[1]
+                st["is_plus"] = "plus_strand" in strand
+            st["has_pipe"] = True
+            word = chrom
+        pos = 0
+        cur = []
+        start = 0
+        for ch in body:
+            if ord(ch) <= 32:
+                continue
+            if ch in cmap:
+                if not cur:
+                    start = pos
+                cur.append(ALPHA[cmap[ch]])
+                pos += 1
+            else:
+                if cur:
+                    out.append((word, start, "".join(cur), pos, dict(st)))
+                    cur = []
+                pos += 1
+        if cur:
+            out.append((word, start, "".join(cur), pos, dict(st)))
+    return out
+
+
+def fmt_records(prof_name, M, consensus, contig, hits, K, lam, total, evalue):
+    name, cstart, letters, rec_len, st = contig
+    recs = []
+    for lnr, m, e, a1, a2, segs in hits:
+        if not segs:
+            continue
+        row1, row2 = [], []
+        pos1, pos2 = segs[0][0], segs[0][1]
+        for s1, s2, ln in segs:
+            while pos1 < s1:
+                row1.append(ALPHA[consensus[pos1]]); pos1 += 1
+            row1.append("-" * (s2 - pos2))
+            while pos1 < s1 + ln:
+                row1.append(ALPHA[consensus[pos1]]); pos1 += 1
+            pos2 = s2 + ln
+        pos1, pos2 = segs[0][0], segs[0][1]
+        for s1, s2, ln in segs:
+            row2.append("-" * (s1 - pos1))
+            row2.append(letters[pos2:s2 + ln])
+            pos2 = s2 + ln
+            pos1 = s1 + ln
+        r1, r2 = "".join(row1), "".join(row2)
+        ev = K * total * math.exp(-lam * lnr)
+        if evalue > 0 and ev > evalue:
+            continue
+        start2 = cstart + segs[0][1]
+        if st["has_pipe"]:
+            start2 = st["w_start"] - 1 + start2 if st["is_plus"] else st["true_length"] - st["w_end"] + start2
+        rlen = st["true_length"] if st["has_pipe"] else rec_len
+        recs.append(dict(bits=lnr / math.log(2), E=ev, a1=a1, a2=cstart + a2, s1=segs[0][0],
+                         span1=len(r1) - r1.count("-"), s2=start2, span2=len(r2) - r2.count("-"),
+                         strand="+" if st["is_plus"] else "-", plen=M, slen=rlen, r1=r1, r2=r2,
+                         pname=prof_name, sname=name))
+    return recs
+
+
+def parse_records(text):
+    recs = []
+    lines = text.splitlines()
+    for k, line in enumerate(lines):
+        m = re.match(r"a score=(\S+) E=(\S+) anchor=(\d+),(\d+)$", line)
+        if not m:
+            continue
+        p = lines[k + 1].split()
+        s = lines[k + 2].split()
+        recs.append(dict(bits=float(m.group(1)), E=float(m.group(2)), a1=int(m.group(3)), a2=int(m.group(4)),
+                         pname=p[1], s1=int(p[2]), span1=int(p[3]), plen=int(p[5]), r1=p[6],
+                         sname=s[1], s2=int(s[2]), span2=int(s[3]), strand=s[4], slen=int(s[5]), r2=s[6]))
+    return recs
+
+
+def key(r):
+    return (r["pname"], r["sname"], r["a1"], r["a2"], r["s1"], r["span1"], r["s2"], r["span2"], r["strand"],
+            r["plen"], r["slen"], r["r1"], r["r2"])
+
+
+def test_cli_records_match_oracle(tmp_path):
+    print("cli end-to-end", flush=True)
+    rng = np.random.default_rng(77)
+    text, prof, dnas = synthetic_case(701, 120, [900, 1500, 700, 2000, 1200, 800], homolog_every=2)
+    # a contig break ('X' is not a sequence letter) and IUPAC letters inside a homolog record
+    dnas[2] = dnas[2][:300] + "XX" + dnas[2][300:]
+    dnas[4] = dnas[4][:100] + "NNRY" + dnas[4][104:]
+    fasta = []
+    for k, d in enumerate(dnas):
+        if k == 3:  # pipeline header: window coordinates, true length, target profile, strand
+            fasta.append(f">chr9/1001-{1000 + len(d)} length=50000 profile=syn701 minus_strand_revcomp\n")
+        else:
+            fasta.append(f">read{k} some description\n")
+        fasta.extend(d[i:i + 60] + "\n" for i in range(0, len(d), 60))
+    fasta_text = "".join(fasta)
+    (tmp_path / "p.hmm").write_text(text)
+    (tmp_path / "s.fa").write_text(fasta_text)
+    evalue = 1.0
+    r = subprocess.run([CLI, "-m", "0", "-t", str(TRIALS), "-l", str(LENGTH), "-b", str(BORDER), "-e", str(evalue),
+                        str(tmp_path / "p.hmm"), str(tmp_path / "s.fa")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = parse_records(r.stdout)
+
+    # oracle: calibration
+    seqs = [O.decode(O.calib_sequence(prof.bg, LENGTH, BORDER, t)) for t in range(TRIALS)]
+    mid, bwd = O.fwdbwd_batch(prof, seqs)
+    mid_ln = [O.es_log(float(m), int(e)) for m, e in mid]
+    flt_ln = [O.es_log(float(m), int(e)) for m, e in bwd]
+    gm, gf = O.gumbel(mid_ln, LENGTH), O.gumbel(flt_ln, LENGTH)
+    lam, K, lamf, Kf = gm["mmL"], gm["mmK"], gf["mmL"], gf["mmK"]
+    m = re.findall(r"# Lambda: (\S+)", r.stdout)
+    assert float(m[0]) == pytest.approx(lam, rel=1e-3) and float(m[1]) == pytest.approx(lamf, rel=1e-3)
+
+    # oracle: search
+    contigs = contigs_of(fasta_text)
+    total = sum(len(c[2]) for c in contigs)
+    assert f"# Total sequence length: {total}" in r.stdout
+    raw = O.parse_hmm(text)[0]
+    consensus = list(raw.consensus)
+    want = []
+    running = 0
+    for c in contigs:
+        running += len(c[2])
+        codes = O.decode(c[2])
+        if len(codes) < 3:
+            continue
+        ratio = float(np.float32((K * running / evalue) ** (1 / lam)))
+        best = O.filter_score(prof, codes)
+        if not best > 0:
+            continue
+        lp = math.log(best)
+        if not (Kf * total / math.exp(lp * lamf) <= evalue or not math.isfinite(lp)):
+            continue
+        res = O.search(prof, codes, ratio)
+        want += fmt_records("syn701", prof.M, consensus, c, res.hits, K, lam, total, evalue)
+    near = lambda rec: abs(rec["E"] - evalue) < 0.01 * evalue
+    gk = {key(x): x for x in got if not near(x)}
+    wk = {key(x): x for x in want if not near(x)}
+    assert len(wk) >= 2, "the case should produce hits"
+    assert set(gk) == set(wk)
+    for k2, w in wk.items():
+        g = gk[k2]
+        assert abs(g["bits"] - w["bits"]) < 1e-4 / math.log(2) + 1e-6
+        assert g["E"] == pytest.approx(w["E"], rel=1e-3)
