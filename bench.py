@@ -177,7 +177,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "uneven reads vs profiles, forward+backward scores: "
                                f"{len(w['reads'])} reads 150-300 bp x {len(w['Ms'])} profiles "
-                               "(lognormal median 150, clipped 30-1023), 0.4% pair mask",
+                               "(lognormal median 150, sigma 0.8, clipped 30-2000), 0.4% pair mask",
                    "pairs": int(len(prof)), "cells_per_step": cells, "dp_device_ms_per_step": round(dp_ms_step, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -20,7 +20,8 @@ constexpr int kWidth = 29;     // values per profile row (20 letters + 9)
 constexpr int kNCode = 25;     // decoded codes 0..24; 24 is the zero-probability pad (zero_idx)
 constexpr int kZeroIdx = 24;
 constexpr int kEmStride = 32;  // floats per emission row in HBM
-constexpr int kRPad = 64;      // record padding columns on each side of a pair's sequence
+constexpr int kRPad = 4;       // record padding columns on each side of a pair's sequence
+                               // (every column beyond them reads the zero record at the edge)
 
 // Backward record for column j: g[j+1], suffixSeed[j], prefixSeed[j], a[j+1].
 struct BRec {
@@ -54,6 +55,7 @@ struct PairInfo {
     int64_t bufOff;  // float offset of this pair's W1 / w0 buffers (full and dump modes)
     float s1, s2, sC;  // per-pair background scales, packed floats (dummest-dp-backward.hh:244-246)
     float pad2;
+    int64_t nullOff;   // null-kernel scratch: entry j of this pair at nullOff + 64*j
 };
 
 struct Unit {
@@ -85,8 +87,7 @@ struct NullArgs {
     PairInfo *pairs;
     BRec *brec;
     FRec *frec;
-    ES *scratchPre;    // per pair, n + 4 entries starting at recOff
-    ES *scratchSuf;
+    ES *scratch;       // suffix values, interleaved by the 64 pairs of a null-kernel wave
     int npairs;
     float bgfs1, bgfs2;  // BACKGROUND_FRAMESHIFT_RATE(_2), as Float sums
 };
@@ -118,11 +119,10 @@ struct DPArgs {
 #ifdef __HIPCC__
 // Implemented in kernels.hip.
 int dp_configs(int *gs, int *rs, int cap);
+int dp_strip_configs(int *gs, int *rs, int cap);
 hipError_t launch_null(const NullArgs &a, hipStream_t s);
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
-// Row-strip kernels (profiles longer than one group covers): G = 64 and R = kStripR.
-constexpr int kStripR = 16;
 #endif
 
 }  // namespace dm

@@ -64,9 +64,9 @@ struct dm_engine {
     std::vector<int32_t> seqLen;
     bool seqDirty = true;
     DevBuf dCodes;
-    DevBuf dPairs, dBrec, dFrec, dPre, dSuf, dUnits, dCounter, dWbuf, dW0buf, dEbuf, dFbuf, dOut;
+    DevBuf dPairs, dBrec, dFrec, dNull, dUnits, dCounter, dWbuf, dW0buf, dEbuf, dFbuf, dOut;
     std::vector<int> cfgG, cfgR;
-    std::vector<char> cfgStrips;  // last entry: the row-strip configuration (G = 64, R = kStripR)
+    std::vector<char> cfgStrips;  // row-strip configurations (G = 64, any profile length)
     std::vector<double> cfgStepNs;  // autotuned device ns per wave-step (0: not measured)
     int forceCfg = -1;              // autotuning: every pair runs this configuration
     DevBuf dBnd, dBndE;
@@ -169,9 +169,15 @@ static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
             bestCost = cost;
         }
     }
-    if (best < 0)
-        for (size_t c = 0; c < e->cfgG.size(); c++)
-            if (e->cfgStrips[c]) best = (int)c;  // profiles longer than one group: row strips
+    if (best < 0)  // profiles longer than one group: the cheapest row-strip kernel
+        for (size_t c = 0; c < e->cfgG.size(); c++) {
+            if (!e->cfgStrips[c]) continue;
+            const double cost = pairCost(e, (int)c, M, n);
+            if (best < 0 || cost < bestCost * (1 - 1e-9)) {
+                best = (int)c;
+                bestCost = cost;
+            }
+        }
     return best;
 }
 
@@ -186,7 +192,23 @@ struct Chunk {
     std::vector<int> unitCfg;    // cfg per unit (units grouped by cfg)
     std::vector<int> cfgFirstUnit, cfgUnits, cfgMaxN, cfgMaxS;
     int64_t records = 0;
+    int64_t nullScratch = 0;  // ES entries of the null kernel's interleaved scratch
 };
+
+// Null-kernel scratch: the pairs of each 64-pair wave share an interleaved
+// block of 64 x (longest n + 4) ExpScores, so the suffix values are written
+// and read with coalesced accesses.
+static void layoutNullScratch(Chunk &C) {
+    int64_t off = 0;
+    for (size_t w = 0; w < C.pairs.size(); w += 64) {
+        const size_t en = std::min(C.pairs.size(), w + 64);
+        int maxN = 0;
+        for (size_t i = w; i < en; i++) maxN = std::max(maxN, C.pairs[i].n);
+        for (size_t i = w; i < en; i++) C.pairs[i].nullOff = off + (int64_t)(i - w);
+        off += 64 * (int64_t)(maxN + 4);
+    }
+    C.nullScratch = std::max<int64_t>(off, 64);
+}
 
 // Build the queue for caller pairs [sorted positions b, e).
 static void buildChunk(const dm_engine *E, const std::vector<int64_t> &sorted, const std::vector<int> &cfgOf,
@@ -248,6 +270,7 @@ static void buildChunk(const dm_engine *E, const std::vector<int64_t> &sorted, c
         }
         (void)G;
     }
+    layoutNullScratch(C);
 }
 
 static int uploadChunk(dm_engine *e, Chunk &C) {
@@ -256,8 +279,7 @@ static int uploadChunk(dm_engine *e, Chunk &C) {
     HIPCHK(e->dUnits.ensure(sizeof(Unit) * C.units.size()));
     HIPCHK(e->dBrec.ensure(sizeof(BRec) * C.records));
     HIPCHK(e->dFrec.ensure(sizeof(FRec) * C.records));
-    HIPCHK(e->dPre.ensure(sizeof(ES) * C.records));
-    HIPCHK(e->dSuf.ensure(sizeof(ES) * C.records));
+    HIPCHK(e->dNull.ensure(sizeof(ES) * C.nullScratch));
     HIPCHK(e->dCounter.ensure(sizeof(int) * std::max<size_t>(64, e->cfgG.size())));
     HIPCHK(hipMemcpyAsync(e->dPairs.p, C.pairs.data(), sizeof(PairInfo) * np, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->dUnits.p, C.units.data(), sizeof(Unit) * C.units.size(), hipMemcpyHostToDevice,
@@ -268,8 +290,7 @@ static int uploadChunk(dm_engine *e, Chunk &C) {
     na.pairs = e->dPairs.as<PairInfo>();
     na.brec = e->dBrec.as<BRec>();
     na.frec = e->dFrec.as<FRec>();
-    na.scratchPre = e->dPre.as<ES>();
-    na.scratchSuf = e->dSuf.as<ES>();
+    na.scratch = e->dNull.as<ES>();
     na.npairs = (int)np;
     na.bgfs1 = e->bgfs1();
     na.bgfs2 = e->bgfs2();
@@ -316,7 +337,7 @@ struct dm_plan {
     int64_t cells = 0;
     int64_t maxChunk = 0;
     std::vector<DevChunk> chunks;
-    DevBuf out, brec, frec, pre, suf, counter, wbuf, ebuf, bnd, bndE;
+    DevBuf out, brec, frec, nullScratch, counter, wbuf, ebuf, bnd, bndE;
     void *hostOut = nullptr;  // pinned staging for results
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -325,7 +346,7 @@ struct dm_plan {
             c.pairs.release();
             c.units.release();
         }
-        for (DevBuf *b : {&out, &brec, &frec, &pre, &suf, &counter, &wbuf, &ebuf, &bnd, &bndE}) b->release();
+        for (DevBuf *b : {&out, &brec, &frec, &nullScratch, &counter, &wbuf, &ebuf, &bnd, &bndE}) b->release();
         if (graphExec) (void)hipGraphExecDestroy(graphExec);
         if (graph) (void)hipGraphDestroy(graph);
         if (hostOut) (void)hipHostFree(hostOut);
@@ -365,7 +386,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         return e->seqLen[seq[a]] > e->seqLen[seq[b]];
     });
     const size_t step = e->plain ? 1 : (size_t)kChunkPairs;
-    int64_t maxRecords = 1, maxW = 1, maxE = 1, maxB = 1;
+    int64_t maxRecords = 1, maxW = 1, maxE = 1, maxB = 1, maxNull = 64;
     const size_t ncfg = e->cfgG.size();
     for (size_t b = 0; b < (size_t)npairs; b += step) {
         const size_t en = std::min((size_t)npairs, b + step);
@@ -417,6 +438,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             D.launches.push_back(L);
         }
         maxRecords = std::max(maxRecords, C.records);
+        maxNull = std::max(maxNull, C.nullScratch);
         P.maxChunk = std::max<int64_t>(P.maxChunk, D.npairs);
         P.chunks.push_back(std::move(D));
     }
@@ -425,8 +447,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
     HIPCHK(hipHostMalloc(&P.hostOut, outBytes, hipHostMallocDefault));
     HIPCHK(P.brec.ensure(sizeof(BRec) * maxRecords));
     HIPCHK(P.frec.ensure(sizeof(FRec) * maxRecords));
-    HIPCHK(P.pre.ensure(sizeof(ES) * maxRecords));
-    HIPCHK(P.suf.ensure(sizeof(ES) * maxRecords));
+    HIPCHK(P.nullScratch.ensure(sizeof(ES) * maxNull));
     HIPCHK(P.counter.ensure(sizeof(int) * std::max<size_t>(64, ncfg)));
     if (mode == kScore) {
         HIPCHK(P.wbuf.ensure(sizeof(float) * maxW));
@@ -462,8 +483,7 @@ static int enqueueChunk(dm_engine *e, dm_plan &P, DevChunk &D, std::vector<std::
     na.pairs = D.pairs.as<PairInfo>();
     na.brec = P.brec.as<BRec>();
     na.frec = P.frec.as<FRec>();
-    na.scratchPre = P.pre.as<ES>();
-    na.scratchSuf = P.suf.as<ES>();
+    na.scratch = P.nullScratch.as<ES>();
     na.npairs = D.npairs;
     na.bgfs1 = e->bgfs1();
     na.bgfs2 = e->bgfs2();
@@ -688,9 +708,12 @@ int dm_engine_create(int device, dm_engine **out) {
         e->cfgR.push_back(rs[i]);
         e->cfgStrips.push_back(0);
     }
-    e->cfgG.push_back(64);
-    e->cfgR.push_back(kStripR);
-    e->cfgStrips.push_back(1);
+    const int ns = dp_strip_configs(gs, rs, 64);
+    for (int i = 0; i < ns && i < 64; i++) {
+        e->cfgG.push_back(gs[i]);
+        e->cfgR.push_back(rs[i]);
+        e->cfgStrips.push_back(1);
+    }
     *out = e;
     return DM_OK;
 }
@@ -701,7 +724,7 @@ void dm_engine_destroy(dm_engine *e) {
     e->dBnd.release();
     e->dBndE.release();
     for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dBrec, &e->dFrec,
-                      &e->dPre, &e->dSuf, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
+                      &e->dNull, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
                       &e->dOut})
         b->release();
     if (e->ev0) (void)hipEventDestroy(e->ev0);

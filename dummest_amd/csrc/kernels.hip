@@ -135,11 +135,19 @@ __device__ __forceinline__ float es_to_float_clamped(const ES &a) {  // dummest-
 // Null model, one thread per pair: dummest-dp-backward.hh:6-188 for one lane
 // (the reference's lane arithmetic is independent of the batch). Writes the
 // pair's record arrays and scales.
+//
+// Three passes over the sequence, each with its recurrence window in
+// registers: the suffix (descending; its values go to the interleaved scratch,
+// entry j at nullOff + 64*j, so a wave's accesses are coalesced), the prefix
+// (ascending; only the last three values are kept, for the totals), and a
+// final ascending pass that recomputes the prefix, forms both seeds and writes
+// the records. The operations and their order are those of the reference, so
+// the results are bit-identical to a single stored pass.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
     const int pi = blockIdx.x * blockDim.x + threadIdx.x;
     if (pi >= a.npairs) return;
-    PairInfo P = a.pairs[pi];
+    const PairInfo P = a.pairs[pi];
     const int n = P.n;
     BRec *br = a.brec + P.recOff;
     FRec *fr = a.frec + P.recOff;
@@ -156,55 +164,65 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
     }
     const uint8_t *cd = a.codes + P.seqOff;
     const float *bgp = a.bg + (int64_t)P.prof * 32;
+    ES *suf = a.scratch + P.nullOff;  // suf(j) = suf[64 * j]
     auto code = [&](int j) -> int { return (j >= 0 && j < n) ? (int)cd[j] : kZeroIdx; };
-    ES *pre = a.scratchPre + P.recOff;  // indices 0 .. n+3
-    ES *suf = a.scratchSuf + P.recOff;
     const double single = (double)a.bgfs1, dbl = (double)a.bgfs2;
     const double codonKeep = 1 - single - dbl;
     const double singleEmit = single * 0.25;
     const double doubleEmit = dbl * 0.0625;
     const double frame = 1.0 / 3.0;
-    const int W = n;
-    for (int j = 0; j < n + 4; j++) { pre[j] = es_zero(); suf[j] = es_zero(); }
-    // buildNullSuffix, dummest-dp-backward.hh:6-55
+
+    // buildNullSuffix, dummest-dp-backward.hh:6-55 (window s1,s2,s3 = suf(i+1..i+3))
     {
         float tm;
         int te;
         split_double(0.5 * codonKeep, tm, te);
-        suf[n] = es_pow2(0);
-        for (int i = W - 1; i >= 0; i--) {
+        ES s1 = es_pow2(0), s2 = es_zero(), s3 = es_zero();
+        suf[(int64_t)64 * n] = s1;
+        for (int i = n - 1; i >= 0; i--) {
             const float bgc = bgp[code(i)];
             const double codonScale = (double)bgc * codonKeep;
-            ES singleIn = suf[i + 1];
-            ES doubleIn = (i + 1 < n) ? suf[i + 2] : es_pow2(2);
-            ES codonRaw = es_bscale(suf[i + 3], codonScale);
-            ES sb = es_bscale(singleIn, singleEmit);
-            ES db = es_bscale(doubleIn, doubleEmit);
-            ES cb = (i + 2 < n) ? codonRaw : ES{tm, te - 2 * (n - i) + 1};
-            suf[i] = es_badd(cb, es_badd(sb, db));
+            const ES doubleIn = (i + 1 < n) ? s2 : es_pow2(2);
+            const ES codonRaw = es_bscale(s3, codonScale);
+            const ES sb = es_bscale(s1, singleEmit);
+            const ES db = es_bscale(doubleIn, doubleEmit);
+            const ES cb = (i + 2 < n) ? codonRaw : ES{tm, te - 2 * (n - i) + 1};
+            const ES v = es_badd(cb, es_badd(sb, db));
+            suf[(int64_t)64 * i] = v;
+            s3 = s2;
+            s2 = s1;
+            s1 = v;
         }
     }
-    // buildNullPrefixFirstCodon, dummest-dp-backward.hh:58-100
-    for (int i = 0; i < W; i++) {
+    // buildNullPrefixFirstCodon, dummest-dp-backward.hh:58-100, as a function of
+    // the window (p1, p2, p3) = prefix(i-1..i-3)
+    auto prefixStep = [&](int i, const ES &p1, const ES &p2, const ES &p3) -> ES {
         const float bgc = bgp[code(i - 2)];
         const double codonScale = (double)bgc * codonKeep;
-        ES ps = (i > 0) ? pre[i - 1] : es_zero();
-        ES pd = (i >= 2) ? pre[i - 2] : es_zero();
+        const ES ps = (i > 0) ? p1 : es_zero();
+        const ES pd = (i >= 2) ? p2 : es_zero();
         ES cb;
-        if (i >= 3) cb = es_bscale(pre[i - 3], codonScale);
+        if (i >= 3) cb = es_bscale(p3, codonScale);
         else if (i == 2) cb = es_scale(es_normalize((double)bgc, 0), frame);
         else if (i == 1) cb = es_scale(es_pow2(-4), frame);
         else cb = es_scale(es_pow2(-2), frame);
-        pre[i] = es_badd(cb, es_badd(es_bscale(ps, singleEmit), es_bscale(pd, doubleEmit)));
+        return es_badd(cb, es_badd(es_bscale(ps, singleEmit), es_bscale(pd, doubleEmit)));
+    };
+    ES p1 = es_zero(), p2 = es_zero(), p3 = es_zero();
+    for (int i = 0; i < n; i++) {
+        const ES v = prefixStep(i, p1, p2, p3);
+        p3 = p2;
+        p2 = p1;
+        p1 = v;
     }
-    // buildNullTotalAndScales, dummest-dp-backward.hh:103-129
+    // buildNullTotalAndScales, dummest-dp-backward.hh:103-129 (pre[n-1..n-3] = p1..p3)
     ES perBase, perBaseInv, s1, s2, sC;
     {
         const double keepSingle = 1.0 - single;
         const double keepBoth = keepSingle - dbl;
-        ES ts = es_scale(pre[n - 2], keepSingle * 0.25);
-        ES td = es_scale(pre[n - 3], keepBoth * 0.0625);
-        ES total = es_add(es_add(pre[n - 1], ts), td);
+        const ES ts = es_scale(p2, keepSingle * 0.25);
+        const ES td = es_scale(p3, keepBoth * 0.0625);
+        const ES total = es_add(es_add(p1, ts), td);
         const float nullLog = (total.m == 0.0f) ? -INFINITY : (float)(log2((double)total.m) + (double)total.e);
         double perBaseLog = -(double)nullLog / (double)(float)n;
         if (!(perBaseLog < 120.0)) perBaseLog = 120.0;
@@ -215,29 +233,36 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
         s2 = es_mul(es_mul(perBase, perBase), es_normalize(0.0625, 0));
         sC = es_mul(es_mul(perBase, perBase), perBase);
     }
-    // buildNullSeedModels, dummest-dp-backward.hh:133-165, and the records.
+    // buildNullSeedModels, dummest-dp-backward.hh:133-165, and the records:
+    // column j holds g[j+1] / g[j-2], the seeds and the codon codes.
     {
-        ES running = es_pow2(0);
-        for (int t = 0; t < n - 1; t++) running = es_bmul(running, perBase);
-        for (int j = 0; j < W; j++) {
-            const float ssd = es_to_float_clamped(es_bmul(suf[j + 1], running));
-            running = es_bmul(running, perBaseInv);
-            br[j].ss = ssd;  // suffix seed (prefix seed filled below)
+        ES runS = es_pow2(0);
+        for (int t = 0; t < n - 1; t++) runS = es_bmul(runS, perBase);
+        ES runP = perBase;
+        for (int j = -kRPad; j < 0; j++) {
+            const int cb = code(j + 1), cf = code(j - 2);
+            br[j] = BRec{bgp[cb], 0.f, 0.f, cb};
+            fr[j] = FRec{bgp[cf], 0.f, cf, 0};
         }
-        running = perBase;
-        for (int j = 0; j < W; j++) {
-            const float psd = es_to_float_clamped(es_bmul(pre[j], running));
-            running = es_bmul(running, perBase);
-            br[j].ps = psd;
+        ES q1 = es_zero(), q2 = es_zero(), q3 = es_zero();
+        for (int j = 0; j < n; j++) {
+            const ES pre = prefixStep(j, q1, q2, q3);
+            q3 = q2;
+            q2 = q1;
+            q1 = pre;
+            const float ssd = es_to_float_clamped(es_bmul(suf[(int64_t)64 * (j + 1)], runS));
+            runS = es_bmul(runS, perBaseInv);
+            const float psd = es_to_float_clamped(es_bmul(pre, runP));
+            runP = es_bmul(runP, perBase);
+            const int cb = code(j + 1), cf = code(j - 2);
+            br[j] = BRec{bgp[cb], ssd, psd, cb};
+            fr[j] = FRec{bgp[cf], psd, cf, 0};
         }
-    }
-    for (int j = -kRPad; j < n + kRPad; j++) {
-        const bool in = (j >= 0 && j < n);
-        const float ss = in ? br[j].ss : 0.f;
-        const float ps = in ? br[j].ps : 0.f;
-        const int cb = code(j + 1), cf = code(j - 2);
-        br[j] = BRec{bgp[cb], ss, ps, cb};
-        fr[j] = FRec{bgp[cf], ps, cf, 0};
+        for (int j = n; j < n + kRPad; j++) {
+            const int cb = code(j + 1), cf = code(j - 2);
+            br[j] = BRec{bgp[cb], 0.f, 0.f, cb};
+            fr[j] = FRec{bgp[cf], 0.f, cf, 0};
+        }
     }
     a.pairs[pi].s1 = es_to_float_clamped(s1);
     a.pairs[pi].s2 = es_to_float_clamped(s2);
@@ -291,6 +316,31 @@ struct ExpMax {
         const int d = se - newOff;
         cur = d > 127 ? FLT_MAX : (d < -150 ? 0.f : ldexpf(sm, d));
         if (!(cur <= FLT_MAX)) cur = FLT_MAX;
+    }
+};
+
+// The same for products of two floats formed in double (exact, and without
+// the float range limits): cur holds the max at offset off.
+struct ExpMaxD {
+    double cur;
+    int off;
+    float sm;
+    int se;
+    __device__ __forceinline__ void init(int o) { cur = 0.0; off = o; sm = 0.f; se = 0; }
+    __device__ __forceinline__ void retire() {
+        if (cur > 0.0) {
+            int ex;
+            const double m = frexp(cur, &ex);
+            ES c{(float)m, ex + off};
+            if (es_less(ES{sm, se}, c)) { sm = c.m; se = c.e; }
+        }
+    }
+    __device__ __forceinline__ void rebase(int newOff) {
+        retire();
+        off = newOff;
+        if (sm == 0.f) { cur = 0.0; return; }
+        const int d = se - newOff;
+        cur = d > 1000 ? DBL_MAX : (d < -1000 ? 0.0 : ldexp((double)sm, d));
     }
 };
 
@@ -391,8 +441,8 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
 
         float enterC[R], d0[R], d1s2[R], d2s1[R], a1s1[R], a2s2[R], eps[R];
         float fbest = 0.f;  // filter mode: max W1 * prefixSeed
-        ExpMax bwdMax;
-        bwdMax.init(kPre);
+        ExpMaxD bwdMax;  // max of W1 * prefixSeed, at offset Eb
+        bwdMax.init(0);
         const float *etk = etab + k * R;
 
         // Per-strip setup: LDS emission rows and per-row coefficients
@@ -472,7 +522,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 for (int q = 0; q < R; q++) Yc[q] *= f;
                 Eb -= s;
                 rs = ldexpf(1.f, -Eb);
-                if (a.wantBwd) bwdMax.rebase(Eb + kPre);
+                if (a.wantBwd) bwdMax.rebase(Eb);
             };
 
             // Column records are loaded kPF steps ahead of their use.
@@ -528,6 +578,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 }
                 const float ssr = rc.ss * rs;
                 const float psp = rc.ps;
+                const double pspD = (double)rc.ps;
                 const float *et = etk + rc.code * ROWS;
                 float smax = 0.f;
 #pragma unroll
@@ -544,12 +595,8 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     w = fmaf(bY, d0[q], w);
                     const float y = fmaf(eps[q], bY, w);
                     const float z = fmaf(b0sC[q], zg, w);
-                    if (MODE == kFilter) {
-                        fbest = fmaxf(fbest, w * psp);
-                    } else {
-                        smax = fmaxf(smax, fmaxf(y, z));  // y, z >= w
-                        if (a.wantBwd) bwdMax.cur = fmaxf(bwdMax.cur, w * (psp * 7.888609052e-31f));
-                    }
+                    if (MODE == kFilter) fbest = fmaxf(fbest, w * psp);
+                    else smax = fmaxf(smax, fmaxf(y, z));  // y, z >= w
                     // row q at columns c+1, c+2, c+3 for row q-1 (same lane, same column)
                     bw1 = H[S1][q];
                     bw2 = H[S2][q];
@@ -558,6 +605,10 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     H[S4][q] = w;
                     Zr[S4][q] = z;
                     Yc[q] = y;
+                }
+                if (MODE != kFilter && a.wantBwd) {  // backward start score: max W1 * prefixSeed
+#pragma unroll
+                    for (int q = 0; q < R; q++) bwdMax.cur = fmax(bwdMax.cur, (double)H[S4][q] * pspD);
                 }
                 if (STR && above && k == 0 && c >= 0 && c < n) {
                     wOut[c + 1] = H[S4][0];
@@ -577,8 +628,9 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 if (MODE != kFilter) {
                     // One exponent per pair per step: lower the scale by 2^-64 when
                     // a value passes 2^100; every 4 steps, raise it by 2^64 when the
-                    // whole live state (the last 4 steps) of the pair is below 2^36,
-                    // so decaying regions keep their precision.
+                    // whole live state (the last 4 steps) of the pair is below 2^36
+                    // and the exponent is at least 64, so regions that decay after
+                    // a strong hit keep their precision.
                     LV[S4] = fmaxf(smax, fmaxf(gw, gy));
                     const bool big = smax > kBig;
                     if (__builtin_expect(__any(big), 0)) {
@@ -587,9 +639,9 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                         for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
                         if (gbig) scaleB(-64);
                     }
-                    if (PH == 3) {
+                    if (PH == 3) {  // never below exponent 0: the seeds enter at 2^-Eb <= 1
                         const float live = fmaxf(fmaxf(LV[0], LV[1]), fmaxf(LV[2], LV[3]));
-                        if (__builtin_expect(group_small<G>(live, grp), 0)) scaleB(64);
+                        if (__builtin_expect(group_small<G>(live, grp), 0) && Eb >= 64) scaleB(64);
                     }
                 }
             };
@@ -604,7 +656,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             if (t < T) bstep(std::integral_constant<int, 1>{}, t++);
             if (t < T) bstep(std::integral_constant<int, 2>{}, t++);
             if (STR) __syncthreads();  // boundary written above is read by the next strip
-            if (a.wantBwd && MODE != kFilter) bwdMax.rebase(kPre);  // next strip starts at exponent 0
+            if (a.wantBwd && MODE != kFilter) bwdMax.rebase(0);  // next strip starts at exponent 0
         }
         if (MODE == kFilter) {
             // group max of the plain float best (no rescale in the filter)
@@ -769,9 +821,9 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
                     if (gbig) scaleF(-64);
                 }
-                if (PH == 3) {
+                if (PH == 3) {  // never below exponent 0: the seeds enter at 2^-Ef <= 1
                     const float live = fmaxf(fmaxf(LVf[0], LVf[1]), fmaxf(LVf[2], LVf[3]));
-                    if (__builtin_expect(group_small<G>(live, grp), 0)) scaleF(64);
+                    if (__builtin_expect(group_small<G>(live, grp), 0) && Ef >= 64) scaleF(64);
                 }
             };
             int t = 0;
@@ -806,18 +858,26 @@ namespace dm {
     X(32, 2) X(32, 3) X(32, 4) X(32, 5) X(32, 6) X(32, 7) X(32, 8)                               \
     X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 6) X(64, 8) X(64, 12) X(64, 16)
 
-int dp_configs(int *gs, int *rs, int cap) {
-    int n = 0;
+// Row-strip kernels: one pair per wave, any profile length, rows in strips of 64*R.
+#define DM_STRIP_CFGS(X) X(64, 4) X(64, 6) X(64, 8) X(64, 16)
+
 #define DM_LIST(g, r)        \
     if (n < cap) {           \
         gs[n] = g;           \
         rs[n] = r;           \
     }                        \
     n++;
+int dp_configs(int *gs, int *rs, int cap) {
+    int n = 0;
     DM_CFGS(DM_LIST)
-#undef DM_LIST
     return n;
 }
+int dp_strip_configs(int *gs, int *rs, int cap) {
+    int n = 0;
+    DM_STRIP_CFGS(DM_LIST)
+    return n;
+}
+#undef DM_LIST
 
 hipError_t launch_null(const NullArgs &a, hipStream_t s) {
     const int blocks = (a.npairs + 63) / 64;
@@ -851,7 +911,7 @@ hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int g
         return launch_one(dp_kernel<g, r, kDump, true>, smem, grid, s, a);                          \
     }
     DM_CFGS(DM_LAUNCH)
-    DM_LAUNCH_STRIPS(64, 16)
+    DM_STRIP_CFGS(DM_LAUNCH_STRIPS)
 #undef DM_LAUNCH
 #undef DM_LAUNCH_STRIPS
     return hipErrorInvalidValue;

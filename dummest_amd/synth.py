@@ -211,7 +211,7 @@ def mutate_np(rng, dna: np.ndarray, indel=0.01, sub=0.0) -> np.ndarray:
     return res
 
 
-def workload_c4(seed=1004, n_profiles=5000, n_reads=100000, pair_rate=0.004, max_len=1023):
+def workload_c4(seed=1004, n_profiles=5000, n_reads=100000, pair_rate=0.004, max_len=2000):
     """Uneven reads against many profiles (the fwd+bwd-at-scale configuration).
 
     profiles: lengths lognormal (median 150, sigma 0.8, clipped to [30, 2000]
@@ -249,7 +249,7 @@ def workload_c4(seed=1004, n_profiles=5000, n_reads=100000, pair_rate=0.004, max
     return dict(Ms=Ms, raws=raws, reads=reads, prof=prof, seq=seq, homolog=homolog, src=src)
 
 
-def workload_c1(seed=1001, n_profiles=1000, n_windows=10000, homolog_frac=0.2, max_len=1023):
+def workload_c1(seed=1001, n_profiles=1000, n_windows=10000, homolog_frac=0.2, max_len=2000):
     """Profiles x windows, all pairs (the forward-only score configuration).
     windows: uniform 1-10 kb, GC uniform 0.35-0.65, 20% carrying an embedded
     frameshifted homolog of a random profile's ancestor."""
