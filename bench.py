@@ -61,6 +61,21 @@ def cpu_baseline(w, pairs_idx, budget_s=15.0, threads=None):
                        f"oracle fwd+bwd scores (FP32, exact ExpScore anchors), {threads} threads")
 
 
+def traffic_of(kernel, cells_per_launch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/dp_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch,
+    scaled by the plain-copy reference), or None when that summary was taken
+    on another kernel or launch size."""
+    path = os.path.join(ROOT, "profiles", "dp_traffic.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or abs(t.get("cells_per_launch", 0) - cells_per_launch) > 0.01 * cells_per_launch:
+        return None
+    return t.get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +86,10 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=400)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune-file", default="",
+                    help="load the autotuned cost table from this file if it exists, else tune and save it there")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only time a plain device copy of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the DP pairs are independent and a multi-GPU run would shard the "
@@ -80,6 +99,13 @@ def main():
     import dummest_amd as D
     import oracle as O
     from dummest_amd import synth
+
+    if args.copy_probe:
+        e = D.Engine(0)
+        ms = e.copy_probe(args.copy_probe)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "ms": ms,
+                          "GB_per_s": 2 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
+        return
 
     t0 = time.perf_counter()
     w = synth.workload_c4(n_profiles=args.profiles, n_reads=args.reads)
@@ -97,9 +123,17 @@ def main():
     cells = int(((M + 1) * n).sum())
 
     t = time.perf_counter()
-    table = e.autotune(300)
-    log(f"autotune: {time.perf_counter() - t:.2f} s; ns per wave-step: " +
-        " ".join(f"G{G}R{R}{'s' if st else ''}={ns:.0f}" for G, R, st, ns in table))
+    if args.tune_file and os.path.exists(args.tune_file):
+        e.set_config_costs(json.load(open(args.tune_file))["ns_per_wave_step"])
+        table = e.config_table()
+        log(f"cost table loaded from {args.tune_file}")
+    else:
+        table = e.autotune(300)
+        log(f"autotune: {time.perf_counter() - t:.2f} s")
+        if args.tune_file:
+            json.dump({"configs": [[G, R, st] for G, R, st, _ in table],
+                       "ns_per_wave_step": [ns for *_, ns in table]}, open(args.tune_file, "w"))
+    log("ns per wave-step: " + " ".join(f"G{G}R{R}{'s' if st else ''}={ns:.2f}" for G, R, st, ns in table))
     t = time.perf_counter()
     plan = e.plan(prof, seq, mode="score", want_bwd=False)
     log(f"plan: {time.perf_counter() - t:.1f} s, {cells:.4g} cells per step")
@@ -119,38 +153,41 @@ def main():
     # dominant kernel: the configuration with the most device time
     agg = {}
     for st in kstats:
-        for G, R, ms, c, launches in st:
-            a = agg.setdefault((G, R), [0.0, 0, 0])
+        for G, R, strips, ms, c, launches in st:
+            a = agg.setdefault((G, R, strips), [0.0, 0, 0])
             a[0] += ms
             a[1] += c
             a[2] += launches
-    for (g, r), (ms_, c_, l_) in sorted(agg.items(), key=lambda kv: -kv[1][0]):
-        log(f"  G={g:2d} R={r:2d}: {ms_ / args.steps:8.2f} ms/step {c_ / args.steps:.3g} cells/step "
-            f"{c_ / (ms_ * 1e-3):.3g} cells/s")
-    (G, R), (ms, c, launches) = max(agg.items(), key=lambda kv: kv[1][0])
+    for (g, r, st), (ms_, c_, l_) in sorted(agg.items(), key=lambda kv: -kv[1][0]):
+        log(f"  G={g:2d} R={r:2d}{' strips' if st else '       '}: {ms_ / args.steps:8.2f} ms/step "
+            f"{c_ / args.steps:.3g} cells/step {c_ / (ms_ * 1e-3):.3g} cells/s")
+    (G, R, ST), (ms, c, launches) = max(agg.items(), key=lambda kv: kv[1][0])
     avg_ms = ms / launches
     flops_per_launch = FLOPS_PER_CELL * c / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     dp_ms_step = sum(v[0] for v in agg.values()) / args.steps
+    kname = f"void dm::dp_kernel<{G}, {R}, 1, {'true' if ST else 'false'}>(dm::DPArgs)"
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": f"dp_kernel<G={G},R={R},score>", "avg_launch_ms": round(avg_ms, 3),
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_of(kname, c / launches),
+                "kernel": kname, "avg_launch_ms": round(avg_ms, 3),
                 "cells_per_launch": int(c / launches), "flops_per_cell": FLOPS_PER_CELL}
 
     # plain per-pair form on a sample
     rng = np.random.default_rng(7)
-    sample = rng.choice(len(prof), size=min(args.plain_pairs, len(prof)), replace=False)
-    e.set_option("plain", 1)
-    pplan = e.plan(prof[sample], seq[sample], mode="score", want_bwd=False)
-    pplan.execute()
-    t = time.perf_counter()
-    pm, pe, _, _ = pplan.execute()
-    plain_s = time.perf_counter() - t
-    e.set_option("plain", 0)
-    pcells = int(((M[sample] + 1) * n[sample]).sum())
-    plain = pcells / plain_s
-    same = float(np.max(np.abs(D.es_log(pm, pe) - D.es_log(mm[sample], me[sample]))))
-    log(f"plain: {plain:.4g} cells/s on {len(sample)} pairs (max |d score| vs batched {same:.2e} nats)")
+    plain, pplan, sample = None, None, []
+    if args.plain_pairs > 0:
+        sample = rng.choice(len(prof), size=min(args.plain_pairs, len(prof)), replace=False)
+        e.set_option("plain", 1)
+        pplan = e.plan(prof[sample], seq[sample], mode="score", want_bwd=False)
+        pplan.execute()
+        t = time.perf_counter()
+        pm, pe, _, _ = pplan.execute()
+        plain_s = time.perf_counter() - t
+        e.set_option("plain", 0)
+        pcells = int(((M[sample] + 1) * n[sample]).sum())
+        plain = pcells / plain_s
+        same = float(np.max(np.abs(D.es_log(pm, pe) - D.es_log(mm[sample], me[sample]))))
+        log(f"plain: {plain:.4g} cells/s on {len(sample)} pairs (max |d score| vs batched {same:.2e} nats)")
 
     cpu = None
     if not args.no_cpu:
@@ -181,13 +218,15 @@ def main():
                    "pairs": int(len(prof)), "cells_per_step": cells, "dp_device_ms_per_step": round(dp_ms_step, 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "plain_kernel": {"value": float(f"{plain:.4g}"), "unit": "cells/s", "pairs": int(len(sample))},
-        "batched_vs_plain": round(value / plain, 2),
+        "plain_kernel": {"value": float(f"{plain:.4g}"), "unit": "cells/s", "pairs": int(len(sample))}
+        if plain else None,
+        "batched_vs_plain": round(value / plain, 2) if plain else None,
         "batched_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
     print(json.dumps(out), flush=True)
     plan.close()
-    pplan.close()
+    if pplan:
+        pplan.close()
     e.close()
 
 

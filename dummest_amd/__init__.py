@@ -80,12 +80,14 @@ def _declare(L):
     L.dm_plan_capture.argtypes = [vp, vp]
     L.dm_plan_destroy.argtypes = [vp, vp]
     L.dm_plan_destroy.restype = None
-    L.dm_kernel_stats.argtypes = [vp, i32, _P(i32), _P(i32), _P(d), _P(i64), _P(i64)]
+    L.dm_kernel_stats.argtypes = [vp, i32, _P(i32), _P(i32), _P(i32), _P(d), _P(i64), _P(i64)]
+    L.dm_copy_probe.argtypes = [vp, i64, _P(d)]
     L.dm_post_process.argtypes = [i32, _P(f), _P(f), _P(f), i32, _P(u8), _P(f), _P(i32), _P(f), _P(i32), _P(f),
                                   _P(f), _P(d), d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
     L.dm_set_option.argtypes = [vp, ctypes.c_char_p, i64]
     L.dm_calibrate.argtypes = [vp, i32, _P(i32), i32, i32, i32, _P(d), _P(d), _P(d)]
     L.dm_autotune.argtypes = [vp, i32, _P(d), i32]
+    L.dm_set_config_costs.argtypes = [vp, i32, _P(d)]
     L.dm_config_table.argtypes = [vp, i32, _P(i32), _P(i32), _P(i32), _P(d)]
     L.dm_calib_sequence.argtypes = [_P(f), _P(d), i32, i32, i32, ctypes.c_char_p]
     L.dm_gumbel_fits.argtypes = [_P(d), i32, d, _P(d)]
@@ -396,6 +398,11 @@ class Engine:
         self._check(n if n < 0 else 0, "dm_autotune")
         return self.config_table()
 
+    def set_config_costs(self, ns_per_step):
+        """Load a cost table saved from autotune() (same device and build)."""
+        ns = np.ascontiguousarray(ns_per_step, np.float64)
+        self._check(lib().dm_set_config_costs(self._h, len(ns), _ptr(ns, ctypes.c_double)), "dm_set_config_costs")
+
     def config_table(self):
         """[(G, R, strips, ns_per_wave_step)] in configuration order."""
         cap = 64
@@ -406,17 +413,25 @@ class Engine:
         return [(int(G[k]), int(R[k]), bool(S[k]), float(ns[k])) for k in range(min(n, cap))]
 
     def kernel_stats(self):
-        """[(G, R, ms, cells, launches)] per kernel configuration of the last execution."""
+        """[(G, R, strips, ms, cells, launches)] per kernel configuration of the last execution."""
         cap = 64
         G = np.zeros(cap, np.int32)
         R = np.zeros(cap, np.int32)
+        S = np.zeros(cap, np.int32)
         ms = np.zeros(cap, np.float64)
         cells = np.zeros(cap, np.int64)
         lc = np.zeros(cap, np.int64)
         i32, i64 = ctypes.c_int32, ctypes.c_int64
-        n = lib().dm_kernel_stats(self._h, cap, _ptr(G, i32), _ptr(R, i32), _ptr(ms, ctypes.c_double),
+        n = lib().dm_kernel_stats(self._h, cap, _ptr(G, i32), _ptr(R, i32), _ptr(S, i32), _ptr(ms, ctypes.c_double),
                                   _ptr(cells, i64), _ptr(lc, i64))
-        return [(int(G[k]), int(R[k]), float(ms[k]), int(cells[k]), int(lc[k])) for k in range(min(n, cap))]
+        return [(int(G[k]), int(R[k]), bool(S[k]), float(ms[k]), int(cells[k]), int(lc[k]))
+                for k in range(min(n, cap))]
+
+    def copy_probe(self, nbytes: int) -> float:
+        """Device ms of one plain copy of nbytes (traffic-counter reference)."""
+        ms = ctypes.c_double()
+        self._check(lib().dm_copy_probe(self._h, int(nbytes), ctypes.byref(ms)), "dm_copy_probe")
+        return ms.value
 
     @property
     def last_dp_ms(self) -> float:

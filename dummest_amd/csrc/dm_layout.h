@@ -123,6 +123,7 @@ int dp_strip_configs(int *gs, int *rs, int cap);
 hipError_t launch_null(const NullArgs &a, hipStream_t s);
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+hipError_t launch_copy_probe(const float *src, float *dst, int64_t n16, hipStream_t s);
 #endif
 
 }  // namespace dm

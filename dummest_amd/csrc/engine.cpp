@@ -1073,8 +1073,8 @@ void dm_plan_destroy(dm_engine *e, dm_plan *P) {
     delete P;
 }
 
-int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, double *ms, int64_t *cells,
-                    int64_t *launches) {
+int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int32_t *strips, double *ms,
+                    int64_t *cells, int64_t *launches) {
     if (!e) return DM_EINVAL;
     int n = 0;
     for (size_t c = 0; c < e->cfgMs.size(); c++) {
@@ -1082,6 +1082,7 @@ int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, dou
         if (n < cap) {
             if (G) G[n] = e->cfgG[c];
             if (R) R[n] = e->cfgR[c];
+            if (strips) strips[n] = e->cfgStrips[c];
             if (ms) ms[n] = e->cfgMs[c];
             if (cells) cells[n] = e->cfgCells[c];
             if (launches) launches[n] = e->cfgLaunches[c];
@@ -1255,6 +1256,19 @@ int dm_autotune(dm_engine *e, int32_t seq_len, double *ns_per_step, int32_t cap)
     return (int)ncfg;
 }
 
+int dm_set_config_costs(dm_engine *e, int32_t n, const double *ns_per_step) {
+    if (!e || (n > 0 && !ns_per_step)) return DM_EINVAL;
+    if (n == 0) {
+        e->cfgStepNs.clear();
+        return DM_OK;
+    }
+    if (n != (int32_t)e->cfgG.size()) return fail(e, DM_EINVAL, "cost table size differs from the configuration table");
+    for (int32_t c = 0; c < n; c++)
+        if (!(ns_per_step[c] > 0)) return fail(e, DM_EINVAL, "costs must be positive");
+    e->cfgStepNs.assign(ns_per_step, ns_per_step + n);
+    return DM_OK;
+}
+
 int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int32_t *strips, double *ns_per_step) {
     if (!e) return DM_EINVAL;
     const size_t ncfg = e->cfgG.size();
@@ -1265,6 +1279,25 @@ int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int
         if (ns_per_step) ns_per_step[c] = c < e->cfgStepNs.size() ? e->cfgStepNs[c] : 0.0;
     }
     return (int)ncfg;
+}
+
+int dm_copy_probe(dm_engine *e, int64_t bytes, double *ms) {
+    if (!e || bytes < 16 || (bytes & 15) || !ms) return DM_EINVAL;
+    HIPCHK(hipSetDevice(e->device));
+    DevBuf src, dst;
+    HIPCHK(src.ensure((size_t)bytes));
+    HIPCHK(dst.ensure((size_t)bytes));
+    HIPCHK(launch_copy_probe(src.as<float>(), dst.as<float>(), bytes / 16, e->stream));  // warm-up
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    HIPCHK(launch_copy_probe(src.as<float>(), dst.as<float>(), bytes / 16, e->stream));
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e->ev0, e->ev1));
+    *ms = t;
+    src.release();
+    dst.release();
+    return DM_OK;
 }
 
 double dm_last_dp_ms(const dm_engine *e) { return e ? e->lastMs : 0.0; }

@@ -22,6 +22,8 @@
 //     [code][row]; per-column inputs (background codon probability, seeds,
 //     codon code) are 16-byte records read once per step per lane.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 #include <float.h>
 #include <limits.h>
@@ -533,6 +535,23 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             BRec rq[kPF];
 #pragma unroll
             for (int f = 0; f < kPF; f++) rq[f] = brecAt(f);
+            // strip kernels: the strip below's top row (W1, Y, exponent), also ahead
+            float gqw[kPF], gqy[kPF];
+            int gqe[kPF];
+            auto ghostAt = [&](int tt, float &w, float &y, int &ge) {
+                if (STR && below) {
+                    const int bi = bidx(Tmine - 1 - tt - k, n);
+                    w = wIn[bi];
+                    y = yIn[bi];
+                    ge = eIn[bi];
+                } else {
+                    w = 0.f;
+                    y = 0.f;
+                    ge = 0;
+                }
+            };
+#pragma unroll
+            for (int f = 0; f < kPF; f++) ghostAt(f, gqw[f], gqy[f], gqe[f]);
 
             auto bstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
@@ -545,9 +564,9 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     // last lane: the strip below's top row from the boundary buffer.
                     // When that row sits far above this strip's scale, raise the
                     // exponent first (before any neighbour value is read).
-                    const int bi = bidx(c, n);
-                    const float rw = wIn[bi], ry = yIn[bi];
-                    const int ge = eIn[bi];
+                    const float rw = gqw[PH % kPF], ry = gqy[PH % kPF];
+                    const int ge = gqe[PH % kPF];
+                    ghostAt(t + kPF, gqw[PH % kPF], gqy[PH % kPF], gqe[PH % kPF]);
                     if (MODE != kFilter) {
                         int need = 0;
                         if (k == G - 1 && (rw > 0.f || ry > 0.f)) {
@@ -736,6 +755,24 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 loadW1(f, Wq[f], Eq[f]);
             }
 
+            // strip kernels: the strip above's last row (carried sum, Y, exponent), also ahead
+            float fqn[kPF], fqy[kPF];
+            int fqe[kPF];
+            auto fghostAt = [&](int tt, float &nv, float &yv, int &ge) {
+                if (STR && above) {
+                    const int bi = bidx(tt - k, n);
+                    nv = fN[bi];
+                    yv = fY[bi];
+                    ge = fE[bi];
+                } else {
+                    nv = 0.f;
+                    yv = 0.f;
+                    ge = 0;
+                }
+            };
+#pragma unroll
+            for (int f = 0; f < kPF; f++) fghostAt(f, fqn[f], fqy[f], fqe[f]);
+
             auto fstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
                 constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
@@ -753,9 +790,9 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 if (STR && above) {
                     // lane 0: the strip above's last row from the boundary buffer;
                     // raise this strip's exponent first when that row is far above it
-                    const int bi = bidx(c, n);
-                    const float rn = fN[bi], ry = fY[bi];
-                    const int ge = fE[bi];
+                    const float rn = fqn[SL], ry = fqy[SL];
+                    const int ge = fqe[SL];
+                    fghostAt(t + kPF, fqn[SL], fqy[SL], fqe[SL]);
                     int need = 0;
                     if (k == 0 && (rn > 0.f || ry > 0.f)) {
                         int ex;
@@ -926,6 +963,20 @@ namespace dm {
 __global__ void zero_ints_kernel(int *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// A plain copy of n16 16-byte words: the reference point for the HBM traffic
+// counters (its algorithmic traffic is exactly 16 bytes read + 16 written per word).
+__global__ void __launch_bounds__(256) copy_probe_kernel(const float4 *src, float4 *dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const float *src, float *dst, int64_t n16, hipStream_t s) {
+    if (n16 <= 0) return hipSuccess;
+    const int blocks = (int)std::min<int64_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n16);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {

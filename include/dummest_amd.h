@@ -125,10 +125,17 @@ void dm_plan_destroy(dm_engine *e, dm_plan *plan);
 int dm_plan_capture(dm_engine *e, dm_plan *plan);
 
 /* Per kernel configuration of the last execution: G (lanes per pair), R (rows
- * per lane), device milliseconds (HIP events around each launch), DP cells,
- * launches. Returns the number of configurations used (fills up to cap). */
-int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, double *ms, int64_t *cells,
-                    int64_t *launches);
+ * per lane), row-strip kernel or not, device milliseconds (HIP events around
+ * each launch), DP cells, launches. Returns the number of configurations used
+ * (fills up to cap). */
+int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int32_t *strips, double *ms,
+                    int64_t *cells, int64_t *launches);
+
+/* Measurement aid with no reference counterpart: one timed launch of a plain
+ * device-to-device copy of `bytes` (a multiple of 16), the reference point
+ * that scales the HBM traffic counters (FETCH_SIZE / WRITE_SIZE) of the DP
+ * kernels. */
+int dm_copy_probe(dm_engine *e, int64_t bytes, double *ms);
 
 /* Post-gate processing on caller-supplied DP matrices (no device): the same
  * host code dm_pair_hits runs after its dump, fed with w0/W1 ExpScore matrices
@@ -170,6 +177,9 @@ int dm_gumbel_fits(const double *scores, int32_t n, double seq_len, double out[8
  * configurations. Replaces the reference's fixed lane packing
  * (simdWidth, DUMMEST_MAX_LANES; dummest-search.hh:85-87). */
 int dm_autotune(dm_engine *e, int32_t seq_len, double *ns_per_step, int32_t cap);
+/* Load a cost table saved from an earlier dm_autotune on the same device (n =
+ * the number of configurations; n = 0 returns to the untuned cost model). */
+int dm_set_config_costs(dm_engine *e, int32_t n, const double *ns_per_step);
 /* The configuration table (G, R, strip kernel or not, tuned ns per wave-step or 0). */
 int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int32_t *strips, double *ns_per_step);
 
