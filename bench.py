@@ -61,17 +61,18 @@ def cpu_baseline(w, pairs_idx, budget_s=15.0, threads=None):
                        f"oracle fwd+bwd scores (FP32, exact ExpScore anchors), {threads} threads")
 
 
-def traffic_of(kernel, cells_per_launch):
+def traffic_of(kernel, cells_per_launch, build):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (profiles/dp_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch,
     scaled by the plain-copy reference), or None when that summary was taken
-    on another kernel or launch size."""
+    on another build, kernel or launch size."""
     path = os.path.join(ROOT, "profiles", "dp_traffic.json")
     try:
         t = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel or abs(t.get("cells_per_launch", 0) - cells_per_launch) > 0.01 * cells_per_launch:
+    if t.get("build_id") != build or t.get("kernel") != kernel or \
+            abs(t.get("cells_per_launch", 0) - cells_per_launch) > 0.01 * cells_per_launch:
         return None
     return t.get("hbm_bytes_per_launch")
 
@@ -168,7 +169,7 @@ def main():
     dp_ms_step = sum(v[0] for v in agg.values()) / args.steps
     kname = f"void dm::dp_kernel<{G}, {R}, 1, {'true' if ST else 'false'}>(dm::DPArgs)"
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_of(kname, c / launches),
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_of(kname, c / launches, D.build_id()),
                 "kernel": kname, "avg_launch_ms": round(avg_ms, 3),
                 "cells_per_launch": int(c / launches), "flops_per_cell": FLOPS_PER_CELL}
 
@@ -222,6 +223,7 @@ def main():
         if plain else None,
         "batched_vs_plain": round(value / plain, 2) if plain else None,
         "batched_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
+        "build_id": D.build_id(),
     }
     print(json.dumps(out), flush=True)
     plan.close()

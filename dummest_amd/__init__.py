@@ -82,6 +82,8 @@ def _declare(L):
     L.dm_plan_destroy.restype = None
     L.dm_kernel_stats.argtypes = [vp, i32, _P(i32), _P(i32), _P(i32), _P(d), _P(i64), _P(i64)]
     L.dm_copy_probe.argtypes = [vp, i64, _P(d)]
+    L.dm_build_id.argtypes = []
+    L.dm_build_id.restype = ctypes.c_char_p
     L.dm_post_process.argtypes = [i32, _P(f), _P(f), _P(f), i32, _P(u8), _P(f), _P(i32), _P(f), _P(i32), _P(f),
                                   _P(f), _P(d), d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
     L.dm_set_option.argtypes = [vp, ctypes.c_char_p, i64]
@@ -189,6 +191,11 @@ def gumbel_fits(scores, seq_len) -> dict:
     if rc:
         raise DummestError(f"dm_gumbel_fits failed ({rc})")
     return dict(zip(GUMBEL_KEYS, out.tolist()))
+
+
+def build_id() -> str:
+    """Hash of the kernel sources the loaded library was built from."""
+    return lib().dm_build_id().decode()
 
 
 def es_log(m, e):

@@ -9,7 +9,7 @@
 //              bg     [p*32 + code]                     (background per codon code, bg_probs+4)
 //   sequences: decoded codes, uint8, concatenated (dummest-sequence.hh:181-204)
 //   pairs    : PairInfo per (profile, sequence) pair, in work-queue order
-//   records  : per pair, columns j in [-RPAD, n+RPAD): BRec (backward) and FRec (forward)
+//   records  : per pair, columns j in [-RPAD, n+RPAD): Rec (12 bytes)
 //   units    : work units = up to 64/G pairs of one profile with similar lengths
 #pragma once
 #include <stdint.h>
@@ -23,19 +23,14 @@ constexpr int kEmStride = 32;  // floats per emission row in HBM
 constexpr int kRPad = 4;       // record padding columns on each side of a pair's sequence
                                // (every column beyond them reads the zero record at the edge)
 
-// Backward record for column j: g[j+1], suffixSeed[j], prefixSeed[j], a[j+1].
-struct BRec {
-    float g;
+// Column record j of a pair (12 bytes): prefix seed, the codon codes at j+1
+// (low byte, used by the backward) and j-2 (second byte, used by the forward),
+// suffix seed. The background codon probabilities g[j+1] and g[j-2] come from
+// the profile's 25-entry table, which the DP kernel keeps in LDS.
+struct Rec {
+    float ps;
+    uint32_t codes;
     float ss;
-    float ps;
-    int code;
-};
-// Forward record for column j: g[j-2], prefixSeed[j], a[j-2].
-struct FRec {
-    float g;
-    float ps;
-    int code;
-    int pad;
 };
 
 struct ProfInfo {
@@ -85,8 +80,7 @@ struct NullArgs {
     const uint8_t *codes;
     const float *bg;   // [prof*32 + code]
     PairInfo *pairs;
-    BRec *brec;
-    FRec *frec;
+    Rec *rec;
     ES *scratch;       // suffix values, interleaved by the 64 pairs of a null-kernel wave
     int npairs;
     float bgfs1, bgfs2;  // BACKGROUND_FRAMESHIFT_RATE(_2), as Float sums
@@ -97,8 +91,8 @@ struct DPArgs {
     const float *params;
     const float *emis;
     const PairInfo *pairs;
-    const BRec *brec;
-    const FRec *frec;
+    const Rec *rec;
+    const float *bg;      // [prof*32 + code], background codon probabilities
     const Unit *units;
     int nUnits;
     int *unitCounter;

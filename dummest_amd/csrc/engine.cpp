@@ -46,7 +46,8 @@ struct DevBuf {
     template <class T> T *as() const { return (T *)p; }
 };
 
-constexpr int kChunkPairs = 1 << 18;          // pairs per queue chunk
+constexpr int kChunkPairs = 1 << 22;          // pairs per queue chunk (at most)
+constexpr int64_t kChunkBytes = 24LL << 30;   // record + null-scratch bytes per chunk (at most)
 constexpr int64_t kScratchBudget = 8LL << 30;  // bytes of W1 scratch per launch
 
 }  // namespace
@@ -64,7 +65,7 @@ struct dm_engine {
     std::vector<int32_t> seqLen;
     bool seqDirty = true;
     DevBuf dCodes;
-    DevBuf dPairs, dBrec, dFrec, dNull, dUnits, dCounter, dWbuf, dW0buf, dEbuf, dFbuf, dOut;
+    DevBuf dPairs, dRec, dNull, dUnits, dCounter, dWbuf, dW0buf, dEbuf, dFbuf, dOut;
     std::vector<int> cfgG, cfgR;
     std::vector<char> cfgStrips;  // row-strip configurations (G = 64, any profile length)
     std::vector<double> cfgStepNs;  // autotuned device ns per wave-step (0: not measured)
@@ -277,8 +278,7 @@ static int uploadChunk(dm_engine *e, Chunk &C) {
     const size_t np = C.pairs.size();
     HIPCHK(e->dPairs.ensure(sizeof(PairInfo) * np));
     HIPCHK(e->dUnits.ensure(sizeof(Unit) * C.units.size()));
-    HIPCHK(e->dBrec.ensure(sizeof(BRec) * C.records));
-    HIPCHK(e->dFrec.ensure(sizeof(FRec) * C.records));
+    HIPCHK(e->dRec.ensure(sizeof(Rec) * C.records));
     HIPCHK(e->dNull.ensure(sizeof(ES) * C.nullScratch));
     HIPCHK(e->dCounter.ensure(sizeof(int) * std::max<size_t>(64, e->cfgG.size())));
     HIPCHK(hipMemcpyAsync(e->dPairs.p, C.pairs.data(), sizeof(PairInfo) * np, hipMemcpyHostToDevice, e->stream));
@@ -288,8 +288,7 @@ static int uploadChunk(dm_engine *e, Chunk &C) {
     na.codes = e->dCodes.as<uint8_t>();
     na.bg = e->dBg.as<float>();
     na.pairs = e->dPairs.as<PairInfo>();
-    na.brec = e->dBrec.as<BRec>();
-    na.frec = e->dFrec.as<FRec>();
+    na.rec = e->dRec.as<Rec>();
     na.scratch = e->dNull.as<ES>();
     na.npairs = (int)np;
     na.bgfs1 = e->bgfs1();
@@ -304,8 +303,8 @@ static DPArgs baseArgs(dm_engine *e) {
     a.params = e->dParams.as<float>();
     a.emis = e->dEmis.as<float>();
     a.pairs = e->dPairs.as<PairInfo>();
-    a.brec = e->dBrec.as<BRec>();
-    a.frec = e->dFrec.as<FRec>();
+    a.rec = e->dRec.as<Rec>();
+    a.bg = e->dBg.as<float>();
     return a;
 }
 
@@ -322,6 +321,7 @@ struct Launch {
 struct DevChunk {
     DevBuf pairs, units;
     int npairs = 0;
+    int64_t outOff = 0;  // first result slot of this chunk in the plan's host staging
     int64_t records = 0;
     std::vector<int64_t> order;
     std::vector<Launch> launches;
@@ -337,7 +337,7 @@ struct dm_plan {
     int64_t cells = 0;
     int64_t maxChunk = 0;
     std::vector<DevChunk> chunks;
-    DevBuf out, brec, frec, nullScratch, counter, wbuf, ebuf, bnd, bndE;
+    DevBuf out, rec, nullScratch, counter, wbuf, ebuf, bnd, bndE;
     void *hostOut = nullptr;  // pinned staging for results
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -346,7 +346,7 @@ struct dm_plan {
             c.pairs.release();
             c.units.release();
         }
-        for (DevBuf *b : {&out, &brec, &frec, &nullScratch, &counter, &wbuf, &ebuf, &bnd, &bndE}) b->release();
+        for (DevBuf *b : {&out, &rec, &nullScratch, &counter, &wbuf, &ebuf, &bnd, &bndE}) b->release();
         if (graphExec) (void)hipGraphExecDestroy(graphExec);
         if (graph) (void)hipGraphDestroy(graph);
         if (hostOut) (void)hipHostFree(hostOut);
@@ -385,16 +385,25 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         if (prof[a] != prof[b]) return prof[a] < prof[b];
         return e->seqLen[seq[a]] > e->seqLen[seq[b]];
     });
-    const size_t step = e->plain ? 1 : (size_t)kChunkPairs;
     int64_t maxRecords = 1, maxW = 1, maxE = 1, maxB = 1, maxNull = 64;
     const size_t ncfg = e->cfgG.size();
-    for (size_t b = 0; b < (size_t)npairs; b += step) {
-        const size_t en = std::min((size_t)npairs, b + step);
+    for (size_t b = 0, en = 0; b < (size_t)npairs; b = en) {
+        // chunk: up to kChunkPairs pairs whose records and null scratch fit kChunkBytes
+        en = b + 1;
+        if (!e->plain) {
+            int64_t bytes = 0;
+            for (en = b; en < (size_t)npairs && en - b < (size_t)kChunkPairs; en++) {
+                const int64_t nn = e->seqLen[seq[sorted[en]]];
+                bytes += (nn + 2 * kRPad) * (int64_t)sizeof(Rec) + (nn + 4) * (int64_t)sizeof(ES);
+                if (bytes > kChunkBytes && en > b) break;
+            }
+        }
         Chunk C;
         buildChunk(e, sorted, cfgOf, prof, seq, b, en, C);
         DevChunk D;
         D.npairs = (int)C.pairs.size();
         D.records = C.records;
+        D.outOff = (int64_t)b;
         std::vector<int64_t> cfgCells(ncfg, 0);
         for (size_t u = 0; u < C.units.size(); u++)
             for (int k = 0; k < C.units[u].count; k++) {
@@ -432,7 +441,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             }
             if (L.strips) {
                 a.bndCols = C.cfgMaxN[cfg] + 2;
-                maxB = std::max(maxB, a.bndCols * grid);
+                maxB = std::max(maxB, a.bndCols * grid * Pp);
             }
             L.grid = grid;
             D.launches.push_back(L);
@@ -442,11 +451,10 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         P.maxChunk = std::max<int64_t>(P.maxChunk, D.npairs);
         P.chunks.push_back(std::move(D));
     }
-    const size_t outBytes = (size_t)std::max<int64_t>(1, P.maxChunk) * sizeof(PairScore);
+    const size_t outBytes = (size_t)std::max<int64_t>(1, npairs) * sizeof(PairScore);
     HIPCHK(P.out.ensure(outBytes));
     HIPCHK(hipHostMalloc(&P.hostOut, outBytes, hipHostMallocDefault));
-    HIPCHK(P.brec.ensure(sizeof(BRec) * maxRecords));
-    HIPCHK(P.frec.ensure(sizeof(FRec) * maxRecords));
+    HIPCHK(P.rec.ensure(sizeof(Rec) * maxRecords));
     HIPCHK(P.nullScratch.ensure(sizeof(ES) * maxNull));
     HIPCHK(P.counter.ensure(sizeof(int) * std::max<size_t>(64, ncfg)));
     if (mode == kScore) {
@@ -462,8 +470,8 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             a.params = e->dParams.as<float>();
             a.emis = e->dEmis.as<float>();
             a.pairs = D.pairs.as<PairInfo>();
-            a.brec = P.brec.as<BRec>();
-            a.frec = P.frec.as<FRec>();
+            a.rec = P.rec.as<Rec>();
+            a.bg = e->dBg.as<float>();
             a.unitCounter = P.counter.as<int>() + L.cfg;
             a.wbuf = P.wbuf.as<float>();
             a.ebuf = P.ebuf.as<int>();
@@ -481,8 +489,7 @@ static int enqueueChunk(dm_engine *e, dm_plan &P, DevChunk &D, std::vector<std::
     na.codes = e->dCodes.as<uint8_t>();
     na.bg = e->dBg.as<float>();
     na.pairs = D.pairs.as<PairInfo>();
-    na.brec = P.brec.as<BRec>();
-    na.frec = P.frec.as<FRec>();
+    na.rec = P.rec.as<Rec>();
     na.scratch = P.nullScratch.as<ES>();
     na.npairs = D.npairs;
     na.bgfs1 = e->bgfs1();
@@ -503,15 +510,16 @@ static int enqueueChunk(dm_engine *e, dm_plan &P, DevChunk &D, std::vector<std::
         }
     }
     const size_t bytes = (size_t)D.npairs * (P.mode == kFilter ? sizeof(float) : sizeof(PairScore));
-    HIPCHK(hipMemcpyAsync(P.hostOut, P.out.p, bytes, hipMemcpyDeviceToHost, e->stream));
+    char *stage = (char *)P.hostOut + D.outOff * (P.mode == kFilter ? sizeof(float) : sizeof(PairScore));
+    HIPCHK(hipMemcpyAsync(stage, P.out.p, bytes, hipMemcpyDeviceToHost, e->stream));
     return DM_OK;
 }
 
 static void scatterChunk(const dm_plan &P, const DevChunk &D, float *filterOut, PairScore *scoreOut) {
     for (int k = 0; k < D.npairs; k++) {
         const int64_t dst = D.order[k];
-        if (P.mode == kFilter) filterOut[dst] = ((const float *)P.hostOut)[k];
-        else scoreOut[dst] = ((const PairScore *)P.hostOut)[k];
+        if (P.mode == kFilter) filterOut[dst] = ((const float *)P.hostOut)[D.outOff + k];
+        else scoreOut[dst] = ((const PairScore *)P.hostOut)[D.outOff + k];
     }
 }
 
@@ -529,24 +537,29 @@ static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *sc
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
         totalMs = ms;
-        scatterChunk(P, P.chunks[0], filterOut, scoreOut);
+        for (const DevChunk &D : P.chunks) scatterChunk(P, D, filterOut, scoreOut);
         e->lastMs = totalMs;
         e->lastCells = P.cells;
         return DM_OK;
     }
+    // every chunk is enqueued before the one synchronization: chunks share the
+    // device buffers but run in stream order, and each copies its results to
+    // its own slots of the host staging
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
     for (DevChunk &D : P.chunks) {
-        std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
         int rc = enqueueChunk(e, P, D, &evs);
         if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(e->stream));
-        for (auto &ev : evs) {
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, ev.second.first, ev.second.second));
-            e->cfgMs[ev.first] += ms;
-            totalMs += ms;
-            (void)hipEventDestroy(ev.second.first);
-            (void)hipEventDestroy(ev.second.second);
-        }
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (auto &ev : evs) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev.second.first, ev.second.second));
+        e->cfgMs[ev.first] += ms;
+        totalMs += ms;
+        (void)hipEventDestroy(ev.second.first);
+        (void)hipEventDestroy(ev.second.second);
+    }
+    for (DevChunk &D : P.chunks) {
         for (const Launch &L : D.launches) {
             e->cfgCells[L.cfg] += L.cells;
             e->cfgLaunches[L.cfg] += 1;
@@ -558,13 +571,16 @@ static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *sc
     return DM_OK;
 }
 
-// Capture a single-chunk plan's launches into a hipGraph; later executions
-// replay the graph.
+// Capture a plan's launches (all chunks, in stream order) into a hipGraph;
+// later executions replay the graph.
 static int capturePlan(dm_engine *e, dm_plan &P) {
-    if (P.chunks.size() != 1) return fail(e, DM_EUNSUP, "only single-chunk plans can be captured");
     if (P.graphExec) return DM_OK;
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueueChunk(e, P, P.chunks[0], nullptr);
+    int rc = DM_OK;
+    for (DevChunk &D : P.chunks) {
+        rc = enqueueChunk(e, P, D, nullptr);
+        if (rc) break;
+    }
     hipGraph_t g = nullptr;
     hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (rc) {
@@ -591,7 +607,7 @@ struct PairDump {
     int G, R, M, n, T;
     std::vector<float> w0, W1;      // [(n+G-1) * G*R]
     std::vector<int> ef, eb;        // [(n+G-1)]
-    std::vector<BRec> brec;         // columns [-kRPad, n+kRPad)
+    std::vector<Rec> rec;           // columns [-kRPad, n+kRPad)
     float s1, s2, sC;
 };
 
@@ -632,8 +648,8 @@ static int dumpPair(dm_engine *e, int32_t prof, int32_t seq, PairDump &D) {
     const bool strips = e->cfgStrips[cfg] != 0;
     if (strips) {
         a.bndCols = n + 2;
-        HIPCHK(e->dBnd.ensure((size_t)(6 * a.bndCols) * sizeof(float)));
-        HIPCHK(e->dBndE.ensure((size_t)(3 * a.bndCols) * sizeof(int)));
+        HIPCHK(e->dBnd.ensure((size_t)(6 * a.bndCols * (64 / G)) * sizeof(float)));
+        HIPCHK(e->dBndE.ensure((size_t)(3 * a.bndCols * (64 / G)) * sizeof(int)));
         a.bnd = e->dBnd.as<float>();
         a.bndE = e->dBndE.as<int>();
     }
@@ -647,13 +663,13 @@ static int dumpPair(dm_engine *e, int32_t prof, int32_t seq, PairDump &D) {
     D.W1.resize(S * T * ROWS);
     D.ef.resize(S * T);
     D.eb.resize(S * T);
-    D.brec.resize(n + 2 * kRPad);
+    D.rec.resize(n + 2 * kRPad);
     PairInfo pi{};
     HIPCHK(hipMemcpyAsync(D.w0.data(), e->dW0buf.p, sizeof(float) * D.w0.size(), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(D.W1.data(), e->dWbuf.p, sizeof(float) * D.W1.size(), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(D.ef.data(), e->dFbuf.p, sizeof(int) * D.ef.size(), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(D.eb.data(), e->dEbuf.p, sizeof(int) * D.eb.size(), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(D.brec.data(), e->dBrec.as<BRec>(), sizeof(BRec) * D.brec.size(), hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(D.rec.data(), e->dRec.as<Rec>(), sizeof(Rec) * D.rec.size(), hipMemcpyDeviceToHost,
                           e->stream));
     HIPCHK(hipMemcpyAsync(&pi, e->dPairs.p, sizeof(PairInfo), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -723,7 +739,7 @@ void dm_engine_destroy(dm_engine *e) {
     (void)hipSetDevice(e->device);
     e->dBnd.release();
     e->dBndE.release();
-    for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dBrec, &e->dFrec,
+    for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dRec,
                       &e->dNull, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
                       &e->dOut})
         b->release();
@@ -906,8 +922,8 @@ int dm_pair_null(dm_engine *e, int32_t prof, int32_t seq, float *prefix_seed, fl
     int rc = dumpPair(e, prof, seq, D);
     if (rc) return rc;
     for (int j = 0; j < D.n; j++) {
-        if (prefix_seed) prefix_seed[j] = D.brec[j + kRPad].ps;
-        if (suffix_seed) suffix_seed[j] = D.brec[j + kRPad].ss;
+        if (prefix_seed) prefix_seed[j] = D.rec[j + kRPad].ps;
+        if (suffix_seed) suffix_seed[j] = D.rec[j + kRPad].ss;
     }
     if (scales) {
         scales[0] = D.s1;
@@ -978,7 +994,7 @@ int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio,
     }
     in.codes = e->codes.data() + e->seqOff[seq];
     in.prefixSeed.resize(D.n);
-    for (int j = 0; j < D.n; j++) in.prefixSeed[j] = D.brec[j + kRPad].ps;
+    for (int j = 0; j < D.n; j++) in.prefixSeed[j] = D.rec[j + kRPad].ps;
     in.s1 = D.s1;
     in.s2 = D.s2;
     in.sC = D.sC;
@@ -1226,7 +1242,7 @@ int dm_autotune(dm_engine *e, int32_t seq_len, double *ns_per_step, int32_t cap)
     for (size_t c = 0; c < ncfg && rc == DM_OK; c++) {
         const int G = e->cfgG[c];
         const int S = e->cfgStrips[c] ? 2 : 1;
-        const int units = e->numCUs * (e->cfgStrips[c] ? 2 : 16);
+        const int units = e->numCUs * 16;
         const int64_t np = (int64_t)units * (64 / G);
         std::vector<int32_t> pp(np, cfgProf[c]), ss(np);
         for (int64_t i = 0; i < np; i++) ss[i] = (int32_t)(seqs0 + (size_t)(i % nseq));
@@ -1299,6 +1315,11 @@ int dm_copy_probe(dm_engine *e, int64_t bytes, double *ms) {
     dst.release();
     return DM_OK;
 }
+
+#ifndef DM_BUILD_ID
+#define DM_BUILD_ID "unknown"
+#endif
+const char *dm_build_id(void) { return DM_BUILD_ID; }
 
 double dm_last_dp_ms(const dm_engine *e) { return e ? e->lastMs : 0.0; }
 int64_t dm_last_cells(const dm_engine *e) { return e ? e->lastCells : 0; }

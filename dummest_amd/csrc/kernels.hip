@@ -151,15 +151,11 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
     if (pi >= a.npairs) return;
     const PairInfo P = a.pairs[pi];
     const int n = P.n;
-    BRec *br = a.brec + P.recOff;
-    FRec *fr = a.frec + P.recOff;
+    Rec *rc = a.rec + P.recOff;
     if (n < 3) {
         // The reference asserts n >= 3 (dummest-dp.hh:84). Zero records make the
         // DP produce nothing; status reports the rejection.
-        for (int j = -kRPad; j < n + kRPad; j++) {
-            br[j] = BRec{0.f, 0.f, 0.f, kZeroIdx};
-            fr[j] = FRec{0.f, 0.f, kZeroIdx, 0};
-        }
+        for (int j = -kRPad; j < n + kRPad; j++) rc[j] = Rec{0.f, kZeroIdx | (kZeroIdx << 8), 0.f};
         a.pairs[pi].status = 1;
         a.pairs[pi].s1 = a.pairs[pi].s2 = a.pairs[pi].sC = 0.f;
         return;
@@ -236,16 +232,12 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
         sC = es_mul(es_mul(perBase, perBase), perBase);
     }
     // buildNullSeedModels, dummest-dp-backward.hh:133-165, and the records:
-    // column j holds g[j+1] / g[j-2], the seeds and the codon codes.
+    // column j holds the seeds and the codon codes at j+1 and j-2.
     {
         ES runS = es_pow2(0);
         for (int t = 0; t < n - 1; t++) runS = es_bmul(runS, perBase);
         ES runP = perBase;
-        for (int j = -kRPad; j < 0; j++) {
-            const int cb = code(j + 1), cf = code(j - 2);
-            br[j] = BRec{bgp[cb], 0.f, 0.f, cb};
-            fr[j] = FRec{bgp[cf], 0.f, cf, 0};
-        }
+        for (int j = -kRPad; j < 0; j++) rc[j] = Rec{0.f, (uint32_t)(code(j + 1) | (code(j - 2) << 8)), 0.f};
         ES q1 = es_zero(), q2 = es_zero(), q3 = es_zero();
         for (int j = 0; j < n; j++) {
             const ES pre = prefixStep(j, q1, q2, q3);
@@ -256,15 +248,9 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
             runS = es_bmul(runS, perBaseInv);
             const float psd = es_to_float_clamped(es_bmul(pre, runP));
             runP = es_bmul(runP, perBase);
-            const int cb = code(j + 1), cf = code(j - 2);
-            br[j] = BRec{bgp[cb], ssd, psd, cb};
-            fr[j] = FRec{bgp[cf], psd, cf, 0};
+            rc[j] = Rec{psd, (uint32_t)(code(j + 1) | (code(j - 2) << 8)), ssd};
         }
-        for (int j = n; j < n + kRPad; j++) {
-            const int cb = code(j + 1), cf = code(j - 2);
-            br[j] = BRec{bgp[cb], 0.f, 0.f, cb};
-            fr[j] = FRec{bgp[cf], 0.f, cf, 0};
-        }
+        for (int j = n; j < n + kRPad; j++) rc[j] = Rec{0.f, (uint32_t)(code(j + 1) | (code(j - 2) << 8)), 0.f};
     }
     a.pairs[pi].s1 = es_to_float_clamped(s1);
     a.pairs[pi].s2 = es_to_float_clamped(s2);
@@ -292,6 +278,24 @@ template <int G> __device__ __forceinline__ float from_below(float v) {
 template <int G> __device__ __forceinline__ float from_above(float v) {
     constexpr int ctrl = (G == 16) ? 0x111 : 0x138;
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, true));
+}
+
+// A column record moved one lane along the group (same DPP as the DP values).
+template <int G> __device__ __forceinline__ Rec rec_from_below(const Rec &r) {
+    constexpr int ctrl = (G == 16) ? 0x101 : 0x130;
+    Rec o;
+    o.ps = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(r.ps), ctrl, 0xf, 0xf, true));
+    o.codes = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r.codes, ctrl, 0xf, 0xf, true);
+    o.ss = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(r.ss), ctrl, 0xf, 0xf, true));
+    return o;
+}
+template <int G> __device__ __forceinline__ Rec rec_from_above(const Rec &r) {
+    constexpr int ctrl = (G == 16) ? 0x111 : 0x138;
+    Rec o;
+    o.ps = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(r.ps), ctrl, 0xf, 0xf, true));
+    o.codes = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r.codes, ctrl, 0xf, 0xf, true);
+    o.ss = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(r.ss), ctrl, 0xf, 0xf, true));
+    return o;
 }
 
 // Exact running maximum of float values whose power-of-two offset changes
@@ -382,7 +386,8 @@ template <int G, int R, int MODE, bool STR>
 __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
     constexpr int ROWS = G * R;
     constexpr int kPF = (R <= 8) ? 4 : 2;  // prefetch distance in steps (divides 4)
-    extern __shared__ float etab[];  // [kNCode][ROWS]
+    extern __shared__ float etab[];  // [kNCode][ROWS], then the profile's background table (32)
+    float *bgl = etab + kNCode * ROWS;
     const int lane = threadIdx.x;
     const int grp = lane / G;
     const int k = lane - grp * G;
@@ -399,8 +404,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
         const int pi = U.first + (active ? grp : 0);
         const PairInfo PR = a.pairs[pi];
         const int n = active ? PR.n : 0;
-        const BRec *br = a.brec + PR.recOff;
-        const FRec *fr = a.frec + PR.recOff;
+        const Rec *recs = a.rec + PR.recOff;
         const float s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
         const int T = U.maxN + G - 1;
         const int Tmine = n + G - 1;
@@ -415,26 +419,27 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             wsc0 = a.wbuf + off;
             esc0 = a.ebuf + off / ROWS;
         }
-        // strip boundary buffers (per wave): two parities of backward (W, Y, E)
+        // strip boundary buffers (per group): two parities of backward (W, Y, E)
         // and one of forward (N, Y, E), n + 2 entries each
         float *bW = nullptr, *bY2 = nullptr, *fN = nullptr, *fY = nullptr;
         int *bE = nullptr, *fE = nullptr;
         if (STR) {
             const int64_t nb = (int64_t)a.bndCols;
-            float *base = a.bnd + (int64_t)blockIdx.x * 6 * nb;
+            const int64_t gslot = (int64_t)blockIdx.x * (64 / G) + grp;
+            float *base = a.bnd + gslot * 6 * nb;
             bW = base;            // [2][nb]
             bY2 = base + 2 * nb;  // [2][nb]
             fN = base + 4 * nb;
             fY = base + 5 * nb;
-            int *ebase = a.bndE + (int64_t)blockIdx.x * 3 * nb;
+            int *ebase = a.bndE + gslot * 3 * nb;
             bE = ebase;           // [2][nb]
             fE = ebase + 2 * nb;
-            for (int c = lane; c < 2 * nb; c += 64) {
+            for (int c = k; c < 2 * nb; c += G) {
                 bW[c] = 0.f;
                 bY2[c] = 0.f;
                 bE[c] = 0;
             }
-            for (int c = lane; c < nb; c += 64) {
+            for (int c = k; c < nb; c += G) {
                 fN[c] = 0.f;
                 fY[c] = 0.f;
                 fE[c] = 0;
@@ -456,6 +461,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 const int c = idx / ROWS, r = rb + idx - c * ROWS;
                 etab[idx] = (r <= M) ? a.emis[PI.emOff + (int64_t)r * kEmStride + c] : 0.f;
             }
+            if (lane < 32) bgl[lane] = a.bg[(int64_t)U.prof * 32 + lane];
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < R; q++) {
@@ -527,14 +533,18 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 if (a.wantBwd) bwdMax.rebase(Eb);
             };
 
-            // Column records are loaded kPF steps ahead of their use.
-            auto brecAt = [&](int tt) {
-                const int c = Tmine - 1 - tt - k;
-                return br[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
+            // Column records: lane k uses at step t the record lane k+1 used at
+            // step t-1, so a record enters the group at its last lane (loaded kPF
+            // steps ahead there) and moves up one lane per step by DPP.
+            auto brecAt = [&](int tt, int kk) {
+                const int c = Tmine - 1 - tt - kk;
+                return recs[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
             };
-            BRec rq[kPF];
+            const bool injB = (k == G - 1);
+            Rec rq[kPF];  // the last lane's records of the next steps (slot = step mod kPF)
 #pragma unroll
-            for (int f = 0; f < kPF; f++) rq[f] = brecAt(f);
+            for (int f = 1; f <= kPF; f++) rq[f % kPF] = injB ? brecAt(f, G - 1) : Rec{0.f, 0u, 0.f};
+            Rec rcur = brecAt(0, k);
             // strip kernels: the strip below's top row (W1, Y, exponent), also ahead
             float gqw[kPF], gqy[kPF];
             int gqe[kPF];
@@ -552,13 +562,20 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             };
 #pragma unroll
             for (int f = 0; f < kPF; f++) ghostAt(f, gqw[f], gqy[f], gqe[f]);
+            // emission values of the next step, read from LDS one step ahead
+            float eq[R];
+#pragma unroll
+            for (int q = 0; q < R; q++) eq[q] = etk[(rcur.codes & 255) * ROWS + q];
+            float gq = bgl[rcur.codes & 255];  // g[c+1] of the next step
+            // lanes whose rows all lie past row M store and load no W1
+            const bool rowsLive = strip * ROWS + k * R <= M;
 
             auto bstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
                 constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
                 const int c = Tmine - 1 - t - k;  // column: D = Tmine-1-t
-                const BRec rc = rq[PH % kPF];
-                rq[PH % kPF] = brecAt(t + kPF);
+                const Rec rc = rcur;
+                const float rcg = gq;
                 float gw = 0.f, gy = 0.f;
                 if (STR && below) {
                     // last lane: the strip below's top row from the boundary buffer.
@@ -574,7 +591,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                             (void)frexpf(fmaxf(rw, ry), &ex);
                             need = ex + ge - Eb - kGhostTop;
                         }
-                        need = __builtin_amdgcn_readlane(need, G - 1);
+                        need = __shfl(need, grp * G + G - 1, 64);  // the group's last lane
                         if (__builtin_expect(need > 0, 0)) scaleB(-((need + 63) & ~63));
                     }
                     gw = ldexpf(rw, ge - Eb);
@@ -598,13 +615,12 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 const float ssr = rc.ss * rs;
                 const float psp = rc.ps;
                 const double pspD = (double)rc.ps;
-                const float *et = etk + rc.code * ROWS;
                 float smax = 0.f;
 #pragma unroll
                 for (int q = R - 1; q >= 0; q--) {
-                    const float e = et[q];
+                    const float e = eq[q];
                     const float eE = e * enterC[q];
-                    const float zg = Zr[S3][q] * rc.g;
+                    const float zg = Zr[S3][q] * rcg;
                     float w = fmaf(bw3, eE, ssr * valid[q]);
                     w = fmaf(bw2, d1s2[q], w);
                     w = fmaf(bw1, d2s1[q], w);
@@ -625,6 +641,18 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     Zr[S4][q] = z;
                     Yc[q] = y;
                 }
+                {
+                    // the next step's record: the lane below's, or a new one at the last lane
+                    Rec nx = rec_from_below<G>(rcur);
+                    constexpr int SN = (PH + 1) % kPF;
+                    if (injB) nx = rq[SN];
+                    rq[SN] = injB ? brecAt(t + 1 + kPF, G - 1) : rq[SN];
+                    rcur = nx;
+                    const int nc = nx.codes & 255;
+#pragma unroll
+                    for (int q = 0; q < R; q++) eq[q] = etk[nc * ROWS + q];
+                    gq = bgl[nc];
+                }
                 if (MODE != kFilter && a.wantBwd) {  // backward start score: max W1 * prefixSeed
 #pragma unroll
                     for (int q = 0; q < R; q++) bwdMax.cur = fmax(bwdMax.cur, (double)H[S4][q] * pspD);
@@ -637,10 +665,12 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 if (MODE != kFilter) {
                     const int D = Tmine - 1 - t;
                     if (D >= 0 && active) {
-                        float *dst = wsc + (int64_t)D * ROWS + k * R;
+                        if (rowsLive && c >= 0 && c < n) {
+                            float *dst = wsc + (int64_t)D * ROWS + k * R;
 #pragma unroll
-                        for (int q = 0; q < R; q++)
-                            dst[q] = (MODE == kScore) ? H[S4][q] * 7.888609052e-31f : H[S4][q];
+                            for (int q = 0; q < R; q++)
+                                dst[q] = (MODE == kScore) ? H[S4][q] * 7.888609052e-31f : H[S4][q];
+                        }
                         if (k == 0) esc[D] = Eb;
                     }
                 }
@@ -736,24 +766,29 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             // Column records and the stored W1 block of a step (with its
             // exponent) are loaded kPF steps ahead of their use.
             constexpr int kNoStep = INT_MIN;
-            auto frecAt = [&](int tt) {
-                const int c = tt - k;
-                return fr[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
+            // Column records move down one lane per step (lane k uses at step t
+            // the record lane k-1 used at t-1); they enter at the group's first lane.
+            auto frecAt = [&](int tt, int kk) {
+                const int c = tt - kk;
+                return recs[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
             };
-            FRec fq[kPF];
+            const bool injF = (k == 0);
+            Rec fq[kPF];  // the first lane's records of the next steps (slot = step mod kPF)
             float Wq[kPF][R];
             int Eq[kPF];
+            const bool rowsLive = strip * ROWS + k * R <= M;
             auto loadW1 = [&](int tt, float *dst, int &e) {
                 const bool in = tt < Tmine;
                 e = in ? esc[tt] : kNoStep;
+                const bool live = in && rowsLive && tt - k >= 0 && tt - k < n;  // the backward stored this block
 #pragma unroll
-                for (int q = 0; q < R; q++) dst[q] = (MODE == kScore && in) ? wsc[(int64_t)tt * ROWS + k * R + q] : 0.f;
+                for (int q = 0; q < R; q++) dst[q] = (MODE == kScore && live) ? wsc[(int64_t)tt * ROWS + k * R + q] : 0.f;
             };
 #pragma unroll
-            for (int f = 0; f < kPF; f++) {
-                fq[f] = frecAt(f);
-                loadW1(f, Wq[f], Eq[f]);
-            }
+            for (int f = 0; f < kPF; f++) loadW1(f, Wq[f], Eq[f]);
+#pragma unroll
+            for (int f = 1; f <= kPF; f++) fq[f % kPF] = injF ? frecAt(f, 0) : Rec{0.f, 0u, 0.f};
+            Rec fcur = frecAt(0, k);
 
             // strip kernels: the strip above's last row (carried sum, Y, exponent), also ahead
             float fqn[kPF], fqy[kPF];
@@ -772,6 +807,10 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             };
 #pragma unroll
             for (int f = 0; f < kPF; f++) fghostAt(f, fqn[f], fqy[f], fqe[f]);
+            float eqf[R];  // emission values of the next step, one step ahead
+#pragma unroll
+            for (int q = 0; q < R; q++) eqf[q] = etk[((fcur.codes >> 8) & 255) * ROWS + q];
+            float gqf = bgl[(fcur.codes >> 8) & 255];  // g[c-2] of the next step
 
             auto fstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
@@ -779,8 +818,8 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 constexpr int SL = PH % kPF;
                 const int D = t;
                 const int c = D - k;
-                const FRec rc = fq[SL];
-                fq[SL] = frecAt(t + kPF);
+                const Rec rc = fcur;
+                const float rcg = gqf;
                 float W1v[R];
 #pragma unroll
                 for (int q = 0; q < R; q++) W1v[q] = Wq[SL][q];
@@ -799,14 +838,14 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                         (void)frexpf(fmaxf(rn, ry), &ex);
                         need = ex + ge - Ef - kGhostTop;
                     }
-                    need = __builtin_amdgcn_readlane(need, 0);
+                    need = __shfl(need, grp * G, 64);  // the group's first lane
                     if (__builtin_expect(need > 0, 0)) scaleF(-((need + 63) & ~63));
                     gn = ldexpf(rn, ge - Ef);
                     gy = ldexpf(ry, ge - Ef);
                 }
                 float Nin = from_above<G>(lastN);
                 float Ya = from_above<G>(lastY);
-                if (STR && above && k == 0) {
+                if (STR && k == 0) {  // the strip above's row, or 0 in the top strip
                     Nin = gn;
                     Ya = gy;
                 }
@@ -816,14 +855,13 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     if (EbD != EbPrev) midMax.rebase(Ef + EbD + kPre);
                 }
                 EbPrev = EbD;
-                const float gsC = rc.g * sC;
+                const float gsC = rcg * sC;
                 const float psr = rc.ps * rf;
-                const float *et = etk + rc.code * ROWS;
                 float smax = 0.f;
 #pragma unroll
                 for (int q = 0; q < R; q++) {
                     const float w1 = Wf[S1][q], w2 = Wf[S2][q], w3 = Wf[S3][q];
-                    const float e = et[q];
+                    const float e = eqf[q];
                     const float X = enterC[q] * e * w3;
                     const float Z0 = gsC * fmaf(b0[q], Zf[S3][q], a0[q] * w3);
                     float w0 = fmaf(a1s1[q], w1, Z0);
@@ -843,6 +881,18 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 }
                 lastN = Nin;
                 lastY = Ya;
+                {
+                    // the next step's record: the lane above's, or a new one at the first lane
+                    Rec nx = rec_from_above<G>(fcur);
+                    constexpr int SN = (PH + 1) % kPF;
+                    if (injF) nx = fq[SN];
+                    fq[SN] = injF ? frecAt(t + 1 + kPF, 0) : fq[SN];
+                    fcur = nx;
+                    const int nc = (nx.codes >> 8) & 255;
+#pragma unroll
+                    for (int q = 0; q < R; q++) eqf[q] = etk[nc * ROWS + q];
+                    gqf = bgl[nc];
+                }
                 if (STR && below && k == G - 1 && c >= 0 && c < n) {
                     fN[c + 1] = lastN;
                     fY[c + 1] = lastY;
@@ -893,10 +943,10 @@ namespace dm {
 #define DM_CFGS(X)                                                                                 \
     X(16, 2) X(16, 3) X(16, 4) X(16, 5) X(16, 6) X(16, 7) X(16, 8)                               \
     X(32, 2) X(32, 3) X(32, 4) X(32, 5) X(32, 6) X(32, 7) X(32, 8)                               \
-    X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 6) X(64, 8) X(64, 12) X(64, 16)
+    X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 6) X(64, 7) X(64, 8) X(64, 12) X(64, 16)
 
-// Row-strip kernels: one pair per wave, any profile length, rows in strips of 64*R.
-#define DM_STRIP_CFGS(X) X(64, 4) X(64, 6) X(64, 8) X(64, 16)
+// Row-strip kernels: any profile length, rows in strips of G*R.
+#define DM_STRIP_CFGS(X) X(32, 4) X(32, 6) X(32, 8) X(64, 4) X(64, 6) X(64, 8) X(64, 16)
 
 #define DM_LIST(g, r)        \
     if (n < cap) {           \
@@ -934,7 +984,7 @@ template <class K> static hipError_t launch_one(K kernel, size_t smem, int grid,
 }
 
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s) {
-    const size_t smem = sizeof(float) * kNCode * G * R;
+    const size_t smem = sizeof(float) * (kNCode * G * R + 32);
 #define DM_LAUNCH(g, r)                                                                             \
     if (G == g && R == r && !strips) {                                                              \
         if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter, false>, smem, grid, s, a);  \

@@ -119,9 +119,9 @@ int dm_plan_create(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, 
 int dm_plan_execute(dm_engine *e, dm_plan *plan, float *best, float *mid_m, int32_t *mid_e, float *bwd_m,
                     int32_t *bwd_e);
 void dm_plan_destroy(dm_engine *e, dm_plan *plan);
-/* Capture a single-chunk plan (up to 2^18 pairs) into a hipGraph; later
- * dm_plan_execute calls replay the graph. The engine's profiles and sequences
- * must not change while a captured plan is in use. */
+/* Capture a plan's launches into a hipGraph; later dm_plan_execute calls
+ * replay the graph. The engine's profiles and sequences must not change while
+ * a captured plan is in use. */
 int dm_plan_capture(dm_engine *e, dm_plan *plan);
 
 /* Per kernel configuration of the last execution: G (lanes per pair), R (rows
@@ -136,6 +136,9 @@ int dm_kernel_stats(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int
  * that scales the HBM traffic counters (FETCH_SIZE / WRITE_SIZE) of the DP
  * kernels. */
 int dm_copy_probe(dm_engine *e, int64_t bytes, double *ms);
+/* Hash of the kernel sources this library was built from (measurement files
+ * record it, so that they are matched to the build they describe). */
+const char *dm_build_id(void);
 
 /* Post-gate processing on caller-supplied DP matrices (no device): the same
  * host code dm_pair_hits runs after its dump, fed with w0/W1 ExpScore matrices
