@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=400)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-profiles", type=int, default=500,
+                    help="profiles in the calibration sweep reported beside the main figure (0: skip)")
     ap.add_argument("--tune-file", default="",
                     help="load the autotuned cost table from this file if it exists, else tune and save it there")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -190,6 +192,21 @@ def main():
         same = float(np.max(np.abs(D.es_log(pm, pe) - D.es_log(mm[sample], me[sample]))))
         log(f"plain: {plain:.4g} cells/s on {len(sample)} pairs (max |d score| vs batched {same:.2e} nats)")
 
+    # calibration sweep (BASELINE configs[2]'s shape): trial DNA on the host, the
+    # scores of all profiles' trials in one graph-captured pass, Gumbel fits
+    calib = None
+    if args.calib_profiles > 0:
+        cp = pids[:args.calib_profiles]
+        t = time.perf_counter()
+        e.calibrate(cp, seq_len=3000, border=100, trials=200)
+        wall = time.perf_counter() - t
+        ccells = sum((w["Ms"][k] + 1) * 3100 * 200 for k in range(len(cp)))
+        calib = {"profiles": len(cp), "trials": 200, "seq_len": 3000, "border": 100,
+                 "device_ms": round(e.last_dp_ms, 2), "wall_s": round(wall, 3),
+                 "dp_cells_per_s": float(f"{ccells / (e.last_dp_ms * 1e-3):.4g}")}
+        log(f"calibration sweep: {len(cp)} profiles x 200 trials of 3.1 kb: {e.last_dp_ms:.1f} ms on the device, "
+            f"{wall:.2f} s in all")
+
     cpu = None
     if not args.no_cpu:
         w["oracle_profiles"] = {}
@@ -224,6 +241,7 @@ def main():
         "batched_vs_plain": round(value / plain, 2) if plain else None,
         "batched_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "build_id": D.build_id(),
+        "calibration_sweep": calib,
     }
     print(json.dumps(out), flush=True)
     plan.close()

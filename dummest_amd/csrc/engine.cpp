@@ -15,6 +15,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dm_layout.h"
@@ -1131,24 +1132,43 @@ int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_l
     const int len = seq_len + border;
     const int64_t npairs = (int64_t)nprof * trials;
     std::vector<int32_t> pp(npairs), ss(npairs);
-    std::vector<char> dna(len);
-    e->codes.reserve(codes0 + (size_t)npairs * len);
-    for (int32_t k = 0; k < nprof; k++) {
-        const HostProfile &P = e->profiles[prof[k]];
-        for (int32_t t = 0; t < trials; t++) {
-            calibSequence(P.bg.data() + 4, (float)e->rates[5], e->bgfs1(), e->bgfs2(), seq_len, border, t, dna.data());
-            const size_t at = e->codes.size();
-            e->codes.resize(at + len);
-            decodeDna(dna.data(), len, e->codes.data() + at);
-            pp[(size_t)k * trials + t] = prof[k];
-            ss[(size_t)k * trials + t] = (int32_t)e->seqLen.size();
-            e->seqOff.push_back((int64_t)at);
-            e->seqLen.push_back(len);
-        }
+    e->codes.resize(codes0 + (size_t)npairs * len);
+    for (int64_t i = 0; i < npairs; i++) {
+        pp[i] = prof[i / trials];
+        ss[i] = (int32_t)(seqs0 + (size_t)i);
+        e->seqOff.push_back((int64_t)(codes0 + (size_t)i * len));
+        e->seqLen.push_back(len);
+    }
+    // the trials are independent (each seeds its own generator): host threads
+    // generate and decode them into their slots
+    {
+        const unsigned hc = std::thread::hardware_concurrency();
+        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(hc ? hc : 1, (npairs + 63) / 64));
+        std::vector<std::thread> pool;
+        for (int th = 0; th < nth; th++)
+            pool.emplace_back([&, th]() {
+                std::vector<char> dna(len);
+                for (int64_t i = th; i < npairs; i += nth) {
+                    const HostProfile &P = e->profiles[pp[i]];
+                    calibSequence(P.bg.data() + 4, (float)e->rates[5], e->bgfs1(), e->bgfs2(), seq_len, border,
+                                  (int)(i % trials), dna.data());
+                    decodeDna(dna.data(), len, e->codes.data() + codes0 + (size_t)i * len);
+                }
+            });
+        for (auto &t : pool) t.join();
     }
     e->seqDirty = true;
     std::vector<PairScore> out(npairs);
-    int rc = runPairs(e, kScore, npairs, pp.data(), ss.data(), 1, nullptr, out.data());
+    // the whole sweep (null model and DP launches of every chunk) is captured
+    // as one hipGraph and replayed
+    int rc;
+    {
+        dm_plan P;
+        rc = buildPlan(e, kScore, npairs, pp.data(), ss.data(), 1, P);
+        if (rc == DM_OK) rc = capturePlan(e, P);
+        if (rc == DM_OK) rc = executePlan(e, P, nullptr, out.data());
+        P.release();
+    }
     e->codes.resize(codes0);
     e->seqOff.resize(seqs0);
     e->seqLen.resize(seqs0);

@@ -543,7 +543,7 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             const bool injB = (k == G - 1);
             Rec rq[kPF];  // the last lane's records of the next steps (slot = step mod kPF)
 #pragma unroll
-            for (int f = 1; f <= kPF; f++) rq[f % kPF] = injB ? brecAt(f, G - 1) : Rec{0.f, 0u, 0.f};
+            for (int f = 1; f <= kPF; f++) rq[f % kPF] = brecAt(f, G - 1);
             Rec rcur = brecAt(0, k);
             // strip kernels: the strip below's top row (W1, Y, exponent), also ahead
             float gqw[kPF], gqy[kPF];
@@ -576,6 +576,16 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 const int c = Tmine - 1 - t - k;  // column: D = Tmine-1-t
                 const Rec rc = rcur;
                 const float rcg = gq;
+                // the next step's record (the lane below's, or a new one at the last
+                // lane) and its emission values, read now so that they arrive in time
+                constexpr int SN = (PH + 1) % kPF;
+                Rec nx = rec_from_below<G>(rcur);
+                if (injB) nx = rq[SN];
+                rq[SN] = brecAt(t + 1 + kPF, G - 1);  // same address in every lane of the group
+                float eqN[R];
+#pragma unroll
+                for (int q = 0; q < R; q++) eqN[q] = etk[(nx.codes & 255) * ROWS + q];
+                const float gqN = bgl[nx.codes & 255];
                 float gw = 0.f, gy = 0.f;
                 if (STR && below) {
                     // last lane: the strip below's top row from the boundary buffer.
@@ -641,18 +651,10 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                     Zr[S4][q] = z;
                     Yc[q] = y;
                 }
-                {
-                    // the next step's record: the lane below's, or a new one at the last lane
-                    Rec nx = rec_from_below<G>(rcur);
-                    constexpr int SN = (PH + 1) % kPF;
-                    if (injB) nx = rq[SN];
-                    rq[SN] = injB ? brecAt(t + 1 + kPF, G - 1) : rq[SN];
-                    rcur = nx;
-                    const int nc = nx.codes & 255;
+                rcur = nx;
 #pragma unroll
-                    for (int q = 0; q < R; q++) eq[q] = etk[nc * ROWS + q];
-                    gq = bgl[nc];
-                }
+                for (int q = 0; q < R; q++) eq[q] = eqN[q];
+                gq = gqN;
                 if (MODE != kFilter && a.wantBwd) {  // backward start score: max W1 * prefixSeed
 #pragma unroll
                     for (int q = 0; q < R; q++) bwdMax.cur = fmax(bwdMax.cur, (double)H[S4][q] * pspD);
@@ -777,17 +779,22 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
             float Wq[kPF][R];
             int Eq[kPF];
             const bool rowsLive = strip * ROWS + k * R <= M;
+            // Loads are unconditional (from a clamped, always-valid step) so that
+            // no select waits for them; which of them count is decided at use.
             auto loadW1 = [&](int tt, float *dst, int &e) {
-                const bool in = tt < Tmine;
-                e = in ? esc[tt] : kNoStep;
-                const bool live = in && rowsLive && tt - k >= 0 && tt - k < n;  // the backward stored this block
+                const int ts = tt < Tmine ? tt : (Tmine > 0 ? Tmine - 1 : 0);
+                e = esc[ts];
+                if (MODE == kScore) {
 #pragma unroll
-                for (int q = 0; q < R; q++) dst[q] = (MODE == kScore && live) ? wsc[(int64_t)tt * ROWS + k * R + q] : 0.f;
+                    for (int q = 0; q < R; q++) dst[q] = wsc[(int64_t)ts * ROWS + k * R + q];
+                }
             };
+            // the backward stored the block of step tt for this lane
+            auto liveW1 = [&](int tt) { return tt < Tmine && rowsLive && tt - k >= 0 && tt - k < n; };
 #pragma unroll
             for (int f = 0; f < kPF; f++) loadW1(f, Wq[f], Eq[f]);
 #pragma unroll
-            for (int f = 1; f <= kPF; f++) fq[f % kPF] = injF ? frecAt(f, 0) : Rec{0.f, 0u, 0.f};
+            for (int f = 1; f <= kPF; f++) fq[f % kPF] = frecAt(f, 0);
             Rec fcur = frecAt(0, k);
 
             // strip kernels: the strip above's last row (carried sum, Y, exponent), also ahead
@@ -820,10 +827,20 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 const int c = D - k;
                 const Rec rc = fcur;
                 const float rcg = gqf;
+                // the next step's record (the lane above's, or a new one at the first
+                // lane) and its emission values, read now so that they arrive in time
+                constexpr int SN = (PH + 1) % kPF;
+                Rec nx = rec_from_above<G>(fcur);
+                if (injF) nx = fq[SN];
+                fq[SN] = frecAt(t + 1 + kPF, 0);  // same address in every lane of the group
+                float eqN[R];
+#pragma unroll
+                for (int q = 0; q < R; q++) eqN[q] = etk[((nx.codes >> 8) & 255) * ROWS + q];
+                const float gqN = bgl[(nx.codes >> 8) & 255];
                 float W1v[R];
 #pragma unroll
-                for (int q = 0; q < R; q++) W1v[q] = Wq[SL][q];
-                const int EbNext = Eq[SL];
+                for (int q = 0; q < R; q++) W1v[q] = (MODE == kScore && liveW1(t)) ? Wq[SL][q] : 0.f;
+                const int EbNext = (t < Tmine) ? Eq[SL] : kNoStep;
                 loadW1(t + kPF, Wq[SL], Eq[SL]);
                 float gn = 0.f, gy = 0.f;
                 if (STR && above) {
@@ -881,18 +898,10 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
                 }
                 lastN = Nin;
                 lastY = Ya;
-                {
-                    // the next step's record: the lane above's, or a new one at the first lane
-                    Rec nx = rec_from_above<G>(fcur);
-                    constexpr int SN = (PH + 1) % kPF;
-                    if (injF) nx = fq[SN];
-                    fq[SN] = injF ? frecAt(t + 1 + kPF, 0) : fq[SN];
-                    fcur = nx;
-                    const int nc = (nx.codes >> 8) & 255;
+                fcur = nx;
 #pragma unroll
-                    for (int q = 0; q < R; q++) eqf[q] = etk[nc * ROWS + q];
-                    gqf = bgl[nc];
-                }
+                for (int q = 0; q < R; q++) eqf[q] = eqN[q];
+                gqf = gqN;
                 if (STR && below && k == G - 1 && c >= 0 && c < n) {
                     fN[c + 1] = lastN;
                     fY[c + 1] = lastY;

@@ -200,25 +200,27 @@ bool readHmmFile(const char *path, int bgType, const double rates[6], std::vecto
 
 // Codon table in TCAG order; non-ACGT letters make the codon '?'.
 void decodeDna(const char *dna, int64_t n, uint8_t *codes) {
+    // lookup tables, built once (thread-safe static initialisation)
+    struct Tables {
+        int8_t base[256];
+        uint8_t num[256];
+        Tables() {
+            std::memset(base, -1, sizeof(base));
+            base['T'] = base['t'] = 0;
+            base['C'] = base['c'] = 1;
+            base['A'] = base['a'] = 2;
+            base['G'] = base['g'] = 3;
+            for (int c = 0; c < 256; c++) num[c] = 22;
+            for (int i = 0; kAlpha[i]; i++) num[(unsigned char)kAlpha[i]] = (uint8_t)i;
+        }
+    };
+    static const Tables T;
     static const char *aa = "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG";
-    static int8_t base[256];
-    static uint8_t num[256];
-    static bool init = false;
-    if (!init) {
-        std::memset(base, -1, sizeof(base));
-        base['T'] = base['t'] = 0;
-        base['C'] = base['c'] = 1;
-        base['A'] = base['a'] = 2;
-        base['G'] = base['g'] = 3;
-        for (int c = 0; c < 256; c++) num[c] = 22;
-        for (int i = 0; kAlpha[i]; i++) num[(unsigned char)kAlpha[i]] = (uint8_t)i;
-        init = true;
-    }
     for (int64_t i = 0; i < n; i++) codes[i] = 0;
     for (int64_t i = 0; i + 2 < n; i++) {
-        const int b0 = base[(unsigned char)dna[i]], b1 = base[(unsigned char)dna[i + 1]],
-                  b2 = base[(unsigned char)dna[i + 2]];
-        codes[i] = (b0 < 0 || b1 < 0 || b2 < 0) ? 22 : num[(unsigned char)aa[b0 * 16 + b1 * 4 + b2]];
+        const int b0 = T.base[(unsigned char)dna[i]], b1 = T.base[(unsigned char)dna[i + 1]],
+                  b2 = T.base[(unsigned char)dna[i + 2]];
+        codes[i] = (b0 < 0 || b1 < 0 || b2 < 0) ? 22 : T.num[(unsigned char)aa[b0 * 16 + b1 * 4 + b2]];
     }
 }
 

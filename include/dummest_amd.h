@@ -153,7 +153,8 @@ int dm_post_process(int32_t M, const float *values, const float *params, const f
  * dummest-calibration.hh:338-600, called per profile from dummest.cc:384-408):
  * per profile, `trials` random frameshifting codon sequences of seq_len +
  * border letters drawn from the profile's background (trial t seeded with
- * 5489 + t), all scored in one batched forward+backward pass. Writes the
+ * 5489 + t), all scored in one batched forward+backward pass captured as one
+ * hipGraph (dm_last_dp_ms gives its device time). Writes the
  * natural-log scores (mid-anchored and filter, nprof x trials each; either
  * may be NULL) and per profile 16 fits: estimateGumbel of the mid scores
  * (MM lambda, MM K, MM K lambda=1, ML lambda, ML K, ML K lambda=1, LM lambda,
