@@ -21,6 +21,8 @@ constexpr int kNCode = 25;     // decoded codes 0..24; 24 is the zero-probabilit
 constexpr int kZeroIdx = 24;
 constexpr int kEmStride = 32;  // floats per emission row in HBM
 constexpr int kRPad = 4;       // record padding columns on each side of a pair's sequence
+constexpr int kRecBlock = 16;  // a pair's records start on a multiple of 16 (192 bytes, 64-byte aligned)
+inline constexpr int64_t recSpan(int n) { return (n + 2 * kRPad + kRecBlock - 1) / kRecBlock * kRecBlock; }
                                // (every column beyond them reads the zero record at the edge)
 
 // Column record j of a pair (12 bytes): prefix seed, the codon codes at j+1
@@ -78,6 +80,7 @@ enum Mode { kFilter = 0, kScore = 1, kDump = 2 };
 
 struct NullArgs {
     const uint8_t *codes;
+    const uint8_t *zeroCode;  // one device byte holding kZeroIdx
     const float *bg;   // [prof*32 + code]
     PairInfo *pairs;
     Rec *rec;

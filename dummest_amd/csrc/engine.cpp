@@ -131,9 +131,13 @@ static int syncProfiles(dm_engine *e) {
 
 static int syncSequences(dm_engine *e) {
     if (!e->seqDirty) return DM_OK;
-    HIPCHK(e->dCodes.ensure(std::max<size_t>(1, e->codes.size())));
+    // The byte after the codes holds the zero-probability code: the null
+    // kernel points its loads of columns outside a sequence there.
+    HIPCHK(e->dCodes.ensure(e->codes.size() + 1));
     if (!e->codes.empty())
         HIPCHK(hipMemcpy(e->dCodes.p, e->codes.data(), e->codes.size(), hipMemcpyHostToDevice));
+    const uint8_t zero = kZeroIdx;
+    HIPCHK(hipMemcpy(e->dCodes.as<uint8_t>() + e->codes.size(), &zero, 1, hipMemcpyHostToDevice));
     e->seqDirty = false;
     return DM_OK;
 }
@@ -260,7 +264,7 @@ static void buildChunk(const dm_engine *E, const std::vector<int64_t> &sorted, c
                 pi.n = E->seqLen[pi.seq];
                 pi.seqOff = E->seqOff[pi.seq];
                 pi.recOff = C.records + kRPad;
-                C.records += pi.n + 2 * kRPad;
+                C.records += recSpan(pi.n);
                 U.maxN = std::max(U.maxN, pi.n);
                 C.pairs.push_back(pi);
                 C.order.push_back(pidx);
@@ -287,6 +291,7 @@ static int uploadChunk(dm_engine *e, Chunk &C) {
                           e->stream));
     NullArgs na{};
     na.codes = e->dCodes.as<uint8_t>();
+    na.zeroCode = e->dCodes.as<uint8_t>() + e->codes.size();
     na.bg = e->dBg.as<float>();
     na.pairs = e->dPairs.as<PairInfo>();
     na.rec = e->dRec.as<Rec>();
@@ -395,7 +400,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             int64_t bytes = 0;
             for (en = b; en < (size_t)npairs && en - b < (size_t)kChunkPairs; en++) {
                 const int64_t nn = e->seqLen[seq[sorted[en]]];
-                bytes += (nn + 2 * kRPad) * (int64_t)sizeof(Rec) + (nn + 4) * (int64_t)sizeof(ES);
+                bytes += recSpan((int)nn) * (int64_t)sizeof(Rec) + (nn + 4) * (int64_t)sizeof(ES);
                 if (bytes > kChunkBytes && en > b) break;
             }
         }
@@ -488,6 +493,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
 static int enqueueChunk(dm_engine *e, dm_plan &P, DevChunk &D, std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> *evs) {
     NullArgs na{};
     na.codes = e->dCodes.as<uint8_t>();
+    na.zeroCode = e->dCodes.as<uint8_t>() + e->codes.size();
     na.bg = e->dBg.as<float>();
     na.pairs = D.pairs.as<PairInfo>();
     na.rec = P.rec.as<Rec>();
@@ -1271,10 +1277,17 @@ int dm_autotune(dm_engine *e, int32_t seq_len, double *ns_per_step, int32_t cap)
         rc = buildPlan(e, kScore, np, pp.data(), ss.data(), 0, P);
         std::vector<PairScore> out(np);
         if (rc == DM_OK) rc = executePlan(e, P, nullptr, out.data());  // warm-up
-        if (rc == DM_OK) rc = executePlan(e, P, nullptr, out.data());
+        // median of three timed runs: one slow or fast outlier would move
+        // every pair of a length class to the wrong configuration
+        double ms[3] = {0, 0, 0};
+        for (int r = 0; r < 3 && rc == DM_OK; r++) {
+            rc = executePlan(e, P, nullptr, out.data());
+            ms[r] = e->cfgMs[c];
+        }
         if (rc == DM_OK) {
+            std::sort(ms, ms + 3);
             const double waveSteps = (double)units * (double)(seq_len + G - 1) * S;
-            tuned[c] = e->cfgMs[c] * 1e6 / waveSteps;
+            tuned[c] = ms[1] * 1e6 / waveSteps;
         }
         P.release();
     }

@@ -79,14 +79,16 @@ __device__ __forceinline__ ES es_mul(const ES &a, const ES &b) {  // dummest-sco
 
 __device__ __forceinline__ ES es_pow2(int e2) { return ES{0.5f, e2 + 1}; }
 
-__device__ __forceinline__ void split_double(double x, float &mOut, int &eAdd) {  // dummest-batch.hh:42-53
-    if (x == 0.0) { mOut = 0.0f; eAdd = 0; return; }
-    uint64_t u = (uint64_t)__double_as_longlong(x);
-    int ef = (int)((u >> 52) & 0x7FFULL);
-    if (ef == 0x7FF) { mOut = (float)x; eAdd = 0; return; }
-    if (ef == 0) { mOut = 0.0f; eAdd = 0; return; }
-    eAdd = ef - 1022;
-    mOut = (float)__longlong_as_double((long long)((u & 0xFFFFFFFFFFFFFULL) | (1022ULL << 52)));
+// dummest-batch.hh:42-53: zero and subnormal x give (0, 0), infinity and NaN
+// give ((float)x, 0), otherwise the mantissa in [0.5, 1) and its exponent.
+// Written with selects so that it compiles to straight-line code.
+__device__ __forceinline__ void split_double(double x, float &mOut, int &eAdd) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const int ef = (int)((u >> 52) & 0x7FFULL);
+    const float mNorm = (float)__longlong_as_double((long long)((u & 0xFFFFFFFFFFFFFULL) | (1022ULL << 52)));
+    const bool special = (ef == 0x7FF), zero = (ef == 0);
+    mOut = zero ? 0.0f : (special ? (float)x : mNorm);
+    eAdd = (zero || special) ? 0 : ef - 1022;
 }
 
 __device__ __forceinline__ ES es_bmul(const ES &A, const ES &B) {  // dummest-batch.hh:58-72
@@ -138,32 +140,100 @@ __device__ __forceinline__ float es_to_float_clamped(const ES &a) {  // dummest-
 // (the reference's lane arithmetic is independent of the batch). Writes the
 // pair's record arrays and scales.
 //
-// Three passes over the sequence, each with its recurrence window in
-// registers: the suffix (descending; its values go to the interleaved scratch,
-// entry j at nullOff + 64*j, so a wave's accesses are coalesced), the prefix
-// (ascending; only the last three values are kept, for the totals), and a
-// final ascending pass that recomputes the prefix, forms both seeds and writes
-// the records. The operations and their order are those of the reference, so
-// the results are bit-identical to a single stored pass.
+// One 64-lane wave takes 64 pairs. Three passes over the columns, each with its
+// recurrence window in registers: the suffix (descending; its values go to the
+// interleaved scratch, entry j at nullOff + 64*j, so a wave's accesses are
+// coalesced), the prefix (ascending; only the last three values are kept, for
+// the totals), and a final ascending pass that recomputes the prefix, forms
+// both seeds and writes the records. The operations and their order are those
+// of the reference, so the results are bit-identical to a single stored pass.
+//
+// Memory: the column loops are wave-uniform (each lane predicated to its own
+// length), so the codes are staged through an LDS window that the wave fills
+// with coalesced loads (64 pairs x 40 columns at a time), and the records
+// are collected in LDS and written 8 columns (96 bytes; every pair's record
+// block starts on a multiple of 16 records, 64-byte aligned) per pair at a
+// time. Per-lane scattered byte loads and 12-byte stores would each touch a
+// different cache line per lane and thrash L2.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
-    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pi >= a.npairs) return;
-    const PairInfo P = a.pairs[pi];
-    const int n = P.n;
-    Rec *rc = a.rec + P.recOff;
-    if (n < 3) {
-        // The reference asserts n >= 3 (dummest-dp.hh:84). Zero records make the
-        // DP produce nothing; status reports the rejection.
-        for (int j = -kRPad; j < n + kRPad; j++) rc[j] = Rec{0.f, kZeroIdx | (kZeroIdx << 8), 0.f};
-        a.pairs[pi].status = 1;
-        a.pairs[pi].s1 = a.pairs[pi].s2 = a.pairs[pi].sC = 0.f;
-        return;
+constexpr int kNullWin = 32;                          // columns per code window step
+constexpr int kNullMargin = 4;                        // window margin on either side
+constexpr int kNullWinWords = (kNullWin + 2 * kNullMargin) / 4;  // 10 dwords of codes per pair
+constexpr int kNullWinStride = kNullWinWords + 1;     // odd stride: conflict-free per-lane reads
+constexpr int kNullFlush = 8;                         // records per pair per flush (96 bytes)
+constexpr int kNullRecWords = kNullFlush * 3;         // 24 dwords
+constexpr int kNullRecStride = kNullRecWords + 1;
+static_assert(sizeof(Rec) == 12, "record flush assumes 12-byte records");
+static_assert(kNullWin % kNullFlush == 0 && kRecBlock % kNullFlush == 0, "window and record blocks must nest");
+static_assert(kNullMargin >= 2 && kNullMargin % 4 == 0, "records read codes at j-2 and j+1");
+
+// About 10 KB per wave, so four waves share a SIMD.
+struct NullShared {
+    int64_t seqOff[64];
+    int n[64];  // -kNullNone for lanes without a pair
+    uint32_t win[64 * kNullWinStride];
+    uint32_t out[64 * kNullRecStride];
+};
+constexpr int kNullNone = 1 << 20;
+
+// Codes of columns [c0 - 4, c0 + 36) of every pair of the wave, kZeroIdx
+// outside [0, n). Each lane assembles one dword of four codes per iteration.
+// A column outside the sequence loads the kZeroIdx byte at zeroCode, so every
+// load is unconditional and all of them are in flight together.
+__device__ __forceinline__ void null_fill(NullShared &S, const uint8_t *codes, const uint8_t *zeroCode, int lane,
+                                          int c0) {
+    __syncthreads();
+    uint32_t v[kNullWinWords];
+#pragma unroll
+    for (int k = 0; k < kNullWinWords; k++) {
+        const int f = k * 64 + lane;
+        const int p = f / kNullWinWords, w = f - p * kNullWinWords;
+        const int np = S.n[p];
+        const uint8_t *cp = codes + S.seqOff[p];
+        const int col = c0 - kNullMargin + 4 * w;
+        v[k] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = col + q;
+            const uint8_t *src = (c >= 0 && c < np) ? cp + c : zeroCode;
+            v[k] |= (uint32_t)*src << (8 * q);
+        }
     }
-    const uint8_t *cd = a.codes + P.seqOff;
-    const float *bgp = a.bg + (int64_t)P.prof * 32;
-    ES *suf = a.scratch + P.nullOff;  // suf(j) = suf[64 * j]
-    auto code = [&](int j) -> int { return (j >= 0 && j < n) ? (int)cd[j] : kZeroIdx; };
+#pragma unroll
+    for (int k = 0; k < kNullWinWords; k++) {
+        const int f = k * 64 + lane;
+        const int p = f / kNullWinWords, w = f - p * kNullWinWords;
+        S.win[p * kNullWinStride + w] = v[k];
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
+    __shared__ NullShared S;
+    const int lane = threadIdx.x;
+    const int pi = blockIdx.x * 64 + lane;
+    const bool valid = pi < a.npairs;
+    PairInfo P{};
+    if (valid) P = a.pairs[pi];
+    const int n = valid ? P.n : -kNullNone;
+    // The reference asserts n >= 3 (dummest-dp.hh:84). Such pairs get records
+    // with zero seeds and zero-probability codes, so the DP produces nothing;
+    // status reports the rejection.
+    const bool ok = valid && n >= 3;
+    S.seqOff[lane] = valid ? P.seqOff : 0;
+    S.n[lane] = n;
+    const int64_t myRecOff = valid ? P.recOff : 0;
+    int N = ok ? n : 0;  // wave-uniform loop bound: the longest sequence of the wave
+    for (int off = 32; off > 0; off >>= 1) N = max(N, __shfl_xor(N, off));
+    int NR = valid ? n : -kRPad;  // record columns run to n + kRPad for short pairs too
+    for (int off = 32; off > 0; off >>= 1) NR = max(NR, __shfl_xor(NR, off));
+
+    const uint8_t *winb = reinterpret_cast<const uint8_t *>(S.win) + lane * 4 * kNullWinStride;
+    int c0 = 0;  // base column of the current window
+    auto code = [&](int j) -> int { return (int)winb[j - c0 + kNullMargin]; };
+
+    const float *bgp = a.bg + (int64_t)(valid ? P.prof : 0) * 32;
+    ES *suf = a.scratch + (valid ? P.nullOff : 0);  // suf(j) = suf[64 * j]
     const double single = (double)a.bgfs1, dbl = (double)a.bgfs2;
     const double codonKeep = 1 - single - dbl;
     const double singleEmit = single * 0.25;
@@ -176,26 +246,34 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
         int te;
         split_double(0.5 * codonKeep, tm, te);
         ES s1 = es_pow2(0), s2 = es_zero(), s3 = es_zero();
-        suf[(int64_t)64 * n] = s1;
-        for (int i = n - 1; i >= 0; i--) {
-            const float bgc = bgp[code(i)];
-            const double codonScale = (double)bgc * codonKeep;
-            const ES doubleIn = (i + 1 < n) ? s2 : es_pow2(2);
-            const ES codonRaw = es_bscale(s3, codonScale);
-            const ES sb = es_bscale(s1, singleEmit);
-            const ES db = es_bscale(doubleIn, doubleEmit);
-            const ES cb = (i + 2 < n) ? codonRaw : ES{tm, te - 2 * (n - i) + 1};
-            const ES v = es_badd(cb, es_badd(sb, db));
-            suf[(int64_t)64 * i] = v;
-            s3 = s2;
-            s2 = s1;
-            s1 = v;
+        if (ok) suf[(int64_t)64 * n] = s1;
+        for (int b = (N - 1) / kNullWin; N > 0 && b >= 0; b--) {
+            c0 = b * kNullWin;
+            null_fill(S, a.codes, a.zeroCode, lane, c0);
+            // the next column's background value is loaded one column ahead
+            float bgNext = bgp[code(c0 + kNullWin - 1)];
+            for (int i = c0 + kNullWin - 1; i >= c0; i--) {
+                const float bgc = bgNext;
+                bgNext = bgp[code(i - 1)];
+                if (!ok || i >= n) continue;
+                const double codonScale = (double)bgc * codonKeep;
+                const ES doubleIn = (i + 1 < n) ? s2 : es_pow2(2);
+                const ES codonRaw = es_bscale(s3, codonScale);
+                const ES sb = es_bscale(s1, singleEmit);
+                const ES db = es_bscale(doubleIn, doubleEmit);
+                const ES cb = (i + 2 < n) ? codonRaw : ES{tm, te - 2 * (n - i) + 1};
+                const ES v = es_badd(cb, es_badd(sb, db));
+                suf[(int64_t)64 * i] = v;
+                s3 = s2;
+                s2 = s1;
+                s1 = v;
+            }
         }
     }
     // buildNullPrefixFirstCodon, dummest-dp-backward.hh:58-100, as a function of
     // the window (p1, p2, p3) = prefix(i-1..i-3)
-    auto prefixStep = [&](int i, const ES &p1, const ES &p2, const ES &p3) -> ES {
-        const float bgc = bgp[code(i - 2)];
+    // (bgc = background value of code(i - 2)).
+    auto prefixStep = [&](int i, float bgc, const ES &p1, const ES &p2, const ES &p3) -> ES {
         const double codonScale = (double)bgc * codonKeep;
         const ES ps = (i > 0) ? p1 : es_zero();
         const ES pd = (i >= 2) ? p2 : es_zero();
@@ -207,15 +285,23 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
         return es_badd(cb, es_badd(es_bscale(ps, singleEmit), es_bscale(pd, doubleEmit)));
     };
     ES p1 = es_zero(), p2 = es_zero(), p3 = es_zero();
-    for (int i = 0; i < n; i++) {
-        const ES v = prefixStep(i, p1, p2, p3);
-        p3 = p2;
-        p2 = p1;
-        p1 = v;
+    for (int b = 0; b * kNullWin < N; b++) {
+        c0 = b * kNullWin;
+        null_fill(S, a.codes, a.zeroCode, lane, c0);
+        float bgNext = bgp[code(c0 - 2)];
+        for (int i = c0; i < c0 + kNullWin; i++) {
+            const float bgc = bgNext;
+            bgNext = bgp[code(i - 1)];
+            if (!ok || i >= n) continue;
+            const ES v = prefixStep(i, bgc, p1, p2, p3);
+            p3 = p2;
+            p2 = p1;
+            p1 = v;
+        }
     }
     // buildNullTotalAndScales, dummest-dp-backward.hh:103-129 (pre[n-1..n-3] = p1..p3)
-    ES perBase, perBaseInv, s1, s2, sC;
-    {
+    ES perBase = es_zero(), perBaseInv = es_zero(), s1 = es_zero(), s2 = es_zero(), sC = es_zero();
+    if (ok) {
         const double keepSingle = 1.0 - single;
         const double keepBoth = keepSingle - dbl;
         const ES ts = es_scale(p2, keepSingle * 0.25);
@@ -235,27 +321,65 @@ __global__ void __launch_bounds__(64) null_kernel(NullArgs a) {
     // column j holds the seeds and the codon codes at j+1 and j-2.
     {
         ES runS = es_pow2(0);
-        for (int t = 0; t < n - 1; t++) runS = es_bmul(runS, perBase);
+        for (int t = 0; ok && t < n - 1; t++) runS = es_bmul(runS, perBase);
         ES runP = perBase;
-        for (int j = -kRPad; j < 0; j++) rc[j] = Rec{0.f, (uint32_t)(code(j + 1) | (code(j - 2) << 8)), 0.f};
         ES q1 = es_zero(), q2 = es_zero(), q3 = es_zero();
-        for (int j = 0; j < n; j++) {
-            const ES pre = prefixStep(j, q1, q2, q3);
-            q3 = q2;
-            q2 = q1;
-            q1 = pre;
-            const float ssd = es_to_float_clamped(es_bmul(suf[(int64_t)64 * (j + 1)], runS));
-            runS = es_bmul(runS, perBaseInv);
-            const float psd = es_to_float_clamped(es_bmul(pre, runP));
-            runP = es_bmul(runP, perBase);
-            rc[j] = Rec{psd, (uint32_t)(code(j + 1) | (code(j - 2) << 8)), ssd};
+        uint32_t *rec = reinterpret_cast<uint32_t *>(a.rec);
+        // suf(j + 1) and the background value of code(j - 2) are loaded one
+        // column ahead; the suffix index stays inside [0, n] of the lane's pair
+        auto sufIdx = [&](int j) -> int64_t { return ok ? (int64_t)64 * min(max(j, 0), n) : 0; };
+        ES sufNext = suf[sufIdx(-kRPad + 1)];
+        float bgNext = 0.f;
+        for (int jb = -kRPad; jb < NR + kRPad; jb += kNullFlush) {
+            if (((jb + kRPad) % kNullWin) == 0) {
+                c0 = jb;
+                null_fill(S, a.codes, a.zeroCode, lane, c0);
+                bgNext = bgp[code(c0 - 2)];
+            }
+            for (int u = 0; u < kNullFlush; u++) {
+                const int j = jb + u;
+                if (j >= n + kRPad) break;
+                const float bgc = bgNext;
+                bgNext = bgp[code(j - 1)];
+                const ES sufj = sufNext;
+                sufNext = suf[sufIdx(j + 2)];
+                const uint32_t cds = ok ? (uint32_t)(code(j + 1) | (code(j - 2) << 8)) : (uint32_t)(kZeroIdx | (kZeroIdx << 8));
+                float psd = 0.f, ssd = 0.f;
+                if (ok && j >= 0 && j < n) {
+                    const ES pre = prefixStep(j, bgc, q1, q2, q3);
+                    q3 = q2;
+                    q2 = q1;
+                    q1 = pre;
+                    ssd = es_to_float_clamped(es_bmul(sufj, runS));
+                    runS = es_bmul(runS, perBaseInv);
+                    psd = es_to_float_clamped(es_bmul(pre, runP));
+                    runP = es_bmul(runP, perBase);
+                }
+                uint32_t *o = S.out + lane * kNullRecStride + 3 * u;
+                o[0] = __float_as_uint(psd);
+                o[1] = cds;
+                o[2] = __float_as_uint(ssd);
+            }
+            __syncthreads();
+            // 64 pairs x 24 dwords, 64 lanes per store: each store covers
+            // contiguous 96-byte record blocks of two or three pairs
+#pragma unroll 4
+            for (int k = 0; k < kNullRecWords; k++) {
+                const int f = k * 64 + lane;
+                const int p = f / kNullRecWords, d = f - p * kNullRecWords;
+                const int col = jb + d / 3;
+                const int64_t ro = __shfl(myRecOff, p);
+                if (col < S.n[p] + kRPad) rec[(ro + col) * 3 + d % 3] = S.out[p * kNullRecStride + d];
+            }
+            __syncthreads();
         }
-        for (int j = n; j < n + kRPad; j++) rc[j] = Rec{0.f, (uint32_t)(code(j + 1) | (code(j - 2) << 8)), 0.f};
     }
-    a.pairs[pi].s1 = es_to_float_clamped(s1);
-    a.pairs[pi].s2 = es_to_float_clamped(s2);
-    a.pairs[pi].sC = es_to_float_clamped(sC);
-    a.pairs[pi].status = 0;
+    if (valid) {
+        a.pairs[pi].s1 = es_to_float_clamped(s1);
+        a.pairs[pi].s2 = es_to_float_clamped(s2);
+        a.pairs[pi].sC = es_to_float_clamped(sC);
+        a.pairs[pi].status = ok ? 0 : 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -382,8 +506,12 @@ constexpr int kPre = 100;               // bulk-mode prescale of stored W1 (2^-1
 // step that produced it; index c+1 for columns c in [-1, n], zero outside [0, n).
 __device__ __forceinline__ int bidx(int c, int n) { return c < -1 ? 0 : (c > n ? n + 1 : c + 1); }
 
+// Two waves per SIMD for the R = 8 single-strip kernels: they fit 256
+// registers without spilling, and the second wave hides latency (about 8-13%
+// faster per step than the unconstrained allocation, which takes one wave).
 template <int G, int R, int MODE, bool STR>
-__global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1)))
+dp_kernel(DPArgs a) {
     constexpr int ROWS = G * R;
     constexpr int kPF = (R <= 8) ? 4 : 2;  // prefetch distance in steps (divides 4)
     extern __shared__ float etab[];  // [kNCode][ROWS], then the profile's background table (32)
@@ -655,9 +783,15 @@ __global__ void __launch_bounds__(64) dp_kernel(DPArgs a) {
 #pragma unroll
                 for (int q = 0; q < R; q++) eq[q] = eqN[q];
                 gq = gqN;
-                if (MODE != kFilter && a.wantBwd) {  // backward start score: max W1 * prefixSeed
+                if (MODE != kFilter && a.wantBwd) {
+                    // backward start score: max W1 * prefixSeed. The seed is
+                    // shared by the lane's rows and non-negative, and a product
+                    // of two floats is exact in double, so the max of the
+                    // products is the product of the float max.
+                    float hm = H[S4][0];
 #pragma unroll
-                    for (int q = 0; q < R; q++) bwdMax.cur = fmax(bwdMax.cur, (double)H[S4][q] * pspD);
+                    for (int q = 1; q < R; q++) hm = fmaxf(hm, H[S4][q]);
+                    bwdMax.cur = fmax(bwdMax.cur, (double)hm * pspD);
                 }
                 if (STR && above && k == 0 && c >= 0 && c < n) {
                     wOut[c + 1] = H[S4][0];
