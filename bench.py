@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-profiles", type=int, default=500,
                     help="profiles in the calibration sweep reported beside the main figure (0: skip)")
+    ap.add_argument("--mask-reads", type=int, default=100000,
+                    help="reads masked with tantan on the device in the sequence-masking figure (0: skip)")
     ap.add_argument("--tune-file", default="",
                     help="load the autotuned cost table from this file if it exists, else tune and save it there")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -207,6 +209,29 @@ def main():
         log(f"calibration sweep: {len(cp)} profiles x 200 trials of 3.1 kb: {e.last_dp_ms:.1f} ms on the device, "
             f"{wall:.2f} s in all")
 
+    # sequence masking (-m 2): tantan repeat probabilities of the reads on the
+    # device, against the oracle's scalar restatement on a sample
+    masking = None
+    if args.mask_reads > 0:
+        reads = [r.tobytes().decode() for r in w["reads"][:args.mask_reads]]
+        letters = sum(len(r) for r in reads)
+        e.tantan_dna(reads[:256])  # warm-up
+        t = time.perf_counter()
+        e.tantan_dna(reads)
+        wall = time.perf_counter() - t
+        dev = e.last_tantan_ms
+        masking = {"reads": len(reads), "letters": letters, "device_ms": round(dev, 3), "wall_s": round(wall, 3),
+                   "letters_per_s": float(f"{letters / (dev * 1e-3):.4g}")}
+        if not args.no_cpu:
+            sample = reads[:2000]
+            t = time.perf_counter()
+            for r in sample:
+                O.mask_dna(r, 0.5)
+            cs = time.perf_counter() - t
+            masking["cpu_port_letters_per_s"] = float(f"{sum(len(r) for r in sample) / cs:.4g}")
+            masking["cpu_sample"] = f"{len(sample)} reads, oracle tantan, 1 thread"
+        log(f"sequence masking: {letters} letters in {dev:.2f} ms on the device ({wall:.2f} s in all)")
+
     cpu = None
     if not args.no_cpu:
         w["oracle_profiles"] = {}
@@ -242,6 +267,7 @@ def main():
         "batched_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "build_id": D.build_id(),
         "calibration_sweep": calib,
+        "sequence_masking": masking,
     }
     print(json.dumps(out), flush=True)
     plan.close()

@@ -62,7 +62,8 @@ def _declare(L):
     L.dm_engine_error.restype = ctypes.c_char_p
     L.dm_set_rates.argtypes = [vp, _P(d)]
     L.dm_read_hmm.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _P(i32), _P(i32)]
-    L.dm_parse_hmm_text.argtypes = [ctypes.c_char_p, ctypes.c_int, _P(d), i32, _P(i32), _P(f), _P(f), _P(f), _P(i32)]
+    L.dm_parse_hmm_text.argtypes = [ctypes.c_char_p, ctypes.c_int, _P(d), d, i32, _P(i32), _P(f), _P(f), _P(f),
+                                    _P(i32), _P(i32)]
     L.dm_add_raw_profiles.argtypes = [vp, i32, _P(i32), _P(f), ctypes.c_int, _P(i32)]
     L.dm_add_profile.argtypes = [vp, i32, _P(f), _P(f), _P(f), _P(i32)]
     L.dm_get_profile.argtypes = [vp, i32, _P(i32), _P(f), _P(f), _P(f)]
@@ -70,6 +71,10 @@ def _declare(L):
     L.dm_decode.argtypes = [ctypes.c_char_p, i64, _P(u8)]
     L.dm_add_sequences.argtypes = [vp, i32, _P(u8), _P(i64), _P(i32), _P(i32)]
     L.dm_clear_sequences.argtypes = [vp]
+    L.dm_set_masking.argtypes = [vp, ctypes.c_int, d]
+    L.dm_profile_masked.argtypes = [vp, i32]
+    L.dm_tantan_dna.argtypes = [vp, i32, ctypes.c_char_p, _P(i64), _P(i32), _P(f)]
+    L.dm_add_dna.argtypes = [vp, i32, ctypes.c_char_p, _P(i64), _P(i32), _P(i32)]
     L.dm_filter_scores.argtypes = [vp, i64, _P(i32), _P(i32), _P(f)]
     L.dm_fwdbwd_scores.argtypes = [vp, i64, _P(i32), _P(i32), ctypes.c_int, _P(f), _P(i32), _P(f), _P(i32)]
     L.dm_pair_matrices.argtypes = [vp, i32, i32, _P(f), _P(i32), _P(f), _P(i32)]
@@ -95,6 +100,8 @@ def _declare(L):
     L.dm_gumbel_fits.argtypes = [_P(d), i32, d, _P(d)]
     L.dm_last_dp_ms.argtypes = [vp]
     L.dm_last_dp_ms.restype = d
+    L.dm_last_tantan_ms.argtypes = [vp]
+    L.dm_last_tantan_ms.restype = d
     L.dm_last_cells.argtypes = [vp]
     L.dm_last_cells.restype = i64
 
@@ -113,14 +120,18 @@ def decode(dna) -> np.ndarray:
     return out
 
 
-def parse_hmm_text(text: str, index: int = 0, bg_type: str = "G", rates=DEFAULT_RATES):
+def parse_hmm_text(text: str, index: int = 0, bg_type: str = "G", rates=DEFAULT_RATES, mask_threshold=None,
+                   with_masked=False):
     """Host-side readProfiles + finalizeProfile of one profile of an HMMER3/f text
-    (no device needed): returns (M, values, params, bg, count)."""
+    (no device needed): returns (M, values, params, bg, count), plus the number
+    of masked positions when with_masked. mask_threshold turns on profile
+    masking (-m 1)."""
     r = np.array(rates, np.float64)
-    M, cnt = ctypes.c_int32(), ctypes.c_int32()
+    M, cnt, nm = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     t = text.encode()
-    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), index, ctypes.byref(M), None, None,
-                                 None, ctypes.byref(cnt))
+    thr = -1.0 if mask_threshold is None else float(mask_threshold)
+    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), thr, index, ctypes.byref(M), None, None,
+                                 None, ctypes.byref(cnt), None)
     if rc:
         raise DummestError(f"dm_parse_hmm_text failed ({rc})")
     m = M.value
@@ -128,10 +139,12 @@ def parse_hmm_text(text: str, index: int = 0, bg_type: str = "G", rates=DEFAULT_
     p = np.zeros((m + 2, 11), np.float32)
     b = np.zeros(WIDTH, np.float32)
     f = ctypes.c_float
-    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), index, ctypes.byref(M), _ptr(v, f),
-                                 _ptr(p, f), _ptr(b, f), ctypes.byref(cnt))
+    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), thr, index, ctypes.byref(M), _ptr(v, f),
+                                 _ptr(p, f), _ptr(b, f), ctypes.byref(cnt), ctypes.byref(nm))
     if rc:
         raise DummestError(f"dm_parse_hmm_text failed ({rc})")
+    if with_masked:
+        return m, v, p, b, cnt.value, nm.value
     return m, v, p, b, cnt.value
 
 
@@ -303,6 +316,41 @@ class Engine:
     def clear_sequences(self):
         self._check(lib().dm_clear_sequences(self._h), "dm_clear_sequences")
 
+    # -- tantan masking ------------------------------------------------------
+    def set_masking(self, mask: int, threshold: float = 0.5):
+        """-m (1 profiles, 2 sequences, 3 both) and --tantan-threshold."""
+        self._check(lib().dm_set_masking(self._h, int(mask), float(threshold)), "dm_set_masking")
+
+    def profile_masked(self, pid: int) -> int:
+        """Positions of profile pid masked by tantan."""
+        return int(lib().dm_profile_masked(self._h, pid))
+
+    @staticmethod
+    def _dna_batch(seqs):
+        bs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+        lens = np.array([len(b) for b in bs], np.int32)
+        offs = np.zeros(len(bs), np.int64)
+        if len(bs) > 1:
+            offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+        return b"".join(bs) or b"\0", offs, lens
+
+    def tantan_dna(self, seqs) -> list:
+        """tantan repeat probabilities of DNA letters, computed on the device."""
+        data, offs, lens = self._dna_batch(seqs)
+        out = np.zeros(max(1, int(lens.sum())), np.float32)
+        self._check(lib().dm_tantan_dna(self._h, len(seqs), data, _ptr(offs, ctypes.c_int64),
+                                        _ptr(lens, ctypes.c_int32), _ptr(out, ctypes.c_float)), "dm_tantan_dna")
+        return [out[o:o + n] for o, n in zip(offs, lens)]
+
+    def add_dna(self, seqs) -> list:
+        """Append DNA given as letters (masked on the device under mask bit 2,
+        then decoded); returns their ids."""
+        data, offs, lens = self._dna_batch(seqs)
+        first = ctypes.c_int32()
+        self._check(lib().dm_add_dna(self._h, len(seqs), data, _ptr(offs, ctypes.c_int64),
+                                     _ptr(lens, ctypes.c_int32), ctypes.byref(first)), "dm_add_dna")
+        return list(range(first.value, first.value + len(seqs)))
+
     # -- scoring ------------------------------------------------------------
     @staticmethod
     def _pairs(prof, seq):
@@ -443,6 +491,10 @@ class Engine:
     @property
     def last_dp_ms(self) -> float:
         return lib().dm_last_dp_ms(self._h)
+
+    @property
+    def last_tantan_ms(self) -> float:
+        return lib().dm_last_tantan_ms(self._h)
 
     @property
     def last_cells(self) -> int:

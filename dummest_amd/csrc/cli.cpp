@@ -15,10 +15,14 @@
 //      (dm_pair_hits), and the records (dummest-search.hh:89-105,
 //      dummest-output.hh:54-107).
 //
+// tantan masking (-m, default 3): profiles are masked on the host when they are
+// read (finalizeProfile, dummest-profile.hh:105-233), sequences per chunk on
+// the GPU (dm_tantan_dna) before they are decoded (dummest.cc:557-573); masked
+// letters print in lowercase (dummest-output.hh:6-30).
+//
 // Not available here (the command fails with a message rather than giving
-// different results): tantan masking (-m 1/2/3; -m 0 is required) and the
-// Gaussian profile filter (-d/-D). There is no calibration cache: every run
-// calibrates (on the GPU).
+// different results): the Gaussian profile filter (-d/-D). There is no
+// calibration cache: every run calibrates (on the GPU).
 #include <getopt.h>
 #include <unistd.h>
 
@@ -200,8 +204,19 @@ struct Target {
     SeqState rec;   // record state when the contig was read
     Contig contig;
     std::string letters;  // alphabet letters (uppercase) of the contig
+    std::vector<uint8_t> nums;  // the reader's letter numbers (alphabet indices, 22 = IUPAC)
+    std::vector<char> masked;   // tantan-masked positions (-m 2)
     std::vector<uint8_t> codes;
 };
+
+// Aligned-row letters (dummest-output.hh:6-30): masked profile positions carry
+// bit 32 of the consensus code; a masked sequence letter prints in lowercase
+// when the mask code (22) is above its own code.
+char profLetter(char code) { return (char)(kAlphabet[code & 31] + (code & 32)); }
+char seqLetter(const Target &t, int pos) {
+    const bool low = !t.masked.empty() && t.masked[pos] && t.nums[pos] < 22;
+    return (char)(t.letters[pos] + (low ? 32 : 0));
+}
 
 struct Options {
     double evalue = 10;
@@ -390,16 +405,13 @@ int main(int argc, char *argv[]) {
         return 1;
     }
     if (filterStdDev > 0) return die("the Gaussian profile filter (-d/-D) is not available in this build");
-    if (o.mask != 0)
-        return die("tantan masking (-m 1/2/3; the reference's default is 3) is not available in this build: "
-                   "pass -m 0");
     if (o.border > INT32_MAX - 64 - o.length) return die("sequence + border is too big");
 
     // ------------------------------------------------------------- profiles
     std::vector<HostProfile> profiles;
     {
         std::string msg;
-        if (!readHmmFile(argv[optind], o.bgType, o.rates, profiles, msg))
+        if (!readHmmFile(argv[optind], o.bgType, o.rates, profiles, msg, (o.mask & 1) ? (float)o.tantan : -1.f))
             return die(msg.empty() ? "can't read the profile data" : msg);
     }
     dm_engine *E = nullptr;
@@ -427,6 +439,8 @@ int main(int argc, char *argv[]) {
               << " tantan-threshold=" << o.tantan << "\n";
     if (o.maxMode) std::cout << "# Max mode: filter pass disabled\n";
     else if (nargs > 1) std::cout << "# Filter E-value <= " << o.filterEvalue << "\n";
+    if (o.mask & 1) std::cout << "# Masking simple regions in profiles\n";
+    if (nargs > 1 && (o.mask & 2)) std::cout << "# Masking simple regions in sequences\n";
     if (nargs > 1 && o.evalue > 0) std::cout << "# E-value <= " << o.evalue << "\n";
 
     // ----------------------------------------------------------- calibration
@@ -462,6 +476,7 @@ int main(int argc, char *argv[]) {
         Kf[i] = f[9];
         lambdaF[i] = f[8];
         std::cout << "\n# Profile name: " << p.name << "\n# Profile length: " << p.M << "\n";
+        if (o.mask & 1) std::cout << "# Positions masked by tantan: " << p.masked << "\n";
         std::cout << "# Background letter probabilities:";
         for (int j = 0; j < 20; j++) std::cout << " " << p.bg[4 + j];
         std::cout << std::endl;
@@ -562,6 +577,34 @@ int main(int argc, char *argv[]) {
 
     auto flush = [&]() -> int {
         if (targets.empty()) return 0;
+        // -m 2: tantan on the chunk's contigs on the GPU; letters at or above
+        // the threshold (a Float, as TANTAN_MASK_THRESHOLD) decode as non-ACGT
+        if (o.mask & 2) {
+            std::string all;
+            std::vector<int64_t> toff;
+            std::vector<int32_t> tlen;
+            for (const Target &t : targets) {
+                toff.push_back((int64_t)all.size());
+                tlen.push_back((int32_t)t.letters.size());
+                all += t.letters;
+            }
+            std::vector<float> probs(std::max<size_t>(1, all.size()));
+            int r = dm_tantan_dna(E, (int32_t)targets.size(), all.data(), toff.data(), tlen.data(), probs.data());
+            if (r) return engineFail(E, "dm_tantan_dna", r);
+            const float thr = (float)o.tantan;
+            for (size_t k = 0; k < targets.size(); k++) {
+                Target &t = targets[k];
+                t.masked.assign(t.letters.size(), 0);
+                for (size_t j = 0; j < t.letters.size(); j++) t.masked[j] = !(probs[toff[k] + j] < thr);
+            }
+        }
+        for (Target &t : targets) {
+            std::string dec = t.letters;
+            for (size_t j = 0; j < t.masked.size(); j++)
+                if (t.masked[j]) dec[j] = 'N';
+            t.codes.resize(dec.size());
+            decodeDna(dec.data(), (int64_t)dec.size(), t.codes.data());
+        }
         std::vector<uint8_t> codes;
         std::vector<int64_t> offs;
         std::vector<int32_t> lens;
@@ -652,9 +695,9 @@ int main(int argc, char *argv[]) {
                     int pos1 = sg[0], pos2 = sg[1];
                     for (int g = 0; g < nseg; g++) {
                         const int s1 = sg[3 * g], s2 = sg[3 * g + 1], ln = sg[3 * g + 2];
-                        for (; pos1 < s1; ++pos1) row1.push_back(kAlphabet[P.consensus[pos1] & 31]);
+                        for (; pos1 < s1; ++pos1) row1.push_back(profLetter(P.consensus[pos1]));
                         row1.append(s2 - pos2, '-');
-                        for (; pos1 < s1 + ln; ++pos1) row1.push_back(kAlphabet[P.consensus[pos1] & 31]);
+                        for (; pos1 < s1 + ln; ++pos1) row1.push_back(profLetter(P.consensus[pos1]));
                         pos2 = s2 + ln;
                     }
                     pos1 = sg[0];
@@ -662,7 +705,7 @@ int main(int argc, char *argv[]) {
                     for (int g = 0; g < nseg; g++) {
                         const int s1 = sg[3 * g], s2 = sg[3 * g + 1], ln = sg[3 * g + 2];
                         row2.append(s1 - pos1, '-');
-                        for (; pos2 < s2 + ln; ++pos2) row2.push_back(T.letters[pos2]);
+                        for (; pos2 < s2 + ln; ++pos2) row2.push_back(seqLetter(T, pos2));
                         pos1 = s1 + ln;
                     }
                     const double ev = K[pi] * (double)totLen * std::exp(-lambda[pi] * lnr);
@@ -713,9 +756,11 @@ int main(int argc, char *argv[]) {
             T.rec = s;
             T.contig = ct;
             T.letters.resize(letters.size());
-            for (size_t j = 0; j < letters.size(); j++) T.letters[j] = kAlphabet[(int)letters[j]];
-            T.codes.resize(letters.size());
-            decodeDna(T.letters.data(), (int64_t)T.letters.size(), T.codes.data());
+            T.nums.resize(letters.size());
+            for (size_t j = 0; j < letters.size(); j++) {
+                T.letters[j] = kAlphabet[(int)letters[j]];
+                T.nums[j] = (uint8_t)letters[j];
+            }
             const int ti = (int)targets.size();
             for (int j = 0; j < np; j++) {
                 if (!s.targetProfile.empty() && profiles[j].name != s.targetProfile) continue;

@@ -19,16 +19,32 @@ struct HostProfile {
     std::vector<float> params;  // (M+2) x 11
     std::vector<float> bg;      // 29
     std::vector<char> consensus;  // letter index per position (dummest-profile.hh:310-312)
+    int masked = 0;               // positions masked by tantan (profile masking)
 };
 
-// readProfiles + finalizeProfile (dummest-profile.hh:105-329), -m 0.
+// readProfiles + finalizeProfile (dummest-profile.hh:105-329). maskThr >= 0
+// turns on profile masking (-m 1) at that tantan threshold; negative = off.
 bool readHmmFile(const char *path, int bgType, const double rates[6], std::vector<HostProfile> &out,
-                 std::string &msg);
+                 std::string &msg, float maskThr = -1.f);
 bool readHmmText(const std::string &text, int bgType, const double rates[6], std::vector<HostProfile> &out,
-                 std::string &msg);
+                 std::string &msg, float maskThr = -1.f);
 
 // finalizeProfile on values as read (probabilities, (M+1) x 29, reader order).
-bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg);
+bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr = -1.f);
+
+// tantan as DUMMEST calls it (tantan-wrapper.hh:13-64, gap-free HMM): the
+// likelihood ratios (BLOSUM62 at lambda 0.324 for proteins, +1/-1 for DNA
+// letters 0..3), the repeat-offset limit K (50 / 100) and the transition
+// probabilities.
+struct TantanParams {
+    int K;
+    double b2b, f2b, f2f0;
+    double b2f[100];
+    double ratio[22][22];
+};
+TantanParams tantanParams(bool isProtein);
+// Posterior repeat probability of every letter (small-number alphabet).
+void tantanRepeatProbs(const uint8_t *seq, int n, bool isProtein, float *out);
 
 // decodeSequence (dummest-sequence.hh:181-204) on raw DNA letters.
 void decodeDna(const char *dna, int64_t n, uint8_t *codes);

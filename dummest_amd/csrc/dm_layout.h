@@ -120,6 +120,23 @@ int dp_strip_configs(int *gs, int *rs, int cap);
 hipError_t launch_null(const NullArgs &a, hipStream_t s);
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+
+// tantan on DNA (sequence masking): bases 0..3 per letter, sequences at
+// offsets; probs in the same layout; scales: lengths/16 doubles per sequence at
+// scaleOff. Sixteen sequences per 64-lane wave, four lanes per sequence.
+constexpr int kTantanK = 100;  // repeat-offset limit for DNA (tantan-wrapper.hh:58)
+struct TantanArgs {
+    const uint8_t *bases;
+    const int64_t *offsets;
+    const int32_t *lengths;
+    const int64_t *scaleOff;
+    float *probs;
+    double *scales;
+    const double *b2f;  // [kTantanK]
+    double b2b, f2b, f2f0;
+    int nseq;
+};
+hipError_t launch_tantan_dna(const TantanArgs &a, hipStream_t s);
 hipError_t launch_copy_probe(const float *src, float *dst, int64_t n16, hipStream_t s);
 #endif
 

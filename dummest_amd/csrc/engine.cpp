@@ -80,6 +80,12 @@ struct dm_engine {
     int64_t lastCells = 0;
     std::vector<double> cfgMs;
     std::vector<int64_t> cfgCells, cfgLaunches;
+    double lastTantanMs = 0;  // device time of the last dm_tantan_dna kernel
+    int maskBits = 0;       // -m: 1 = profiles, 2 = sequences
+    float maskThr = 0.5f;   // --tantan-threshold (TANTAN_MASK_THRESHOLD, a Float)
+    DevBuf dTanBases, dTanProbs, dTanScales, dTanMeta, dTanB2f;
+
+    float profileMaskThr() const { return (maskBits & 1) ? maskThr : -1.f; }
 
     float bgfs1() const { return (float)rates[0] + (float)rates[3]; }  // INSERT1 + DELETE2 (Float)
     float bgfs2() const { return (float)rates[1] + (float)rates[2]; }  // INSERT2 + DELETE1 (Float)
@@ -770,7 +776,7 @@ int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, 
     if (!e || !path) return DM_EINVAL;
     std::vector<HostProfile> ps;
     std::string msg;
-    if (!readHmmFile(path, bg_type, e->rates, ps, msg)) return fail(e, DM_EFORMAT, msg);
+    if (!readHmmFile(path, bg_type, e->rates, ps, msg, e->profileMaskThr())) return fail(e, DM_EFORMAT, msg);
     if (first_id) *first_id = (int32_t)e->profiles.size();
     if (count) *count = (int32_t)ps.size();
     for (auto &p : ps) e->profiles.push_back(std::move(p));
@@ -778,12 +784,13 @@ int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, 
     return DM_OK;
 }
 
-int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], int32_t index, int32_t *M, float *values,
-                      float *params, float *bg, int32_t *count) {
-    if (!text || !rates) return DM_EINVAL;
+int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], double mask_threshold, int32_t index,
+                      int32_t *M, float *values, float *params, float *bg, int32_t *count, int32_t *masked) {
+    if (!text || !rates || mask_threshold > 1) return DM_EINVAL;
     std::vector<HostProfile> ps;
     std::string msg;
-    if (!readHmmText(text, bg_type, rates, ps, msg)) return DM_EFORMAT;
+    if (!readHmmText(text, bg_type, rates, ps, msg, mask_threshold >= 0 ? (float)mask_threshold : -1.f))
+        return DM_EFORMAT;
     if (count) *count = (int32_t)ps.size();
     if (index < 0 || index >= (int)ps.size()) return ps.empty() ? DM_EFORMAT : DM_OK;
     const HostProfile &p = ps[index];
@@ -791,6 +798,7 @@ int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], int3
     if (values) std::memcpy(values, p.values.data(), sizeof(float) * p.values.size());
     if (params) std::memcpy(params, p.params.data(), sizeof(float) * p.params.size());
     if (bg) std::memcpy(bg, p.bg.data(), sizeof(float) * p.bg.size());
+    if (masked) *masked = p.masked;
     return DM_OK;
 }
 
@@ -812,7 +820,7 @@ int dm_add_raw_profiles(dm_engine *e, int32_t nprof, const int32_t *M, const flo
             p.consensus[i] = (char)(std::max_element(r, r + 20) - r);
         }
         std::string msg;
-        if (!finalizeRaw(p, bg_type, e->rates, msg)) return fail(e, DM_EFORMAT, msg);
+        if (!finalizeRaw(p, bg_type, e->rates, msg, e->profileMaskThr())) return fail(e, DM_EFORMAT, msg);
         e->profiles.push_back(std::move(p));
     }
     e->profDirty = true;
@@ -850,6 +858,126 @@ int dm_decode(const char *dna, int64_t n, uint8_t *codes) {
     if (n < 0 || (n > 0 && (!dna || !codes))) return DM_EINVAL;
     decodeDna(dna, n, codes);
     return DM_OK;
+}
+
+int dm_set_masking(dm_engine *e, int mask, double threshold) {
+    if (!e || mask < 0 || mask > 3 || !(threshold >= 0 && threshold <= 1)) return DM_EINVAL;
+    e->maskBits = mask;
+    e->maskThr = (float)threshold;
+    return DM_OK;
+}
+
+int dm_profile_masked(const dm_engine *e, int32_t id) {
+    if (!e || id < 0 || id >= (int)e->profiles.size()) return -1;
+    return e->profiles[id].masked;
+}
+
+// tantan's letter numbers for DNA (makeMaskedSequence, dummest-sequence.hh:206-230):
+// A, C, G, T (either case) are 0..3, every other letter counts as A.
+static inline uint8_t tantanLetter(char c) {
+    switch (c) {
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 0;
+    }
+}
+
+int dm_tantan_dna(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+                  float *probs) {
+    if (!e || nseq < 0 || (nseq > 0 && (!dna || !offsets || !lengths || !probs))) return DM_EINVAL;
+    HIPCHK(hipSetDevice(e->device));
+    // sequences of a wave run in lockstep: order them by length, longest first
+    std::vector<int32_t> ord(nseq);
+    int64_t total = 0, nscale = 0;
+    for (int32_t i = 0; i < nseq; i++) {
+        if (lengths[i] < 0 || offsets[i] < 0) return fail(e, DM_EINVAL, "negative sequence length or offset");
+        ord[i] = i;
+        total += lengths[i];
+        nscale += lengths[i] / 16;
+    }
+    if (total == 0) return DM_OK;
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return lengths[a] > lengths[b]; });
+    // packed device layout: sequence k (in length order) at pOff[k]
+    std::vector<int64_t> meta(3 * (size_t)nseq);  // offsets, scale offsets, lengths (as int64 pairs)
+    std::vector<uint8_t> bases(total);
+    int64_t at = 0, sat = 0;
+    std::vector<int32_t> len32(nseq);
+    for (int32_t k = 0; k < nseq; k++) {
+        const int32_t i = ord[k];
+        meta[k] = at;
+        meta[nseq + k] = sat;
+        len32[k] = lengths[i];
+        for (int32_t j = 0; j < lengths[i]; j++) bases[at + j] = tantanLetter(dna[offsets[i] + j]);
+        at += lengths[i];
+        sat += lengths[i] / 16;
+    }
+    const TantanParams tp = tantanParams(false);
+    HIPCHK(e->dTanBases.ensure(total));
+    HIPCHK(e->dTanProbs.ensure(sizeof(float) * total));
+    HIPCHK(e->dTanScales.ensure(sizeof(double) * std::max<int64_t>(1, nscale)));
+    HIPCHK(e->dTanMeta.ensure(sizeof(int64_t) * 2 * nseq + sizeof(int32_t) * nseq));
+    HIPCHK(e->dTanB2f.ensure(sizeof(double) * kTantanK));
+    HIPCHK(hipMemcpyAsync(e->dTanBases.p, bases.data(), total, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->dTanMeta.p, meta.data(), sizeof(int64_t) * 2 * nseq, hipMemcpyHostToDevice, e->stream));
+    int32_t *dLen = (int32_t *)((int64_t *)e->dTanMeta.p + 2 * nseq);
+    HIPCHK(hipMemcpyAsync(dLen, len32.data(), sizeof(int32_t) * nseq, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->dTanB2f.p, tp.b2f, sizeof(double) * kTantanK, hipMemcpyHostToDevice, e->stream));
+    TantanArgs ta{};
+    ta.bases = e->dTanBases.as<uint8_t>();
+    ta.offsets = e->dTanMeta.as<int64_t>();
+    ta.scaleOff = e->dTanMeta.as<int64_t>() + nseq;
+    ta.lengths = dLen;
+    ta.probs = e->dTanProbs.as<float>();
+    ta.scales = e->dTanScales.as<double>();
+    ta.b2f = e->dTanB2f.as<double>();
+    ta.b2b = tp.b2b;
+    ta.f2b = tp.f2b;
+    ta.f2f0 = tp.f2f0;
+    ta.nseq = nseq;
+    if (!e->ev0) HIPCHK(hipEventCreate(&e->ev0));
+    if (!e->ev1) HIPCHK(hipEventCreate(&e->ev1));
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    HIPCHK(launch_tantan_dna(ta, e->stream));
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    std::vector<float> packed(total);
+    HIPCHK(hipMemcpyAsync(packed.data(), e->dTanProbs.p, sizeof(float) * total, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    float tms = 0.f;
+    HIPCHK(hipEventElapsedTime(&tms, e->ev0, e->ev1));
+    e->lastTantanMs = tms;
+    for (int32_t k = 0; k < nseq; k++) {
+        const int32_t i = ord[k];
+        std::memcpy(probs + offsets[i], packed.data() + meta[k], sizeof(float) * lengths[i]);
+    }
+    return DM_OK;
+}
+
+int dm_add_dna(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+               int32_t *first_id) {
+    if (!e || nseq < 0 || (nseq > 0 && (!dna || !offsets || !lengths))) return DM_EINVAL;
+    std::vector<int64_t> poff(nseq);
+    int64_t total = 0;
+    for (int32_t i = 0; i < nseq; i++) {
+        if (lengths[i] < 0 || offsets[i] < 0) return fail(e, DM_EINVAL, "negative sequence length or offset");
+        poff[i] = total;
+        total += lengths[i];
+    }
+    std::vector<char> letters(std::max<int64_t>(1, total));
+    for (int32_t i = 0; i < nseq; i++) std::memcpy(letters.data() + poff[i], dna + offsets[i], lengths[i]);
+    if (e->maskBits & 2) {
+        // -m 2: letters whose tantan repeat probability reaches the threshold
+        // become a non-ACGT letter, so every codon over them decodes to '?'
+        // (makeMaskedSequence + decodeSequence, dummest.cc:557-573)
+        std::vector<float> probs(std::max<int64_t>(1, total));
+        const int rc = dm_tantan_dna(e, nseq, letters.data(), poff.data(), lengths, probs.data());
+        if (rc != DM_OK) return rc;
+        for (int64_t j = 0; j < total; j++)
+            if (!(probs[j] < e->maskThr)) letters[j] = 'N';
+    }
+    std::vector<uint8_t> codes(std::max<int64_t>(1, total));
+    for (int32_t i = 0; i < nseq; i++) decodeDna(letters.data() + poff[i], lengths[i], codes.data() + poff[i]);
+    return dm_add_sequences(e, nseq, codes.data(), poff.data(), lengths, first_id);
 }
 
 int dm_add_sequences(dm_engine *e, int32_t nseq, const uint8_t *codes, const int64_t *offsets, const int32_t *lengths,
@@ -1355,6 +1483,7 @@ int dm_copy_probe(dm_engine *e, int64_t bytes, double *ms) {
 const char *dm_build_id(void) { return DM_BUILD_ID; }
 
 double dm_last_dp_ms(const dm_engine *e) { return e ? e->lastMs : 0.0; }
+double dm_last_tantan_ms(const dm_engine *e) { return e ? e->lastTantanMs : 0.0; }
 int64_t dm_last_cells(const dm_engine *e) { return e ? e->lastCells : 0; }
 
 }  // extern "C"

@@ -1165,6 +1165,159 @@ __global__ void __launch_bounds__(256) copy_probe_kernel(const float4 *src, floa
         dst[i] = src[i];
 }
 
+// ---------------------------------------------------------------------------
+// tantan on DNA (sequence masking, -m 2): makeMaskedSequence
+// (dummest-sequence.hh:206-236) -> calcTantanProbabilities(isProtein = false)
+// (tantan-wrapper.hh:13-64) -> the gap-free forward-backward of tantan.cc
+// (40-151, 222-397): per letter, 100 repeat-offset states and one background
+// state in FP64, rescaled every 16 letters.
+//
+// Four lanes per sequence: lane l of a group holds the offsets i = 4k + l
+// (k < 25), which is exactly the partial sum the reference's AVX2 lane l
+// accumulates (mcf_simd.hh:74). The partial sums meet as (s0+s2)+(s1+s3)
+// through two lane exchanges (mcf_simd.hh:175-179) and the offsets past the
+// last full quad (the first 100 letters) are added afterwards in order, as the
+// reference's scalar tail does; multiply-adds are fused where the reference's
+// x86-64-v3 build fuses them. The last 100 letters sit in 2-bit fields of
+// seven registers (field i = the letter i+1 back), so the likelihood ratio of
+// offset i is one field extract. Loops are wave-uniform (sequences of a wave
+// are length-sorted by the host); a group's state is left unchanged at
+// positions past its own length.
+// ---------------------------------------------------------------------------
+constexpr int kTQ = kTantanK / 4;  // offsets per lane
+
+__device__ __forceinline__ int tantan_base(const uint8_t *bs, int n, int p) {
+    return (n > 0) ? (int)bs[p < 0 ? 0 : (p >= n ? n - 1 : p)] : 0;
+}
+
+__global__ void __launch_bounds__(64) tantan_dna_kernel(TantanArgs a) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x;
+    const int l = lane & 3;
+    const int g0 = lane & ~3;  // first lane of the group
+    const int sq = blockIdx.x * 16 + (lane >> 2);
+    const bool valid = sq < a.nseq;
+    const int n = valid ? a.lengths[sq] : 0;
+    const int64_t off = valid ? a.offsets[sq] : 0;
+    const uint8_t *bs = a.bases + off;
+    float *out = a.probs + off;
+    double *sc = a.scales + (valid ? a.scaleOff[sq] : 0);
+    int N = n;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) N = max(N, __shfl_xor(N, s));
+
+    // the offset weights are the same for every sequence: LDS, read at use
+    __shared__ double sb2f[kTantanK];
+    for (int i = lane; i < kTantanK; i += 64) sb2f[i] = a.b2f[i];
+    __syncthreads();
+    auto b2f = [&](int k) { return sb2f[4 * k + l]; };
+    double fp[kTQ];
+#pragma unroll
+    for (int k = 0; k < kTQ; ++k) fp[k] = 0.0;
+    uint32_t h[7] = {0, 0, 0, 0, 0, 0, 0};  // field i (bits 2i, 2i+1): letter pos-1-i
+    auto field = [&](int k) -> int { return (int)((h[k >> 2] >> (8 * (k & 3) + 2 * l)) & 3u); };
+    const double f2f0 = a.f2f0, f2b = a.f2b, b2b = a.b2b;
+
+    // forward (tantan.cc:283-330 with rescaleForward 364-371)
+    double bg = 1.0;
+    int curNext = tantan_base(bs, n, 0);
+    for (int pos = 0; pos < N; ++pos) {
+        const bool act = pos < n;
+        const int cur = curNext;
+        curNext = tantan_base(bs, n, pos + 1);
+        const int m = pos < kTantanK ? pos : kTantanK;
+        const int q4 = m >> 2, t = m & 3;
+        const double b = bg;
+        double s = 0.0, tailf = 0.0;
+#pragma unroll
+        for (int k = 0; k < kTQ; ++k) {
+            const double r = (field(k) == cur) ? 3.0 : (1.0 / 3.0);
+            const double f = fp[k];
+            if (k < q4) s += f;
+            if (k == q4 && l < t) tailf = f;
+            const double nf = __fma_rn(b, b2f(k), f * f2f0) * r;
+            fp[k] = (act && 4 * k + l < m) ? nf : f;
+        }
+        double x = s + __shfl_xor(s, 2);
+        x = x + __shfl_xor(x, 1);
+        for (int u = 0; u < t; ++u) x += __shfl(tailf, g0 + u);
+        double nb = __fma_rn(b, b2b, x * f2b);
+        if ((pos & 15) == 15) {
+            const double scl = 1.0 / nb;
+            if (act && l == 0) sc[pos >> 4] = scl;
+            nb *= scl;
+#pragma unroll
+            for (int k = 0; k < kTQ; ++k) fp[k] = act ? fp[k] * scl : fp[k];
+        }
+        bg = act ? nb : bg;
+        if (act && l == 0) out[pos] = (float)bg;
+#pragma unroll
+        for (int w = 6; w > 0; --w) h[w] = (h[w] << 2) | (h[w - 1] >> 30);
+        h[0] = (h[0] << 2) | (uint32_t)cur;
+    }
+    // forwardTotal (tantan.cc:135-141): the offsets summed in order
+    double all = 0.0;
+#pragma unroll
+    for (int i = 0; i < kTantanK; ++i) all += __shfl(fp[i >> 2], g0 + (i & 3));
+    const double z = __fma_rn(all, f2b, bg * b2b);
+
+    // backward (tantan.cc:147-151, 332-361, 384-397)
+    bg = b2b;
+#pragma unroll
+    for (int k = 0; k < kTQ; ++k) fp[k] = f2b;
+    auto shiftBack = [&](int p) {  // window of position p-1 from that of p
+#pragma unroll
+        for (int w = 0; w < 6; ++w) h[w] = (h[w] >> 2) | (h[w + 1] << 30);
+        h[6] = ((h[6] >> 2) & ~(3u << 6)) | ((uint32_t)tantan_base(bs, n, p - 101) << 6);
+    };
+    shiftBack(N);
+    for (int pos = N - 1; pos >= 0; --pos) {
+        const bool act = pos < n;
+        const int cur = tantan_base(bs, n, pos);
+        if (act && l == 0) {
+            const double nonRepeat = (double)out[pos] * bg / z;
+            out[pos] = 1.0f - (float)nonRepeat;
+        }
+        if ((pos & 15) == 15) {
+            double scl = (act && l == 0) ? sc[pos >> 4] : 1.0;
+            scl = __shfl(scl, g0);
+            if (act) {
+                bg *= scl;
+#pragma unroll
+                for (int k = 0; k < kTQ; ++k) fp[k] *= scl;
+            }
+        }
+        const double toBg = f2b * bg;
+        const int m = pos < kTantanK ? pos : kTantanK;
+        const int q4 = m >> 2, t = m & 3;
+        double s = 0.0, tb = 0.0, tf = 0.0;
+#pragma unroll
+        for (int k = 0; k < kTQ; ++k) {
+            const double r = (field(k) == cur) ? 3.0 : (1.0 / 3.0);
+            const double f = fp[k] * r;
+            if (k < q4) s = __fma_rn(b2f(k), f, s);
+            if (k == q4 && l < t) {
+                tb = b2f(k);
+                tf = f;
+            }
+            const double nf = __fma_rn(f, f2f0, toBg);
+            fp[k] = (act && 4 * k + l < m) ? nf : fp[k];
+        }
+        double x = s + __shfl_xor(s, 2);
+        x = x + __shfl_xor(x, 1);
+        for (int u = 0; u < t; ++u) x = __fma_rn(__shfl(tb, g0 + u), __shfl(tf, g0 + u), x);
+        if (act) bg = __fma_rn(b2b, bg, x);
+        shiftBack(pos);
+    }
+}
+
+hipError_t launch_tantan_dna(const TantanArgs &a, hipStream_t s) {
+    const int blocks = (a.nseq + 15) / 16;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(tantan_dna_kernel, dim3(blocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy_probe(const float *src, float *dst, int64_t n16, hipStream_t s) {
     if (n16 <= 0) return hipSuccess;
     const int blocks = (int)std::min<int64_t>((n16 + 255) / 256, 8192);

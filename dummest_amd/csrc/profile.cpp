@@ -4,10 +4,11 @@
 // transitions m->i, i->i, m->d, d->d and 20 match emissions per node, stored
 // as probabilities exp(-x) ("*" = 0, probFromText dummest-profile.hh:14-19).
 // Row r holds node r's transitions and node r+1's emissions.
-// Transform: finalizeProfile (dummest-profile.hh:105-233) without tantan
-// masking. The arithmetic keeps the reference's Float/double mix (Float =
-// float for the FP32 engine): rate products are formed in float, the
-// per-row Params in double and narrowed to float.
+// Transform: finalizeProfile (dummest-profile.hh:105-233), with the profile
+// masking of -m 1 (tantan on the consensus) when a threshold is given. The
+// arithmetic keeps the reference's Float/double mix (Float = float for the
+// FP32 engine): rate products are formed in float, the per-row Params in
+// double and narrowed to float.
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -37,31 +38,65 @@ double probFromWord(const std::string &w) {
     return std::exp(-std::strtod(w.c_str(), nullptr));
 }
 
-double backgroundMean(const float *v, int length, int step, int type) {  // myMean, dummest-profile.hh:37-62
+// myMean, dummest-profile.hh:37-62: positions whose repeat probability reaches
+// the threshold are left out (profile masking).
+double backgroundMean(const float *v, int length, int step, int type, const float *rep = nullptr,
+                      float thr = 0.f) {
     double mean = 0;
+    int n = 0;
     std::vector<float> med;
     for (int i = 0; i < length; ++i) {
+        if (rep && rep[i] >= thr) continue;
         const double x = v[(size_t)i * step];
         if (type == 'G') mean += std::log(std::max(x, 1e-6));
         else if (type == 'A') mean += x;
         else med.push_back((float)x);
+        ++n;
     }
-    if (type == 'G') return std::exp(mean / length);
-    if (type == 'A') return mean / length;
+    if (n == 0) return NAN;  // the reference asserts n > 0 (dummest-profile.hh:53)
+    if (type == 'G') return std::exp(mean / n);
+    if (type == 'A') return mean / n;
     std::sort(med.begin(), med.end());
-    return med[length / 2];
+    return med[n / 2];
 }
 
-bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string &msg) {
+bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr) {
     const float ins1 = (float)rates[0], ins2 = (float)rates[1], del1 = (float)rates[2], del2 = (float)rates[3];
     const float stop = (float)rates[4], bgStop = (float)rates[5];
     const int M = p.M;
     float *v = p.values.data();
     float *end = v + (size_t)kWidth * M;
     if (end[3] <= 0) end[3] = (float)backgroundMean(v + kWidth + 3, M - 1, kWidth, 'G');
+    // Profile masking (-m 1, dummest-profile.hh:122-125): tantan repeat
+    // probabilities of the consensus (the first most probable letter of each
+    // node); positions at or above the threshold leave the background means
+    // and emit like the background.
+    std::vector<float> rep;
+    p.masked = 0;
+    if (maskThr >= 0.f) {
+        if ((int)p.consensus.size() != M) {
+            p.consensus.assign(M, 0);
+            for (int i = 0; i < M; ++i) {
+                const float *e = v + (size_t)i * kWidth + 4;
+                p.consensus[i] = (char)(std::max_element(e, e + 20) - e);
+            }
+        }
+        rep.resize(M);
+        tantanRepeatProbs(reinterpret_cast<const uint8_t *>(p.consensus.data()), M, true, rep.data());
+        for (int i = 0; i < M; ++i)
+            if (rep[i] >= maskThr) {
+                ++p.masked;
+                p.consensus[i] |= 32;  // printed in lowercase (dummest-profile.hh:225-226)
+            }
+    }
+    const float *rp = rep.empty() ? nullptr : rep.data();
     double sum = 0;
     for (int k = 4; k < 24; ++k) {
-        const double m = backgroundMean(v + k, M, kWidth, bgType);
+        const double m = backgroundMean(v + k, M, kWidth, bgType, rp, maskThr);
+        if (!(m == m)) {
+            msg = "every profile position is masked";
+            return false;
+        }
         end[k] = (float)m;
         sum += m;
     }
@@ -99,9 +134,10 @@ bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string
         }
         row[2] = (float)delta;
         row[3] = (float)eps;
+        const bool masked = rp && rp[i] >= maskThr;
         for (int k = 4; k < 24; ++k) {
             const double x = row[k];
-            row[k] = (float)((1 - stop) * x / synonymousCodons(kAlpha[k - 4]));
+            row[k] = masked ? end[k] : (float)((1 - stop) * x / synonymousCodons(kAlpha[k - 4]));
         }
         row[24] = row[5];   // U = C
         row[25] = row[12];  // O = K
@@ -115,7 +151,7 @@ bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string
 }  // namespace
 
 bool readHmmText(const std::string &text, int bgType, const double rates[6], std::vector<HostProfile> &out,
-                 std::string &msg) {
+                 std::string &msg, float maskThr) {
     std::istringstream in(text);
     std::string line;
     int state = 0;
@@ -152,7 +188,7 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
                 if (cur.M < 2) return bad("profile shorter than 2 positions");
                 vals.insert(vals.end(), kWidth - 4, 0.f);
                 cur.values = vals;
-                if (!finalizeImpl(cur, bgType, rates, msg)) return false;
+                if (!finalizeImpl(cur, bgType, rates, msg, maskThr)) return false;
                 out.push_back(std::move(cur));
                 cur = HostProfile();
                 vals.clear();
@@ -183,11 +219,12 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
     return true;
 }
 
-bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg) {
-    return finalizeImpl(p, bgType, rates, msg);
+bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr) {
+    return finalizeImpl(p, bgType, rates, msg, maskThr);
 }
 
-bool readHmmFile(const char *path, int bgType, const double rates[6], std::vector<HostProfile> &out, std::string &msg) {
+bool readHmmFile(const char *path, int bgType, const double rates[6], std::vector<HostProfile> &out, std::string &msg,
+                 float maskThr) {
     std::ifstream f(path, std::ios::binary);
     if (!f) {
         msg = std::string("can't open file: ") + path;
@@ -195,7 +232,7 @@ bool readHmmFile(const char *path, int bgType, const double rates[6], std::vecto
     }
     std::stringstream ss;
     ss << f.rdbuf();
-    return readHmmText(ss.str(), bgType, rates, out, msg);
+    return readHmmText(ss.str(), bgType, rates, out, msg, maskThr);
 }
 
 // Codon table in TCAG order; non-ACGT letters make the codon '?'.
@@ -221,6 +258,127 @@ void decodeDna(const char *dna, int64_t n, uint8_t *codes) {
         const int b0 = T.base[(unsigned char)dna[i]], b1 = T.base[(unsigned char)dna[i + 1]],
                   b2 = T.base[(unsigned char)dna[i + 2]];
         codes[i] = (b0 < 0 || b1 < 0 || b2 < 0) ? 22 : T.num[(unsigned char)aa[b0 * 16 + b1 * 4 + b2]];
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// tantan repeat probabilities on the host (profile masking): the gap-free
+// forward-backward of tantan/tantan.cc:40-151, 222-397 with the parameters of
+// calcTantanProbabilities (tantan-wrapper.hh:13-64). Four interleaved partial
+// sums over the repeat offsets reduced as (s0+s2)+(s1+s3), and the fused
+// multiply-adds written out, as the reference's x86-64-v3 build evaluates
+// them (the DNA kernel in kernels.hip does the same on the device).
+// ---------------------------------------------------------------------------
+namespace {
+const signed char kBlosum62[20][20] = {
+    {4, 0, -2, -1, -2, 0, -2, -1, -1, -1, -1, -2, -1, -1, -1, 1, 0, 0, -3, -2},
+    {0, 9, -3, -4, -2, -3, -3, -1, -3, -1, -1, -3, -3, -3, -3, -1, -1, -1, -2, -2},
+    {-2, -3, 6, 2, -3, -1, -1, -3, -1, -4, -3, 1, -1, 0, -2, 0, -1, -3, -4, -3},
+    {-1, -4, 2, 5, -3, -2, 0, -3, 1, -3, -2, 0, -1, 2, 0, 0, -1, -2, -3, -2},
+    {-2, -2, -3, -3, 6, -3, -1, 0, -3, 0, 0, -3, -4, -3, -3, -2, -2, -1, 1, 3},
+    {0, -3, -1, -2, -3, 6, -2, -4, -2, -4, -3, 0, -2, -2, -2, 0, -2, -3, -2, -3},
+    {-2, -3, -1, 0, -1, -2, 8, -3, -1, -3, -2, 1, -2, 0, 0, -1, -2, -3, -2, 2},
+    {-1, -1, -3, -3, 0, -4, -3, 4, -3, 2, 1, -3, -3, -3, -3, -2, -1, 3, -3, -1},
+    {-1, -3, -1, 1, -3, -2, -1, -3, 5, -2, -1, 0, -1, 1, 2, 0, -1, -2, -3, -2},
+    {-1, -1, -4, -3, 0, -4, -3, 2, -2, 4, 2, -3, -3, -2, -2, -2, -1, 1, -2, -1},
+    {-1, -1, -3, -2, 0, -3, -2, 1, -1, 2, 5, -2, -2, 0, -1, -1, -1, 1, -1, -1},
+    {-2, -3, 1, 0, -3, 0, 1, -3, 0, -3, -2, 6, -2, 0, 0, 1, 0, -3, -4, -2},
+    {-1, -3, -1, -1, -4, -2, -2, -3, -1, -3, -2, -2, 7, -1, -2, -1, -1, -2, -4, -3},
+    {-1, -3, 0, 2, -3, -2, 0, -3, 1, -2, 0, 0, -1, 5, 1, 0, -1, -2, -2, -1},
+    {-1, -3, -2, 0, -3, -2, 0, -3, 2, -2, -1, 0, -2, 1, 5, -1, -1, -3, -3, -2},
+    {1, -1, 0, 0, -2, 0, -1, -2, 0, -2, -1, 1, -1, 0, -1, 4, 1, -2, -3, -2},
+    {0, -1, -1, -1, -2, -2, -2, -1, -1, -1, -1, 0, -1, -1, -1, 1, 5, 0, -2, -2},
+    {0, -1, -3, -2, -1, -3, -3, 3, -2, 1, 1, -3, -2, -2, -3, -2, 0, 4, -3, -1},
+    {-3, -2, -4, -3, 1, -2, -2, -3, -3, -2, -1, -4, -4, -2, -3, -3, -2, -3, 11, 2},
+    {-2, -2, -3, -2, 3, -3, 2, -1, -2, -1, -1, -2, -3, -1, -2, -2, -2, -1, 2, 7},
+};
+}  // namespace
+
+TantanParams tantanParams(bool isProtein) {
+    TantanParams t;
+    t.K = isProtein ? 50 : 100;
+    for (int i = 0; i < 22; ++i)
+        for (int j = 0; j < 22; ++j)
+            t.ratio[i][j] = isProtein ? std::exp(0.32403216734804408 * ((i < 20 && j < 20) ? kBlosum62[i][j] : -4))
+                                      : ((i < 4 && i == j) ? 3.0 : 1.0 / 3.0);
+    const double repeatProb = 0.005, repeatEndProb = 0.05, decay = 0.9;
+    t.b2b = 1 - repeatProb;
+    t.f2b = repeatEndProb;
+    t.f2f0 = 1 - repeatEndProb;
+    // firstRepeatOffsetProb (tantan.cc:25-30) for the decaying offset weights
+    const double first = (decay < 1 || decay > 1) ? (1 - decay) / (1 - std::pow(decay, t.K)) : 1.0 / t.K;
+    double q = repeatProb * first;
+    for (int i = 0; i < t.K; ++i) {
+        t.b2f[i] = q;
+        q *= decay;
+    }
+    return t;
+}
+
+void tantanRepeatProbs(const uint8_t *seq, int n, bool isProtein, float *out) {
+    const TantanParams t = tantanParams(isProtein);
+    const int K = t.K;
+    std::vector<double> fp(K, 0.0), scales(n / 16);
+    double bg = 1.0;
+    for (int pos = 0; pos < n; ++pos) {  // forward, with a rescale every 16 letters
+        const double b = bg;
+        const double *lr = t.ratio[seq[pos]];
+        const int m = std::min(pos, K);
+        double s4[4] = {0, 0, 0, 0};
+        int i = 0;
+        for (; i + 4 <= m; i += 4)
+            for (int l = 0; l < 4; ++l) {
+                const double f = fp[i + l];
+                s4[l] += f;
+                fp[i + l] = std::fma(b, t.b2f[i + l], f * t.f2f0) * lr[seq[pos - i - l - 1]];
+            }
+        double sum = (s4[0] + s4[2]) + (s4[1] + s4[3]);
+        for (; i < m; ++i) {
+            const double f = fp[i];
+            sum += f;
+            fp[i] = std::fma(b, t.b2f[i], f * t.f2f0) * lr[seq[pos - i - 1]];
+        }
+        bg = std::fma(b, t.b2b, sum * t.f2b);
+        if (pos % 16 == 15) {
+            const double sc = 1 / bg;
+            scales[pos / 16] = sc;
+            bg *= sc;
+            for (double &f : fp) f *= sc;
+        }
+        out[pos] = (float)bg;
+    }
+    double all = 0.0;
+    for (double f : fp) all += f;
+    const double z = std::fma(all, t.f2b, bg * t.b2b);
+    bg = t.b2b;
+    std::fill(fp.begin(), fp.end(), t.f2b);
+    for (int pos = n - 1; pos >= 0; --pos) {  // backward: posterior of the non-repeat state
+        const double nonRepeat = (double)out[pos] * bg / z;
+        out[pos] = 1 - (float)nonRepeat;
+        if (pos % 16 == 15) {
+            const double sc = scales[pos / 16];
+            bg *= sc;
+            for (double &f : fp) f *= sc;
+        }
+        const double toBg = t.f2b * bg;
+        const double *lr = t.ratio[seq[pos]];
+        const int m = std::min(pos, K);
+        double s4[4] = {0, 0, 0, 0};
+        int i = 0;
+        for (; i + 4 <= m; i += 4)
+            for (int l = 0; l < 4; ++l) {
+                const double f = fp[i + l] * lr[seq[pos - i - l - 1]];
+                s4[l] = std::fma(t.b2f[i + l], f, s4[l]);
+                fp[i + l] = std::fma(f, t.f2f0, toBg);
+            }
+        double tf = (s4[0] + s4[2]) + (s4[1] + s4[3]);
+        for (; i < m; ++i) {
+            const double f = fp[i] * lr[seq[pos - i - 1]];
+            tf = std::fma(t.b2f[i], f, tf);
+            fp[i] = std::fma(f, t.f2f0, toBg);
+        }
+        bg = std::fma(t.b2b, bg, tf);
     }
 }
 

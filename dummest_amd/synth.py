@@ -65,9 +65,10 @@ def evolve(rng, anc: np.ndarray, sub=0.30, indel=0.02):
     return np.array(out, np.int64), np.array(aligned, np.int64), ins_ev, del_ev
 
 
-def make_profile(rng, L: int, name: str, ndesc=50):
-    """(HMMER3 text, ancestral protein as letter indices, descendants)."""
-    anc = rng.integers(0, 20, size=L)
+def make_profile(rng, L: int, name: str, ndesc=50, anc=None):
+    """(HMMER3 text, ancestral protein as letter indices, descendants). anc
+    (letter indices, length L) fixes the ancestor, e.g. a tandem repeat."""
+    anc = rng.integers(0, 20, size=L) if anc is None else np.asarray(anc)
     counts = np.ones((L, 20))
     ins = np.zeros(L + 1)
     dels = np.zeros(L + 1)

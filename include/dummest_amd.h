@@ -39,16 +39,18 @@ const char *dm_engine_error(const dm_engine *e);
 int dm_set_rates(dm_engine *e, const double rates[6]);
 
 /* Read HMMER3/f profiles and finalize them (readProfiles + finalizeProfile,
- * dummest-profile.hh:105-329; tantan masking off, i.e. -m 0). bg_type is 'G'
+ * dummest-profile.hh:105-329; profile masking per dm_set_masking). bg_type is 'G'
  * (geometric mean, default), 'A' (arithmetic) or 'M' (median). Profiles get
  * consecutive ids starting at *first_id. */
 int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, int32_t *count);
 
 /* Parse HMMER3/f text and finalize it without a device (as dm_read_hmm);
  * writes profile number `index` (M, values (M+1)x29, params (M+2)x11, bg 29;
- * pass NULL arrays to query M) and the number of profiles in the text. */
-int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], int32_t index, int32_t *M,
-                      float *values, float *params, float *bg, int32_t *count);
+ * pass NULL arrays to query M), the number of profiles in the text and the
+ * positions masked. mask_threshold >= 0 turns on profile masking (-m 1) at that
+ * tantan threshold; negative leaves it off. */
+int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], double mask_threshold, int32_t index,
+                      int32_t *M, float *values, float *params, float *bg, int32_t *count, int32_t *masked);
 
 /* Add profiles from values as the reader stores them (probabilities, (M+1)x29
  * per profile, concatenated: transitions m->i i->i m->d d->d then node i+1's
@@ -75,6 +77,26 @@ int dm_decode(const char *dna, int64_t n, uint8_t *codes);
 int dm_add_sequences(dm_engine *e, int32_t nseq, const uint8_t *codes, const int64_t *offsets,
                      const int32_t *lengths, int32_t *first_id);
 int dm_clear_sequences(dm_engine *e);
+
+/* Masking of simple regions with tantan (dummest.cc -m and --tantan-threshold,
+ * :181-184, :257-261). mask bit 1: profiles read or added afterwards are masked
+ * (finalizeProfile's isMask, dummest-profile.hh:105-233); bit 2: DNA added with
+ * dm_add_dna is masked (makeMaskedSequence, dummest-sequence.hh:206-236).
+ * threshold in [0, 1] (default 0.5). */
+int dm_set_masking(dm_engine *e, int mask, double threshold);
+/* Profile positions masked by tantan ("# Positions masked by tantan", dummest.cc:389-395);
+ * -1 for an invalid id. */
+int dm_profile_masked(const dm_engine *e, int32_t id);
+/* tantan repeat probabilities of DNA letters on the device (calcTantanProbabilities
+ * with isProtein = false, tantan-wrapper.hh:13-64; A,C,G,T in either case, every
+ * other letter counts as A): probs[offsets[k] + j] for letter j of sequence k. */
+int dm_tantan_dna(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+                  float *probs);
+/* Append DNA sequences given as letters: masked on the device when mask bit 2
+ * is set, then decoded (replaces makeMaskedSequence + decodeSequence per
+ * contig, dummest.cc:557-573). Ids as dm_add_sequences. */
+int dm_add_dna(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+               int32_t *first_id);
 
 /* Score-only filter DP for (profile, sequence) pairs: best over cells of
  * W1 * prefixSeed in FP32 with no rescaling (+inf when it overflows).
@@ -195,6 +217,8 @@ int dm_set_option(dm_engine *e, const char *name, int64_t value);
  * engine's stream), in milliseconds, and the number of DP cells it covered
  * (sum over pairs of (M+1)*n). */
 double dm_last_dp_ms(const dm_engine *e);
+/* Device time of the tantan kernel of the last dm_tantan_dna / masked dm_add_dna call, ms. */
+double dm_last_tantan_ms(const dm_engine *e);
 int64_t dm_last_cells(const dm_engine *e);
 
 #ifdef __cplusplus

@@ -49,8 +49,11 @@ def _p(t):
 
 def _declare(L):
     f, d, i, u8, c = ctypes.c_float, ctypes.c_double, ctypes.c_int, ctypes.c_uint8, ctypes.c_char
-    L.or_finalize_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), i]
-    L.or_finalize_f64.argtypes = [i, _p(d), _p(d), _p(d), _p(d), i]
+    L.or_finalize_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), i, _p(f), d]
+    L.or_finalize_f64.argtypes = [i, _p(d), _p(d), _p(d), _p(d), i, _p(f), d]
+    L.or_tantan.argtypes = [_p(u8), i, i, _p(f)]
+    L.or_blosum62.argtypes = [_p(ctypes.c_int8)]
+    L.or_mask_dna.argtypes = [ctypes.c_char_p, i, d, ctypes.c_char_p, _p(f)]
     L.or_decode.argtypes = [ctypes.c_char_p, i, _p(u8)]
     for suf, t in (("f32", f), ("f64", d)):
         getattr(L, "or_null_" + suf).argtypes = [_p(t), _p(u8), i, _p(d), _p(t), _p(t), _p(t),
@@ -190,10 +193,12 @@ class Profile:
     params: np.ndarray   # (M+2, 11) Float
     bg: np.ndarray       # (29,) Float
     consensus: list
+    masked: int = 0      # positions masked by tantan (profile masking)
 
 
-def finalize(rp: RawProfile, dtype=np.float32, opts=DEFAULT_OPTS, bg_type="G") -> Profile:
-    """finalizeProfile (dummest-profile.hh:105-233) with -m 0."""
+def finalize(rp: RawProfile, dtype=np.float32, opts=DEFAULT_OPTS, bg_type="G", mask_threshold=None) -> Profile:
+    """finalizeProfile (dummest-profile.hh:105-233); mask_threshold turns on
+    the profile masking of -m 1 (tantan on the consensus, :122-125)."""
     dt = np.dtype(dtype)
     vals = np.ascontiguousarray(rp.raw.astype(dt))
     params = np.zeros((rp.length + 2, 11), dtype=dt)
@@ -201,10 +206,58 @@ def finalize(rp: RawProfile, dtype=np.float32, opts=DEFAULT_OPTS, bg_type="G") -
     o = np.array(opts, dtype=np.float64)
     fn = lib().or_finalize_f32 if dt == np.float32 else lib().or_finalize_f64
     t = _ct(dt)
-    ok = fn(rp.length, _ptr(vals, t), _ptr(params, t), _ptr(bg, t), _ptr(o, ctypes.c_double), ord(bg_type))
+    rep = None
+    if mask_threshold is not None:
+        rep = tantan(np.array(rp.consensus, np.uint8), True)
+    rp_ptr = _ptr(rep, ctypes.c_float) if rep is not None else None
+    ok = fn(rp.length, _ptr(vals, t), _ptr(params, t), _ptr(bg, t), _ptr(o, ctypes.c_double), ord(bg_type),
+            rp_ptr, float(mask_threshold or 0.0))
     if not ok:
         raise ValueError("finalizeProfile rejected the profile")
-    return Profile(rp.name, rp.length, vals, params, bg, list(rp.consensus))
+    prof = Profile(rp.name, rp.length, vals, params, bg, list(rp.consensus))
+    prof.masked = 0 if rep is None else int(np.sum(rep >= np.float32(mask_threshold)))
+    return prof
+
+
+def blosum62() -> np.ndarray:
+    """The oracle's BLOSUM62 table (tantan-wrapper.hh:16-36 order: ACDEFGHIKLMNPQRSTVWY)."""
+    out = np.zeros((20, 20), np.int8)
+    lib().or_blosum62(_ptr(out, ctypes.c_int8))
+    return out
+
+
+def tantan(seq: np.ndarray, is_protein: bool) -> np.ndarray:
+    """Repeat probability per letter (calcTantanProbabilities, tantan-wrapper.hh:13-64)."""
+    c = np.ascontiguousarray(seq, dtype=np.uint8)
+    out = np.zeros(max(1, len(c)), np.float32)
+    lib().or_tantan(_ptr(c, ctypes.c_uint8), len(c), int(is_protein), _ptr(out, ctypes.c_float))
+    return out[:len(c)]
+
+
+def mask_dna(dna: str | bytes, threshold=0.5):
+    """makeMaskedSequence (dummest-sequence.hh:206-236) on DNA letters: the
+    masked letters become 'N'. Returns (masked letters, probabilities)."""
+    b = dna.encode() if isinstance(dna, str) else bytes(dna)
+    out = ctypes.create_string_buffer(len(b) + 1)
+    probs = np.zeros(max(1, len(b)), np.float32)
+    lib().or_mask_dna(b, len(b), float(threshold), out, _ptr(probs, ctypes.c_float))
+    return out.raw[:len(b)].decode(), probs[:len(b)]
+
+
+REF_TANTAN = os.path.join(HERE, "_ref", "libtantan_ref.so")
+
+
+def ref_tantan(seq: np.ndarray, is_protein: bool):
+    """The reference's own tantan (oracle/_ref, built from its sources by
+    `make ref`), or None when that library was not built."""
+    if not os.path.exists(REF_TANTAN):
+        return None
+    L = ctypes.CDLL(REF_TANTAN)
+    L.ref_tantan_probs.argtypes = [_p(ctypes.c_uint8), ctypes.c_int, ctypes.c_int, _p(ctypes.c_float)]
+    c = np.ascontiguousarray(seq, dtype=np.uint8)
+    out = np.zeros(max(1, len(c)), np.float32)
+    L.ref_tantan_probs(_ptr(c, ctypes.c_uint8), len(c), int(is_protein), _ptr(out, ctypes.c_float))
+    return out[:len(c)]
 
 
 def decode(dna: str | bytes) -> np.ndarray:

@@ -287,11 +287,13 @@ static int codonCount(char aa) {  // dummest-sequence.hh:103-124 (sizes)
 // bg out: 29 entries (bg_probs, dummest-profile.hh:135-148).
 // ----------------------------------------------------------------------------
 template <class F>
-static double myMean(const F *values, int length, int step, int meanType) {
+static double myMean(const F *values, int length, int step, int meanType, const float *tantan = nullptr,
+                     F thr = 0) {
     double mean = 0;
     int n = 0;
     std::vector<F> med;
     for (int i = 0; i < length; ++i) {
+        if (tantan && tantan[i] >= thr) continue;  // dummest-profile.hh:41-43
         double v = values[i * step];
         if (meanType == 'G') mean += std::log(std::max(v, 1e-6));
         if (meanType == 'A') mean += v;
@@ -305,13 +307,18 @@ static double myMean(const F *values, int length, int step, int meanType) {
 }
 
 template <class F>
-static int finalizeProfile(int M, F *values, F *params, F *bgOut, const Rates<F> &R, int bgType) {
+static int finalizeProfile(int M, F *values, F *params, F *bgOut, const Rates<F> &R, int bgType,
+                           const float *tantan = nullptr, F thr = 0) {
+    // tantan: repeat probabilities of the consensus (profile masking, -m 1),
+    // computed after the final epsilon (dummest-profile.hh:116-126); positions
+    // at or above thr leave the background means and emit like the background
+    // (dummest-profile.hh:209-211, 225-226).
     const int W = kWidth;
     F *end = values + W * M;
     if (end[3] <= 0) end[3] = (F)myMean<F>(values + W + 3, M - 1, W, 'G');
     double sumOfMeans = 0;
     for (int k = 4; k < 24; ++k) {
-        double mean = myMean<F>(values + k, M, W, bgType);
+        double mean = myMean<F>(values + k, M, W, bgType, tantan, thr);
         end[k] = (F)mean;
         sumOfMeans += mean;
     }
@@ -353,7 +360,12 @@ static int finalizeProfile(int M, F *values, F *params, F *bgOut, const Rates<F>
         if (epsilon >= 1) return 0;
         probs[2] = (F)delta;
         probs[3] = (F)epsilon;
+        const bool masked = tantan && tantan[i] >= thr;
         for (int k = 4; k < 24; ++k) {
+            if (masked) {
+                probs[k] = end[k];
+                continue;
+            }
             double p = probs[k];
             int cc = codonCount(kAlphabet[k - 4]);
             probs[k] = (F)((1 - R.stop) * p / cc);
@@ -365,6 +377,155 @@ static int finalizeProfile(int M, F *values, F *params, F *bgOut, const Rates<F>
         probs[28] = 0;
     }
     return 1;
+}
+
+// ----------------------------------------------------------------------------
+// tantan repeat probabilities, as DUMMEST calls them: calcTantanProbabilities
+// (tantan-wrapper.hh:13-64) -> tantan::getProbabilities -> Tantan::calcRepeatProbs
+// (tantan/tantan.cc:40-130 set-up, 222-330 the gap-free forward and backward
+// steps, 332-392 rescaling and the driver) with firstGapProb = otherGapProb = 0.
+// The reference is built with -O3 -march=x86-64-v3, where the sums over repeat
+// offsets run in four interleaved partial sums (the AVX2 lanes of
+// tantan/mcf_simd.hh:74) reduced as (s0+s2)+(s1+s3) (mcf_simd.hh:175-179), and
+// GCC fuses these multiply-adds; the restatement spells both out (pinned
+// bit-exactly against oracle/_ref, the reference's own tantan built from its
+// sources):
+//   forward   fp = fma(b, b2f[i], f * f2f0) * r        bg = fma(b, b2b, sum * f2b)
+//   backward  f = fp * r   s = fma(b2f[i], f, s)   fp = fma(f, f2f0, f2b * bg)
+//             bg = fma(b2b, bg, s)
+//   total     z = fma(sum(fp), f2b, bg * b2b)
+// ----------------------------------------------------------------------------
+static const signed char kBlosum62[20][20] = {
+    // A   C   D   E   F   G   H   I   K   L   M   N   P   Q   R   S   T   V   W   Y
+    {4, 0, -2, -1, -2, 0, -2, -1, -1, -1, -1, -2, -1, -1, -1, 1, 0, 0, -3, -2},      // A
+    {0, 9, -3, -4, -2, -3, -3, -1, -3, -1, -1, -3, -3, -3, -3, -1, -1, -1, -2, -2},  // C
+    {-2, -3, 6, 2, -3, -1, -1, -3, -1, -4, -3, 1, -1, 0, -2, 0, -1, -3, -4, -3},     // D
+    {-1, -4, 2, 5, -3, -2, 0, -3, 1, -3, -2, 0, -1, 2, 0, 0, -1, -2, -3, -2},        // E
+    {-2, -2, -3, -3, 6, -3, -1, 0, -3, 0, 0, -3, -4, -3, -3, -2, -2, -1, 1, 3},      // F
+    {0, -3, -1, -2, -3, 6, -2, -4, -2, -4, -3, 0, -2, -2, -2, 0, -2, -3, -2, -3},    // G
+    {-2, -3, -1, 0, -1, -2, 8, -3, -1, -3, -2, 1, -2, 0, 0, -1, -2, -3, -2, 2},      // H
+    {-1, -1, -3, -3, 0, -4, -3, 4, -3, 2, 1, -3, -3, -3, -3, -2, -1, 3, -3, -1},     // I
+    {-1, -3, -1, 1, -3, -2, -1, -3, 5, -2, -1, 0, -1, 1, 2, 0, -1, -2, -3, -2},      // K
+    {-1, -1, -4, -3, 0, -4, -3, 2, -2, 4, 2, -3, -3, -2, -2, -2, -1, 1, -2, -1},     // L
+    {-1, -1, -3, -2, 0, -3, -2, 1, -1, 2, 5, -2, -2, 0, -1, -1, -1, 1, -1, -1},      // M
+    {-2, -3, 1, 0, -3, 0, 1, -3, 0, -3, -2, 6, -2, 0, 0, 1, 0, -3, -4, -2},          // N
+    {-1, -3, -1, -1, -4, -2, -2, -3, -1, -3, -2, -2, 7, -1, -2, -1, -1, -2, -4, -3}, // P
+    {-1, -3, 0, 2, -3, -2, 0, -3, 1, -2, 0, 0, -1, 5, 1, 0, -1, -2, -2, -1},         // Q
+    {-1, -3, -2, 0, -3, -2, 0, -3, 2, -2, -1, 0, -2, 1, 5, -1, -1, -3, -3, -2},      // R
+    {1, -1, 0, 0, -2, 0, -1, -2, 0, -2, -1, 1, -1, 0, -1, 4, 1, -2, -3, -2},         // S
+    {0, -1, -1, -1, -2, -2, -2, -1, -1, -1, -1, 0, -1, -1, -1, 1, 5, 0, -2, -2},     // T
+    {0, -1, -3, -2, -1, -3, -3, 3, -2, 1, 1, -3, -2, -2, -3, -2, 0, 4, -3, -1},      // V
+    {-3, -2, -4, -3, 1, -2, -2, -3, -3, -2, -1, -4, -4, -2, -3, -3, -2, -3, 11, 2},  // W
+    {-2, -2, -3, -2, 3, -3, 2, -1, -2, -1, -1, -2, -3, -1, -2, -2, -2, -1, 2, 7},    // Y
+};
+
+static void tantanProbs(const uint8_t *seq, int n, bool isProtein, float *out) {
+    // likelihood ratios (tantan-wrapper.hh:43-56): BLOSUM62 at lambda
+    // 0.32403216734804408 (letters >= 20 score -4), or +1/-1 for DNA
+    double ratio[22][22];
+    for (int i = 0; i < 22; ++i)
+        for (int j = 0; j < 22; ++j) {
+            if (isProtein) {
+                const int sc = (i < 20 && j < 20) ? kBlosum62[i][j] : -4;
+                ratio[i][j] = std::exp(0.32403216734804408 * sc);
+            } else {
+                ratio[i][j] = (i < 4 && i == j) ? 3.0 : 1.0 / 3.0;
+            }
+        }
+    const int K = isProtein ? 50 : 100;
+    const double repeatProb = 0.005, repeatEndProb = 0.05, decay = 0.9;
+    const double b2b = 1 - repeatProb, f2b = repeatEndProb, f2f0 = 1 - repeatEndProb;
+    auto firstOffsetProb = [&](double pm) {  // tantan.cc:25-30
+        if (pm < 1 || pm > 1) return (1 - pm) / (1 - std::pow(pm, K));
+        return 1.0 / K;
+    };
+    const double b2fFirst = repeatProb * firstOffsetProb(decay);
+    std::vector<double> b2f(K), fp(K, 0.0), scales(n / 16);
+    double p = b2fFirst;
+    for (int i = 0; i < K; ++i) {
+        b2f[i] = p;
+        p *= decay;
+    }
+    // forward (tantan.cc:283-330, 375-381)
+    double bg = 1.0;
+    for (int pos = 0; pos < n; ++pos) {
+        const double b = bg;
+        const double *lr = ratio[seq[pos]];
+        const int m = std::min(pos, K);
+        double s4[4] = {0, 0, 0, 0};
+        int i = 0;
+        for (; i + 4 <= m; i += 4)
+            for (int l = 0; l < 4; ++l) {
+                const double f = fp[i + l];
+                s4[l] += f;
+                fp[i + l] = std::fma(b, b2f[i + l], f * f2f0) * lr[seq[pos - i - l - 1]];
+            }
+        double sum = (s4[0] + s4[2]) + (s4[1] + s4[3]);
+        for (; i < m; ++i) {
+            const double f = fp[i];
+            sum += f;
+            fp[i] = std::fma(b, b2f[i], f * f2f0) * lr[seq[pos - i - 1]];
+        }
+        bg = std::fma(b, b2b, sum * f2b);
+        if (pos % 16 == 15) {  // tantan.cc:364-371
+            const double scale = 1 / bg;
+            scales[pos / 16] = scale;
+            bg *= scale;
+            for (double &f : fp) f *= scale;
+        }
+        out[pos] = (float)bg;
+    }
+    double all = 0.0;
+    for (double f : fp) all += f;
+    const double z = std::fma(all, f2b, bg * b2b);  // forwardTotal, tantan.cc:135-141
+    // backward (tantan.cc:147-151, 332-361, 384-397)
+    bg = b2b;
+    std::fill(fp.begin(), fp.end(), f2b);
+    for (int pos = n - 1; pos >= 0; --pos) {
+        const double nonRepeat = (double)out[pos] * bg / z;
+        out[pos] = 1 - (float)nonRepeat;
+        if (pos % 16 == 15) {
+            const double scale = scales[pos / 16];
+            bg *= scale;
+            for (double &f : fp) f *= scale;
+        }
+        const double toBg = f2b * bg;
+        const double *lr = ratio[seq[pos]];
+        const int m = std::min(pos, K);
+        double s4[4] = {0, 0, 0, 0};
+        int i = 0;
+        for (; i + 4 <= m; i += 4)
+            for (int l = 0; l < 4; ++l) {
+                const double f = fp[i + l] * lr[seq[pos - i - l - 1]];
+                s4[l] = std::fma(b2f[i + l], f, s4[l]);
+                fp[i + l] = std::fma(f, f2f0, toBg);
+            }
+        double tf = (s4[0] + s4[2]) + (s4[1] + s4[3]);
+        for (; i < m; ++i) {
+            const double f = fp[i] * lr[seq[pos - i - 1]];
+            tf = std::fma(b2f[i], f, tf);
+            fp[i] = std::fma(f, f2f0, toBg);
+        }
+        bg = std::fma(b2b, bg, tf);
+    }
+}
+
+// DNA masking (makeMaskedSequence, dummest-sequence.hh:206-236): tantan runs on
+// A,C,G,T = 0..3 (every other letter counts as A); positions whose repeat
+// probability reaches thr become 'N' (the reference's mask value decodes like
+// any non-ACGT letter: the codons over it are '?').
+static void maskDna(const char *dna, int n, double thr, char *outDna, float *probs) {
+    std::vector<uint8_t> x(n);
+    for (int i = 0; i < n; ++i) {
+        switch (std::toupper((unsigned char)dna[i])) {
+        case 'C': x[i] = 1; break;
+        case 'G': x[i] = 2; break;
+        case 'T': x[i] = 3; break;
+        default: x[i] = 0;
+        }
+    }
+    tantanProbs(x.data(), n, false, probs);
+    for (int i = 0; i < n; ++i) outDna[i] = (probs[i] < (float)thr) ? dna[i] : 'N';
 }
 
 // ----------------------------------------------------------------------------
@@ -1302,16 +1463,22 @@ static void totalsImpl(int M, const F *values, const F *params, const F *bg, con
 
 extern "C" {
 
-int or_finalize_f32(int M, float *values, float *params, float *bg, const double *opts, int bgType) {
+int or_finalize_f32(int M, float *values, float *params, float *bg, const double *opts, int bgType,
+                    const float *tantan, double thr) {
     Rates<float> R;
     R.init(opts);
-    return finalizeProfile<float>(M, values, params, bg, R, bgType);
+    return finalizeProfile<float>(M, values, params, bg, R, bgType, tantan, (float)thr);
 }
-int or_finalize_f64(int M, double *values, double *params, double *bg, const double *opts, int bgType) {
+int or_finalize_f64(int M, double *values, double *params, double *bg, const double *opts, int bgType,
+                    const float *tantan, double thr) {
     Rates<double> R;
     R.init(opts);
-    return finalizeProfile<double>(M, values, params, bg, R, bgType);
+    return finalizeProfile<double>(M, values, params, bg, R, bgType, tantan, thr);
 }
+
+void or_tantan(const uint8_t *seq, int n, int isProtein, float *out) { tantanProbs(seq, n, isProtein != 0, out); }
+void or_blosum62(signed char *out) { std::memcpy(out, kBlosum62, sizeof(kBlosum62)); }
+void or_mask_dna(const char *dna, int n, double thr, char *outDna, float *probs) { maskDna(dna, n, thr, outDna, probs); }
 
 void or_decode(const char *dna, int n, uint8_t *codes) { decodeDna(dna, n, codes); }
 

@@ -22,18 +22,25 @@ def test_help_lists_reference_options():
         assert opt in r.stdout, opt
 
 
-@pytest.mark.parametrize("args", [["-e", "-1"], ["-t", "0"], ["--insert1", "1.5"], ["-m", "7"], ["-s", "3"]])
+@pytest.mark.parametrize("args", [["-e", "-1"], ["-t", "0"], ["--insert1", "1.5"], ["-m", "7"], ["-m", "-1"],
+                                  ["-s", "3"], ["--tantan-threshold", "1.5"], ["--tantan-threshold", "-0.1"]])
 def test_bad_option_values_fail(args, tmp_path):
     r = run(*args, str(tmp_path / "x.hmm"))
     assert r.returncode == 1
+    assert "usage" in r.stderr or "bad option" in r.stderr.lower() or "invalid" in r.stderr.lower()
 
 
-def test_masking_is_refused_not_ignored(tmp_path):
-    # tantan masking is not restated: the default -m 3 must fail loudly
-    r = run(str(tmp_path / "x.hmm"))
-    assert r.returncode == 1 and "-m 0" in r.stderr
-    r = run("-d", "5", "-m", "0", str(tmp_path / "x.hmm"))
+def test_gaussian_filter_is_refused_not_ignored(tmp_path):
+    # the Gaussian profile filter is not restated: it must fail loudly
+    r = run("-d", "5", str(tmp_path / "x.hmm"))
     assert r.returncode == 1 and "-d" in r.stderr
+
+
+@pytest.mark.parametrize("mask", ["0", "1", "2", "3"])
+def test_masking_options_accepted(mask, tmp_path):
+    # every -m value passes option checking (the run then stops at the missing file)
+    r = run("-m", mask, "--tantan-threshold", "0.4", str(tmp_path / "x.hmm"))
+    assert r.returncode == 1 and "can't open" in r.stderr
 
 
 def test_usage_without_arguments():

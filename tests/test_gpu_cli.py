@@ -89,8 +89,15 @@ def contigs_of(fasta_text):
     return out
 
 
-def fmt_records(prof_name, M, consensus, contig, hits, K, lam, total, evalue):
+def prof_letter(c):
+    """profileLetter (dummest-output.hh:6-8): bit 32 marks a masked position."""
+    return ALPHA[c & 31].lower() if c & 32 else ALPHA[c & 31]
+
+
+def fmt_records(prof_name, M, consensus, contig, hits, K, lam, total, evalue, shown=None):
+    """shown: the contig letters as printed (lowercase where masked), default the letters."""
     name, cstart, letters, rec_len, st = contig
+    shown = letters if shown is None else shown
     recs = []
     for lnr, m, e, a1, a2, segs in hits:
         if not segs:
@@ -99,15 +106,15 @@ def fmt_records(prof_name, M, consensus, contig, hits, K, lam, total, evalue):
         pos1, pos2 = segs[0][0], segs[0][1]
         for s1, s2, ln in segs:
             while pos1 < s1:
-                row1.append(ALPHA[consensus[pos1]]); pos1 += 1
+                row1.append(prof_letter(consensus[pos1])); pos1 += 1
             row1.append("-" * (s2 - pos2))
             while pos1 < s1 + ln:
-                row1.append(ALPHA[consensus[pos1]]); pos1 += 1
+                row1.append(prof_letter(consensus[pos1])); pos1 += 1
             pos2 = s2 + ln
         pos1, pos2 = segs[0][0], segs[0][1]
         for s1, s2, ln in segs:
             row2.append("-" * (s1 - pos1))
-            row2.append(letters[pos2:s2 + ln])
+            row2.append(shown[pos2:s2 + ln])
             pos2 = s2 + ln
             pos1 = s1 + ln
         r1, r2 = "".join(row1), "".join(row2)
@@ -141,17 +148,26 @@ def parse_records(text):
 
 
 def key(r):
-    return (r["pname"], r["sname"], r["a1"], r["a2"], r["s1"], r["span1"], r["s2"], r["span2"], r["strand"],
+    """Everything of a record but its score, E-value and anchor cell."""
+    return (r["pname"], r["sname"], r["s1"], r["span1"], r["s2"], r["span2"], r["strand"],
             r["plen"], r["slen"], r["r1"], r["r2"])
 
 
-def test_cli_records_match_oracle(tmp_path):
-    print("cli end-to-end", flush=True)
+@pytest.mark.parametrize("mask", [0, 3])
+def test_cli_records_match_oracle(tmp_path, mask):
+    """-m 0, and -m 3 (tantan on the profile and on every contig, masked
+    letters decoding as '?' and printing in lowercase)."""
+    print(f"cli end-to-end -m {mask}", flush=True)
     rng = np.random.default_rng(77)
     text, prof, dnas = synthetic_case(701, 120, [900, 1500, 700, 2000, 1200, 800], homolog_every=2)
     # a contig break ('X' is not a sequence letter) and IUPAC letters inside a homolog record
     dnas[2] = dnas[2][:300] + "XX" + dnas[2][300:]
     dnas[4] = dnas[4][:100] + "NNRY" + dnas[4][104:]
+    # microsatellites (masked under -m 2) in homolog and non-homolog records
+    for k, at in ((0, 450), (1, 200), (4, 600)):
+        dnas[k] = dnas[k][:at] + "CA" * 30 + dnas[k][at + 60:]
+    if mask & 1:
+        prof = O.finalize(O.parse_hmm(text)[0], np.float32, mask_threshold=0.5)
     fasta = []
     for k, d in enumerate(dnas):
         if k == 3:  # pipeline header: window coordinates, true length, target profile, strand
@@ -163,7 +179,7 @@ def test_cli_records_match_oracle(tmp_path):
     (tmp_path / "p.hmm").write_text(text)
     (tmp_path / "s.fa").write_text(fasta_text)
     evalue = 1.0
-    r = subprocess.run([CLI, "-m", "0", "-t", str(TRIALS), "-l", str(LENGTH), "-b", str(BORDER), "-e", str(evalue),
+    r = subprocess.run([CLI, "-m", str(mask), "-t", str(TRIALS), "-l", str(LENGTH), "-b", str(BORDER), "-e", str(evalue),
                         str(tmp_path / "p.hmm"), str(tmp_path / "s.fa")], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     got = parse_records(r.stdout)
@@ -184,11 +200,26 @@ def test_cli_records_match_oracle(tmp_path):
     assert f"# Total sequence length: {total}" in r.stdout
     raw = O.parse_hmm(text)[0]
     consensus = list(raw.consensus)
+    if mask & 1:
+        rep = O.tantan(np.array(consensus, np.uint8), True)
+        consensus = [c | 32 if p >= np.float32(0.5) else c for c, p in zip(consensus, rep)]
+        assert f"# Positions masked by tantan: {prof.masked}" in r.stdout
     want = []
     running = 0
+    masked_letters = 0
     for c in contigs:
         running += len(c[2])
-        codes = O.decode(c[2])
+        shown = c[2]
+        if mask & 2:
+            # makeMaskedSequence (dummest-sequence.hh:206-236): masked letters take the
+            # mask code 22, print in lowercase when 22 is above their own code
+            m_letters, probs = O.mask_dna(c[2], 0.5)
+            masked_letters += m_letters.count("N") - c[2].count("N")
+            shown = "".join(ch.lower() if (p >= np.float32(0.5) and ch != "?") else ch
+                            for ch, p in zip(c[2], probs))
+            codes = O.decode(m_letters)
+        else:
+            codes = O.decode(c[2])
         if len(codes) < 3:
             continue
         ratio = float(np.float32((K * running / evalue) ** (1 / lam)))
@@ -199,7 +230,12 @@ def test_cli_records_match_oracle(tmp_path):
         if not (Kf * total / math.exp(lp * lamf) <= evalue or not math.isfinite(lp)):
             continue
         res = O.search(prof, codes, ratio)
-        want += fmt_records("syn701", prof.M, consensus, c, res.hits, K, lam, total, evalue)
+        recs = fmt_records("syn701", prof.M, consensus, c, res.hits, K, lam, total, evalue, shown)
+        for rec in recs:
+            rec["codes"], rec["cstart"] = codes, c[1]
+        want += recs
+    if mask & 2:
+        assert masked_letters >= 120  # the microsatellites are masked
     near = lambda rec: abs(rec["E"] - evalue) < 0.01 * evalue
     gk = {key(x): x for x in got if not near(x)}
     wk = {key(x): x for x in want if not near(x)}
@@ -209,3 +245,13 @@ def test_cli_records_match_oracle(tmp_path):
         g = gk[k2]
         assert abs(g["bits"] - w["bits"]) < 1e-4 / math.log(2) + 1e-6
         assert g["E"] == pytest.approx(w["E"], rel=1e-3)
+        if (g["a1"], g["a2"]) != (w["a1"], w["a2"]):
+            # The anchor is the argmax of w0*W1 over a ridge of near-equal cells
+            # along the alignment; FP32 summation order can move it along the
+            # ridge (the reference's FP32 test leaves the anchor unchecked,
+            # tests/test_overflow.sh:66-71). The oracle's value at the engine's
+            # anchor cell must equal the hit score.
+            (w0m, w0e), (W1m, W1e), _, _ = O.matrices(prof, w["codes"])
+            i, j = g["a1"], g["a2"] - w["cstart"]
+            cell = O.es_log(float(w0m[i, j]), int(w0e[i, j])) + O.es_log(float(W1m[i, j]), int(W1e[i, j]))
+            assert abs(cell - w["bits"] * math.log(2)) < 1e-4, (g["a1"], g["a2"], w["a1"], w["a2"])
