@@ -121,6 +121,16 @@ hipError_t launch_null(const NullArgs &a, hipStream_t s);
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 
+// Waves per workgroup of a DP configuration. Single-strip kernels whose
+// emission table is large (192 < G*R <= 512 rows: 19-51 KB of LDS) run two
+// (up to 384 rows) or four waves per workgroup on as many work units of the
+// same profile, sharing one table, so LDS stops limiting their occupancy; the
+// host groups the units (same profile, similar lengths) and pads a profile's
+// last group with empty units.
+inline constexpr int dp_wg(int G, int R, bool strips, int mode) {
+    return (strips || mode == kDump || G * R <= 192 || G * R > 512) ? 1 : (G * R > 384 ? 4 : 2);
+}
+
 // tantan on DNA (sequence masking): bases 0..3 per letter, sequences at
 // offsets; probs in the same layout; scales: lengths/16 doubles per sequence at
 // scaleOff. Sixteen sequences per 64-lane wave, four lanes per sequence.

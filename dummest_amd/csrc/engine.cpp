@@ -166,7 +166,9 @@ static double pairCost(const dm_engine *e, int c, int M, int n) {
 }
 
 static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
-    if (e->forceCfg >= 0) return e->forceCfg;
+    // a forced configuration (autotuning, the "config" option) applies where its rows fit
+    if (e->forceCfg >= 0 && (e->cfgStrips[e->forceCfg] || e->cfgG[e->forceCfg] * e->cfgR[e->forceCfg] >= M + 1))
+        return e->forceCfg;
     int best = -1;
     double bestCost = 0;
     const bool tuned = !e->cfgStepNs.empty();
@@ -223,7 +225,7 @@ static void layoutNullScratch(Chunk &C) {
 }
 
 // Build the queue for caller pairs [sorted positions b, e).
-static void buildChunk(const dm_engine *E, const std::vector<int64_t> &sorted, const std::vector<int> &cfgOf,
+static void buildChunk(const dm_engine *E, int mode, const std::vector<int64_t> &sorted, const std::vector<int> &cfgOf,
                        const int32_t *prof, const int32_t *seq, size_t b, size_t en, Chunk &C) {
     const int ncfg = (int)E->cfgG.size();
     C = Chunk();
@@ -251,14 +253,47 @@ static void buildChunk(const dm_engine *E, const std::vector<int64_t> &sorted, c
     for (int cfg = 0; cfg < ncfg; cfg++) {
         auto &us = unitsByCfg[cfg];
         const int G = E->cfgG[cfg];
-        // longest first (units hold length-sorted pairs, so pair 0 is the longest)
-        std::stable_sort(us.begin(), us.end(), [&](const std::vector<int64_t> &x, const std::vector<int64_t> &y) {
-            return E->seqLen[seq[x[0]]] > E->seqLen[seq[y[0]]];
-        });
+        const int WG = dp_wg(G, E->cfgR[cfg], E->cfgStrips[cfg] != 0, mode);
+        auto lenOf = [&](const std::vector<int64_t> &x) { return x.empty() ? -1 : E->seqLen[seq[x[0]]]; };
+        if (WG == 1) {
+            // longest first (units hold length-sorted pairs, so pair 0 is the longest)
+            std::stable_sort(us.begin(), us.end(), [&](const std::vector<int64_t> &x, const std::vector<int64_t> &y) {
+                return lenOf(x) > lenOf(y);
+            });
+        } else {
+            // items of WG units of one profile (the kernel's waves share its
+            // emission table), longest items first; a profile's last item is
+            // padded with empty units. Units of a profile arrive consecutive
+            // and length-sorted, so an item's units have similar lengths.
+            std::vector<std::vector<std::vector<int64_t>>> items;
+            for (size_t a0 = 0; a0 < us.size();) {
+                std::vector<std::vector<int64_t>> it;
+                const int32_t pr = prof[us[a0][0]];
+                while (a0 < us.size() && (int)it.size() < WG && prof[us[a0][0]] == pr) it.push_back(std::move(us[a0++]));
+                items.push_back(std::move(it));
+            }
+            std::stable_sort(items.begin(), items.end(), [&](const auto &x, const auto &y) {
+                return lenOf(x[0]) > lenOf(y[0]);
+            });
+            us.clear();
+            for (auto &it : items) {
+                const size_t real = it.size();
+                for (size_t w = 0; w < (size_t)WG; w++) us.push_back(w < real ? std::move(it[w]) : std::vector<int64_t>());
+            }
+        }
         C.cfgFirstUnit[cfg] = (int)C.units.size();
         C.cfgUnits[cfg] = (int)us.size();
-        for (auto &u : us) {
+        for (size_t ui = 0; ui < us.size(); ui++) {
+            auto &u = us[ui];
             Unit U{};
+            if (u.empty()) {  // padding of a work item: the partner unit's profile, no pairs
+                U = C.units.back();
+                U.count = 0;
+                U.maxN = 0;
+                C.units.push_back(U);
+                C.unitCfg.push_back(cfg);
+                continue;
+            }
             U.prof = prof[u[0]];
             U.first = (int)C.pairs.size();
             U.count = (int)u.size();
@@ -411,7 +446,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             }
         }
         Chunk C;
-        buildChunk(e, sorted, cfgOf, prof, seq, b, en, C);
+        buildChunk(e, mode, sorted, cfgOf, prof, seq, b, en, C);
         DevChunk D;
         D.npairs = (int)C.pairs.size();
         D.records = C.records;
@@ -442,18 +477,20 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             a.nUnits = nu;
             a.wantBwd = wantBwd;
             a.stripSteps = C.cfgMaxN[cfg] + G - 1;
-            int grid = e->gridOpt > 0 ? e->gridOpt : e->numCUs * 8;
-            grid = std::min(grid, nu);
+            // grid in workgroups of WG waves; scratch per wave
+            const int WG = dp_wg(G, R, L.strips, mode);
+            int grid = e->gridOpt > 0 ? e->gridOpt : e->numCUs * 8 / WG;
+            grid = std::min(grid, nu / WG);
             if (mode == kScore) {
                 a.waveFloats = (int64_t)C.cfgMaxS[cfg] * a.stripSteps * ROWS * Pp;
-                const int64_t perWaveBytes = a.waveFloats * (int64_t)sizeof(float);
+                const int64_t perWaveBytes = a.waveFloats * (int64_t)sizeof(float) * WG;
                 grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, kScratchBudget / perWaveBytes));
-                maxW = std::max(maxW, a.waveFloats * grid);
-                maxE = std::max(maxE, a.waveFloats * grid / ROWS + 1);
+                maxW = std::max(maxW, a.waveFloats * grid * WG);
+                maxE = std::max(maxE, a.waveFloats * grid * WG / ROWS + 1);
             }
             if (L.strips) {
                 a.bndCols = C.cfgMaxN[cfg] + 2;
-                maxB = std::max(maxB, a.bndCols * grid * Pp);
+                maxB = std::max(maxB, a.bndCols * grid * WG * Pp);
             }
             L.grid = grid;
             D.launches.push_back(L);
@@ -637,7 +674,7 @@ static int dumpPair(dm_engine *e, int32_t prof, int32_t seq, PairDump &D) {
     std::vector<int64_t> sorted{0};
     std::vector<int> cfgOf{cfg};
     Chunk C;
-    buildChunk(e, sorted, cfgOf, &prof, &seq, 0, 1, C);
+    buildChunk(e, kDump, sorted, cfgOf, &prof, &seq, 0, 1, C);
     const int G = e->cfgG[cfg], R = e->cfgR[cfg], ROWS = G * R;
     const int S = stripsOf(e, cfg, M);
     const int64_t T = n + G - 1;
@@ -1247,6 +1284,10 @@ int dm_set_option(dm_engine *e, const char *name, int64_t value) {
     if (!e || !name) return DM_EINVAL;
     if (!std::strcmp(name, "plain")) e->plain = value != 0;
     else if (!std::strcmp(name, "grid")) e->gridOpt = (int)value;
+    else if (!std::strcmp(name, "config")) {
+        if (value < -1 || value >= (int64_t)e->cfgG.size()) return fail(e, DM_EINVAL, "configuration index out of range");
+        e->forceCfg = (int)value;
+    }
     else return fail(e, DM_EINVAL, std::string("unknown option ") + name);
     return DM_OK;
 }

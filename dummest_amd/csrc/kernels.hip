@@ -510,20 +510,35 @@ __device__ __forceinline__ int bidx(int c, int n) { return c < -1 ? 0 : (c > n ?
 // registers without spilling, and the second wave hides latency (about 8-13%
 // faster per step than the unconstrained allocation, which takes one wave).
 template <int G, int R, int MODE, bool STR>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1)))
-dp_kernel(DPArgs a) {
+__global__ void __launch_bounds__(64 * dp_wg(G, R, STR, MODE))
+__attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs a) {
     constexpr int ROWS = G * R;
+    constexpr int WG = dp_wg(G, R, STR, MODE);  // waves per workgroup (sharing the emission table)
     constexpr int kPF = (R <= 8) ? 4 : 2;  // prefetch distance in steps (divides 4)
     extern __shared__ float etab[];  // [kNCode][ROWS], then the profile's background table (32)
     float *bgl = etab + kNCode * ROWS;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wv = (WG > 1) ? (int)(threadIdx.x >> 6) : 0;  // wave of the workgroup
+    const int wslot = (int)blockIdx.x * WG + wv;             // this wave's scratch slot
     const int grp = lane / G;
     const int k = lane - grp * G;
+    __shared__ int sItem;
     for (;;) {
-        int got = 0;
-        if (lane == 0) got = atomicAdd(a.unitCounter, 1);
-        const int u = __builtin_amdgcn_readfirstlane(got);
-        if (u >= a.nUnits) break;
+        int u;
+        if (WG == 1) {
+            int got = 0;
+            if (lane == 0) got = atomicAdd(a.unitCounter, 1);
+            u = __builtin_amdgcn_readfirstlane(got);
+            if (u >= a.nUnits) break;
+        } else {
+            // one item = WG consecutive units of one profile (the host pairs them)
+            __syncthreads();  // the previous item is done with sItem and the table
+            if (threadIdx.x == 0) sItem = atomicAdd(a.unitCounter, 1);
+            __syncthreads();
+            const int item = sItem;
+            if (item * WG >= a.nUnits) break;
+            u = item * WG + wv;
+        }
         const Unit U = a.units[u];
         const ProfInfo PI = a.profs[U.prof];
         const int M = PI.M;
@@ -543,7 +558,7 @@ dp_kernel(DPArgs a) {
             wsc0 = a.wbuf + PR.bufOff;
             esc0 = a.ebuf + PR.bufOff / ROWS;
         } else {
-            const int64_t off = (int64_t)blockIdx.x * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G));
+            const int64_t off = (int64_t)wslot * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G));
             wsc0 = a.wbuf + off;
             esc0 = a.ebuf + off / ROWS;
         }
@@ -553,7 +568,7 @@ dp_kernel(DPArgs a) {
         int *bE = nullptr, *fE = nullptr;
         if (STR) {
             const int64_t nb = (int64_t)a.bndCols;
-            const int64_t gslot = (int64_t)blockIdx.x * (64 / G) + grp;
+            const int64_t gslot = (int64_t)wslot * (64 / G) + grp;
             float *base = a.bnd + gslot * 6 * nb;
             bW = base;            // [2][nb]
             bY2 = base + 2 * nb;  // [2][nb]
@@ -585,11 +600,11 @@ dp_kernel(DPArgs a) {
         auto loadStrip = [&](int strip, float *a0v, float *b0v, float *valid) {
             const int rb = strip * ROWS;
             __syncthreads();
-            for (int idx = lane; idx < kNCode * ROWS; idx += 64) {
+            for (int idx = (int)threadIdx.x; idx < kNCode * ROWS; idx += 64 * WG) {
                 const int c = idx / ROWS, r = rb + idx - c * ROWS;
                 etab[idx] = (r <= M) ? a.emis[PI.emOff + (int64_t)r * kEmStride + c] : 0.f;
             }
-            if (lane < 32) bgl[lane] = a.bg[(int64_t)U.prof * 32 + lane];
+            if (threadIdx.x < 32) bgl[threadIdx.x] = a.bg[(int64_t)U.prof * 32 + threadIdx.x];
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < R; q++) {
@@ -1117,28 +1132,31 @@ hipError_t launch_null(const NullArgs &a, hipStream_t s) {
 }
 
 // Tables above 64 KiB of LDS need the per-kernel dynamic-LDS limit raised.
-template <class K> static hipError_t launch_one(K kernel, size_t smem, int grid, hipStream_t s, const DPArgs &a) {
+// grid = workgroups; wg = waves per workgroup (dp_wg).
+template <class K>
+static hipError_t launch_one(K kernel, size_t smem, int grid, int wg, hipStream_t s, const DPArgs &a) {
     if (smem > 65536) {
         hipError_t r = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (r != hipSuccess) return r;
     }
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(64), smem, s, a);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(64 * wg), smem, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s) {
     const size_t smem = sizeof(float) * (kNCode * G * R + 32);
-#define DM_LAUNCH(g, r)                                                                             \
-    if (G == g && R == r && !strips) {                                                              \
-        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter, false>, smem, grid, s, a);  \
-        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore, false>, smem, grid, s, a);    \
-        return launch_one(dp_kernel<g, r, kDump, false>, smem, grid, s, a);                         \
+    const int wg = dp_wg(G, R, strips, mode);
+#define DM_LAUNCH(g, r)                                                                                 \
+    if (G == g && R == r && !strips) {                                                                  \
+        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter, false>, smem, grid, wg, s, a);  \
+        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore, false>, smem, grid, wg, s, a);    \
+        return launch_one(dp_kernel<g, r, kDump, false>, smem, grid, wg, s, a);                         \
     }
-#define DM_LAUNCH_STRIPS(g, r)                                                                      \
-    if (G == g && R == r && strips) {                                                               \
-        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter, true>, smem, grid, s, a);   \
-        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore, true>, smem, grid, s, a);     \
-        return launch_one(dp_kernel<g, r, kDump, true>, smem, grid, s, a);                          \
+#define DM_LAUNCH_STRIPS(g, r)                                                                          \
+    if (G == g && R == r && strips) {                                                                   \
+        if (mode == kFilter) return launch_one(dp_kernel<g, r, kFilter, true>, smem, grid, wg, s, a);   \
+        if (mode == kScore) return launch_one(dp_kernel<g, r, kScore, true>, smem, grid, wg, s, a);     \
+        return launch_one(dp_kernel<g, r, kDump, true>, smem, grid, wg, s, a);                          \
     }
     DM_CFGS(DM_LAUNCH)
     DM_STRIP_CFGS(DM_LAUNCH_STRIPS)

@@ -210,7 +210,9 @@ int dm_set_config_costs(dm_engine *e, int32_t n, const double *ns_per_step);
 int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int32_t *strips, double *ns_per_step);
 
 /* Options: "plain" = 1 launches the DP once per pair (the unbatched form);
- * "grid" = persistent grid size (0 = automatic). */
+ * "grid" = persistent grid size (0 = automatic); "config" = index into the
+ * configuration table (dm_config_table) that every pair whose profile fits it
+ * runs on (-1 = automatic choice). */
 int dm_set_option(dm_engine *e, const char *name, int64_t value);
 
 /* Device time of the DP kernels of the last scoring call (HIP events on the

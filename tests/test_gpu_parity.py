@@ -197,3 +197,38 @@ def test_calibration_matches_oracle(engine):
             assert fits[k]["mid"][key] == pytest.approx(ref, rel=1e-3), key
         for key, ref in O.gumbel(ref_flt, seq_len).items():
             assert fits[k]["filter"][key] == pytest.approx(ref, rel=1e-3), key
+
+
+def test_every_configuration(engine):
+    """Each compiled configuration forced in turn: single-strip kernels with one
+    or two waves per workgroup (two waves share the emission table of two units
+    of one profile; an odd unit count leaves a padded item) and the row-strip
+    kernels, on pairs whose profile fits, in filter and score modes."""
+    table = engine.config_table()
+    print(f"every configuration: {len(table)}", flush=True)
+    cases = {}
+    try:
+        for idx, (G, R, strips, _) in enumerate(table):
+            rows = G * R
+            M = int(rows * 2.3) if strips else rows - 3
+            npairs = 3 * (64 // G)  # three units of one profile
+            if M not in cases:
+                lengths = [150 + (37 * k) % 260 for k in range(npairs)]
+                pid, prof, codes, sids = load_case(engine, 900 + M, M, lengths, homolog_every=2)
+                rm, rb = O.fwdbwd_batch(prof, codes)
+                flt = [O.filter_score(prof, c) for c in codes]
+                cases[M] = (pid, sids, rm, rb, flt)
+            pid, sids, rm, rb, flt = cases[M]
+            sids = sids[:npairs]
+            engine.set_option("config", idx)
+            mm, me, bm, be = engine.fwdbwd_scores(pid, sids, want_bwd=True)
+            best = engine.find_similarities_backward_only(pid, sids)
+            for k in range(len(sids)):
+                assert abs(ln_es(mm[k], me[k]) - ln_es(*rm[k])) < TOL_NATS, (G, R, strips, k)
+                assert abs(ln_es(bm[k], be[k]) - ln_es(*rb[k])) < TOL_NATS, (G, R, strips, k)
+                if np.isfinite(flt[k]):
+                    assert abs(np.log(best[k]) - np.log(flt[k])) < TOL_NATS, (G, R, strips, k)
+                else:
+                    assert not np.isfinite(best[k])
+    finally:
+        engine.set_option("config", -1)
