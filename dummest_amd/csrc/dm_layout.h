@@ -89,6 +89,15 @@ struct NullArgs {
     float bgfs1, bgfs2;  // BACKGROUND_FRAMESHIFT_RATE(_2), as Float sums
 };
 
+// A strip-edge value of one column: the row value (W1 of the strip's top row
+// for the backward, the carried row sum for the forward), its Y, and the
+// exponent of the step that produced them.
+struct Bnd {
+    float v;
+    float y;
+    int e;
+};
+
 struct DPArgs {
     const ProfInfo *profs;
     const float *params;
@@ -105,8 +114,7 @@ struct DPArgs {
     int *fbuf;            // dump mode: per-step forward exponents
     int64_t waveFloats;   // score mode: scratch floats per wave
     int64_t stripSteps;   // steps per row strip in wbuf/ebuf (>= maxN + G - 1)
-    float *bnd;           // strip kernels: per-wave boundary values (6 x bndCols floats)
-    int *bndE;            // strip kernels: per-wave boundary exponents (3 x bndCols ints)
+    struct Bnd *bnd;      // strip kernels: per-group boundary records (3 x bndCols)
     int64_t bndCols;      // >= maxN + 2
     float *filterOut;     // filter mode: best W1*prefixSeed per pair
     PairScore *scoreOut;  // score mode
@@ -121,14 +129,16 @@ hipError_t launch_null(const NullArgs &a, hipStream_t s);
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 
-// Waves per workgroup of a DP configuration. Single-strip kernels whose
-// emission table is large (192 < G*R <= 512 rows: 19-51 KB of LDS) run two
-// (up to 384 rows) or four waves per workgroup on as many work units of the
-// same profile, sharing one table, so LDS stops limiting their occupancy; the
-// host groups the units (same profile, similar lengths) and pads a profile's
-// last group with empty units.
+// Waves per workgroup of a DP configuration. Kernels whose emission table is
+// large (192 < G*R <= 512 rows: 19-51 KB of LDS) run two (up to 384 rows) or
+// four waves per workgroup on as many work units of the same profile, sharing
+// one table (in strip kernels, one strip's table: the units of one profile run
+// the same strips in the same order), so LDS stops limiting their occupancy;
+// the host groups the units (same profile, similar lengths) and pads a
+// profile's last group with empty units.
 inline constexpr int dp_wg(int G, int R, bool strips, int mode) {
-    return (strips || mode == kDump || G * R <= 192 || G * R > 512) ? 1 : (G * R > 384 ? 4 : 2);
+    (void)strips;
+    return (mode == kDump || G * R <= 192 || G * R > 512) ? 1 : (G * R > 384 ? 4 : 2);
 }
 
 // tantan on DNA (sequence masking): bases 0..3 per letter, sequences at

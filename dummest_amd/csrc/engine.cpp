@@ -71,7 +71,7 @@ struct dm_engine {
     std::vector<char> cfgStrips;  // row-strip configurations (G = 64, any profile length)
     std::vector<double> cfgStepNs;  // autotuned device ns per wave-step (0: not measured)
     int forceCfg = -1;              // autotuning: every pair runs this configuration
-    DevBuf dBnd, dBndE;
+    DevBuf dBnd;
     bool plain = false;
     int gridOpt = 0;
     int numCUs = 256;
@@ -384,7 +384,7 @@ struct dm_plan {
     int64_t cells = 0;
     int64_t maxChunk = 0;
     std::vector<DevChunk> chunks;
-    DevBuf out, rec, nullScratch, counter, wbuf, ebuf, bnd, bndE;
+    DevBuf out, rec, nullScratch, counter, wbuf, ebuf, bnd;
     void *hostOut = nullptr;  // pinned staging for results
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -393,7 +393,7 @@ struct dm_plan {
             c.pairs.release();
             c.units.release();
         }
-        for (DevBuf *b : {&out, &rec, &nullScratch, &counter, &wbuf, &ebuf, &bnd, &bndE}) b->release();
+        for (DevBuf *b : {&out, &rec, &nullScratch, &counter, &wbuf, &ebuf, &bnd}) b->release();
         if (graphExec) (void)hipGraphExecDestroy(graphExec);
         if (graph) (void)hipGraphDestroy(graph);
         if (hostOut) (void)hipHostFree(hostOut);
@@ -510,8 +510,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         HIPCHK(P.wbuf.ensure(sizeof(float) * maxW));
         HIPCHK(P.ebuf.ensure(sizeof(int) * maxE));
     }
-    HIPCHK(P.bnd.ensure(sizeof(float) * 6 * maxB));
-    HIPCHK(P.bndE.ensure(sizeof(int) * 3 * maxB));
+    HIPCHK(P.bnd.ensure(sizeof(Bnd) * 3 * maxB));
     for (DevChunk &D : P.chunks)
         for (Launch &L : D.launches) {
             DPArgs &a = L.args;
@@ -524,8 +523,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             a.unitCounter = P.counter.as<int>() + L.cfg;
             a.wbuf = P.wbuf.as<float>();
             a.ebuf = P.ebuf.as<int>();
-            a.bnd = P.bnd.as<float>();
-            a.bndE = P.bndE.as<int>();
+            a.bnd = P.bnd.as<Bnd>();
             a.scoreOut = P.out.as<PairScore>();
             a.filterOut = P.out.as<float>();
         }
@@ -698,10 +696,8 @@ static int dumpPair(dm_engine *e, int32_t prof, int32_t seq, PairDump &D) {
     const bool strips = e->cfgStrips[cfg] != 0;
     if (strips) {
         a.bndCols = n + 2;
-        HIPCHK(e->dBnd.ensure((size_t)(6 * a.bndCols * (64 / G)) * sizeof(float)));
-        HIPCHK(e->dBndE.ensure((size_t)(3 * a.bndCols * (64 / G)) * sizeof(int)));
-        a.bnd = e->dBnd.as<float>();
-        a.bndE = e->dBndE.as<int>();
+        HIPCHK(e->dBnd.ensure((size_t)(3 * a.bndCols * (64 / G)) * sizeof(Bnd)));
+        a.bnd = e->dBnd.as<Bnd>();
     }
     HIPCHK(launch_dp(G, R, kDump, strips, a, 1, e->stream));
     D.G = G;
@@ -788,7 +784,6 @@ void dm_engine_destroy(dm_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     e->dBnd.release();
-    e->dBndE.release();
     for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dRec,
                       &e->dNull, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
                       &e->dOut})

@@ -514,7 +514,9 @@ __global__ void __launch_bounds__(64 * dp_wg(G, R, STR, MODE))
 __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs a) {
     constexpr int ROWS = G * R;
     constexpr int WG = dp_wg(G, R, STR, MODE);  // waves per workgroup (sharing the emission table)
-    constexpr int kPF = (R <= 8) ? 4 : 2;  // prefetch distance in steps (divides 4)
+    // prefetch distance in steps (divides 4); the strip kernels keep 2, which
+    // lets all but G = 32, R = 8 fit 256 registers (two waves per SIMD)
+    constexpr int kPF = (R <= 8 && (!STR || (G == 32 && R == 8))) ? 4 : 2;
     extern __shared__ float etab[];  // [kNCode][ROWS], then the profile's background table (32)
     float *bgl = etab + kNCode * ROWS;
     const int lane = threadIdx.x & 63;
@@ -562,31 +564,15 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             wsc0 = a.wbuf + off;
             esc0 = a.ebuf + off / ROWS;
         }
-        // strip boundary buffers (per group): two parities of backward (W, Y, E)
-        // and one of forward (N, Y, E), n + 2 entries each
-        float *bW = nullptr, *bY2 = nullptr, *fN = nullptr, *fY = nullptr;
-        int *bE = nullptr, *fE = nullptr;
+        // strip boundary buffers (per group), one (value, Y, exponent) record per
+        // column, n + 2 entries each: two parities for the backward, one for the forward
+        Bnd *bB = nullptr, *bF = nullptr;
         if (STR) {
             const int64_t nb = (int64_t)a.bndCols;
             const int64_t gslot = (int64_t)wslot * (64 / G) + grp;
-            float *base = a.bnd + gslot * 6 * nb;
-            bW = base;            // [2][nb]
-            bY2 = base + 2 * nb;  // [2][nb]
-            fN = base + 4 * nb;
-            fY = base + 5 * nb;
-            int *ebase = a.bndE + gslot * 3 * nb;
-            bE = ebase;           // [2][nb]
-            fE = ebase + 2 * nb;
-            for (int c = k; c < 2 * nb; c += G) {
-                bW[c] = 0.f;
-                bY2[c] = 0.f;
-                bE[c] = 0;
-            }
-            for (int c = k; c < nb; c += G) {
-                fN[c] = 0.f;
-                fY[c] = 0.f;
-                fE[c] = 0;
-            }
+            bB = a.bnd + gslot * 3 * nb;  // [2][nb]
+            bF = bB + 2 * nb;
+            for (int c = k; c < 3 * nb; c += G) bB[c] = Bnd{0.f, 0.f, 0};
         }
 
         float enterC[R], d0[R], d1s2[R], d2s1[R], a1s1[R], a2s2[R], eps[R];
@@ -597,7 +583,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
 
         // Per-strip setup: LDS emission rows and per-row coefficients
         // (dummest-dp-backward.hh:481-491, dummest-dp-forward.hh:72-80).
-        auto loadStrip = [&](int strip, float *a0v, float *b0v, float *valid) {
+        auto loadStrip = [&](int strip, float *a0v, float *b0v) {
             const int rb = strip * ROWS;
             __syncthreads();
             for (int idx = (int)threadIdx.x; idx < kNCode * ROWS; idx += 64 * WG) {
@@ -620,14 +606,16 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 a2s2[q] = ok ? pp[2] * s2 : 0.f;
                 b0v[q] = ok ? pp[3] : 0.f;
                 eps[q] = ok ? pp[9] : 0.f;
-                if (valid) valid[q] = ok ? 1.f : 0.f;
             }
         };
 
         // ------------------------------------------------------------- backward
         for (int strip = S - 1; strip >= 0; strip--) {
-            float a0sC[R], b0sC[R], valid[R];
-            loadStrip(strip, a0sC, b0sC, valid);
+            float a0sC[R], b0sC[R];
+            loadStrip(strip, a0sC, b0sC);
+            // rows of this lane that exist (r <= M): a prefix of its R rows; the
+            // seed enters only there
+            const int nValid = active ? max(0, min(R, M - (strip * ROWS + k * R) + 1)) : 0;
 #pragma unroll
             for (int q = 0; q < R; q++) {  // backward alpha0Coeff/beta0Coeff carry sC
                 a0sC[q] *= sC;
@@ -638,12 +626,8 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             const bool below = STR && strip < S - 1;  // a strip below feeds lane G-1
             const bool above = STR && strip > 0;      // lane 0 feeds the strip above
             const int par = strip & 1;
-            float *wOut = STR ? bW + par * a.bndCols : nullptr;
-            float *yOut = STR ? bY2 + par * a.bndCols : nullptr;
-            int *eOut = STR ? bE + par * a.bndCols : nullptr;
-            const float *wIn = STR ? bW + (par ^ 1) * a.bndCols : nullptr;
-            const float *yIn = STR ? bY2 + (par ^ 1) * a.bndCols : nullptr;
-            const int *eIn = STR ? bE + (par ^ 1) * a.bndCols : nullptr;
+            Bnd *bOut = STR ? bB + par * a.bndCols : nullptr;
+            const Bnd *bIn = STR ? bB + (par ^ 1) * a.bndCols : nullptr;
 
             float H[4][R], Zr[4][R], Yc[R], GW[4], LV[4];
 #pragma unroll
@@ -689,22 +673,13 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             for (int f = 1; f <= kPF; f++) rq[f % kPF] = brecAt(f, G - 1);
             Rec rcur = brecAt(0, k);
             // strip kernels: the strip below's top row (W1, Y, exponent), also ahead
-            float gqw[kPF], gqy[kPF];
-            int gqe[kPF];
-            auto ghostAt = [&](int tt, float &w, float &y, int &ge) {
-                if (STR && below) {
-                    const int bi = bidx(Tmine - 1 - tt - k, n);
-                    w = wIn[bi];
-                    y = yIn[bi];
-                    ge = eIn[bi];
-                } else {
-                    w = 0.f;
-                    y = 0.f;
-                    ge = 0;
-                }
+            Bnd gq4[kPF];
+            auto ghostAt = [&](int tt) -> Bnd {
+                if (STR && below) return bIn[bidx(Tmine - 1 - tt - k, n)];
+                return Bnd{0.f, 0.f, 0};
             };
 #pragma unroll
-            for (int f = 0; f < kPF; f++) ghostAt(f, gqw[f], gqy[f], gqe[f]);
+            for (int f = 0; f < kPF; f++) gq4[f] = ghostAt(f);
             // emission values of the next step, read from LDS one step ahead
             float eq[R];
 #pragma unroll
@@ -734,9 +709,9 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     // last lane: the strip below's top row from the boundary buffer.
                     // When that row sits far above this strip's scale, raise the
                     // exponent first (before any neighbour value is read).
-                    const float rw = gqw[PH % kPF], ry = gqy[PH % kPF];
-                    const int ge = gqe[PH % kPF];
-                    ghostAt(t + kPF, gqw[PH % kPF], gqy[PH % kPF], gqe[PH % kPF]);
+                    const float rw = gq4[PH % kPF].v, ry = gq4[PH % kPF].y;
+                    const int ge = gq4[PH % kPF].e;
+                    gq4[PH % kPF] = ghostAt(t + kPF);
                     if (MODE != kFilter) {
                         int need = 0;
                         if (k == G - 1 && (rw > 0.f || ry > 0.f)) {
@@ -774,7 +749,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     const float e = eq[q];
                     const float eE = e * enterC[q];
                     const float zg = Zr[S3][q] * rcg;
-                    float w = fmaf(bw3, eE, ssr * valid[q]);
+                    float w = fmaf(bw3, eE, q < nValid ? ssr : 0.f);
                     w = fmaf(bw2, d1s2[q], w);
                     w = fmaf(bw1, d2s1[q], w);
                     w = fmaf(zg, a0sC[q], w);
@@ -809,9 +784,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     bwdMax.cur = fmax(bwdMax.cur, (double)hm * pspD);
                 }
                 if (STR && above && k == 0 && c >= 0 && c < n) {
-                    wOut[c + 1] = H[S4][0];
-                    yOut[c + 1] = Yc[0];
-                    eOut[c + 1] = Eb;
+                    bOut[c + 1] = Bnd{H[S4][0], Yc[0], Eb};
                 }
                 if (MODE != kFilter) {
                     const int D = Tmine - 1 - t;
@@ -878,7 +851,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
         midMax.init(kPre);
         for (int strip = 0; strip < S; strip++) {
             float a0[R], b0[R];
-            loadStrip(strip, a0, b0, nullptr);
+            loadStrip(strip, a0, b0);
             const float *wsc = wsc0 + (int64_t)strip * strideS;
             const int *esc = esc0 + (int64_t)strip * a.stripSteps;
             const bool above = STR && strip > 0;      // lane 0 reads the strip above
@@ -947,22 +920,13 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             Rec fcur = frecAt(0, k);
 
             // strip kernels: the strip above's last row (carried sum, Y, exponent), also ahead
-            float fqn[kPF], fqy[kPF];
-            int fqe[kPF];
-            auto fghostAt = [&](int tt, float &nv, float &yv, int &ge) {
-                if (STR && above) {
-                    const int bi = bidx(tt - k, n);
-                    nv = fN[bi];
-                    yv = fY[bi];
-                    ge = fE[bi];
-                } else {
-                    nv = 0.f;
-                    yv = 0.f;
-                    ge = 0;
-                }
+            Bnd fq4[kPF];
+            auto fghostAt = [&](int tt) -> Bnd {
+                if (STR && above) return bF[bidx(tt - k, n)];
+                return Bnd{0.f, 0.f, 0};
             };
 #pragma unroll
-            for (int f = 0; f < kPF; f++) fghostAt(f, fqn[f], fqy[f], fqe[f]);
+            for (int f = 0; f < kPF; f++) fq4[f] = fghostAt(f);
             float eqf[R];  // emission values of the next step, one step ahead
 #pragma unroll
             for (int q = 0; q < R; q++) eqf[q] = etk[((fcur.codes >> 8) & 255) * ROWS + q];
@@ -995,9 +959,9 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 if (STR && above) {
                     // lane 0: the strip above's last row from the boundary buffer;
                     // raise this strip's exponent first when that row is far above it
-                    const float rn = fqn[SL], ry = fqy[SL];
-                    const int ge = fqe[SL];
-                    fghostAt(t + kPF, fqn[SL], fqy[SL], fqe[SL]);
+                    const float rn = fq4[SL].v, ry = fq4[SL].y;
+                    const int ge = fq4[SL].e;
+                    fq4[SL] = fghostAt(t + kPF);
                     int need = 0;
                     if (k == 0 && (rn > 0.f || ry > 0.f)) {
                         int ex;
@@ -1052,9 +1016,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 for (int q = 0; q < R; q++) eqf[q] = eqN[q];
                 gqf = gqN;
                 if (STR && below && k == G - 1 && c >= 0 && c < n) {
-                    fN[c + 1] = lastN;
-                    fY[c + 1] = lastY;
-                    fE[c + 1] = Ef;
+                    bF[c + 1] = Bnd{lastN, lastY, Ef};
                 }
                 if (MODE == kDump && inD && active && k == 0) efd[D] = Ef;
                 // exponent moves as in the backward (up at 2^100, down below 2^36)
