@@ -104,6 +104,13 @@ def _declare(L):
     L.dm_last_tantan_ms.restype = d
     L.dm_last_cells.argtypes = [vp]
     L.dm_last_cells.restype = i64
+    L.dm_xxh3_128.argtypes = [ctypes.c_char_p, i64, _P(ctypes.c_uint64)]
+    L.dm_calibration_key.argtypes = [ctypes.c_char_p, i32, _P(f), ctypes.c_char_p, i32, i32, i32, _P(d), d,
+                                     ctypes.c_char_p]
+    L.dm_cache_file_path.argtypes = [ctypes.c_char_p, i32]
+    L.dm_cache_read.argtypes = [ctypes.c_char_p, i64, ctypes.c_char_p, i32, _P(d)]
+    L.dm_cache_read.restype = i64
+    L.dm_cache_write.argtypes = [ctypes.c_char_p, i64, ctypes.c_char_p, i32, _P(d)]
 
 
 def _ptr(a, t):
@@ -204,6 +211,78 @@ def gumbel_fits(scores, seq_len) -> dict:
     if rc:
         raise DummestError(f"dm_gumbel_fits failed ({rc})")
     return dict(zip(GUMBEL_KEYS, out.tolist()))
+
+
+def xxh3_128(data: bytes) -> int:
+    """XXH3 128-bit hash (seed 0) as one integer, high half in the top 64 bits."""
+    out = (ctypes.c_uint64 * 2)()
+    rc = lib().dm_xxh3_128(data, len(data), out)
+    if rc:
+        raise DummestError(f"dm_xxh3_128 failed ({rc})")
+    return out[0] | (out[1] << 64)
+
+
+# Field order of a calibration-cache entry (CacheEntry, dummest-calibration.hh:148-181).
+CACHE_FIELDS = tuple(f"{m}{a}{q}" for m in ("MM", "ML", "LM") for q in ("L", "K", "Ksimple")
+                     for a in ("end", "beg", "mid") if not (m == "LM" and q == "Ksimple")) + (
+    "flt_mid_l", "flt_mid_k", "flt_mid_k_simple")
+
+
+def calibration_key(name: str, M: int, values, seq_len: int, border: int, trials: int, rates=DEFAULT_RATES,
+                    tantan_threshold: float = 0.5, binary_hash: str = "") -> str:
+    """computeCacheKey (dummest-calibration.hh:201-226) of a finalized profile
+    (values (M+1) x 29) and the calibration options."""
+    v = np.ascontiguousarray(values, np.float32)
+    if v.size != 29 * (M + 1):
+        raise ValueError("values must hold (M+1) x 29 floats")
+    r = np.array(rates, np.float64)
+    out = ctypes.create_string_buffer(33)
+    rc = lib().dm_calibration_key(name.encode(), M, _ptr(v, ctypes.c_float), binary_hash.encode(), seq_len, border,
+                                  trials, _ptr(r, ctypes.c_double), float(tantan_threshold), out)
+    if rc:
+        raise DummestError(f"dm_calibration_key failed ({rc})")
+    return out.value.decode()
+
+
+def cache_file_path() -> str:
+    """The calibration cache file the reference's search order picks (creates its directory)."""
+    out = ctypes.create_string_buffer(4096)
+    rc = lib().dm_cache_file_path(out, 4096)
+    if rc:
+        raise DummestError(f"dm_cache_file_path failed ({rc})")
+    return out.value.decode()
+
+
+def cache_read(path: str):
+    """Entries of a calibration cache file, in file order: [(key, 27 doubles)]."""
+    n = lib().dm_cache_read(path.encode(), 0, None, 0, None)
+    if n < 0:
+        raise DummestError(f"dm_cache_read failed ({n})")
+    stride = 256
+    keys = ctypes.create_string_buffer(max(1, n) * stride)
+    vals = np.zeros((max(1, n), 27), np.float64)
+    rc = lib().dm_cache_read(path.encode(), n, keys, stride, _ptr(vals, ctypes.c_double))
+    if rc != n:
+        raise DummestError(f"dm_cache_read failed ({rc})")
+    raw = keys.raw
+    return [(raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode(), vals[i].copy()) for i in range(n)]
+
+
+def cache_write(path: str, entries) -> None:
+    """Writes [(key, 27 doubles)] as a calibration cache file, in the given order."""
+    stride = 256
+    n = len(entries)
+    keys = ctypes.create_string_buffer(max(1, n) * stride)
+    vals = np.zeros((max(1, n), 27), np.float64)
+    for i, (k, v) in enumerate(entries):
+        kb = k.encode()
+        if len(kb) >= stride:
+            raise ValueError("key too long")
+        ctypes.memmove(ctypes.addressof(keys) + i * stride, kb, len(kb))
+        vals[i] = np.asarray(v, np.float64)
+    rc = lib().dm_cache_write(path.encode(), n, keys, stride, _ptr(vals, ctypes.c_double))
+    if rc:
+        raise DummestError(f"dm_cache_write failed ({rc})")
 
 
 def build_id() -> str:

@@ -20,9 +20,12 @@
 // the GPU (dm_tantan_dna) before they are decoded (dummest.cc:557-573); masked
 // letters print in lowercase (dummest-output.hh:6-30).
 //
+// Calibration cache (ProfileCache, dummest-calibration.hh:191-336): the same
+// file and keys as the reference's, so a profile calibrated once is looked up
+// instead of calibrated again (unless -S); only the misses go to the GPU.
+//
 // Not available here (the command fails with a message rather than giving
-// different results): the Gaussian profile filter (-d/-D). There is no
-// calibration cache: every run calibrates (on the GPU).
+// different results): the Gaussian profile filter (-d/-D).
 #include <getopt.h>
 #include <unistd.h>
 
@@ -31,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -38,6 +42,7 @@
 #include <sstream>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "dm_host.h"
@@ -242,6 +247,74 @@ int die(const std::string &msg) {
     return 1;
 }
 
+// The process's calibration cache, as the reference's global ProfileCache
+// (dummest-calibration.hh:191-336): read on first use, written back when the
+// process ends. The reference prints the file name when it first reads it
+// (in the first profile's calibration step, or at exit when no profile got
+// that far); announce() prints it at that point here, since the lookups of
+// all profiles happen before the one batched calibration pass.
+class CalibrationCache {
+public:
+    ~CalibrationCache() {
+        load();
+        announce();
+        std::vector<std::pair<std::string, CacheEntry>> v(entries_.begin(), entries_.end());
+        writeCalibrationCache(path_, v);  // an unwritable file is skipped silently, as in save()
+    }
+    void load() {
+        if (loaded_) return;
+        loaded_ = true;
+        path_ = calibrationCachePath();
+        std::vector<std::pair<std::string, CacheEntry>> v;
+        std::string err;
+        if (readCalibrationCache(path_, v, err)) {
+            for (auto &kv : v) entries_.emplace(std::move(kv.first), kv.second);  // the first of equal keys stays
+        } else {
+            std::cerr << "# Cache Load Error: " << err << ".\n";
+            std::error_code ec;
+            std::filesystem::remove(path_, ec);
+        }
+    }
+    void announce() {
+        if (announced_) return;
+        announced_ = true;
+        std::cout << "# Cache file: " << std::filesystem::path(path_) << std::endl;
+    }
+    bool find(const std::string &key, CacheEntry &e) const {
+        const auto it = entries_.find(key);
+        if (it == entries_.end()) return false;
+        e = it->second;
+        return true;
+    }
+    void store(const std::string &key, const CacheEntry &e) { entries_[key] = e; }
+    const std::string &binaryHashHex() {
+        if (binHash_.empty()) binHash_ = binaryHash();
+        return binHash_;
+    }
+
+private:
+    bool loaded_ = false, announced_ = false;
+    std::string path_, binHash_;
+    std::unordered_map<std::string, CacheEntry> entries_;
+} gCache;
+
+// The cache entry of one calibration: the mid-anchored fits f[0..7] and the
+// filter's method-of-moments fits f[8..10] of dm_calibrate, and the fits x of
+// the all -inf end- and start-anchored samples (estimateK's entry,
+// dummest-calibration.hh:584-599).
+CacheEntry entryFromFits(const double *f, const double *x) {
+    CacheEntry e;
+    for (int g = 0; g < 8; g++) {
+        e.v[3 * g] = x[g];
+        e.v[3 * g + 1] = x[g];
+        e.v[3 * g + 2] = f[g];
+    }
+    e.v[24] = f[8];
+    e.v[25] = f[9];
+    e.v[26] = f[10];
+    return e;
+}
+
 int engineFail(dm_engine *e, const char *what, int rc) {
     std::cerr << "dummest-amd: " << what << " failed (" << rc << "): " << dm_engine_error(e) << "\n";
     return 1;
@@ -271,13 +344,13 @@ int main(int argc, char *argv[]) {
         "  -e E, --evalue E  find similarities with E-value <= this (default: 10)\n"
         "  -N N, --total-length N  override total search space size (default: sum of input lengths)\n"
         "  -s S, --strand S  accepted for compatibility; each record is searched as given\n"
-        "  -m M, --mask M    simple-region masking: only 0 (none) is available here\n"
+        "  -m M, --mask M    mask simple regions of: 0=neither, 1=profile, 2=sequence, 3=both (default: 3)\n"
         "  -t T, --trials T  random sequences for calibration (default: 1000)\n"
         "  -l L, --length L  length of each random sequence (default: 5000)\n"
         "  -b B, --border B  border added to each random sequence (default: 100)\n"
         "  --batch N         stream sequences in chunks of N\n"
         "  -S F, --scores-file F  write bit scores of the random sequences to F and exit\n"
-        "  --warmup          calibrate and exit\n"
+        "  --warmup          calibrate cache-missing profiles and exit\n"
         "  --barithmetic / --bgeometric (default) / --bmedian  background probabilities\n"
         "  --insert1 F --insert2 F --delete1 F --delete2 F  frameshift rates per base\n"
         "  --stop-codon-prob F  --bg-stop-codon-prob F  --tantan-threshold F\n"
@@ -444,15 +517,48 @@ int main(int argc, char *argv[]) {
     if (nargs > 1 && o.evalue > 0) std::cout << "# E-value <= " << o.evalue << "\n";
 
     // ----------------------------------------------------------- calibration
+    // Profiles are looked up in profile order (not with -S), so a profile with
+    // the key of an earlier one calibrated in this run is a hit as in the
+    // reference's one-by-one loop; the misses are calibrated in one GPU pass.
     const int np = (int)profiles.size();
-    std::vector<double> fits((size_t)np * 16), midS, fltS;
-    if (o.scoresFile) {
-        midS.resize((size_t)np * o.trials);
-        fltS.resize((size_t)np * o.trials);
-    }
+    std::vector<std::string> keys(np);
+    std::vector<CacheEntry> entry(np);
+    std::vector<int> from(np, -1);  // -1: calibrated here, -2: cached, j >= 0: the entry of profile j
+    std::vector<int32_t> calPids;
+    std::vector<int> calIdx;
     if (np > 0) {
+        const char *ign = std::getenv("DUMMEST_CACHE_IGNORE_BINARY_HASH");
+        const bool ignoreBin = ign && *ign && std::strcmp(ign, "0") != 0;
+        const std::string bin = ignoreBin ? std::string() : gCache.binaryHashHex();
+        gCache.load();
+        std::unordered_map<std::string, int> firstHere;
+        for (int i = 0; i < np; i++) {
+            keys[i] = calibrationKey(bin, profiles[i], o.length, o.border, o.trials, o.rates, (float)o.tantan);
+            if (!o.scoresFile) {
+                if (gCache.find(keys[i], entry[i])) {
+                    from[i] = -2;
+                    continue;
+                }
+                const auto it = firstHere.find(keys[i]);
+                if (it != firstHere.end()) {
+                    from[i] = it->second;
+                    continue;
+                }
+                firstHere.emplace(keys[i], i);
+            }
+            calIdx.push_back(i);
+            calPids.push_back(pids[i]);
+        }
+    }
+    const int nc = (int)calIdx.size();
+    std::vector<double> fits((size_t)nc * 16), midS, fltS;
+    if (o.scoresFile) {
+        midS.resize((size_t)nc * o.trials);
+        fltS.resize((size_t)nc * o.trials);
+    }
+    if (nc > 0) {
         if (o.trials < 2) return die("calibration needs at least 2 trials");
-        rc = dm_calibrate(E, np, pids.data(), o.length, o.border, o.trials, o.scoresFile ? midS.data() : nullptr,
+        rc = dm_calibrate(E, nc, calPids.data(), o.length, o.border, o.trials, o.scoresFile ? midS.data() : nullptr,
                           o.scoresFile ? fltS.data() : nullptr, fits.data());
         if (rc) return engineFail(E, "dm_calibrate", rc);
     }
@@ -467,55 +573,63 @@ int main(int argc, char *argv[]) {
     // (dummest-dp-forward.hh:536-543); their fits are those of such a sample.
     double endFits[8];
     gumbelFits(std::vector<double>(std::max(2, o.trials), -INFINITY), (double)o.length, endFits);
+    for (int k = 0; k < nc; k++) entry[calIdx[k]] = entryFromFits(&fits[(size_t)k * 16], endFits);
+    for (int i = 0; i < np; i++)
+        if (from[i] >= 0) entry[i] = entry[from[i]];
+    // Stats block of one cache entry (printEntryStats, dummest-calibration.hh:605-638).
+    auto printEntry = [&](const CacheEntry &c) {
+        const double *e = c.v;
+        if (printVerbosity > 1) {
+            static const char *rows[8] = {"#lamMM", "#kMM", "#kMM1", "#lamML", "#kML", "#kML1", "#lamLM", "#kLM"};
+            std::cout << "#\tend-\tstart-\tmid-anchored\n";
+            for (int g = 0; g < 8; g++)
+                std::cout << rows[g] << "\t" << e[3 * g] << "\t" << e[3 * g + 1] << "\t" << e[3 * g + 2] << "\n";
+        } else if (printVerbosity > 0) {
+            std::cout << "# K: " << e[6] << " " << e[7] << " " << e[8] << "\n";
+        } else {
+            std::cout << "# K: " << e[8] << "\n";
+        }
+        std::cout << "# Lambda: " << e[2] << "\n";
+    };
     std::vector<double> K(np), lambda(np), Kf(np), lambdaF(np);
+    int warmHits = 0, warmCalibrated = 0, calK = 0;
     for (int i = 0; i < np; i++) {
         const HostProfile &p = profiles[i];
-        const double *f = &fits[(size_t)i * 16];
-        K[i] = f[1];
-        lambda[i] = f[0];
-        Kf[i] = f[9];
-        lambdaF[i] = f[8];
+        const CacheEntry &c = entry[i];
+        K[i] = c.v[5];
+        lambda[i] = c.v[2];
+        Kf[i] = c.v[25];
+        lambdaF[i] = c.v[24];
         std::cout << "\n# Profile name: " << p.name << "\n# Profile length: " << p.M << "\n";
         if (o.mask & 1) std::cout << "# Positions masked by tantan: " << p.masked << "\n";
         std::cout << "# Background letter probabilities:";
         for (int j = 0; j < 20; j++) std::cout << " " << p.bg[4 + j];
         std::cout << std::endl;
-        // sum of the amino-acid weights of the trial generator (dummest-calibration.hh:365-376)
-        float sum = 0;
-        static const int ncod[20] = {4, 2, 2, 2, 2, 4, 2, 3, 2, 6, 1, 2, 4, 2, 6, 6, 4, 4, 1, 2};
-        for (int j = 0; j < 20; j++) sum += (float)(p.bg[4 + j] * ncod[j]);
-        sum += (float)o.rates[5];
-        std::cout << "# sum is " << sum << std::endl;
-        // Stats block of one cache entry (dummest-calibration.hh:603-649):
-        // end- and start-anchored fits come from all -inf samples, and the
-        // filter block replaces only the method-of-moments fields.
-        auto printEntry = [&](const double *mm, const double *rest) {
-            const double *x = endFits;
-            if (printVerbosity > 1) {
-                std::cout << "#\tend-\tstart-\tmid-anchored\n";
-                std::cout << "#lamMM\t" << mm[0] << "\t" << mm[0] << "\t" << mm[1] << "\n"
-                          << "#kMM\t" << mm[2] << "\t" << mm[2] << "\t" << mm[3] << "\n"
-                          << "#kMM1\t" << mm[4] << "\t" << mm[4] << "\t" << mm[5] << "\n";
-                std::cout << "#lamML\t" << x[3] << "\t" << x[3] << "\t" << rest[3] << "\n"
-                          << "#kML\t" << x[4] << "\t" << x[4] << "\t" << rest[4] << "\n"
-                          << "#kML1\t" << x[5] << "\t" << x[5] << "\t" << rest[5] << "\n";
-                std::cout << "#lamLM\t" << x[6] << "\t" << x[6] << "\t" << rest[6] << "\n"
-                          << "#kLM\t" << x[7] << "\t" << x[7] << "\t" << rest[7] << "\n";
-            } else if (printVerbosity > 0) {
-                std::cout << "# K: " << mm[4] << " " << mm[4] << " " << mm[5] << "\n";
-            } else {
-                std::cout << "# K: " << mm[5] << "\n";
-            }
-            std::cout << "# Lambda: " << mm[1] << "\n";
-        };
-        // (end/start, mid) pairs of lambda, K, K with lambda 1
-        const double fullMM[6] = {endFits[0], f[0], endFits[1], f[1], endFits[2], f[2]};
-        const double fltMM[6] = {f[8], f[8], f[9], f[9], f[10], f[10]};
-        printEntry(fullMM, f);
-        printEntry(fltMM, f);
+        const bool cached = from[i] != -1;
+        if (!o.scoresFile) gCache.announce();  // the lookup reads the cache
+        if (cached) {
+            if (printVerbosity > 1) std::cout << "# Warning: using cached results\n";
+        } else {
+            // sum of the amino-acid weights of the trial generator (dummest-calibration.hh:362-377)
+            float sum = 0;
+            static const int ncod[20] = {4, 2, 2, 2, 2, 4, 2, 3, 2, 6, 1, 2, 4, 2, 6, 6, 4, 4, 1, 2};
+            for (int j = 0; j < 20; j++) sum += (float)(p.bg[4 + j] * ncod[j]);
+            sum += (float)o.rates[5];
+            std::cout << "# sum is " << sum << std::endl;
+            gCache.announce();  // with -S the store is the first use
+            gCache.store(keys[i], c);
+        }
+        printEntry(c);
+        CacheEntry flt = c;
+        for (int j = 0; j < 3; j++) {
+            flt.v[j] = c.v[24];
+            flt.v[3 + j] = c.v[25];
+            flt.v[6 + j] = c.v[26];
+        }
+        printEntry(flt);
         if (o.scoresFile) {
             for (int t = 0; t < o.trials; t++) {
-                const size_t k = (size_t)i * o.trials + t;
+                const size_t k = (size_t)calK * o.trials + t;
                 scores << "full\t" << p.name << "\t" << t + 1 << "\tend\t-inf\n";
                 scores << "full\t" << p.name << "\t" << t + 1 << "\tstart\t-inf\n";
                 scores << "full\t" << p.name << "\t" << t + 1 << "\tmid\t" << scoreStr(natsToLog2(midS[k])) << "\n";
@@ -523,8 +637,15 @@ int main(int argc, char *argv[]) {
                        << "\n";
             }
         }
+        if (!cached) calK++;
+        if (o.warmup) {
+            std::cout << "# Warmup: " << p.name << (cached ? " cache hit" : " calibrated") << "\n";
+            (cached ? warmHits : warmCalibrated)++;
+        }
     }
-    if (o.warmup) std::cout << "# Warmup: " << np << " profiles, 0 cache hits, " << np << " calibrated\n";
+    if (o.warmup)
+        std::cout << "# Warmup: " << np << " profiles, " << warmHits << " cache hits, " << warmCalibrated
+                  << " calibrated\n";
     if (o.scoresFile || o.warmup) return 0;
     if (nargs < 2 || np < 1) return 0;
     std::cout << std::endl;

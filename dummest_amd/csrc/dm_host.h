@@ -88,4 +88,32 @@ void calibSequence(const float *letterFreqs20, float bgStop, float bgfs1, float 
 // L-moment lambda, L-moment K.
 void gumbelFits(std::vector<double> scores, double seqLen, double out[8]);
 
+// Calibration cache (ProfileCache, dummest-calibration.hh:106-336).
+struct Hash128 {
+    uint64_t lo, hi;
+};
+// XXH3 128-bit hash, seed 0, default secret.
+Hash128 xxh3_128(const void *data, size_t len);
+// high 64 bits then low 64 bits, 16 lowercase hex digits each
+std::string hashHex(const Hash128 &h);
+// Hash of the running executable and of this library.
+std::string binaryHash();
+// The 27 doubles of a cache entry, in the reference's CacheEntry order:
+// lambda, K and K (lambda = 1) of the end-, start- and mid-anchored samples for
+// the method of moments, maximum likelihood (lambda, K, K1) and L-moments
+// (lambda, K), then the filter's method-of-moments mid lambda, K and K1.
+struct CacheEntry {
+    double v[27];
+};
+// computeCacheKey (dummest-calibration.hh:201-226) for a finalized profile.
+std::string calibrationKey(const std::string &binHash, const HostProfile &p, int seqLen, int border, int trials,
+                           const double rates[6], float tantanThr);
+// cache.bin in the first usable cache directory (getCacheFilePath, :233-265).
+std::string calibrationCachePath();
+// cereal BinaryArchive layout of unordered_map<string, CacheEntry>. A missing or
+// empty file reads as no entries; a malformed one returns false with err.
+bool readCalibrationCache(const std::string &path, std::vector<std::pair<std::string, CacheEntry>> &out,
+                          std::string &err);
+bool writeCalibrationCache(const std::string &path, const std::vector<std::pair<std::string, CacheEntry>> &entries);
+
 }  // namespace dm

@@ -24,6 +24,7 @@ extern "C" {
 #define DM_EHIP -2     /* HIP runtime error (allocation, launch, copy) */
 #define DM_EFORMAT -3  /* unreadable profile or sequence data */
 #define DM_EUNSUP -4   /* input outside what the kernels cover */
+#define DM_EIO -5      /* a file could not be written */
 
 typedef struct dm_engine dm_engine;
 
@@ -193,6 +194,29 @@ int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_l
  * of n scores (dummest-calibration.hh:7-104; out as in dm_calibrate's 8 fits). */
 int dm_calib_sequence(const float *bg, const double rates[6], int32_t L, int32_t border, int32_t trial, char *out);
 int dm_gumbel_fits(const double *scores, int32_t n, double seq_len, double out[8]);
+
+/* Calibration cache shared with the reference engine (ProfileCache,
+ * dummest-calibration.hh:191-336).
+ * dm_xxh3_128: XXH3 128-bit hash, seed 0 (out[0] low half, out[1] high half;
+ * the reference's Hash128, :106-146).
+ * dm_calibration_key: computeCacheKey (:201-226) for a finalized profile
+ * (values (M+1) x 29 as dm_get_profile returns them) and the calibration
+ * options: 32 hex digits and a NUL. binary_hash is the hex hash of the
+ * executable (getBinaryHash, :148-159), or NULL / "" when it is left out
+ * (DUMMEST_CACHE_IGNORE_BINARY_HASH).
+ * dm_cache_file_path: the cache file that getCacheFilePath (:233-265) picks
+ * (creates its directory).
+ * dm_cache_read / dm_cache_write: the cache file's layout (cereal binary
+ * archive of unordered_map<string, CacheEntry>, 27 doubles per entry in
+ * CacheEntry order, :148-181). dm_cache_read returns the number of entries
+ * (a missing or empty file has none) and copies up to cap of them (keys
+ * NUL-terminated at key_stride bytes each); DM_EFORMAT for a malformed file. */
+int dm_xxh3_128(const void *data, int64_t len, uint64_t out[2]);
+int dm_calibration_key(const char *name, int32_t M, const float *values, const char *binary_hash, int32_t seq_len,
+                       int32_t border, int32_t trials, const double rates[6], double tantan_threshold, char out[33]);
+int dm_cache_file_path(char *out, int32_t cap);
+int64_t dm_cache_read(const char *path, int64_t cap, char *keys, int32_t key_stride, double *values);
+int dm_cache_write(const char *path, int64_t n, const char *keys, int32_t key_stride, const double *values);
 
 /* Autotuner: times every kernel configuration (G lanes per pair, R rows per
  * lane, and the row-strip kernel) on this device with synthetic pairs of

@@ -260,6 +260,87 @@ def ref_tantan(seq: np.ndarray, is_protein: bool):
     return out[:len(c)]
 
 
+def cache_key(name: str, M: int, values, seq_len: int, border: int, trials: int, opts=DEFAULT_OPTS,
+              tantan_threshold=0.5, binary_hash: str = "") -> str:
+    """computeCacheKey (dummest-calibration.hh:201-226): the XXH3-128 hash of
+    the binary hash text (left out under DUMMEST_CACHE_IGNORE_BINARY_HASH),
+    the profile name, width and length (int), the finalized values ((M+1) x
+    width floats), the background letter frequencies (the end row's 20
+    letters), sequence length, border and trial count (int), then the float
+    options INSERT1 INSERT2 DELETE1 DELETE2, BACKGROUND_FRAMESHIFT_RATE (=
+    INSERT1 + DELETE2) and _2 (= INSERT2 + DELETE1) summed in float
+    (dummest-core.hh:118-124), STOP, BG_STOP, TANTAN_MASK_THRESHOLD; printed
+    high 64 bits first (Hash128::to_string, :134-143). The hash is the
+    xxhash package's XXH3 (libxxhash 0.8.2; the reference vendors xxHash 0.8
+    as include/xxhash.h)."""
+    import struct
+
+    import xxhash
+    v = np.ascontiguousarray(values, np.float32).reshape(M + 1, WIDTH)
+    f32 = np.float32
+    i1, i2, d1, d2 = (f32(x) for x in opts[:4])
+    b = binary_hash.encode() + name.encode() + struct.pack("<ii", WIDTH, M) + v.tobytes()
+    b += v[M, 4:24].tobytes() + struct.pack("<iii", seq_len, border, trials)
+    b += np.array([i1, i2, d1, d2, i1 + d2, i2 + d1, opts[4], opts[5], tantan_threshold], np.float32).tobytes()
+    return "%032x" % xxhash.xxh3_128(b).intdigest()
+
+
+REF_CACHE = os.path.join(HERE, "_ref", "libcache_ref.so")
+
+
+def ref_cache():
+    """The reference's vendored XXH3 and cereal (oracle/_ref/libcache_ref.so,
+    built by `make ref`), or None when that library was not built."""
+    if not os.path.exists(REF_CACHE):
+        return None
+    L = ctypes.CDLL(REF_CACHE)
+    L.ref_xxh3_128_pieces.argtypes = [ctypes.c_char_p, _p(ctypes.c_int64), ctypes.c_int, _p(ctypes.c_uint64)]
+    L.ref_cache_write.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int,
+                                  _p(ctypes.c_double)]
+    L.ref_cache_read.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int,
+                                 _p(ctypes.c_double)]
+    L.ref_cache_read.restype = ctypes.c_int64
+    return L
+
+
+def ref_xxh3_pieces(data: bytes, sizes) -> int:
+    """XXH3_128bits of data fed in pieces of the given sizes (the Hash128
+    add() sequence), by the reference's vendored xxhash.h."""
+    L = ref_cache()
+    sz = np.ascontiguousarray(sizes, np.int64)
+    assert int(sz.sum()) == len(data)
+    out = (ctypes.c_uint64 * 2)()
+    L.ref_xxh3_128_pieces(data, _ptr(sz, ctypes.c_int64), len(sz), out)
+    return out[0] | (out[1] << 64)
+
+
+def ref_cache_write(path: str, entries) -> None:
+    """Writes {key: 27 doubles} through the reference's vendored cereal."""
+    L = ref_cache()
+    n, stride = len(entries), 256
+    keys = ctypes.create_string_buffer(max(1, n) * stride)
+    vals = np.zeros((max(1, n), 27), np.float64)
+    for i, (k, v) in enumerate(entries):
+        ctypes.memmove(ctypes.addressof(keys) + i * stride, k.encode(), len(k))
+        vals[i] = v
+    assert L.ref_cache_write(path.encode(), n, keys, stride, _ptr(vals, ctypes.c_double)) == 0
+
+
+def ref_cache_read(path: str):
+    """{key: 27 doubles} read through the reference's vendored cereal, or None
+    when cereal throws."""
+    L = ref_cache()
+    n = L.ref_cache_read(path.encode(), 0, None, 1, None)
+    if n < 0:
+        return None
+    stride = 256
+    keys = ctypes.create_string_buffer(max(1, n) * stride)
+    vals = np.zeros((max(1, n), 27), np.float64)
+    L.ref_cache_read(path.encode(), n, keys, stride, _ptr(vals, ctypes.c_double))
+    raw = keys.raw
+    return {raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode(): vals[i].copy() for i in range(n)}
+
+
 def decode(dna: str | bytes) -> np.ndarray:
     b = dna.encode() if isinstance(dna, str) else dna
     out = np.zeros(len(b), dtype=np.uint8)

@@ -18,3 +18,10 @@ def engine():
     e = D.Engine(0)
     yield e
     e.close()
+
+
+@pytest.fixture(autouse=True)
+def _calibration_cache_dir(tmp_path_factory, monkeypatch):
+    # The engine executable reads and writes a calibration cache when it ends;
+    # every test gets a fresh cache directory of its own.
+    monkeypatch.setenv("DUMMEST_CACHE_DIR", str(tmp_path_factory.mktemp("cache")))
