@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <iosfwd>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -87,6 +89,40 @@ void calibSequence(const float *letterFreqs20, float bgStop, float bgfs1, float 
 // out = MM lambda, MM K, MM K (lambda 1), ML lambda, ML K, ML K (lambda 1),
 // L-moment lambda, L-moment K.
 void gumbelFits(std::vector<double> scores, double seqLen, double out[8]);
+
+// Profile builder (dummest-build, dummest-build.cc).
+struct BuildGapPrior {  // match, insStart, delStart, insEnd, insExtend, delEnd, delExtend pseudocounts
+    double match = 0, insStart = 0, delStart = 0, insEnd = 0, insExt = 0, delEnd = 0, delExt = 0;
+};
+struct BuildOptions {
+    bool hand = false;       // --hand: RF annotation defines the positions
+    bool counts = false;     // --counts: print weighted counts
+    bool countOnly = false;  // --countonly: no Baum-Welch
+    bool pnone = false;      // --pnone: no priors
+    double symfrac = 0.5;    // --symfrac
+    double ere = 0;          // --ere (0: 0.59 protein, 0.62 DNA)
+    double esigma = 45.0;    // --esigma (1e37: --enone)
+    double maxIter = 1000;   // --maxiter
+    double maxDiff = 1e-6;   // --maxdiff
+    int verbosity = 0;
+    std::vector<double> mixture;  // --dmix: per component the coefficient then the alphas
+    int mixtureComponents = 0;
+    bool hasGapPrior = false;  // --gapprior
+    BuildGapPrior gapPrior;
+    std::string versionText = "DUMMEST-AMD 1";  // in the header's brackets
+    std::string bmText;                          // the BM line after "BM   "
+};
+// A malformed alignment (fatal: the reference prints "FATAL: ..." and aborts)
+// or a Baum-Welch overflow (the reference prints the message and exits 1).
+struct BuildError : std::runtime_error {
+    explicit BuildError(const char *m, bool fatalError = true) : std::runtime_error(m), fatal(fatalError) {}
+    bool fatal;
+};
+bool readGapPrior(std::istream &in, BuildGapPrior &gp);
+bool readMixture(std::istream &in, std::vector<double> &params, int &components);
+// Builds a profile per alignment of `in`, writes them to out and progress to
+// log; returns 0 or 1 (after an error message on log); throws BuildError.
+int buildProfiles(std::istream &in, const BuildOptions &o, std::ostream &out, std::ostream &log);
 
 // Calibration cache (ProfileCache, dummest-calibration.hh:106-336).
 struct Hash128 {
