@@ -104,6 +104,10 @@ def _declare(L):
     L.dm_last_tantan_ms.restype = d
     L.dm_last_cells.argtypes = [vp]
     L.dm_last_cells.restype = i64
+    L.dm_bw_prepare.argtypes = [vp, i32, i32, i32, _P(u8), _P(i64), _P(i32), _P(d)]
+    L.dm_bw_estep.argtypes = [vp, _P(d), _P(d), _P(d)]
+    L.dm_last_bw_ms.argtypes = [vp]
+    L.dm_last_bw_ms.restype = d
     L.dm_xxh3_128.argtypes = [ctypes.c_char_p, i64, _P(ctypes.c_uint64)]
     L.dm_calibration_key.argtypes = [ctypes.c_char_p, i32, _P(f), ctypes.c_char_p, i32, i32, i32, _P(d), d,
                                      ctypes.c_char_p]
@@ -429,6 +433,32 @@ class Engine:
         self._check(lib().dm_add_dna(self._h, len(seqs), data, _ptr(offs, ctypes.c_int64),
                                      _ptr(lens, ctypes.c_int32), ctypes.byref(first)), "dm_add_dna")
         return list(range(first.value, first.value + len(seqs)))
+
+    # -- profile builder -----------------------------------------------------
+    def bw_prepare(self, A: int, L: int, seqs, weights):
+        """Upload one alignment's ungapped sequences (letter codes 0..A) and
+        weights for Baum-Welch against a profile of L positions."""
+        lens = np.array([len(x) for x in seqs], np.int32)
+        offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64) if len(seqs) else np.zeros(0, np.int64)
+        data = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint8) for x in seqs] + [np.zeros(1, np.uint8)]))
+        w = np.ascontiguousarray(weights, np.float64)
+        self._bw = (A, L, len(seqs))
+        self._check(lib().dm_bw_prepare(self._h, A, L, len(seqs), _ptr(data, ctypes.c_uint8),
+                                        _ptr(offs, ctypes.c_int64), _ptr(lens, ctypes.c_int32),
+                                        _ptr(w, ctypes.c_double)), "dm_bw_prepare")
+
+    def bw_estep(self, probs):
+        """One Baum-Welch E-step on the device: (counts (L+1) x (7+A), forward totals)."""
+        A, L, n = self._bw
+        p = np.ascontiguousarray(probs, np.float64)
+        counts = np.zeros((L + 1, 7 + A), np.float64)
+        totals = np.zeros(max(1, n), np.float64)
+        self._check(lib().dm_bw_estep(self._h, _ptr(p, ctypes.c_double), _ptr(counts, ctypes.c_double),
+                                      _ptr(totals, ctypes.c_double)), "dm_bw_estep")
+        return counts, totals[:n]
+
+    def last_bw_ms(self) -> float:
+        return lib().dm_last_bw_ms(self._h)
 
     # -- scoring ------------------------------------------------------------
     @staticmethod

@@ -19,16 +19,20 @@
 //
 // Letters outside the alphabet (X, B, Z, ...) get a zero likelihood ratio in
 // Baum-Welch; the reference reads past its probability table for them.
+// Baum-Welch's expected counts run on the GPU (bw.hip); without a HIP device
+// only --countonly works.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <numeric>
 #include <sstream>
 #include <string>
 #include <vector>
 
+#include "../../include/dummest_amd.h"
 #include "dm_host.h"
 
 namespace dm {
@@ -71,10 +75,6 @@ const double kWheeler4[4 * 5] = {
 
 // HMMER 3.4 defaults (dummest-build.cc:26-34)
 constexpr double kEreAmino = 0.59, kEreNucleotide = 0.62;
-
-// Forward/backward values are kept scaled down by 2^-480 so that long
-// alignments do not overflow (dummest-build.cc:36-38).
-const double kScale = std::ldexp(1.0, -480);
 
 struct Mixture {
     const double *params;
@@ -388,128 +388,21 @@ double capForEntropy(const Mixture &mix, const BuildGapPrior &gp, int A, double 
 }
 
 // ----------------------------------------------------------- Baum-Welch
-// One sequence against the profile (probs, width 7 + A), in the reference's
-// state space: X (match), Y (delete), Z (insert) forward values and their
-// backward counterparts, scaled by 2^-480 per step (forward, backward,
-// calculateTransitionCounts, :561-736).
-struct BWWork {
-    std::vector<double> X, Y, Z, Wb, Yb, Zb;
-};
-
-struct PositionRates {
-    double a, b, d, e;  // insert start', insert extend, delete start', delete extend'
-};
-
-inline PositionRates ratesAt(const double *probs, int width, int L, int row) {
-    const double ins = probs[row * width + 1], ext = probs[row * width + 4];
-    const double del = probs[row * width + 2], dext = probs[row * width + 6];
-    const double nextDext = (row == L) ? 0.0 : probs[(row + 1) * width + 6];
-    PositionRates r;
-    r.a = ins * (1.0 - ext);
-    r.b = ext;
-    r.d = del * (1 - nextDext);
-    r.e = dext * (1 - nextDext) / (1 - dext);
-    if (!std::isfinite(r.e)) r.e = 0.0;
-    return r;
-}
-
-// Match-state likelihood ratio of letter x at profile row `row`.
-inline double matchRatio(const double *probs, int width, int L, int A, int row, int x) {
-    if (x >= A) return 0.0;
-    const double *p = probs + row * width;
-    double s = (1 - p[1] - p[2]) * p[7 + x] / probs[(size_t)L * width + 7 + x];
-    return std::isfinite(s) ? s : 0.0;
-}
-
-double forwardPass(const unsigned char *seq, int n, const double *probs, int L, int A, BWWork &w) {
-    const int width = 7 + A, cols = n + 2;
-    double total = 0.0;
-    std::fill_n(&w.X[0], n + 1, 0.0);
-    std::fill_n(&w.Y[1], n + 1, 0.0);
-    for (int i = 1; i <= L + 1; i++) {
-        const PositionRates r = ratesAt(probs, width, L, i - 1);
-        w.X[(size_t)(i - 1) * cols] = 0.0;
-        double z = w.Z[(size_t)i * cols] = 0.0;
-        for (int j = 1; j <= n + 1; j++) {
-            const double S = matchRatio(probs, width, L, A, i - 1, seq[j - 1]);
-            const double y = w.Y[(size_t)(i - 1) * cols + j];
-            const double v = w.X[(size_t)(i - 1) * cols + (j - 1)] + y + z + kScale;
-            z = r.a * v + r.b * z;
-            total += v;
-            w.X[(size_t)i * cols + j] = S * v;
-            w.Y[(size_t)i * cols + j] = r.d * v + r.e * y;
-            w.Z[(size_t)i * cols + j] = z;
-        }
-    }
-    return total;
-}
-
-void backwardPass(const unsigned char *seq, int n, const double *probs, int L, int A, BWWork &w) {
-    const int width = 7 + A, cols = n + 2;
-    std::fill_n(&w.Wb[(size_t)(L + 1) * cols + 1], n + 1, 0.0);
-    std::fill_n(&w.Yb[(size_t)(L + 1) * cols + 0], n + 1, 0.0);
-    for (int i = L; i >= 0; i--) {
-        const PositionRates r = ratesAt(probs, width, L, i);
-        w.Wb[(size_t)(i + 2) * cols - 1] = 0.0;
-        double z = w.Zb[(size_t)(i + 1) * cols - 1] = 0.0;
-        for (int j = n; j >= 0; j--) {
-            const double S = matchRatio(probs, width, L, A, i, seq[j]);
-            const double x = S * w.Wb[(size_t)(i + 1) * cols + (j + 1)];
-            const double y = w.Yb[(size_t)(i + 1) * cols + j];
-            const double v = x + r.d * y + r.a * z + kScale;
-            z = v + r.b * z;
-            w.Wb[(size_t)i * cols + j] = v;
-            w.Yb[(size_t)i * cols + j] = v + r.e * y;
-            w.Zb[(size_t)i * cols + j] = z;
-        }
-    }
-}
-
-// Adds one sequence's expected event counts, weighted by wt.
-void expectedCounts(const unsigned char *seq, int n, int L, int A, double wt, const BWWork &w, double *counts) {
-    const int width = 7 + A, cols = n + 2;
-    double prevMatch = 0;
-    for (int i = 1; i <= L + 1; i++) {
-        double emis[32] = {0};
-        double insEnd = 0, insExt = 0, delEnd = 0, delExt = 0, notIns = 0;
-        for (int j = 1; j <= n + 1; j++) {
-            const size_t up = (size_t)(i - 1) * cols, here = (size_t)i * cols;
-            insEnd += w.Z[here + (j - 1)] * w.Wb[up + (j - 1)];
-            insExt += w.Z[here + (j - 1)] * w.Zb[up + (j - 1)];
-            delEnd += w.Y[up + j] * w.Wb[up + (j - 1)];
-            delExt += w.Y[up + j] * w.Yb[up + (j - 1)];
-            notIns += w.Wb[up + (j - 1)];
-            emis[seq[j - 1]] += w.X[here + j] * w.Wb[here + j];
-        }
-        insExt -= insEnd;
-        delExt -= delEnd;
-        const double match = sumOf(emis, A + 1);
-        const double notInsert = notIns * kScale + prevMatch + delEnd;
-        prevMatch = match;
-        double *c = counts + (size_t)(i - 1) * width;
-        c[0] += match * wt;
-        c[1] += notInsert * wt;
-        c[3] += insEnd * wt;
-        c[4] += insExt * wt;
-        c[5] += delEnd * wt;
-        c[6] += delExt * wt;
-        if (i == L + 1) continue;
-        for (int k = 0; k < A; k++) c[7 + k] += emis[k] * wt;
-    }
-}
-
 // Iterates expected counts -> probabilities until no probability moves by
-// more than maxDiff (baumWelch, :738-822).
+// more than maxDiff (baumWelch, :738-822). The expected counts of each
+// iteration (forward, backward and calculateTransitionCounts over every
+// sequence, :561-736) come from the gfx950 kernels (dm_bw_estep); the delete
+// starts, the priors and the convergence test stay here.
 void baumWelch(std::vector<double> &counts, const Alignment &a, int A, const Mixture &mix, const BuildGapPrior &gp,
-               int L, const std::vector<double> &weights, double maxIter, double maxDiff, std::ostream &log) {
+               int L, const std::vector<double> &weights, double maxIter, double maxDiff, dm_engine *E,
+               std::ostream &log) {
     const int width = 7 + A;
     std::vector<double> oldP(counts.size()), newP(counts.size());
-    const size_t area = (size_t)(L + 2) * (a.ncol + 2);
-    BWWork w;
-    for (auto *v : {&w.X, &w.Y, &w.Z, &w.Wb, &w.Yb, &w.Zb}) v->assign(area, 0.0);
-    std::vector<unsigned char> letters;
-    std::vector<int> lengths(a.nseq, 0);
-    for (int s = 0; s < a.nseq; s++)
+    std::vector<uint8_t> letters;
+    std::vector<int64_t> offsets(a.nseq);
+    std::vector<int32_t> lengths(a.nseq, 0);
+    for (int s = 0; s < a.nseq; s++) {
+        offsets[s] = (int64_t)letters.size();
         for (int c = 0; c < a.ncol; c++) {
             const unsigned char x = a.cells[(size_t)s * a.ncol + c];
             if (x <= A) {
@@ -517,21 +410,18 @@ void baumWelch(std::vector<double> &counts, const Alignment &a, int A, const Mix
                 lengths[s]++;
             }
         }
-    letters.push_back(0);
+    }
+    if (dm_bw_prepare(E, A, L, a.nseq, letters.data(), offsets.data(), lengths.data(), weights.data()))
+        throw BuildError((std::string("Baum-Welch on the GPU: ") + dm_engine_error(E)).c_str(), false);
+    std::vector<double> totals(std::max(1, a.nseq));
     probabilities(mix, gp, A, 1e37, L, counts.data(), oldP.data());
     bool converged = false;
     int iter = 0;
     do {
-        std::fill(counts.begin(), counts.end(), 0.0);
-        const unsigned char *seq = letters.data();
-        for (int s = 0; s < a.nseq; s++) {
-            const int n = lengths[s];
-            const double total = forwardPass(seq, n, oldP.data(), L, A, w);
-            if (!std::isfinite(total)) throw BuildError("numbers overflowed to infinity in Baum-Welch: quitting", false);
-            backwardPass(seq, n, oldP.data(), L, A, w);
-            expectedCounts(seq, n, L, A, weights[s] / (total * kScale), w, counts.data());
-            seq += n;
-        }
+        if (dm_bw_estep(E, oldP.data(), counts.data(), totals.data()))
+            throw BuildError((std::string("Baum-Welch on the GPU: ") + dm_engine_error(E)).c_str(), false);
+        for (int s = 0; s < a.nseq; s++)
+            if (!std::isfinite(totals[s])) throw BuildError("numbers overflowed to infinity in Baum-Welch: quitting", false);
         // delete starts from the delete ends and extensions around them
         for (int i = 1; i <= L; i++) {
             const double d = counts[(size_t)i * width + 5] + counts[(size_t)i * width + 6] -
@@ -630,6 +520,8 @@ bool readMixture(std::istream &in, std::vector<double> &params, int &components)
 int buildProfiles(std::istream &in, const BuildOptions &o, std::ostream &out, std::ostream &log) {
     Alignment a;
     unsigned long count = 0;
+    // the GPU is opened when the first Baum-Welch starts (--countonly needs none)
+    std::unique_ptr<dm_engine, void (*)(dm_engine *)> engine(nullptr, dm_engine_destroy);
     for (;;) {
         const bool more = readAlignment(in, a);
         if (!more) break;
@@ -688,7 +580,14 @@ int buildProfiles(std::istream &in, const BuildOptions &o, std::ostream &out, st
         std::vector<int> columns;
         std::vector<double> counts = eventCounts(a, A, o.hand, o.symfrac, weights, wsum, gp, columns);
         const int L = (int)columns.size();
-        if (!o.countOnly) baumWelch(counts, a, A, mix, gp, L, weights, o.maxIter, o.maxDiff, log);
+        if (!o.countOnly) {
+            if (!engine) {
+                dm_engine *E = nullptr;
+                if (dm_engine_create(o.device, &E)) throw BuildError("no usable HIP device for Baum-Welch", false);
+                engine.reset(E);
+            }
+            baumWelch(counts, a, A, mix, gp, L, weights, o.maxIter, o.maxDiff, engine.get(), log);
+        }
         if (o.counts) {
             writeProfile(out, counts.data(), columns, alphabet, L, a, wsum, true, o);
             continue;

@@ -111,6 +111,7 @@ struct BuildOptions {
     BuildGapPrior gapPrior;
     std::string versionText = "DUMMEST-AMD 1";  // in the header's brackets
     std::string bmText;                          // the BM line after "BM   "
+    int device = 0;                              // HIP device for Baum-Welch
 };
 // A malformed alignment (fatal: the reference prints "FATAL: ..." and aborts)
 // or a Baum-Welch overflow (the reference prints the message and exits 1).

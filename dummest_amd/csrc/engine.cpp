@@ -84,6 +84,13 @@ struct dm_engine {
     int maskBits = 0;       // -m: 1 = profiles, 2 = sequences
     float maskThr = 0.5f;   // --tantan-threshold (TANTAN_MASK_THRESHOLD, a Float)
     DevBuf dTanBases, dTanProbs, dTanScales, dTanMeta, dTanB2f;
+    // profile builder's Baum-Welch (dm_bw_prepare / dm_bw_estep)
+    int bwL = -1, bwA = 0, bwN = 0;
+    std::vector<int> bwBatch;       // first sequence of each batch, then nseq
+    int64_t bwMatCells = 0;         // cells of the largest batch (per matrix)
+    int64_t bwBudget = (int64_t)8 << 30;  // bytes of forward/backward matrices per batch
+    DevBuf dBwLetters, dBwMeta, dBwTables, dBwMat, dBwRowTot, dBwRaw, dBwSeqW, dBwOut;
+    double lastBwMs = 0;
 
     float profileMaskThr() const { return (maskBits & 1) ? maskThr : -1.f; }
 
@@ -786,7 +793,9 @@ void dm_engine_destroy(dm_engine *e) {
     e->dBnd.release();
     for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dRec,
                       &e->dNull, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
-                      &e->dOut})
+                      &e->dOut, &e->dTanBases, &e->dTanProbs, &e->dTanScales, &e->dTanMeta, &e->dTanB2f,
+                      &e->dBwLetters, &e->dBwMeta, &e->dBwTables, &e->dBwMat, &e->dBwRowTot, &e->dBwRaw,
+                      &e->dBwSeqW, &e->dBwOut})
         b->release();
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
@@ -1279,6 +1288,10 @@ int dm_set_option(dm_engine *e, const char *name, int64_t value) {
     if (!e || !name) return DM_EINVAL;
     if (!std::strcmp(name, "plain")) e->plain = value != 0;
     else if (!std::strcmp(name, "grid")) e->gridOpt = (int)value;
+    else if (!std::strcmp(name, "bw_budget")) {
+        if (value < 1) return fail(e, DM_EINVAL, "bw_budget must be positive");
+        e->bwBudget = value;
+    }
     else if (!std::strcmp(name, "config")) {
         if (value < -1 || value >= (int64_t)e->cfgG.size()) return fail(e, DM_EINVAL, "configuration index out of range");
         e->forceCfg = (int)value;
@@ -1520,6 +1533,152 @@ const char *dm_build_id(void) { return DM_BUILD_ID; }
 
 double dm_last_dp_ms(const dm_engine *e) { return e ? e->lastMs : 0.0; }
 double dm_last_tantan_ms(const dm_engine *e) { return e ? e->lastTantanMs : 0.0; }
+double dm_last_bw_ms(const dm_engine *e) { return e ? e->lastBwMs : 0.0; }
+
+// ------------------------------------------------ profile builder: Baum-Welch
+int dm_bw_prepare(dm_engine *e, int32_t A, int32_t L, int32_t nseq, const uint8_t *letters, const int64_t *offsets,
+                  const int32_t *lengths, const double *weights) {
+    if (!e || (A != 4 && A != 20) || L < 0 || nseq < 0 ||
+        (nseq > 0 && (!letters || !offsets || !lengths || !weights)))
+        return DM_EINVAL;
+    HIPCHK(hipSetDevice(e->device));
+    e->bwL = -1;
+    // letters packed back to back, one zero pad after the last (the recurrences
+    // read one letter past a sequence and multiply it by zero)
+    std::vector<int64_t> loff(nseq), coff(nseq);
+    std::vector<int32_t> len(lengths, lengths + nseq);
+    int64_t total = 0;
+    for (int32_t s = 0; s < nseq; s++) {
+        if (lengths[s] < 0 || offsets[s] < 0) return fail(e, DM_EINVAL, "negative sequence length or offset");
+        loff[s] = total;
+        total += lengths[s];
+    }
+    std::vector<uint8_t> packed((size_t)total + 1, 0);
+    for (int32_t s = 0; s < nseq; s++)
+        for (int32_t j = 0; j < lengths[s]; j++) {
+            const uint8_t x = letters[offsets[s] + j];
+            if (x > A) return fail(e, DM_EINVAL, "letter code above the alphabet size");
+            packed[loff[s] + j] = x;
+        }
+    // batches whose six matrices fit the budget
+    e->bwBatch.clear();
+    e->bwMatCells = 0;
+    int64_t cells = 0;
+    for (int32_t s = 0; s < nseq; s++) {
+        const int64_t c = (int64_t)(L + 2) * (lengths[s] + 2);
+        if (e->bwBatch.empty() || (cells > 0 && (cells + c) * 6 * 8 > e->bwBudget)) {
+            e->bwBatch.push_back(s);
+            cells = 0;
+        }
+        coff[s] = cells;
+        cells += c;
+        e->bwMatCells = std::max(e->bwMatCells, cells);
+    }
+    e->bwBatch.push_back(nseq);
+    const size_t metaBytes = sizeof(int64_t) * 2 * nseq + sizeof(int32_t) * nseq;
+    HIPCHK(e->dBwLetters.ensure(packed.size()));
+    HIPCHK(e->dBwMeta.ensure(metaBytes));
+    HIPCHK(e->dBwMat.ensure(sizeof(double) * 6 * std::max<int64_t>(1, e->bwMatCells)));
+    HIPCHK(e->dBwRowTot.ensure(sizeof(double) * (size_t)std::max(1, nseq) * (L + 2)));
+    HIPCHK(e->dBwRaw.ensure(sizeof(double) * (size_t)std::max(1, nseq) * (L + 1) * (kBwRaw + A)));
+    HIPCHK(e->dBwSeqW.ensure(sizeof(double) * 3 * std::max(1, nseq)));
+    HIPCHK(e->dBwTables.ensure(sizeof(double) * (size_t)(L + 1) * (4 + A + 1)));
+    HIPCHK(e->dBwOut.ensure(sizeof(double) * (size_t)(L + 1) * (7 + A)));
+    HIPCHK(hipMemcpyAsync(e->dBwLetters.p, packed.data(), packed.size(), hipMemcpyHostToDevice, e->stream));
+    int64_t *m = e->dBwMeta.as<int64_t>();
+    if (nseq > 0) {
+        HIPCHK(hipMemcpyAsync(m, loff.data(), sizeof(int64_t) * nseq, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(m + nseq, coff.data(), sizeof(int64_t) * nseq, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(m + 2 * nseq, len.data(), sizeof(int32_t) * nseq, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(e->dBwSeqW.p, weights, sizeof(double) * nseq, hipMemcpyHostToDevice, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->bwL = L;
+    e->bwA = A;
+    e->bwN = nseq;
+    return DM_OK;
+}
+
+int dm_bw_estep(dm_engine *e, const double *probs, double *counts, double *totals) {
+#pragma clang fp contract(off)
+    if (!e || !probs || !counts) return DM_EINVAL;
+    if (e->bwL < 0) return fail(e, DM_EINVAL, "dm_bw_prepare first");
+    HIPCHK(hipSetDevice(e->device));
+    const int L = e->bwL, A = e->bwA, N = e->bwN, width = 7 + A;
+    // per profile row: a' = ins (1 - ext), ext, d' = del (1 - next dext),
+    // e' = dext (1 - next dext) / (1 - dext) (0 when not finite), and the
+    // match likelihood ratio (1 - ins - del) p[x] / bg[x] (0 when not finite,
+    // and for letters outside the alphabet) -- dummest-build.cc:566-601
+    std::vector<double> tab((size_t)(L + 1) * (4 + A + 1));
+    double *rates = tab.data(), *ratio = tab.data() + (size_t)(L + 1) * 4;
+    const double *bg = probs + (size_t)L * width + 7;
+    for (int r = 0; r <= L; r++) {
+        const double *p = probs + (size_t)r * width;
+        const double nextDext = (r == L) ? 0.0 : probs[(size_t)(r + 1) * width + 6];
+        rates[4 * r] = p[1] * (1.0 - p[4]);
+        rates[4 * r + 1] = p[4];
+        rates[4 * r + 2] = p[2] * (1 - nextDext);
+        double ep = p[6] * (1 - nextDext) / (1 - p[6]);
+        rates[4 * r + 3] = std::isfinite(ep) ? ep : 0.0;
+        for (int x = 0; x < A; x++) {
+            const double v = (1 - p[1] - p[2]) * p[7 + x] / bg[x];
+            ratio[(size_t)r * (A + 1) + x] = std::isfinite(v) ? v : 0.0;
+        }
+        ratio[(size_t)r * (A + 1) + A] = 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(e->dBwTables.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, e->stream));
+    if (!e->ev0) HIPCHK(hipEventCreate(&e->ev0));
+    if (!e->ev1) HIPCHK(hipEventCreate(&e->ev1));
+    const int64_t *m = e->dBwMeta.as<int64_t>();
+    double *mat = e->dBwMat.as<double>();
+    const int64_t C = std::max<int64_t>(1, e->bwMatCells);
+    BWArgs a{};
+    a.L = L;
+    a.A = A;
+    a.letters = e->dBwLetters.as<uint8_t>();
+    a.letterOff = m;
+    a.cellOff = m + N;
+    a.len = (const int *)(m + 2 * N);
+    a.rates = e->dBwTables.as<double>();
+    a.ratio = e->dBwTables.as<double>() + (size_t)(L + 1) * 4;
+    a.X = mat;
+    a.Y = mat + C;
+    a.Z = mat + 2 * C;
+    a.Wb = mat + 3 * C;
+    a.Yb = mat + 4 * C;
+    a.Zb = mat + 5 * C;
+    a.rowTotal = e->dBwRowTot.as<double>();
+    a.raw = e->dBwRaw.as<double>();
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    for (size_t b = 0; b + 1 < e->bwBatch.size(); b++) {
+        a.seqBase = e->bwBatch[b];
+        const int n = e->bwBatch[b + 1] - e->bwBatch[b];
+        HIPCHK(launch_bw_forward(a, n, e->stream));
+        HIPCHK(launch_bw_backward(a, n, e->stream));
+        HIPCHK(launch_bw_counts(a, n, e->stream));
+    }
+    BWReduceArgs ra{};
+    ra.L = L;
+    ra.A = A;
+    ra.nseq = N;
+    ra.weights = e->dBwSeqW.as<double>();
+    ra.total = e->dBwSeqW.as<double>() + std::max(1, N);
+    ra.wt = e->dBwSeqW.as<double>() + 2 * std::max(1, N);
+    ra.rowTotal = a.rowTotal;
+    ra.raw = a.raw;
+    ra.counts = e->dBwOut.as<double>();
+    HIPCHK(launch_bw_reduce(ra, e->stream));
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipMemcpyAsync(counts, ra.counts, sizeof(double) * (size_t)(L + 1) * width, hipMemcpyDeviceToHost,
+                          e->stream));
+    if (totals && N > 0)
+        HIPCHK(hipMemcpyAsync(totals, ra.total, sizeof(double) * N, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->lastBwMs = ms;
+    return DM_OK;
+}
 int64_t dm_last_cells(const dm_engine *e) { return e ? e->lastCells : 0; }
 
 }  // extern "C"

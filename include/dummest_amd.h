@@ -218,6 +218,22 @@ int dm_cache_file_path(char *out, int32_t cap);
 int64_t dm_cache_read(const char *path, int64_t cap, char *keys, int32_t key_stride, double *values);
 int dm_cache_write(const char *path, int64_t n, const char *keys, int32_t key_stride, const double *values);
 
+/* Profile builder's Baum-Welch E-step on the GPU (dummest-build.cc:561-736).
+ * dm_bw_prepare: the ungapped sequences of one alignment (letter codes 0..A,
+ * A = unknown letter; A = 20 protein or 4 DNA), their weights and the profile
+ * length L; uploads them once for the iterations that follow.
+ * dm_bw_estep: one E-step with the profile probs ((L+1) x (7+A) in the
+ * builder's order: match, insStart, delStart, insEnd, insExt, delEnd, delExt,
+ * letters): forward, backward and calculateTransitionCounts over every
+ * sequence, weighted by weight / (total 2^-480), summed over sequences in
+ * order into counts ((L+1) x (7+A); the delete-start column is left 0 for the
+ * caller to derive), and each sequence's forward total (totals, nseq; may be
+ * NULL). dm_last_bw_ms: device time of the last E-step. */
+int dm_bw_prepare(dm_engine *e, int32_t A, int32_t L, int32_t nseq, const uint8_t *letters, const int64_t *offsets,
+                  const int32_t *lengths, const double *weights);
+int dm_bw_estep(dm_engine *e, const double *probs, double *counts, double *totals);
+double dm_last_bw_ms(const dm_engine *e);
+
 /* Autotuner: times every kernel configuration (G lanes per pair, R rows per
  * lane, and the row-strip kernel) on this device with synthetic pairs of
  * seq_len letters whose profile fills the configuration's rows, and keeps the
@@ -236,7 +252,8 @@ int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int
 /* Options: "plain" = 1 launches the DP once per pair (the unbatched form);
  * "grid" = persistent grid size (0 = automatic); "config" = index into the
  * configuration table (dm_config_table) that every pair whose profile fits it
- * runs on (-1 = automatic choice). */
+ * runs on (-1 = automatic choice); "bw_budget" = bytes of Baum-Welch matrices
+ * per launch batch (default 8 GiB; dm_bw_prepare splits the sequences by it). */
 int dm_set_option(dm_engine *e, const char *name, int64_t value);
 
 /* Device time of the DP kernels of the last scoring call (HIP events on the

@@ -55,6 +55,7 @@ def _declare(L):
     L.or_blosum62.argtypes = [_p(ctypes.c_int8)]
     L.or_mask_dna.argtypes = [ctypes.c_char_p, i, d, ctypes.c_char_p, _p(f)]
     L.or_decode.argtypes = [ctypes.c_char_p, i, _p(u8)]
+    L.or_bw_estep.argtypes = [i, i, _p(d), i, _p(u8), _p(i), _p(d), _p(d), _p(d)]
     for suf, t in (("f32", f), ("f64", d)):
         getattr(L, "or_null_" + suf).argtypes = [_p(t), _p(u8), i, _p(d), _p(t), _p(t), _p(t),
                                                  _p(f), _p(i), _p(f), _p(i)]
@@ -258,6 +259,22 @@ def ref_tantan(seq: np.ndarray, is_protein: bool):
     out = np.zeros(max(1, len(c)), np.float32)
     L.ref_tantan_probs(_ptr(c, ctypes.c_uint8), len(c), int(is_protein), _ptr(out, ctypes.c_float))
     return out[:len(c)]
+
+
+def bw_estep(L: int, A: int, probs, letters, lengths, weights):
+    """One Baum-Welch E-step of dummest-build (dummest-build.cc:561-736): the
+    expected counts ((L+1) x (7+A), delete starts 0) and each sequence's
+    forward total."""
+    p = np.ascontiguousarray(probs, np.float64)
+    assert p.shape == (L + 1, 7 + A)
+    lt = np.concatenate([np.ascontiguousarray(letters, np.uint8), np.zeros(1, np.uint8)])
+    ln = np.ascontiguousarray(lengths, np.int32)
+    w = np.ascontiguousarray(weights, np.float64)
+    counts = np.zeros((L + 1, 7 + A), np.float64)
+    totals = np.zeros(max(1, len(ln)), np.float64)
+    lib().or_bw_estep(L, A, _ptr(p, ctypes.c_double), len(ln), _ptr(lt, ctypes.c_uint8), _ptr(ln, ctypes.c_int),
+                      _ptr(w, ctypes.c_double), _ptr(counts, ctypes.c_double), _ptr(totals, ctypes.c_double))
+    return counts, totals[:len(ln)]
 
 
 def cache_key(name: str, M: int, values, seq_len: int, border: int, trials: int, opts=DEFAULT_OPTS,

@@ -41,6 +41,7 @@
 #include <random>
 #include <string>
 #include <unordered_map>
+#include <numeric>
 #include <vector>
 #include <xmmintrin.h>
 #include <pmmintrin.h>
@@ -1624,6 +1625,115 @@ void or_calib_sequence_f32(const float *letterFreqs20, const double *opts, int L
 }
 
 // ExpScore primitives for contract tests (dummest-score.hh, dummest-batch.hh).
+// Baum-Welch E-step of dummest-build (dummest-build.cc:561-736), per sequence
+// in order: forward (:561-613, the total v summed cell by cell in row-major
+// order), backward (:615-662), calculateTransitionCounts (:664-727) with
+// weight / (v 2^-480). probs: (L+1) x (7+A); letters: codes 0..A back to back
+// plus one pad (the recurrences read one letter past a sequence). counts get
+// the sums of every column but the delete starts (left 0); totals[s] = v.
+// A letter code A (outside the alphabet) gets a zero likelihood ratio.
+void or_bw_estep(int L, int A, const double *probs, int nseq, const unsigned char *letters, const int *lengths,
+                 const double *weights, double *counts, double *totals) {
+    const int width = 7 + A;
+    const double scale = std::ldexp(1.0, -480);  // (1/2^30)^16, dummest-build.cc:36-38
+    int maxN = 0;
+    for (int s = 0; s < nseq; s++) maxN = std::max(maxN, lengths[s]);
+    const size_t area = (size_t)(L + 2) * (maxN + 2);
+    std::vector<double> X(area), Y(area), Z(area), Wb(area), Yb(area), Zb(area);
+    std::fill(counts, counts + (size_t)(L + 1) * width, 0.0);
+    auto rowRates = [&](int r, double &ap, double &bp, double &dp, double &ep) {
+        const double alpha = probs[r * width + 1], beta = probs[r * width + 4];
+        const double delta = probs[r * width + 2], eps = probs[r * width + 6];
+        const double eps1 = (r == L) ? 0.0 : probs[(r + 1) * width + 6];
+        ap = alpha * (1.0 - beta);
+        bp = beta;
+        dp = delta * (1 - eps1);
+        ep = eps * (1 - eps1) / (1 - eps);
+        if (!std::isfinite(ep)) ep = 0.0;
+    };
+    auto ratio = [&](int r, int x) {
+        if (x >= A) return 0.0;
+        double S = (1 - probs[r * width + 1] - probs[r * width + 2]) * probs[r * width + 7 + x] /
+                   probs[(size_t)L * width + 7 + x];
+        return std::isfinite(S) ? S : 0.0;
+    };
+    const unsigned char *seq = letters;
+    for (int s = 0; s < nseq; s++) {
+        const int n = lengths[s], cols = n + 2;
+        // forward
+        double v = 0.0;
+        std::fill_n(&X[0], n + 1, 0.0);
+        std::fill_n(&Y[1], n + 1, 0.0);
+        for (int i = 1; i <= L + 1; i++) {
+            double ap, bp, dp, ep;
+            rowRates(i - 1, ap, bp, dp, ep);
+            X[(size_t)(i - 1) * cols] = 0.0;
+            double z = Z[(size_t)i * cols] = 0.0;
+            for (int j = 1; j <= n + 1; j++) {
+                const double S = ratio(i - 1, seq[j - 1]);
+                const double y = Y[(size_t)(i - 1) * cols + j];
+                const double w = X[(size_t)(i - 1) * cols + (j - 1)] + y + z + scale;
+                z = ap * w + bp * z;
+                v += w;
+                X[(size_t)i * cols + j] = S * w;
+                Y[(size_t)i * cols + j] = dp * w + ep * y;
+                Z[(size_t)i * cols + j] = z;
+            }
+        }
+        totals[s] = v;
+        // backward
+        std::fill_n(&Wb[(size_t)(L + 1) * cols + 1], n + 1, 0.0);
+        std::fill_n(&Yb[(size_t)(L + 1) * cols + 0], n + 1, 0.0);
+        for (int i = L; i >= 0; i--) {
+            double ap, bp, dp, ep;
+            rowRates(i, ap, bp, dp, ep);
+            Wb[(size_t)(i + 2) * cols - 1] = 0.0;
+            double z = Zb[(size_t)(i + 1) * cols - 1] = 0.0;
+            for (int j = n; j >= 0; j--) {
+                const double S = ratio(i, seq[j]);
+                const double x = S * Wb[(size_t)(i + 1) * cols + (j + 1)];
+                const double y = Yb[(size_t)(i + 1) * cols + j];
+                const double w = x + dp * y + ap * z + scale;
+                z = w + bp * z;
+                Wb[(size_t)i * cols + j] = w;
+                Yb[(size_t)i * cols + j] = w + ep * y;
+                Zb[(size_t)i * cols + j] = z;
+            }
+        }
+        // expected counts
+        const double wt = weights[s] / (v * scale);
+        double oldGamma = 0;
+        for (int i = 1; i <= L + 1; i++) {
+            double emis[32] = {0};
+            double betap = 0, beta = 0, epsp = 0, eps = 0, etap = 0;
+            for (int j = 1; j <= n + 1; j++) {
+                const size_t up = (size_t)(i - 1) * cols, here = (size_t)i * cols;
+                betap += Z[here + (j - 1)] * Wb[up + (j - 1)];
+                beta += Z[here + (j - 1)] * Zb[up + (j - 1)];
+                epsp += Y[up + j] * Wb[up + (j - 1)];
+                eps += Y[up + j] * Yb[up + (j - 1)];
+                etap += Wb[up + (j - 1)];
+                emis[seq[j - 1]] += X[here + j] * Wb[here + j];
+            }
+            beta -= betap;
+            eps -= epsp;
+            const double gamma = std::accumulate(emis, emis + A + 1, 0.0);
+            const double notIns = etap * scale + oldGamma + epsp;
+            oldGamma = gamma;
+            double *c = counts + (size_t)(i - 1) * width;
+            c[0] += gamma * wt;
+            c[1] += notIns * wt;
+            c[3] += betap * wt;
+            c[4] += beta * wt;
+            c[5] += epsp * wt;
+            c[6] += eps * wt;
+            if (i == L + 1) continue;
+            for (int k = 0; k < A; k++) c[7 + k] += emis[k] * wt;
+        }
+        seq += n;
+    }
+}
+
 void or_es_normalize(double m, int e, float *om, int *oe) { ES r = normalizeScore(m, e); *om = r.m; *oe = r.e; }
 void or_es_add(float am, int ae, float bm, int be, float *om, int *oe) { ES r = addScores({am, ae}, {bm, be}); *om = r.m; *oe = r.e; }
 void or_es_badd(float am, int ae, float bm, int be, float *om, int *oe) { ES r = batchAdd({am, ae}, {bm, be}); *om = r.m; *oe = r.e; }
