@@ -2,17 +2,17 @@
 // profiles the search reads (dummest-build, dummest-build.cc:1-1167).
 //
 // Per alignment:
-//   1. read it (readMultipleAlignment, :297-364), decide protein or DNA
-//      (isProteinAlignment, :366-387), map letters and gaps, mark end gaps;
-//   2. position-based sequence weights (makeSequenceWeights, :405-441);
+//   1. read it (readMultipleAlignment, :297-365), decide protein or DNA
+//      (isProteinAlignment, :367-389), map letters and gaps, mark end gaps;
+//   2. position-based sequence weights (makeSequenceWeights, :406-444);
 //   3. count match/insert/delete events over the columns kept as positions,
 //      splitting adjacent gaps into extensions and restarts by a fixed point
-//      (countEvents, :443-549);
+//      (countEvents, :446-544);
 //   4. Baum-Welch over the ungapped sequences with the profile's own
-//      forward/backward (baumWelch, :561-795), unless --countonly;
+//      forward/backward (baumWelch, :748-834), unless --countonly;
 //   5. counts to probabilities with the Dirichlet mixture and gap priors,
 //      with the total count capped so the relative entropy meets its target
-//      (countsToProbs, :171-200; entropyWeight, :219-245);
+//      (countsToProbs, :177-203; entropyWeight, :220-245);
 //   6. print (printProfile, :862-914).
 // The arithmetic follows the reference's order of operations so that the
 // printed values agree with its output.
@@ -99,7 +99,7 @@ void normalizeInPlace(double *x, int n) {
     for (int i = 0; i < n; i++) x[i] /= s;
 }
 
-// Checks a finished block of sequence lines (checkAlignmentBlock, :279-295).
+// Checks a finished block of sequence lines (checkAlignmentBlock, :275-295).
 void endBlock(Alignment &a, size_t lineTotal, const std::vector<std::string> &rf, size_t lastLen, int blockLines) {
     if (blockLines) {
         if (a.nseq && blockLines != a.nseq) fatal("alignment blocks should have same number of sequences");
@@ -157,7 +157,7 @@ bool readAlignment(std::istream &in, Alignment &a) {
     return more;
 }
 
-// Protein unless at least 90% of the letters are nucleotide-like (:366-387).
+// Protein unless at least 90% of the letters are nucleotide-like (:367-389).
 bool looksLikeProtein(const Alignment &a) {
     long counts[256] = {0};
     for (unsigned char c : a.cells) ++counts[c];
@@ -172,7 +172,7 @@ inline bool isMatchColumn(bool hand, const Alignment &a, int col, double nonGap,
     return hand ? !isGapChar(a.rf[col]) : (nonGap > 0 && nonGap >= symfrac * total);
 }
 
-// Henikoff position-based weights, scaled so the largest is 1 (:405-441).
+// Henikoff position-based weights, scaled so the largest is 1 (:406-444).
 std::vector<double> sequenceWeights(const Alignment &a, int A, bool hand, double symfrac) {
     const int midGap = A + 1, endGap = A + 2;
     std::vector<double> w(a.nseq, 0.0);
@@ -203,7 +203,7 @@ std::vector<double> sequenceWeights(const Alignment &a, int A, bool hand, double
 }
 
 // (count1 r + pseudo1) / ((count1 + count2) r + pseudo1 + pseudo2), where r
-// caps the total count at maxTotal (getProb, :104-111).
+// caps the total count at maxTotal (getProb, :107-114).
 double cappedRatio(double c1, double c2, double p1, double p2, double maxTotal) {
     const double s = c1 + c2;
     if (s + (p1 + p2) == 0.0) return 0.5;
@@ -211,7 +211,7 @@ double cappedRatio(double c1, double c2, double p1, double p2, double maxTotal) 
     return (c1 * r + p1) / (s * r + (p1 + p2));
 }
 
-// Event counts per position, width 7 + A (countEvents, :443-549): match,
+// Event counts per position, width 7 + A (countEvents, :446-544): match,
 // not-insert, delete start, insert end (= insert start), insert extend, delete
 // end, delete extend, then the letter counts; one more row for the end.
 std::vector<double> eventCounts(const Alignment &a, int A, bool hand, double symfrac, const std::vector<double> &w,
@@ -286,7 +286,7 @@ std::vector<double> eventCounts(const Alignment &a, int A, bool hand, double sym
 }
 
 // Posterior mean of a Dirichlet mixture given letter counts, with the total
-// count capped at maxTotal (applyDirichletMixture, :113-142).
+// count capped at maxTotal (applyDirichletMixture, :116-147).
 void mixtureEstimate(const Mixture &mix, int A, double maxTotal, const double *counts, double *out) {
     const int stride = 1 + A;
     const double total = sumOf(counts, A);
@@ -311,7 +311,7 @@ void mixtureEstimate(const Mixture &mix, int A, double maxTotal, const double *c
 }
 
 // Transition probabilities of one position from its event counts
-// (gapCountsToProbs, :144-169).
+// (gapCountsToProbs, :149-175).
 void gapProbs(const BuildGapPrior &gp, double maxTotal, const double *c, double *p) {
     const double match = c[0], notIns = c[1], delBeg = c[2], insBeg = c[3], insExt = c[4], delEnd = c[5],
                  delExt = c[6];
@@ -328,7 +328,7 @@ void gapProbs(const BuildGapPrior &gp, double maxTotal, const double *c, double 
     p[6] = e;
 }
 
-// All probabilities from counts (countsToProbs, :171-200): the background
+// All probabilities from counts (countsToProbs, :177-203): the background
 // row is the normalized geometric mean of the letter probabilities.
 void probabilities(const Mixture &mix, const BuildGapPrior &gp, int A, double maxTotal, int L, const double *counts,
                    double *probs) {
@@ -365,7 +365,7 @@ double relativeEntropyBits(const double *probs, int A, int L) {
 }
 
 // Bisection on the total-count cap until the relative entropy meets the
-// target; returns the effective sequence number (entropyWeight, :219-245).
+// target; returns the effective sequence number (entropyWeight, :220-245).
 double capForEntropy(const Mixture &mix, const BuildGapPrior &gp, int A, double rangeEnd, double target, int L,
                      const double *counts, double *probs, int verbosity, std::ostream &log) {
     double lo = 0, len = rangeEnd, cap = 1e37;
@@ -389,7 +389,7 @@ double capForEntropy(const Mixture &mix, const BuildGapPrior &gp, int A, double 
 
 // ----------------------------------------------------------- Baum-Welch
 // Iterates expected counts -> probabilities until no probability moves by
-// more than maxDiff (baumWelch, :738-822). The expected counts of each
+// more than maxDiff (baumWelch, :748-834). The expected counts of each
 // iteration (forward, backward and calculateTransitionCounts over every
 // sequence, :561-736) come from the gfx950 kernels (dm_bw_estep); the delete
 // starts, the priors and the convergence test stay here.
@@ -440,7 +440,7 @@ void baumWelch(std::vector<double> &counts, const Alignment &a, int A, const Mix
 }
 
 // Most probable letter; uppercase when it holds at least half (protein) or
-// 90% (DNA) of the position's probability (getConsensusAt, :824-841).
+// 90% (DNA) of the position's probability (getConsensusAt, :843-850).
 char consensusLetter(const double *p, int A, const char *alphabet) {
     const double thr = (A > 4) ? 0.5 : 0.9;
     const int k = (int)(std::max_element(p, p + A) - p);

@@ -1627,7 +1627,7 @@ void or_calib_sequence_f32(const float *letterFreqs20, const double *opts, int L
 // ExpScore primitives for contract tests (dummest-score.hh, dummest-batch.hh).
 // Baum-Welch E-step of dummest-build (dummest-build.cc:561-736), per sequence
 // in order: forward (:561-613, the total v summed cell by cell in row-major
-// order), backward (:615-662), calculateTransitionCounts (:664-727) with
+// order), backward (:615-666), calculateTransitionCounts (:668-736) with
 // weight / (v 2^-480). probs: (L+1) x (7+A); letters: codes 0..A back to back
 // plus one pad (the recurrences read one letter past a sequence). counts get
 // the sums of every column but the delete starts (left 0); totals[s] = v.
