@@ -158,39 +158,6 @@ struct TantanArgs {
 };
 hipError_t launch_tantan_dna(const TantanArgs &a, hipStream_t s);
 
-// Baum-Welch E-step of the profile builder (dummest-build.cc:561-736), FP64.
-// Per sequence of n letters the six (L+2) x (n+2) matrices X, Y, Z (forward:
-// match, delete, insert) and Wb, Yb, Zb (backward), row-major with pitch n+2,
-// at cellOff[s] from each base pointer; `raw` holds per sequence and profile
-// row i-1 (i = 1..L+1) the sums gamma, etap, betap, beta, epsilonp, epsilon,
-// emis[0..A) of calculateTransitionCounts (kBwRaw + A values).
-constexpr int kBwRaw = 6;
-struct BWArgs {
-    int L, A;
-    int seqBase;               // first sequence of this launch
-    const uint8_t *letters;    // letter codes (0..A), all sequences back to back, one pad after
-    const int64_t *letterOff;  // per sequence
-    const int *len;            // per sequence
-    const int64_t *cellOff;    // per sequence, relative to the matrix bases
-    const double *rates;       // (L+1) x 4 per profile row: a' = ins (1 - ext), ext, d', e'
-    const double *ratio;       // (L+1) x (A+1): match likelihood ratio per row and letter (0 for A)
-    double *X, *Y, *Z, *Wb, *Yb, *Zb;
-    double *rowTotal;  // per sequence (L+2): forward sum of rows 1..L+1
-    double *raw;       // per sequence (L+1) x (kBwRaw + A)
-};
-struct BWReduceArgs {
-    int L, A, nseq;
-    const double *weights;   // sequence weights
-    const double *rowTotal;  // as in BWArgs
-    double *total;           // out: per sequence forward total
-    double *wt;              // out: weights / (total * 2^-480)
-    const double *raw;
-    double *counts;  // out: (L+1) x (7 + A) expected counts (delete starts left 0)
-};
-hipError_t launch_bw_forward(const BWArgs &a, int nseq, hipStream_t s);
-hipError_t launch_bw_backward(const BWArgs &a, int nseq, hipStream_t s);
-hipError_t launch_bw_counts(const BWArgs &a, int nseq, hipStream_t s);
-hipError_t launch_bw_reduce(const BWReduceArgs &a, hipStream_t s);
 hipError_t launch_copy_probe(const float *src, float *dst, int64_t n16, hipStream_t s);
 #endif
 

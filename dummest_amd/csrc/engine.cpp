@@ -18,6 +18,7 @@
 #include <thread>
 #include <vector>
 
+#include "bw.h"
 #include "dm_layout.h"
 #include "dm_host.h"
 #include "../../include/dummest_amd.h"
@@ -86,10 +87,11 @@ struct dm_engine {
     DevBuf dTanBases, dTanProbs, dTanScales, dTanMeta, dTanB2f;
     // profile builder's Baum-Welch (dm_bw_prepare / dm_bw_estep)
     int bwL = -1, bwA = 0, bwN = 0;
+    std::vector<int> bwLen;         // sequence lengths
     std::vector<int> bwBatch;       // first sequence of each batch, then nseq
     int64_t bwMatCells = 0;         // cells of the largest batch (per matrix)
     int64_t bwBudget = (int64_t)8 << 30;  // bytes of forward/backward matrices per batch
-    DevBuf dBwLetters, dBwMeta, dBwTables, dBwMat, dBwRowTot, dBwRaw, dBwSeqW, dBwOut;
+    DevBuf dBwLetters, dBwMeta, dBwTables, dBwMat, dBwRaw, dBwSeqW, dBwOut;
     double lastBwMs = 0;
 
     float profileMaskThr() const { return (maskBits & 1) ? maskThr : -1.f; }
@@ -794,7 +796,7 @@ void dm_engine_destroy(dm_engine *e) {
     for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dRec,
                       &e->dNull, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
                       &e->dOut, &e->dTanBases, &e->dTanProbs, &e->dTanScales, &e->dTanMeta, &e->dTanB2f,
-                      &e->dBwLetters, &e->dBwMeta, &e->dBwTables, &e->dBwMat, &e->dBwRowTot, &e->dBwRaw,
+                      &e->dBwLetters, &e->dBwMeta, &e->dBwTables, &e->dBwMat, &e->dBwRaw,
                       &e->dBwSeqW, &e->dBwOut})
         b->release();
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -1550,6 +1552,8 @@ int dm_bw_prepare(dm_engine *e, int32_t A, int32_t L, int32_t nseq, const uint8_
     int64_t total = 0;
     for (int32_t s = 0; s < nseq; s++) {
         if (lengths[s] < 0 || offsets[s] < 0) return fail(e, DM_EINVAL, "negative sequence length or offset");
+        if (lengths[s] > kBwMaxLen)
+            return fail(e, DM_EUNSUP, "Baum-Welch takes sequences of up to " + std::to_string(kBwMaxLen) + " letters");
         loff[s] = total;
         total += lengths[s];
     }
@@ -1565,7 +1569,7 @@ int dm_bw_prepare(dm_engine *e, int32_t A, int32_t L, int32_t nseq, const uint8_
     e->bwMatCells = 0;
     int64_t cells = 0;
     for (int32_t s = 0; s < nseq; s++) {
-        const int64_t c = (int64_t)(L + 2) * (lengths[s] + 2);
+        const int64_t c = bwCells(L, lengths[s]);
         if (e->bwBatch.empty() || (cells > 0 && (cells + c) * 6 * 8 > e->bwBudget)) {
             e->bwBatch.push_back(s);
             cells = 0;
@@ -1575,11 +1579,11 @@ int dm_bw_prepare(dm_engine *e, int32_t A, int32_t L, int32_t nseq, const uint8_
         e->bwMatCells = std::max(e->bwMatCells, cells);
     }
     e->bwBatch.push_back(nseq);
+    e->bwLen.assign(lengths, lengths + nseq);
     const size_t metaBytes = sizeof(int64_t) * 2 * nseq + sizeof(int32_t) * nseq;
     HIPCHK(e->dBwLetters.ensure(packed.size()));
     HIPCHK(e->dBwMeta.ensure(metaBytes));
     HIPCHK(e->dBwMat.ensure(sizeof(double) * 6 * std::max<int64_t>(1, e->bwMatCells)));
-    HIPCHK(e->dBwRowTot.ensure(sizeof(double) * (size_t)std::max(1, nseq) * (L + 2)));
     HIPCHK(e->dBwRaw.ensure(sizeof(double) * (size_t)std::max(1, nseq) * (L + 1) * (kBwRaw + A)));
     HIPCHK(e->dBwSeqW.ensure(sizeof(double) * 3 * std::max(1, nseq)));
     HIPCHK(e->dBwTables.ensure(sizeof(double) * (size_t)(L + 1) * (4 + A + 1)));
@@ -1647,24 +1651,24 @@ int dm_bw_estep(dm_engine *e, const double *probs, double *counts, double *total
     a.Wb = mat + 3 * C;
     a.Yb = mat + 4 * C;
     a.Zb = mat + 5 * C;
-    a.rowTotal = e->dBwRowTot.as<double>();
+    a.weights = e->dBwSeqW.as<double>();
+    a.total = e->dBwSeqW.as<double>() + std::max(1, N);
+    a.wt = e->dBwSeqW.as<double>() + 2 * std::max(1, N);
     a.raw = e->dBwRaw.as<double>();
     HIPCHK(hipEventRecord(e->ev0, e->stream));
     for (size_t b = 0; b + 1 < e->bwBatch.size(); b++) {
         a.seqBase = e->bwBatch[b];
         const int n = e->bwBatch[b + 1] - e->bwBatch[b];
-        HIPCHK(launch_bw_forward(a, n, e->stream));
-        HIPCHK(launch_bw_backward(a, n, e->stream));
+        const int maxLen = *std::max_element(e->bwLen.begin() + e->bwBatch[b], e->bwLen.begin() + e->bwBatch[b + 1]);
+        HIPCHK(launch_bw_forward(a, n, maxLen, e->stream));
+        HIPCHK(launch_bw_backward(a, n, maxLen, e->stream));
         HIPCHK(launch_bw_counts(a, n, e->stream));
     }
     BWReduceArgs ra{};
     ra.L = L;
     ra.A = A;
     ra.nseq = N;
-    ra.weights = e->dBwSeqW.as<double>();
-    ra.total = e->dBwSeqW.as<double>() + std::max(1, N);
-    ra.wt = e->dBwSeqW.as<double>() + 2 * std::max(1, N);
-    ra.rowTotal = a.rowTotal;
+    ra.wt = a.wt;
     ra.raw = a.raw;
     ra.counts = e->dBwOut.as<double>();
     HIPCHK(launch_bw_reduce(ra, e->stream));
@@ -1672,7 +1676,7 @@ int dm_bw_estep(dm_engine *e, const double *probs, double *counts, double *total
     HIPCHK(hipMemcpyAsync(counts, ra.counts, sizeof(double) * (size_t)(L + 1) * width, hipMemcpyDeviceToHost,
                           e->stream));
     if (totals && N > 0)
-        HIPCHK(hipMemcpyAsync(totals, ra.total, sizeof(double) * N, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(totals, a.total, sizeof(double) * N, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
