@@ -495,8 +495,19 @@ constexpr int kGhostTop = 100;          // strip ghost values are kept below 2^1
 template <int G> __device__ __forceinline__ bool group_small(float live, int grp) {
     const unsigned long long bs = __ballot(live < kSmall);
     const unsigned long long bn = __ballot(live > 0.f);
-    const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (grp * G));
+    const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G % 64)) - 1ull) << (grp * G));
     return ((bs & gm) == gm) && ((bn & gm) != 0ull);
+}
+// The value of lane `at` of this lane's group, in every lane of the group:
+// scalar reads of each group's lane (no LDS round trip).
+template <int G> __device__ __forceinline__ int group_lane(int v, int grp, int at) {
+    int r = __builtin_amdgcn_readlane(v, at);
+#pragma unroll
+    for (int g = 1; g < 64 / G; g++) {
+        const int x = __builtin_amdgcn_readlane(v, g * G + at);
+        r = (grp == g) ? x : r;
+    }
+    return r;
 }
 constexpr float kStep = 5.421011e-20f;  // 2^-64
 constexpr int kPre = 100;               // bulk-mode prescale of stored W1 (2^-100)
@@ -719,7 +730,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                             (void)frexpf(fmaxf(rw, ry), &ex);
                             need = ex + ge - Eb - kGhostTop;
                         }
-                        need = __shfl(need, grp * G + G - 1, 64);  // the group's last lane
+                        need = group_lane<G>(need, grp, G - 1);  // the group's last lane
                         if (__builtin_expect(need > 0, 0)) scaleB(-((need + 63) & ~63));
                     }
                     gw = ldexpf(rw, ge - Eb);
@@ -968,7 +979,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                         (void)frexpf(fmaxf(rn, ry), &ex);
                         need = ex + ge - Ef - kGhostTop;
                     }
-                    need = __shfl(need, grp * G, 64);  // the group's first lane
+                    need = group_lane<G>(need, grp, 0);  // the group's first lane
                     if (__builtin_expect(need > 0, 0)) scaleF(-((need + 63) & ~63));
                     gn = ldexpf(rn, ge - Ef);
                     gy = ldexpf(ry, ge - Ef);
