@@ -194,7 +194,7 @@ def main():
         same = float(np.max(np.abs(D.es_log(pm, pe) - D.es_log(mm[sample], me[sample]))))
         log(f"plain: {plain:.4g} cells/s on {len(sample)} pairs (max |d score| vs batched {same:.2e} nats)")
 
-    # calibration sweep (BASELINE configs[2]'s shape): trial DNA on the host, the
+    # calibration sweep (500 profiles x 200 trials of 3 kb plus border): trial DNA on the host, the
     # scores of all profiles' trials in one graph-captured pass, Gumbel fits
     calib = None
     if args.calib_profiles > 0:
