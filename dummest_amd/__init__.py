@@ -66,6 +66,7 @@ def _declare(L):
                                     _P(i32), _P(i32)]
     L.dm_add_raw_profiles.argtypes = [vp, i32, _P(i32), _P(f), ctypes.c_int, _P(i32)]
     L.dm_add_profile.argtypes = [vp, i32, _P(f), _P(f), _P(f), _P(i32)]
+    L.dm_set_profile.argtypes = [vp, i32, i32, _P(f), _P(f), _P(f)]
     L.dm_get_profile.argtypes = [vp, i32, _P(i32), _P(f), _P(f), _P(f)]
     L.dm_profile_count.argtypes = [vp]
     L.dm_decode.argtypes = [ctypes.c_char_p, i64, _P(u8)]
@@ -108,6 +109,11 @@ def _declare(L):
     L.dm_bw_estep.argtypes = [vp, _P(d), _P(d), _P(d)]
     L.dm_last_bw_ms.argtypes = [vp]
     L.dm_last_bw_ms.restype = d
+    L.dm_grad.argtypes = [vp, i32, i32, ctypes.c_char_p, _P(i64), _P(i32), _P(d), _P(d), _P(d)]
+    L.dm_last_grad_ms.argtypes = [vp]
+    L.dm_last_grad_ms.restype = d
+    L.dm_last_grad_cells.argtypes = [vp]
+    L.dm_last_grad_cells.restype = i64
     L.dm_xxh3_128.argtypes = [ctypes.c_char_p, i64, _P(ctypes.c_uint64)]
     L.dm_calibration_key.argtypes = [ctypes.c_char_p, i32, _P(f), ctypes.c_char_p, i32, i32, i32, _P(d), d,
                                      ctypes.c_char_p]
@@ -225,6 +231,9 @@ def xxh3_128(data: bytes) -> int:
         raise DummestError(f"dm_xxh3_128 failed ({rc})")
     return out[0] | (out[1] << 64)
 
+
+# Columns of dm_grad's counts: the 11 Params of a row, then 65 codon bins.
+GRAD_WIDTH = 76
 
 # Field order of a calibration-cache entry (CacheEntry, dummest-calibration.hh:148-181).
 CACHE_FIELDS = tuple(f"{m}{a}{q}" for m in ("MM", "ML", "LM") for q in ("L", "K", "Ksimple")
@@ -363,6 +372,15 @@ class Engine:
                     "dm_add_profile")
         return out.value
 
+    def set_profile(self, pid: int, M, values, params, bg):
+        """Replace profile pid with new finalized arrays."""
+        v = np.ascontiguousarray(values, np.float32)
+        p = np.ascontiguousarray(params, np.float32)
+        b = np.ascontiguousarray(bg, np.float32)
+        assert v.size == (M + 1) * WIDTH and p.size == (M + 2) * 11 and b.size == WIDTH
+        f = ctypes.c_float
+        self._check(lib().dm_set_profile(self._h, pid, int(M), _ptr(v, f), _ptr(p, f), _ptr(b, f)), "dm_set_profile")
+
     def get_profile(self, pid: int):
         M = ctypes.c_int32()
         self._check(lib().dm_get_profile(self._h, pid, ctypes.byref(M), None, None, None), "dm_get_profile")
@@ -459,6 +477,26 @@ class Engine:
 
     def last_bw_ms(self) -> float:
         return lib().dm_last_bw_ms(self._h)
+
+    # -- gradient of the summed log-likelihood --------------------------------
+    def grad_counts(self, prof: int, seqs, weights=None):
+        """(ln Z per sequence, expected counts (M+1) x 76) of one profile over
+        DNA sequences (see dm_grad)."""
+        data, offs, lens = self._dna_batch(seqs)
+        M = self.profile_length(prof)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        lnz = np.zeros(max(1, len(seqs)), np.float64)
+        counts = np.zeros((M + 1, GRAD_WIDTH), np.float64)
+        self._check(lib().dm_grad(self._h, prof, len(seqs), data, _ptr(offs, ctypes.c_int64), _ptr(lens, ctypes.c_int32),
+                                  None if w is None else _ptr(w, ctypes.c_double), _ptr(lnz, ctypes.c_double),
+                                  _ptr(counts, ctypes.c_double)), "dm_grad")
+        return lnz[:len(seqs)], counts
+
+    def last_grad_ms(self) -> float:
+        return lib().dm_last_grad_ms(self._h)
+
+    def last_grad_cells(self) -> int:
+        return lib().dm_last_grad_cells(self._h)
 
     # -- scoring ------------------------------------------------------------
     @staticmethod

@@ -20,6 +20,7 @@
 
 #include "bw.h"
 #include "dm_layout.h"
+#include "grad.h"
 #include "dm_host.h"
 #include "../../include/dummest_amd.h"
 
@@ -93,6 +94,11 @@ struct dm_engine {
     int64_t bwBudget = (int64_t)8 << 30;  // bytes of forward/backward matrices per batch
     DevBuf dBwLetters, dBwMeta, dBwTables, dBwMat, dBwRaw, dBwSeqW, dBwOut;
     double lastBwMs = 0;
+    // gradient of the summed log-likelihood (dm_grad)
+    int64_t gradBudget = (int64_t)8 << 30;  // bytes of FP64 matrices per launch batch
+    DevBuf dGradCodon, dGradMat, dGradBnd, dGradMeta, dGradRaw, dGradW, dGradOut;
+    double lastGradMs = 0;
+    int64_t lastGradCells = 0;
 
     float profileMaskThr() const { return (maskBits & 1) ? maskThr : -1.f; }
 
@@ -797,7 +803,8 @@ void dm_engine_destroy(dm_engine *e) {
                       &e->dNull, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
                       &e->dOut, &e->dTanBases, &e->dTanProbs, &e->dTanScales, &e->dTanMeta, &e->dTanB2f,
                       &e->dBwLetters, &e->dBwMeta, &e->dBwTables, &e->dBwMat, &e->dBwRaw,
-                      &e->dBwSeqW, &e->dBwOut})
+                      &e->dBwSeqW, &e->dBwOut, &e->dGradCodon, &e->dGradMat, &e->dGradBnd, &e->dGradMeta,
+                      &e->dGradRaw, &e->dGradW, &e->dGradOut})
         b->release();
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
@@ -881,6 +888,20 @@ int dm_add_profile(dm_engine *e, int32_t M, const float *values, const float *pa
     p.consensus.assign(M, 0);
     if (id) *id = (int32_t)e->profiles.size();
     e->profiles.push_back(std::move(p));
+    e->profDirty = true;
+    return DM_OK;
+}
+
+int dm_set_profile(dm_engine *e, int32_t id, int32_t M, const float *values, const float *params, const float *bg) {
+    if (!e || M < 2 || !values || !params || !bg) return DM_EINVAL;
+    if (id < 0 || id >= (int)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
+    HostProfile &p = e->profiles[id];
+    p.M = M;
+    p.values.assign(values, values + (size_t)(M + 1) * kWidth);
+    p.params.assign(params, params + (size_t)(M + 2) * 11);
+    p.bg.assign(bg, bg + kWidth);
+    p.consensus.assign(M, 0);
+    p.masked = 0;
     e->profDirty = true;
     return DM_OK;
 }
@@ -1290,6 +1311,10 @@ int dm_set_option(dm_engine *e, const char *name, int64_t value) {
     if (!e || !name) return DM_EINVAL;
     if (!std::strcmp(name, "plain")) e->plain = value != 0;
     else if (!std::strcmp(name, "grid")) e->gridOpt = (int)value;
+    else if (!std::strcmp(name, "grad_budget")) {
+        if (value < 1) return fail(e, DM_EINVAL, "grad_budget must be positive");
+        e->gradBudget = value;
+    }
     else if (!std::strcmp(name, "bw_budget")) {
         if (value < 1) return fail(e, DM_EINVAL, "bw_budget must be positive");
         e->bwBudget = value;
@@ -1536,6 +1561,187 @@ const char *dm_build_id(void) { return DM_BUILD_ID; }
 double dm_last_dp_ms(const dm_engine *e) { return e ? e->lastMs : 0.0; }
 double dm_last_tantan_ms(const dm_engine *e) { return e ? e->lastTantanMs : 0.0; }
 double dm_last_bw_ms(const dm_engine *e) { return e ? e->lastBwMs : 0.0; }
+double dm_last_grad_ms(const dm_engine *e) { return e ? e->lastGradMs : 0.0; }
+int64_t dm_last_grad_cells(const dm_engine *e) { return e ? e->lastGradCells : 0; }
+
+// ------------------------------------- gradient of the summed log-likelihood
+int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+            const double *weights, double *lnZ, double *counts) {
+    if (!e || nseq < 0 || !counts || (nseq > 0 && (!dna || !offsets || !lengths))) return DM_EINVAL;
+    if (prof < 0 || prof >= (int)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
+    for (int32_t s = 0; s < nseq; s++) {
+        if (offsets[s] < 0 || lengths[s] < 3)
+            return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
+        if (lengths[s] > kGradMaxN)
+            return fail(e, DM_EUNSUP, "the gradient takes sequences of up to " + std::to_string(kGradMaxN) + " letters");
+    }
+    HIPCHK(hipSetDevice(e->device));
+    int rc = syncProfiles(e);
+    if (rc) return rc;
+    const int M = e->profiles[prof].M;
+    const int WR = kGradRaw - 1;
+    if (nseq == 0) {
+        std::fill(counts, counts + (size_t)(M + 1) * WR, 0.0);
+        return DM_OK;
+    }
+    // The sequences go into the engine's store for the duration of the call:
+    // decoded codes for the null model and the DP, codon indices for the
+    // emission counts (16 b0 + 4 b1 + b2 over ACGT, 64 otherwise).
+    const size_t codes0 = e->codes.size(), seqs0 = e->seqLen.size();
+    int64_t total = 0;
+    for (int32_t s = 0; s < nseq; s++) total += lengths[s];
+    e->codes.resize(codes0 + (size_t)total);
+    std::vector<uint8_t> codon((size_t)total);
+    auto base4 = [](char c) -> int {
+        switch (c) {
+        case 'A': case 'a': return 0;
+        case 'C': case 'c': return 1;
+        case 'G': case 'g': return 2;
+        case 'T': case 't': return 3;
+        default: return -1;
+        }
+    };
+    std::vector<int32_t> pp(nseq, prof), ss(nseq);
+    int64_t at = 0;
+    for (int32_t s = 0; s < nseq; s++) {
+        const char *d = dna + offsets[s];
+        const int n = lengths[s];
+        decodeDna(d, n, e->codes.data() + codes0 + at);
+        for (int j = 0; j < n; j++) {
+            int c = 64;
+            if (j + 2 < n) {
+                const int x = base4(d[j]), y = base4(d[j + 1]), z = base4(d[j + 2]);
+                if (x >= 0 && y >= 0 && z >= 0) c = 16 * x + 4 * y + z;
+            }
+            codon[at + j] = (uint8_t)c;
+        }
+        e->seqOff.push_back((int64_t)(codes0 + at));
+        e->seqLen.push_back(n);
+        ss[s] = (int32_t)(seqs0 + s);
+        at += n;
+    }
+    e->seqDirty = true;
+    auto restore = [&]() {
+        e->codes.resize(codes0);
+        e->seqOff.resize(seqs0);
+        e->seqLen.resize(seqs0);
+        e->seqDirty = true;
+    };
+    auto run = [&]() -> int {
+        int r = syncSequences(e);
+        if (r) return r;
+        HIPCHK(e->dGradCodon.ensure((size_t)total));
+        HIPCHK(hipMemcpyAsync(e->dGradCodon.p, codon.data(), (size_t)total, hipMemcpyHostToDevice, e->stream));
+        // batches whose six matrices fit the budget
+        std::vector<size_t> batch{0};
+        int64_t cells = 0, maxCells = 1, maxPairs = 1, maxBnd = 1;
+        int64_t bnd = 0;
+        for (int32_t s = 0; s < nseq; s++) {
+            const int64_t c = gradCells(M, lengths[s]);
+            if (cells > 0 && (cells + c) * 6 * 8 > e->gradBudget) {
+                batch.push_back((size_t)s);
+                cells = 0;
+                bnd = 0;
+            }
+            cells += c;
+            bnd += lengths[s];
+            maxCells = std::max(maxCells, cells);
+            maxBnd = std::max(maxBnd, bnd);
+            maxPairs = std::max<int64_t>(maxPairs, (int64_t)s + 1 - (int64_t)batch.back());
+        }
+        batch.push_back((size_t)nseq);
+        HIPCHK(e->dGradMat.ensure(sizeof(double) * 6 * (size_t)maxCells));
+        HIPCHK(e->dGradBnd.ensure(sizeof(double) * 2 * (size_t)maxBnd));
+        HIPCHK(e->dGradMeta.ensure((sizeof(int64_t) * 2 + sizeof(int)) * (size_t)maxPairs));
+        HIPCHK(e->dGradRaw.ensure(sizeof(double) * (size_t)nseq * (M + 1) * kGradRaw));
+        HIPCHK(e->dGradW.ensure(sizeof(double) * 3 * (size_t)nseq));
+        HIPCHK(e->dGradOut.ensure(sizeof(double) * (size_t)(M + 1) * WR));
+        std::vector<double> w(nseq, 1.0);
+        if (weights) std::copy(weights, weights + nseq, w.begin());
+        HIPCHK(hipMemcpyAsync(e->dGradW.p, w.data(), sizeof(double) * nseq, hipMemcpyHostToDevice, e->stream));
+        std::vector<int64_t> sorted(nseq);
+        std::iota(sorted.begin(), sorted.end(), 0);
+        const std::vector<int> cfgOf(nseq, 0);  // only the pair records and the null model are used
+        double *mat = e->dGradMat.as<double>();
+        const int64_t C6 = maxCells;
+        if (!e->ev0) HIPCHK(hipEventCreate(&e->ev0));
+        if (!e->ev1) HIPCHK(hipEventCreate(&e->ev1));
+        float msAll = 0.f;
+        for (size_t b = 0; b + 1 < batch.size(); b++) {
+            Chunk C;
+            buildChunk(e, kScore, sorted, cfgOf, pp.data(), ss.data(), batch[b], batch[b + 1], C);
+            int r2 = uploadChunk(e, C);  // pair records and the null model
+            if (r2) return r2;
+            const int np = (int)C.pairs.size();
+            std::vector<int64_t> meta(2 * (size_t)np);
+            std::vector<int> rawIdx(np);
+            int64_t co = 0, bo = 0;
+            int maxN = 3;
+            for (int k = 0; k < np; k++) {
+                const int n = C.pairs[k].n;
+                meta[k] = co;
+                meta[np + k] = bo;
+                rawIdx[k] = (int)C.order[k];
+                co += gradCells(M, n);
+                bo += n;
+                maxN = std::max(maxN, n);
+            }
+            int64_t *dm = e->dGradMeta.as<int64_t>();
+            int *dIdx = (int *)(dm + 2 * np);
+            HIPCHK(hipMemcpyAsync(dm, meta.data(), sizeof(int64_t) * 2 * np, hipMemcpyHostToDevice, e->stream));
+            HIPCHK(hipMemcpyAsync(dIdx, rawIdx.data(), sizeof(int) * np, hipMemcpyHostToDevice, e->stream));
+            GradArgs ga{};
+            ga.prof = e->dProfs.as<ProfInfo>() + prof;
+            ga.params = e->dParams.as<float>();
+            ga.emis = e->dEmis.as<float>();
+            ga.bg = e->dBg.as<float>() + (int64_t)prof * 32;
+            ga.pairs = e->dPairs.as<PairInfo>();
+            ga.rec = e->dRec.as<Rec>();
+            ga.codon = e->dGradCodon.as<uint8_t>() - codes0;  // indexed by the pairs' seqOff
+            ga.pairBase = 0;
+            ga.cellOff = dm;
+            ga.bndOff = dm + np;
+            ga.rawIdx = dIdx;
+            ga.w0 = mat;
+            ga.Y = mat + C6;
+            ga.Z0 = mat + 2 * C6;
+            ga.W1 = mat + 3 * C6;
+            ga.Yb = mat + 4 * C6;
+            ga.Zb = mat + 5 * C6;
+            ga.bndA = e->dGradBnd.as<double>();
+            ga.bndB = e->dGradBnd.as<double>() + maxBnd;
+            ga.raw = e->dGradRaw.as<double>();
+            HIPCHK(hipEventRecord(e->ev0, e->stream));
+            HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
+            HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
+            HIPCHK(launch_grad_counts(ga, np, M, e->stream));
+            HIPCHK(hipEventRecord(e->ev1, e->stream));
+            HIPCHK(hipEventSynchronize(e->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+            msAll += ms;
+        }
+        GradReduceArgs ra{};
+        ra.M = M;
+        ra.npairs = nseq;
+        ra.raw = e->dGradRaw.as<double>();
+        ra.weight = e->dGradW.as<double>();
+        ra.lnZ = e->dGradW.as<double>() + nseq;
+        ra.counts = e->dGradOut.as<double>();
+        HIPCHK(launch_grad_reduce(ra, e->dGradW.as<double>() + 2 * (size_t)nseq, e->stream));
+        HIPCHK(hipMemcpyAsync(counts, ra.counts, sizeof(double) * (size_t)(M + 1) * WR, hipMemcpyDeviceToHost,
+                              e->stream));
+        if (lnZ) HIPCHK(hipMemcpyAsync(lnZ, ra.lnZ, sizeof(double) * nseq, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        e->lastGradMs = msAll;
+        e->lastGradCells = 0;
+        for (int32_t s = 0; s < nseq; s++) e->lastGradCells += gradCells(M, lengths[s]);
+        return DM_OK;
+    };
+    rc = run();
+    restore();
+    return rc;
+}
 
 // ------------------------------------------------ profile builder: Baum-Welch
 int dm_bw_prepare(dm_engine *e, int32_t A, int32_t L, int32_t nseq, const uint8_t *letters, const int64_t *offsets,

@@ -65,6 +65,11 @@ int dm_add_raw_profiles(dm_engine *e, int32_t nprof, const int32_t *M, const flo
 int dm_add_profile(dm_engine *e, int32_t M, const float *values, const float *params, const float *bg,
                    int32_t *id);
 
+/* Replace profile `id` with new finalized arrays (as dm_add_profile; the
+ * length may change). For callers that re-evaluate a profile many times, such
+ * as a training loop over dm_grad; no counterpart in the reference. */
+int dm_set_profile(dm_engine *e, int32_t id, int32_t M, const float *values, const float *params, const float *bg);
+
 /* Copy out a profile's finalized arrays (sizes as in dm_add_profile). */
 int dm_get_profile(const dm_engine *e, int32_t id, int32_t *M, float *values, float *params, float *bg);
 int dm_profile_count(const dm_engine *e);
@@ -233,6 +238,26 @@ int dm_bw_prepare(dm_engine *e, int32_t A, int32_t L, int32_t nseq, const uint8_
                   const int32_t *lengths, const double *weights);
 int dm_bw_estep(dm_engine *e, const double *probs, double *counts, double *totals);
 double dm_last_bw_ms(const dm_engine *e);
+
+/* Gradient of the summed forward log-likelihood of one profile over DNA
+ * sequences, on the GPU (FP64 forward, backward and expected counts over the
+ * reference's recurrences, dummest-dp-forward.hh:9-250 and
+ * dummest-dp-backward.hh:444-615, without rescaling: a pair's ln Z must stay
+ * below ~700 nats; sequences of 3..4096 letters). Per sequence s the total of
+ * all local alignments Z_s = sum_{i,j} prefixSeed(j) W1(i,j) (ln_z[s]; may be
+ * NULL), and counts = sum_s weights[s] (NULL = 1) times the expected uses of
+ * each parameter: (M+1) x 76 doubles, per profile row the 11 Params of
+ * dm_add_profile (d ln Z / d ln param; entries 1, 2, 7, 8 are the global
+ * frameshift rates) then the match emissions by codon (64 codons in ACGT
+ * order, then codons holding another letter; d ln Z / d ln e of that codon).
+ * The background and the null model stay fixed. dm_last_grad_ms: device time
+ * of the DP and count kernels of the last call; dm_last_grad_cells: its cells
+ * (sum of (M+1) n). "grad_budget" (dm_set_option) bounds the FP64 matrices of
+ * one launch batch (default 8 GiB). */
+int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+            const double *weights, double *ln_z, double *counts);
+double dm_last_grad_ms(const dm_engine *e);
+int64_t dm_last_grad_cells(const dm_engine *e);
 
 /* Autotuner: times every kernel configuration (G lanes per pair, R rows per
  * lane, and the row-strip kernel) on this device with synthetic pairs of

@@ -1,0 +1,163 @@
+"""Gradient of the summed forward log-likelihood on the GPU (grad.hip,
+dm_grad) against the oracle: ln Z of every pair against the oracle's FP64
+total of all local paths (tests/test_oracle.py pins that total: forward and
+backward sums agree), and d F / d logits against FP64 central finite
+differences of the oracle's sum of ln Z over every coordinate of a small
+profile and over 64 random coordinates of a 300-position one."""
+import numpy as np
+import pytest
+
+import dummest_amd as D
+import oracle as O
+from dummest_amd import grad as G
+from dummest_amd import synth
+
+pytestmark = pytest.mark.gpu
+COMP = str.maketrans("ACGT", "TGCA")
+
+
+def make_case(rng, M, lens, homolog_every=2):
+    el = rng.normal(size=(M, 20)) * 1.2
+    tl = rng.normal(size=(M + 1, 7)) * 0.3 + np.array([3.0, 0, 0, 0.8, 0, 0.8, 0])
+    raw = np.zeros((M + 1, 29))
+    MtoI, MtoD, ItoI, DtoD, _ = G.transitions(tl)
+    raw[:, 0], raw[:, 1], raw[:, 2], raw[:, 3] = MtoI, ItoI, MtoD, DtoD
+    raw[:M, 4:24] = G._softmax(el)
+    bg = O.finalize(O.RawProfile("g", M, raw), dtype=np.float64).bg
+    p = G._softmax(el)
+    seqs = []
+    for k, n in enumerate(lens):
+        host = synth.random_dna(rng, n)
+        if k % homolog_every == 0:
+            prot = np.array([rng.choice(20, p=p[r]) for r in range(M)])
+            hom = synth.mutate_dna(rng, synth.reverse_translate(rng, prot), indel=0.01)
+            if k % 4 == 2:
+                hom = hom.translate(COMP)[::-1]
+            host = synth.embed(rng, host, hom[: max(3, n - 10)])
+        seqs.append(host)
+    return el, tl, bg, seqs
+
+
+def oracle_lnz(el, tl, bg, seqs, rounded=True):
+    v, pa = G.profile_from_logits(el, tl, bg)
+    b = bg
+    if rounded:  # the engine holds the profile in float
+        v, pa, b = (x.astype(np.float32).astype(np.float64) for x in (v, pa, bg))
+    prof = O.Profile("g", el.shape[0], np.ascontiguousarray(v), np.ascontiguousarray(pa), np.ascontiguousarray(b), [])
+    return np.array([O.totals(prof, O.decode(s))[0] for s in seqs])
+
+
+@pytest.mark.parametrize("M,lens", [(10, [40, 77, 120, 61]), (63, [200, 150]), (64, [180, 90]), (65, [210, 64]),
+                                    (130, [300, 120, 256])])
+def test_lnz_matches_oracle(engine, M, lens):
+    rng = np.random.default_rng(M)
+    el, tl, bg, seqs = make_case(rng, M, lens)
+    print("grad lnZ case", M, lens, flush=True)
+    L = G.SummedLogLikelihood(engine, seqs, bg)
+    F, ge, gt, lnz = L.evaluate(el, tl)
+    want = oracle_lnz(el, tl, bg, seqs)
+    # the engine's null model hands FP32 seeds and scales to the DP (as the
+    # reference's FP32 build does): ln Z within the score tolerance, 1e-4 nats
+    print("max |ln Z - oracle| =", np.abs(lnz - want).max())
+    assert np.abs(lnz - want).max() < 1e-4, (lnz, want)
+    assert F == pytest.approx(lnz.sum(), abs=1e-9)
+
+
+def test_counts_identities(engine):
+    rng = np.random.default_rng(5)
+    el, tl, bg, seqs = make_case(rng, 20, [90, 140, 60])
+    v, pa = G.profile_from_logits(el, tl, bg)
+    pid = engine.add_profile(20, v, pa, bg)
+    lnz, c = engine.grad_counts(pid, seqs)
+    # every match use emits one codon
+    assert np.allclose(c[:, 11:].sum(1), c[:, 10], rtol=1e-12, atol=1e-300)
+    # weights scale linearly; an empty list gives zeros
+    lnz2, c2 = engine.grad_counts(pid, seqs, weights=[2.0, 0.5, 0.0])
+    _, ca = engine.grad_counts(pid, seqs[:1])
+    _, cb = engine.grad_counts(pid, seqs[1:2])
+    assert np.allclose(c2, 2 * ca + 0.5 * cb, rtol=1e-12, atol=1e-300)
+    assert np.array_equal(lnz, lnz2)
+    # batches of the matrix budget give the same numbers
+    engine.set_option("grad_budget", 6 * 8 * 21 * 100)
+    try:
+        lnz3, c3 = engine.grad_counts(pid, seqs)
+    finally:
+        engine.set_option("grad_budget", 8 << 30)
+    assert np.array_equal(lnz3, lnz) and np.array_equal(c3, c)
+
+
+def test_gradient_matches_oracle_finite_differences(engine):
+    # every coordinate of a small profile, FP64 central differences (h = 1e-4)
+    rng = np.random.default_rng(11)
+    M = 6
+    el, tl, bg, seqs = make_case(rng, M, [45, 70, 33])
+    L = G.SummedLogLikelihood(engine, seqs, bg)
+    F, ge, gt, _ = L.evaluate(el, tl)
+    h = 1e-4
+    scale = max(np.abs(ge).max(), np.abs(gt).max())
+
+    def fd(which, r, k):
+        a, b = (el.copy(), el.copy()) if which == 0 else (tl.copy(), tl.copy())
+        a[r, k] += h
+        b[r, k] -= h
+        if which == 0:
+            return (oracle_lnz(a, tl, bg, seqs, False).sum() - oracle_lnz(b, tl, bg, seqs, False).sum()) / (2 * h)
+        return (oracle_lnz(el, a, bg, seqs, False).sum() - oracle_lnz(el, b, bg, seqs, False).sum()) / (2 * h)
+
+    worst = 0.0
+    for r in range(M):
+        for k in range(20):
+            worst = max(worst, abs(fd(0, r, k) - ge[r, k]))
+    for r in range(M + 1):
+        for k in range(7):
+            worst = max(worst, abs(fd(1, r, k) - gt[r, k]))
+    print("max |GPU - FD| / max |grad| =", worst / scale)
+    assert worst <= 2e-5 * scale
+
+
+def test_gradient_300_positions_random_coordinates(engine):
+    # the differentiation workload's profile size: M = 300, 1-3 kb sequences
+    # (half carrying frameshifted homologs); 64 random coordinates, h = 1e-3,
+    # differences of the GPU's own FP64 sum of ln Z (pinned to the oracle above)
+    rng = np.random.default_rng(300)
+    M = 300
+    lens = rng.integers(1000, 3001, 12)
+    el, tl, bg, seqs = make_case(rng, M, lens)
+    L = G.SummedLogLikelihood(engine, seqs, bg)
+    F, ge, gt, lnz = L.evaluate(el, tl)
+    want = oracle_lnz(el, tl, bg, seqs[:2])
+    # FP32 null-model scales (sC = scale^3 rounded once, applied at every codon
+    # step) put ln Z about 4e-8 nats per column from the FP64 oracle's
+    print("max |ln Z - oracle| =", np.abs(lnz[:2] - want).max(), "at", lens[:2], "letters")
+    assert np.abs(lnz[:2] - want).max() < 2e-7 * max(lens[:2]) + 1e-5
+    h = 1e-3
+    scale = max(np.abs(ge).max(), np.abs(gt).max())
+    worst = 0.0
+    for _ in range(64):
+        which = int(rng.integers(0, 2))
+        r = int(rng.integers(0, M if which == 0 else M + 1))
+        k = int(rng.integers(0, 20 if which == 0 else 7))
+        a, b = (el.copy(), el.copy()) if which == 0 else (tl.copy(), tl.copy())
+        a[r, k] += h
+        b[r, k] -= h
+        fp = L.evaluate(a, tl)[0] if which == 0 else L.evaluate(el, a)[0]
+        fm = L.evaluate(b, tl)[0] if which == 0 else L.evaluate(el, b)[0]
+        g = ge[r, k] if which == 0 else gt[r, k]
+        worst = max(worst, abs((fp - fm) / (2 * h) - g))
+    print("max |GPU - FD| / max |grad| =", worst / scale)
+    assert worst <= 1e-4 * scale
+
+
+def test_autograd_function(engine):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(8)
+    el, tl, bg, seqs = make_case(rng, 9, [60, 80])
+    L = G.SummedLogLikelihood(engine, seqs, bg)
+    F, ge, gt, _ = L.evaluate(el, tl)
+    Fn = L.function()
+    e = torch.tensor(el, dtype=torch.float64, requires_grad=True)
+    t = torch.tensor(tl, dtype=torch.float64, requires_grad=True)
+    out = 2.5 * Fn.apply(e, t)
+    out.backward()
+    assert out.item() == pytest.approx(2.5 * F)
+    assert np.allclose(e.grad.numpy(), 2.5 * ge) and np.allclose(t.grad.numpy(), 2.5 * gt)
