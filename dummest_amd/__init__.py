@@ -475,6 +475,7 @@ class Engine:
                                       _ptr(totals, ctypes.c_double)), "dm_bw_estep")
         return counts, totals[:n]
 
+    @property
     def last_bw_ms(self) -> float:
         return lib().dm_last_bw_ms(self._h)
 
@@ -492,9 +493,11 @@ class Engine:
                                   _ptr(counts, ctypes.c_double)), "dm_grad")
         return lnz[:len(seqs)], counts
 
+    @property
     def last_grad_ms(self) -> float:
         return lib().dm_last_grad_ms(self._h)
 
+    @property
     def last_grad_cells(self) -> int:
         return lib().dm_last_grad_cells(self._h)
 

@@ -272,3 +272,31 @@ def workload_c1(seed=1001, n_profiles=1000, n_windows=10000, homolog_frac=0.2, m
             dna[at:at + len(ins)] = ins
         reads.append(dna)
     return dict(Ms=Ms, raws=raws, reads=reads)
+
+
+def workload_c3(seed=1003, M=300, n_seqs=4096, homolog_frac=0.5, frameshift=0.01):
+    """One profile of M positions (built as raw_profile builds them) and DNA
+    sequences of length uniform 1-3 kb: a fraction carries the profile's
+    ancestor reverse-translated with 1-nt insertions and deletions at rate
+    `frameshift`, embedded at a random place; the rest is random (GC 0.5).
+    Also the logits at which the gradient is taken: the log of the profile's
+    match emissions (M x 20) and of its transitions per state ((M+1) x 7:
+    m->m, m->i, m->d, i->m, i->i, d->m, d->d).
+    Returns dict(M, raw, seqs (str), emis_logits, trans_logits)."""
+    rng = np.random.default_rng(seed)
+    raw, anc = raw_profile(rng, M)
+    seqs = []
+    for _ in range(n_seqs):
+        n = int(rng.integers(1000, 3001))
+        dna = random_dna_np(rng, n)
+        if rng.random() < homolog_frac:
+            ins = mutate_np(rng, reverse_translate_np(rng, anc), indel=frameshift)[:n]
+            at = int(rng.integers(0, n - len(ins) + 1))
+            dna[at:at + len(ins)] = ins
+        seqs.append(dna.tobytes().decode())
+    tiny = 1e-6
+    MtoI, ItoI, MtoD, DtoD = (np.maximum(raw[:, k], tiny) for k in range(4))
+    trans = np.log(np.stack([np.maximum(1 - MtoI - MtoD, tiny), MtoI, MtoD,
+                             np.maximum(1 - ItoI, tiny), ItoI, np.maximum(1 - DtoD, tiny), DtoD], 1))
+    emis = np.log(np.maximum(raw[:M, 4:24], tiny))
+    return dict(M=M, raw=raw, seqs=seqs, emis_logits=emis, trans_logits=trans)

@@ -12,6 +12,8 @@ import oracle as O
 from dummest_amd import grad as G
 from dummest_amd import synth
 
+import oracle_flows as OF
+
 pytestmark = pytest.mark.gpu
 COMP = str.maketrans("ACGT", "TGCA")
 
@@ -84,6 +86,33 @@ def test_counts_identities(engine):
     finally:
         engine.set_option("grad_budget", 8 << 30)
     assert np.array_equal(lnz3, lnz) and np.array_equal(c3, c)
+
+
+def test_counts_match_oracle_matrices(engine):
+    # expected counts from the oracle's own forward and backward matrices
+    # (tests/oracle_flows.py; tests/test_grad.py pins those flows against
+    # finite differences of the oracle's FP64 ln Z): match entry (also by
+    # codon), a1, a2, d1, d2 per row
+    rng = np.random.default_rng(21)
+    M = 40
+    el, tl, bg, seqs = make_case(rng, M, [150, 260, 91])
+    v, pa = G.profile_from_logits(el, tl, bg)
+    v, pa, b = (np.ascontiguousarray(x.astype(np.float32)) for x in (v, pa, bg))
+    pid = engine.add_profile(M, v, pa, b)
+    prof = O.Profile("g", M, v, pa, b, [])
+    for s in seqs:
+        print("counts vs oracle matrices, M", M, "n", len(s), flush=True)
+        lnz, got = engine.grad_counts(pid, [s])
+        want, by_codon, lnz_o = OF.matrix_flows(prof, s)
+        # the oracle's matrices are FP32: counts within 1e-4 of the row maximum
+        for k, w in want.items():
+            err = np.abs(got[:, OF.PARAM_INDEX[k]] - w).max()
+            print(f"  {k}: max |GPU - oracle| = {err:.3g} of max {w.max():.3g}")
+            assert err <= 1e-4 * max(w.max(), 1e-6), k
+        err = np.abs(got[:, 11:] - by_codon).max()
+        print(f"  by codon: max |GPU - oracle| = {err:.3g}; ln Z {lnz[0]:.9f} vs {lnz_o:.9f}")
+        assert err <= 1e-4 * max(by_codon.max(), 1e-6)
+        assert abs(lnz[0] - lnz_o) < 1e-4
 
 
 def test_gradient_matches_oracle_finite_differences(engine):
