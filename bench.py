@@ -12,7 +12,10 @@ Reported beside it, in the same run:
   * the CPU oracle (the repo's C++ restatement of the reference engine) on a
     bounded sample, threaded over the host cores;
   * the roofline of the dominant DP kernel: 43 algorithmic FP32 flops per
-    forward+backward cell against the 157.3 TFLOP/s FP32 peak.
+    forward+backward cell against the 157.3 TFLOP/s FP32 peak;
+  * the calibration sweep, sequence masking, and the gradient of the summed
+    forward log-likelihood (one 300-position profile x 4,096 sequences of
+    1-3 kb) with respect to the emission and transition logits.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--profiles P] [--reads R]
 """
@@ -91,6 +94,10 @@ def main():
                     help="profiles in the calibration sweep reported beside the main figure (0: skip)")
     ap.add_argument("--mask-reads", type=int, default=100000,
                     help="reads masked with tantan on the device in the sequence-masking figure (0: skip)")
+    ap.add_argument("--grad-seqs", type=int, default=4096,
+                    help="sequences in the summed-likelihood gradient figure (M = 300 profile; 0: skip)")
+    ap.add_argument("--grad-budget", type=int, default=64,
+                    help="GiB of FP64 matrices per gradient launch batch")
     ap.add_argument("--tune-file", default="",
                     help="load the autotuned cost table from this file if it exists, else tune and save it there")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -232,6 +239,32 @@ def main():
             masking["cpu_sample"] = f"{len(sample)} reads, oracle tantan, 1 thread"
         log(f"sequence masking: {letters} letters in {dev:.2f} ms on the device ({wall:.2f} s in all)")
 
+    # gradient of the summed forward log-likelihood of one 300-position profile
+    # over 1-3 kb sequences (half carrying frameshifted homologs): FP64
+    # forward, backward and expected counts on the device, then the chain rule
+    # to the emission and transition logits on the host
+    gradient = None
+    if args.grad_seqs > 0:
+        from dummest_amd import grad as G
+        g3 = synth.workload_c3(n_seqs=args.grad_seqs)
+        gp = e.add_raw_profiles([g3["M"]], [g3["raw"]])[0]
+        gbg = e.get_profile(gp)[3].astype(np.float64)
+        e.set_option("grad_budget", args.grad_budget << 30)
+        L = G.SummedLogLikelihood(e, g3["seqs"], gbg)
+        L.evaluate(g3["emis_logits"], g3["trans_logits"])  # warm-up: device buffers
+        t = time.perf_counter()
+        F, ge, gt, lnz = L.evaluate(g3["emis_logits"], g3["trans_logits"])
+        wall = time.perf_counter() - t
+        gcells, gms = e.last_grad_cells, e.last_grad_ms
+        gradient = {"workload": f"d sum ln Z / d logits: one profile of {g3['M']} positions x "
+                                f"{len(g3['seqs'])} sequences of 1-3 kb (half frameshifted homologs)",
+                    "cells": int(gcells), "device_ms": round(gms, 2), "wall_s": round(wall, 3),
+                    "cells_per_s": float(f"{gcells / (gms * 1e-3):.4g}"), "dtype": "f64",
+                    "batch_budget_gib": args.grad_budget, "max_ln_Z": round(float(lnz.max()), 3),
+                    "finite": bool(np.isfinite(lnz).all() and np.isfinite(ge).all() and np.isfinite(gt).all())}
+        log(f"gradient: {gcells:.4g} cells, {gms:.1f} ms on the device, {wall:.2f} s in all, "
+            f"max ln Z {lnz.max():.2f}")
+
     cpu = None
     if not args.no_cpu:
         w["oracle_profiles"] = {}
@@ -268,6 +301,7 @@ def main():
         "build_id": D.build_id(),
         "calibration_sweep": calib,
         "sequence_masking": masking,
+        "gradient": gradient,
     }
     print(json.dumps(out), flush=True)
     plan.close()

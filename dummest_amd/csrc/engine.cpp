@@ -96,7 +96,7 @@ struct dm_engine {
     double lastBwMs = 0;
     // gradient of the summed log-likelihood (dm_grad)
     int64_t gradBudget = (int64_t)8 << 30;  // bytes of FP64 matrices per launch batch
-    DevBuf dGradCodon, dGradMat, dGradBnd, dGradMeta, dGradRaw, dGradW, dGradOut;
+    DevBuf dGradCodon, dGradMat, dGradBnd, dGradMeta, dGradRaw, dGradW, dGradOut, dGradExp;
     double lastGradMs = 0;
     int64_t lastGradCells = 0;
 
@@ -804,7 +804,7 @@ void dm_engine_destroy(dm_engine *e) {
                       &e->dOut, &e->dTanBases, &e->dTanProbs, &e->dTanScales, &e->dTanMeta, &e->dTanB2f,
                       &e->dBwLetters, &e->dBwMeta, &e->dBwTables, &e->dBwMat, &e->dBwRaw,
                       &e->dBwSeqW, &e->dBwOut, &e->dGradCodon, &e->dGradMat, &e->dGradBnd, &e->dGradMeta,
-                      &e->dGradRaw, &e->dGradW, &e->dGradOut})
+                      &e->dGradRaw, &e->dGradW, &e->dGradOut, &e->dGradExp})
         b->release();
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
@@ -1579,7 +1579,7 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
     int rc = syncProfiles(e);
     if (rc) return rc;
     const int M = e->profiles[prof].M;
-    const int WR = kGradRaw - 1;
+    const int WR = kGradRaw;
     if (nseq == 0) {
         std::fill(counts, counts + (size_t)(M + 1) * WR, 0.0);
         return DM_OK;
@@ -1634,27 +1634,33 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
         HIPCHK(hipMemcpyAsync(e->dGradCodon.p, codon.data(), (size_t)total, hipMemcpyHostToDevice, e->stream));
         // batches whose six matrices fit the budget
         std::vector<size_t> batch{0};
-        int64_t cells = 0, maxCells = 1, maxPairs = 1, maxBnd = 1;
-        int64_t bnd = 0;
+        int64_t cells = 0, maxCells = 1, maxPairs = 1, maxBnd = 1, maxExp = 1;
+        int64_t bnd = 0, nexp = 0;
         for (int32_t s = 0; s < nseq; s++) {
             const int64_t c = gradCells(M, lengths[s]);
             if (cells > 0 && (cells + c) * 6 * 8 > e->gradBudget) {
                 batch.push_back((size_t)s);
                 cells = 0;
                 bnd = 0;
+                nexp = 0;
             }
             cells += c;
             bnd += lengths[s];
+            nexp += gradExps(M, lengths[s]);
             maxCells = std::max(maxCells, cells);
             maxBnd = std::max(maxBnd, bnd);
+            maxExp = std::max(maxExp, nexp);
             maxPairs = std::max<int64_t>(maxPairs, (int64_t)s + 1 - (int64_t)batch.back());
         }
         batch.push_back((size_t)nseq);
         HIPCHK(e->dGradMat.ensure(sizeof(double) * 6 * (size_t)maxCells));
-        HIPCHK(e->dGradBnd.ensure(sizeof(double) * 2 * (size_t)maxBnd));
-        HIPCHK(e->dGradMeta.ensure((sizeof(int64_t) * 2 + sizeof(int)) * (size_t)maxPairs));
+        // hand-over: three doubles and an exponent per column, and the column sums
+        HIPCHK(e->dGradBnd.ensure((sizeof(double) * 4 + sizeof(int)) * (size_t)maxBnd));
+        HIPCHK(e->dGradExp.ensure(sizeof(int) * 2 * (size_t)maxExp));
+        HIPCHK(e->dGradMeta.ensure((sizeof(int64_t) * 3 + sizeof(int)) * (size_t)maxPairs));
         HIPCHK(e->dGradRaw.ensure(sizeof(double) * (size_t)nseq * (M + 1) * kGradRaw));
-        HIPCHK(e->dGradW.ensure(sizeof(double) * 3 * (size_t)nseq));
+        // per sequence: weight, ln Z, factor, Z as mantissa and exponent
+        HIPCHK(e->dGradW.ensure(sizeof(double) * 5 * (size_t)nseq));
         HIPCHK(e->dGradOut.ensure(sizeof(double) * (size_t)(M + 1) * WR));
         std::vector<double> w(nseq, 1.0);
         if (weights) std::copy(weights, weights + nseq, w.begin());
@@ -1673,22 +1679,24 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
             int r2 = uploadChunk(e, C);  // pair records and the null model
             if (r2) return r2;
             const int np = (int)C.pairs.size();
-            std::vector<int64_t> meta(2 * (size_t)np);
+            std::vector<int64_t> meta(3 * (size_t)np);  // cell, hand-over and exponent offsets
             std::vector<int> rawIdx(np);
-            int64_t co = 0, bo = 0;
+            int64_t co = 0, bo = 0, eo = 0;
             int maxN = 3;
             for (int k = 0; k < np; k++) {
                 const int n = C.pairs[k].n;
                 meta[k] = co;
                 meta[np + k] = bo;
+                meta[2 * np + k] = eo;
                 rawIdx[k] = (int)C.order[k];
                 co += gradCells(M, n);
                 bo += n;
+                eo += gradExps(M, n);
                 maxN = std::max(maxN, n);
             }
             int64_t *dm = e->dGradMeta.as<int64_t>();
-            int *dIdx = (int *)(dm + 2 * np);
-            HIPCHK(hipMemcpyAsync(dm, meta.data(), sizeof(int64_t) * 2 * np, hipMemcpyHostToDevice, e->stream));
+            int *dIdx = (int *)(dm + 3 * np);
+            HIPCHK(hipMemcpyAsync(dm, meta.data(), sizeof(int64_t) * 3 * np, hipMemcpyHostToDevice, e->stream));
             HIPCHK(hipMemcpyAsync(dIdx, rawIdx.data(), sizeof(int) * np, hipMemcpyHostToDevice, e->stream));
             GradArgs ga{};
             ga.prof = e->dProfs.as<ProfInfo>() + prof;
@@ -1701,6 +1709,7 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
             ga.pairBase = 0;
             ga.cellOff = dm;
             ga.bndOff = dm + np;
+            ga.expOff = dm + 2 * np;
             ga.rawIdx = dIdx;
             ga.w0 = mat;
             ga.Y = mat + C6;
@@ -1709,11 +1718,18 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
             ga.Yb = mat + 4 * C6;
             ga.Zb = mat + 5 * C6;
             ga.bndA = e->dGradBnd.as<double>();
-            ga.bndB = e->dGradBnd.as<double>() + maxBnd;
+            ga.bndB = ga.bndA + maxBnd;
+            ga.bndC = ga.bndA + 2 * maxBnd;
+            ga.colSum = ga.bndA + 3 * maxBnd;
+            ga.bndE = (int *)(ga.bndA + 4 * maxBnd);
+            ga.expF = e->dGradExp.as<int>();
+            ga.expB = ga.expF + maxExp;
+            ga.zInfo = e->dGradW.as<double>() + 3 * (size_t)nseq;
             ga.raw = e->dGradRaw.as<double>();
             HIPCHK(hipEventRecord(e->ev0, e->stream));
             HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
             HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
+            HIPCHK(launch_grad_z(ga, np, e->stream));
             HIPCHK(launch_grad_counts(ga, np, M, e->stream));
             HIPCHK(hipEventRecord(e->ev1, e->stream));
             HIPCHK(hipEventSynchronize(e->ev1));
@@ -1725,6 +1741,7 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
         ra.M = M;
         ra.npairs = nseq;
         ra.raw = e->dGradRaw.as<double>();
+        ra.zInfo = e->dGradW.as<double>() + 3 * (size_t)nseq;
         ra.weight = e->dGradW.as<double>();
         ra.lnZ = e->dGradW.as<double>() + nseq;
         ra.counts = e->dGradOut.as<double>();

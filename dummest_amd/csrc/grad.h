@@ -11,18 +11,22 @@ namespace dm {
 // Per pair (profile row i = 0..M, DNA column j = 0..n-1) six FP64 matrices,
 // row-major with pitch n: the forward w0, its delete chain Y and insert chain
 // Z0, and the backward W1, Yb, Zb (dummest-dp-forward.hh:9-250,
-// dummest-dp-backward.hh:444-615, here without rescaling: FP64 holds every
-// value of a pair whose ln Z stays below ~700 nats). Expected counts per row:
-// the 11 parameters of the row's Params (alpha'[3], beta'[3], delta'[3], eps',
-// enter) and the emission by codon (64 codons, 64 = a codon with a non-ACGT
-// letter), then the row's share of Z = sum_j prefixSeed(j) W1(i, j).
+// dummest-dp-backward.hh:444-615). Values are stored scaled by a power of two
+// per 64-row strip and 64-column block (expF / expB: true value = stored x
+// 2^exponent), so any ln Z fits. Expected counts per row: the 11 parameters of
+// the row's Params (alpha'[3], beta'[3], delta'[3], eps', enter) and the
+// emission by codon (64 codons, 64 = a codon with a non-ACGT letter).
 constexpr int kGradParams = 11;
 constexpr int kGradCodons = 65;
-constexpr int kGradRaw = kGradParams + kGradCodons + 1;
+constexpr int kGradRaw = kGradParams + kGradCodons;
 // the pair's per-column codes and seeds sit in LDS: sequences up to kGradMaxN
 constexpr int kGradMaxN = 4096;
 
 inline int64_t gradCells(int M, int n) { return (int64_t)(M + 1) * n; }
+inline int gradStrips(int M) { return (M + 64) / 64; }  // 64-row strips covering rows 0..M
+inline int gradBlocks(int n) { return (n + 63) / 64; }  // 64-column blocks
+// exponents of one pair: forward and backward each strips x blocks
+inline int64_t gradExps(int M, int n) { return (int64_t)gradStrips(M) * gradBlocks(n); }
 
 struct GradArgs {
     const ProfInfo *prof;  // the profile (one per call)
@@ -35,24 +39,38 @@ struct GradArgs {
     int pairBase;          // first pair of this launch
     const int64_t *cellOff;  // per pair, relative to the matrix bases
     double *w0, *Y, *Z0, *W1, *Yb, *Zb;
-    double *bndA, *bndB;  // per pair (n each, at the pair's bndOff): strip hand-over of two values per column
+    // per pair (n each, at the pair's bndOff): strip hand-over of the edge row
+    // (forward: inflow and Y; backward: W1, Yb and the column sum of W1) and
+    // the exponent each column was stored at; colSum: the column sums of W1
+    // over all rows (at the top strip's backward exponents)
+    double *bndA, *bndB, *bndC, *colSum;
+    int *bndE;
     const int64_t *bndOff;
-    const int *rawIdx;  // per pair: its slot in raw (the caller's order)
-    double *raw;        // per slot (M+1) x kGradRaw
+    // per pair (strips x blocks at expOff): forward strips from row 0 down
+    // (strip s = rows 64s..64s+63, block c = columns 64c..64c+63), backward
+    // strips from row M up (strip s = rows M-64s down to M-64s-63, block c =
+    // columns n-1-64c down to n-64-64c)
+    int *expF, *expB;
+    const int64_t *expOff;
+    double *zInfo;      // per slot: Z = zInfo[0] x 2^zInfo[1]
+    const int *rawIdx;  // per pair: its slot in raw and zInfo (the caller's order)
+    double *raw;        // per slot (M+1) x kGradRaw: flows / 2^zInfo[1]
 };
 
 struct GradReduceArgs {
     int M, npairs;
     const double *raw;
+    const double *zInfo;
     const double *weight;  // per pair
     double *lnZ;           // out: per pair
-    double *counts;        // out: (M+1) x (kGradRaw - 1), weighted sums of the expected counts
+    double *counts;        // out: (M+1) x kGradRaw, weighted sums of the expected counts
 };
 
 hipError_t launch_grad_forward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
+hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s);
 hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s);
-// factor: npairs doubles of scratch (weight / Z per pair)
+// factor: npairs doubles of scratch (weight / Z mantissa per pair)
 hipError_t launch_grad_reduce(const GradReduceArgs &a, double *factor, hipStream_t s);
 
 }  // namespace dm

@@ -6,8 +6,10 @@
 // (forward value into the use) theta (backward value out of it).
 //
 // The recurrences are the reference's (forward dummest-dp-forward.hh:9-250,
-// backward dummest-dp-backward.hh:444-615) in FP64 without row rescaling. Per
-// use (row i, column j):
+// backward dummest-dp-backward.hh:444-615) in FP64. In place of the reference's
+// per-row rescaling (dummest-dp-backward.hh:560-573), every 64-row strip and
+// 64-column block of a pass has its own power-of-two exponent. Per use (row i,
+// column j):
 //   match:   w0(i,j) enter_i sC e_i(code(j+1)) W1(i+1, j+3)   (enter_i, and the
 //            emission of the codon at j+1)
 //   delete:  w0(i,j) d0_i Yb(i+1,j);  Y(i-1,j) eps_i Yb(i+1,j)
@@ -64,6 +66,34 @@ __device__ __forceinline__ RowParams row_params(const GradArgs &a, int row) {
     return RowParams{pr[0], pr[1], pr[2], pr[3], pr[6], pr[7], pr[8], pr[9], pr[10]};
 }
 
+constexpr int kNoExp = -1000000;  // binary exponent given to zero
+constexpr int kRaise = 64;        // a block whose largest value passes 2^kRaise gets a higher exponent
+constexpr int kLower = 256;       // one whose values all stay below 2^-kLower gets a lower one (not below 0)
+
+__device__ __forceinline__ int exp_of(double v) { return v > 0.0 ? ilogb(v) : kNoExp; }
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Exponent of a strip's next 64-column block. When lane 0 enters the block,
+// every lane still holds values of the previous block (stored at E); m is the
+// largest binary exponent among them and among the block's incoming edge
+// values, relative to E. Within 64 columns values grow by less than 2^160 (a
+// codon step multiplies by at most the largest emission odds), so stored
+// values stay below 2^230 and products of two below 2^460; values more than
+// 2^1000 below a block's largest are negligible against Z and may flush to 0.
+__device__ __forceinline__ int next_block_exp(int E, int m) {
+    if (m > kRaise) return E + m;
+    if (m < -kLower && E > 0) return max(0, E + m);
+    return E;
+}
+
+// Forward: w0, Y, Z0 of every cell, strip by strip from row 0 down. Lane k of
+// a strip runs row base+k at column t-k in step t. Each 64-column block of
+// the strip has one exponent (expF), chosen when lane 0 enters the block; a
+// lane converts its three columns of history when it enters the block itself.
 __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
@@ -71,7 +101,7 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
     extern __shared__ float sCol[];  // prefixSeed per column, then code(j-2) per column (bytes)
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
-    const int n = PR.n, M = a.prof->M;
+    const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
     const double s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
     uint8_t *sCode = reinterpret_cast<uint8_t *>(sCol + n);
     const Rec *rec = a.rec + PR.recOff;
@@ -83,7 +113,9 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
     double *w0 = a.w0 + a.cellOff[p], *Yf = a.Y + a.cellOff[p], *Z0 = a.Z0 + a.cellOff[p];
     double *hA = a.bndA + a.bndOff[p], *hB = a.bndB + a.bndOff[p];
-    for (int base = 0; base <= M; base += 64) {
+    int *hE = a.bndE + a.bndOff[p];
+    int *expF = a.expF + a.expOff[p];
+    for (int base = 0, strip = 0; base <= M; base += 64, strip++) {
         const int i = base + lane;
         const bool rowOk = i <= M;
         const RowParams P = row_params(a, rowOk ? i : 0);
@@ -93,17 +125,43 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
         __syncthreads();
         const double *Er = sE + lane * kNCode;
         // row base-1 (the previous strip's last row): its contribution to row
-        // base at column j and its Y, for step t = j, in chunks of 64 columns
+        // base at column j, its Y, and their exponent, in chunks of 64 columns
         const bool edge = base > 0;
         auto chunkA = [&](int c) { return edge ? hA[min(64 * c + lane, n - 1)] : 0.0; };
         auto chunkB = [&](int c) { return edge ? hB[min(64 * c + lane, n - 1)] : 0.0; };
+        auto chunkE = [&](int c) { return edge ? hE[min(64 * c + lane, n - 1)] : 0; };
         double cA = chunkA(0), cB = chunkB(0), nA = chunkA(1), nB = chunkB(1);
+        int cE = chunkE(0), nE = chunkE(1);
         double w1v = 0.0, w2v = 0.0, w3v = 0.0, z1 = 0.0, z2 = 0.0, z3 = 0.0;
         double lastC = 0.0, lastY = 0.0;
+        int Ecur = 0, Eprev = 0, dConv = 0;  // exponents of the newest block and the one before
+        auto decide = [&](int c) {
+            const double mx = fmax(fmax(fmax(w1v, w2v), fmax(w3v, z1)), fmax(fmax(z2, z3), fmax(lastC, lastY)));
+            int m = exp_of(mx);
+            if (edge) m = max(m, exp_of(fmax(cA, cB)) + (cE - Ecur));
+            m = wave_max(m);
+            Eprev = Ecur;
+            Ecur = next_block_exp(Ecur, m);
+            dConv = Eprev - Ecur;
+            if (edge) {
+                cA = ldexp(cA, cE - Ecur);
+                cB = ldexp(cB, cE - Ecur);
+            }
+            if (lane == 0) expF[strip * NB + c] = Ecur;
+        };
         auto step = [&](int t) {
+            if (dConv != 0 && lane == (t & 63)) {  // this lane's first column of the newest block
+                w1v = ldexp(w1v, dConv);
+                w2v = ldexp(w2v, dConv);
+                w3v = ldexp(w3v, dConv);
+                z1 = ldexp(z1, dConv);
+                z2 = ldexp(z2, dConv);
+                z3 = ldexp(z3, dConv);
+            }
             const double inC = lane_from_above(lastC), inY = lane_from_above(lastY);
             const double eA = lane_read(cA, t & 63), eB = lane_read(cB, t & 63);
             const int j = t - lane;
+            const int El = (t & 63) >= lane ? Ecur : Eprev;  // the exponent of column j's block
             double oC = lastC, oY = lastY;
             if (rowOk && j >= 0 && j < n) {
                 const double W0in = lane == 0 ? eA : inC, Yup = lane == 0 ? eB : inY;
@@ -111,7 +169,7 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
                 const double e = Er[code], g = sBg[code];
                 const double X = enterC * e * w3v;
                 const double z0 = g * sC * (P.a0 * w3v + P.b0 * z3);
-                const double w = W0in + (z0 + a1s1 * w1v + a2s2 * w2v + (double)sCol[j]);
+                const double w = W0in + (z0 + a1s1 * w1v + a2s2 * w2v + ldexp((double)sCol[j], -El));
                 const double y = P.d0 * w + P.eps * Yup;
                 oC = X + y + d1s2 * w2v + d2s1 * w1v;  // row i+1's inflow at column j
                 oY = y;
@@ -122,6 +180,7 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
                 if (lane == 63) {
                     hA[j] = oC;
                     hB[j] = oY;
+                    hE[j] = El;
                 }
                 w3v = w2v;
                 w2v = w1v;
@@ -133,29 +192,43 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
             lastC = oC;
             lastY = oY;
         };
+        // at step 64c: the edge chunk of block c, then block c's exponent (no
+        // block starts at or past column n: the lanes stay at the last one)
+        auto boundary = [&](int t) {
+            if (t > 0) {
+                cA = nA;
+                cB = nB;
+                cE = nE;
+                nA = chunkA((t >> 6) + 1);
+                nB = chunkB((t >> 6) + 1);
+                nE = chunkE((t >> 6) + 1);
+            }
+            if (t < n) {
+                decide(t >> 6);
+            } else {
+                Eprev = Ecur;
+                dConv = 0;
+            }
+        };
         const int T = n + 63;
         int t = 0;
         for (; t + 4 <= T; t += 4) {
-            if ((t & 63) == 0 && t > 0) {
-                cA = nA;
-                cB = nB;
-                nA = chunkA((t >> 6) + 1);
-                nB = chunkB((t >> 6) + 1);
-            }
+            if ((t & 63) == 0) boundary(t);
             step(t);
             step(t + 1);
             step(t + 2);
             step(t + 3);
         }
-        if ((t & 63) == 0 && t > 0) {
-            cA = nA;
-            cB = nB;
-        }
+        if ((t & 63) == 0) boundary(t);
         for (; t < T; t++) step(t);
         __threadfence();  // the next strip reads this strip's hand-over values
     }
 }
 
+// Backward: W1, Yb, Zb of every cell, strip by strip from row M up; lane k of
+// a strip runs row top-k at column n-1-t+k in step t. Blocks of 64 columns
+// from the right (expB) as in the forward. The column sums of W1 run down
+// the lanes with Yb and reach row 0 as sum_i W1(i, j) (colSum).
 __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
@@ -163,7 +236,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
     extern __shared__ float sCol[];  // suffixSeed per column, then code(j+1) per column
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
-    const int n = PR.n, M = a.prof->M;
+    const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
     const double s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
     uint8_t *sCode = reinterpret_cast<uint8_t *>(sCol + n);
     const Rec *rec = a.rec + PR.recOff;
@@ -174,8 +247,11 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
     }
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
     double *W1 = a.W1 + a.cellOff[p], *Yb = a.Yb + a.cellOff[p], *Zb = a.Zb + a.cellOff[p];
-    double *hA = a.bndA + a.bndOff[p], *hB = a.bndB + a.bndOff[p];
-    for (int top = M; top >= 0; top -= 64) {
+    double *hA = a.bndA + a.bndOff[p], *hB = a.bndB + a.bndOff[p], *hC = a.bndC + a.bndOff[p];
+    double *cs = a.colSum + a.bndOff[p];
+    int *hE = a.bndE + a.bndOff[p];
+    int *expB = a.expB + a.expOff[p];
+    for (int top = M, strip = 0; top >= 0; top -= 64, strip++) {
         const int i = top - lane;
         const bool rowOk = i >= 0;
         const RowParams P = row_params(a, rowOk ? i : 0);
@@ -185,23 +261,54 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
         stage_emissions(sE, a, M, lane, [&](int rr) { return top - rr; });
         __syncthreads();
         const double *Er = sE + lane * kNCode;
-        // row top+1 (the previous strip's last row): W1 and Yb at column
-        // j = n-1-t for step t, in chunks of 64 columns
+        // row top+1 (the previous strip's last row): W1, Yb, the column sum and
+        // their exponent at column j = n-1-t for step t, in chunks of 64 columns
         const bool edge = top < M;
         auto chunkA = [&](int c) { return edge ? hA[max(n - 1 - 64 * c - lane, 0)] : 0.0; };
         auto chunkB = [&](int c) { return edge ? hB[max(n - 1 - 64 * c - lane, 0)] : 0.0; };
-        double cA = chunkA(0), cB = chunkB(0), nA = chunkA(1), nB = chunkB(1);
+        auto chunkC = [&](int c) { return edge ? hC[max(n - 1 - 64 * c - lane, 0)] : 0.0; };
+        auto chunkE = [&](int c) { return edge ? hE[max(n - 1 - 64 * c - lane, 0)] : 0; };
+        double cA = chunkA(0), cB = chunkB(0), cC = chunkC(0), nA = chunkA(1), nB = chunkB(1), nC = chunkC(1);
+        int cE = chunkE(0), nE = chunkE(1);
         double r1 = 0.0, r2 = 0.0, r3 = 0.0;  // W1(i+1, j+1..j+3), received earlier
         double c1 = 0.0, c2 = 0.0;            // W1(i, j+1), W1(i, j+2)
         double zb1 = 0.0, zb2 = 0.0, zb3 = 0.0;  // Zb(i, j+1..j+3)
-        double lastW = 0.0, lastY = 0.0;
+        double lastW = 0.0, lastY = 0.0, lastS = 0.0;
+        int Ecur = 0, Eprev = 0, dConv = 0;
+        auto decide = [&](int c) {
+            const double mx = fmax(fmax(fmax(fmax(r1, r2), fmax(r3, c1)), fmax(fmax(c2, zb1), fmax(zb2, zb3))),
+                                   fmax(fmax(lastW, lastY), lastS));
+            int m = exp_of(mx);
+            if (edge) m = max(m, exp_of(fmax(fmax(cA, cB), cC)) + (cE - Ecur));
+            m = wave_max(m);
+            Eprev = Ecur;
+            Ecur = next_block_exp(Ecur, m);
+            dConv = Eprev - Ecur;
+            if (edge) {
+                cA = ldexp(cA, cE - Ecur);
+                cB = ldexp(cB, cE - Ecur);
+                cC = ldexp(cC, cE - Ecur);
+            }
+            if (lane == 0) expB[strip * NB + c] = Ecur;
+        };
         auto step = [&](int t) {
-            const double inW = lane_from_above(lastW), inY = lane_from_above(lastY);
-            const double eA = lane_read(cA, t & 63), eB = lane_read(cB, t & 63);
+            if (dConv != 0 && lane == (t & 63)) {  // this lane's first column of the newest block
+                r1 = ldexp(r1, dConv);
+                r2 = ldexp(r2, dConv);
+                r3 = ldexp(r3, dConv);
+                c1 = ldexp(c1, dConv);
+                c2 = ldexp(c2, dConv);
+                zb1 = ldexp(zb1, dConv);
+                zb2 = ldexp(zb2, dConv);
+                zb3 = ldexp(zb3, dConv);
+            }
+            const double inW = lane_from_above(lastW), inY = lane_from_above(lastY), inS = lane_from_above(lastS);
+            const double eA = lane_read(cA, t & 63), eB = lane_read(cB, t & 63), eC = lane_read(cC, t & 63);
             const int j = n - 1 - t + lane;
-            double oW = lastW, oY = lastY;
+            const int El = (t & 63) >= lane ? Ecur : Eprev;
+            double oW = lastW, oY = lastY, oS = lastS;
             if (rowOk && j >= 0 && j < n) {
-                const double Wdn = lane == 0 ? eA : inW, Ynext = lane == 0 ? eB : inY;
+                const double Wdn = lane == 0 ? eA : inW, Ynext = lane == 0 ? eB : inY, Sdn = lane == 0 ? eC : inS;
                 const int code = sCode[j];
                 const double e = Er[code], g = sBg[code];
                 double w = r3 * e * enterC + Ynext * P.d0;
@@ -210,9 +317,10 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 w = w + zb3 * g * a0sC;
                 w = w + c2 * a2s2;
                 w = w + c1 * a1s1;
-                w = w + (double)sCol[j];
+                w = w + ldexp((double)sCol[j], -El);
                 const double y = __fma_rn(P.eps, Ynext, w);
                 const double zb = __fma_rn(b0sC, zb3 * g, w);
+                const double sum = Sdn + w;
                 const size_t at = (size_t)i * n + j;
                 W1[at] = w;
                 Yb[at] = y;
@@ -220,7 +328,10 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 if (lane == 63) {
                     hA[j] = w;
                     hB[j] = y;
+                    hC[j] = sum;
+                    hE[j] = El;
                 }
+                if (i == 0) cs[j] = sum;
                 r3 = r2;
                 r2 = r1;
                 r1 = Wdn;
@@ -231,41 +342,83 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 zb1 = zb;
                 oW = w;
                 oY = y;
+                oS = sum;
             }
             lastW = oW;
             lastY = oY;
+            lastS = oS;
+        };
+        auto boundary = [&](int t) {
+            if (t > 0) {
+                cA = nA;
+                cB = nB;
+                cC = nC;
+                cE = nE;
+                nA = chunkA((t >> 6) + 1);
+                nB = chunkB((t >> 6) + 1);
+                nC = chunkC((t >> 6) + 1);
+                nE = chunkE((t >> 6) + 1);
+            }
+            if (t < n) {
+                decide(t >> 6);
+            } else {
+                Eprev = Ecur;
+                dConv = 0;
+            }
         };
         const int T = n + 63;
         int t = 0;
         for (; t + 4 <= T; t += 4) {
-            if ((t & 63) == 0 && t > 0) {
-                cA = nA;
-                cB = nB;
-                nA = chunkA((t >> 6) + 1);
-                nB = chunkB((t >> 6) + 1);
-            }
+            if ((t & 63) == 0) boundary(t);
             step(t);
             step(t + 1);
             step(t + 2);
             step(t + 3);
         }
-        if ((t & 63) == 0 && t > 0) {
-            cA = nA;
-            cB = nB;
-        }
+        if ((t & 63) == 0) boundary(t);
         for (; t < T; t++) step(t);
         __threadfence();
     }
 }
 
+// Per pair: Z = sum_j prefixSeed(j) colSum(j) as a mantissa and a binary
+// exponent (zInfo), the columns added left to right.
+__global__ void grad_z_kernel(GradArgs a, int npairs) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs) return;
+    const int p = a.pairBase + k;
+    const PairInfo PR = a.pairs[p];
+    const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
+    const double *cs = a.colSum + a.bndOff[p];
+    const int *eb = a.expB + a.expOff[p] + (int64_t)(M / 64) * NB;  // the strip holding row 0
+    const Rec *rec = a.rec + PR.recOff;
+    double zm = 0.0;
+    int ze = kNoExp;
+    for (int j = 0; j < n; j++) {
+        const double v = (double)rec[j].ps * cs[j];
+        if (!(v > 0.0)) continue;
+        const int e = eb[(n - 1 - j) >> 6];
+        if (e > ze) {
+            zm = ldexp(zm, ze - e) + v;
+            ze = e;
+        } else {
+            zm += ldexp(v, e - ze);
+        }
+    }
+    double *zi = a.zInfo + 2 * (size_t)a.rawIdx[p];
+    zi[0] = zm;
+    zi[1] = (double)ze;
+}
+
 // Per row (one lane): the flows of the row's parameters and codons over all
-// columns, and the row's share of Z (unnormalized).
+// columns, each as its stored product times 2^(forward exponent + backward
+// exponent - Z's exponent).
 __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double emis[kGradCodons][64];
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
-    const int n = PR.n, M = a.prof->M;
+    const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
     const int i = 64 * blockIdx.y + lane;
     for (int c = 0; c < kGradCodons; c++) emis[c][lane] = 0.0;
     if (i > M) return;
@@ -282,11 +435,29 @@ __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
     const double *w0 = a.w0 + off + here, *Z0 = a.Z0 + off + here, *Yup = a.Y + off + above;
     const double *W1 = a.W1 + off + here, *W1d = a.W1 + off + below, *Ybd = a.Yb + off + below;
     const double *Zbh = a.Zb + off + here;
+    const int ZE = (int)a.zInfo[2 * (size_t)a.rawIdx[p] + 1];
+    // exponent rows: forward strips of rows i and i-1, backward strips of rows i and i+1
+    const int *eF = a.expF + a.expOff[p], *eB = a.expB + a.expOff[p];
+    const int *eFh = eF + (int64_t)(i >> 6) * NB, *eFu = eF + (int64_t)((hasAbove ? i - 1 : i) >> 6) * NB;
+    const int *eBh = eB + (int64_t)((M - i) >> 6) * NB, *eBd = eB + (int64_t)((M - (hasBelow ? i + 1 : i)) >> 6) * NB;
+    int ef = 0, efu = 0, rb = 0, bh0 = 0, bh1 = 0, bd0 = 0, bd1 = 0;
     double cnt[kGradParams] = {0.0};
-    double zrow = 0.0;
     for (int j = 0; j < n; j++) {
-        const Rec r = rec[j];
-        const int code = (int)(r.codes & 255);  // code(j+1)
+        if ((j & 63) == 0) {
+            ef = eFh[j >> 6];
+            efu = eFu[j >> 6];
+        }
+        if (j == 0 || ((n - 1 - j) & 63) == 63) {  // backward block of column j, and the block right of it
+            rb = (n - 1 - j) >> 6;
+            bh0 = eBh[rb];
+            bd0 = eBd[rb];
+            bh1 = rb > 0 ? eBh[rb - 1] : bh0;
+            bd1 = rb > 0 ? eBd[rb - 1] : bd0;
+        }
+        auto inBlock = [&](int d) { return ((n - 1 - j - d) >> 6) == rb; };
+        const int eh1 = inBlock(1) ? bh0 : bh1, eh2 = inBlock(2) ? bh0 : bh1, eh3 = inBlock(3) ? bh0 : bh1;
+        const int ed1 = inBlock(1) ? bd0 : bd1, ed2 = inBlock(2) ? bd0 : bd1, ed3 = inBlock(3) ? bd0 : bd1;
+        const int code = (int)(rec[j].codes & 255);  // code(j+1)
         const double w = w0[j];
         const double e = Er[code], g = a.bg[code];
         const double wd1 = (hasBelow && j + 1 < n) ? W1d[j + 1] : 0.0;
@@ -296,44 +467,40 @@ __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
         const double wh1 = (j + 1 < n) ? W1[j + 1] : 0.0, wh2 = (j + 2 < n) ? W1[j + 2] : 0.0;
         const double zb3 = (j + 3 < n) ? Zbh[j + 3] : 0.0;
         const double yup = hasAbove ? Yup[j] : 0.0;
-        const double fm = w * enterC * e * wd3;
+        const int f = ef - ZE;
+        const double fm = ldexp(w * enterC * e * wd3, f + ed3);
         cnt[10] += fm;
         emis[(j + 1 < n) ? codon[j + 1] : kGradCodons - 1][lane] += fm;
-        cnt[7] += w * d1s2 * wd2;
-        cnt[8] += w * d2s1 * wd1;
-        cnt[6] += w * P.d0 * ybd;
-        cnt[9] += yup * P.eps * ybd;
-        cnt[1] += w * a1s1 * wh1;
-        cnt[2] += w * a2s2 * wh2;
-        cnt[0] += w * a0sC * g * zb3;
-        cnt[3] += Z0[j] * b0sC * g * zb3;
-        zrow += (double)r.ps * W1[j];
+        cnt[7] += ldexp(w * d1s2 * wd2, f + ed2);
+        cnt[8] += ldexp(w * d2s1 * wd1, f + ed1);
+        cnt[6] += ldexp(w * P.d0 * ybd, f + bd0);
+        cnt[9] += ldexp(yup * P.eps * ybd, efu - ZE + bd0);
+        cnt[1] += ldexp(w * a1s1 * wh1, f + eh1);
+        cnt[2] += ldexp(w * a2s2 * wh2, f + eh2);
+        cnt[0] += ldexp(w * a0sC * g * zb3, f + eh3);
+        cnt[3] += ldexp(Z0[j] * b0sC * g * zb3, f + eh3);
     }
     double *out = a.raw + ((size_t)a.rawIdx[p] * (M + 1) + i) * kGradRaw;
     for (int k = 0; k < kGradParams; k++) out[k] = cnt[k];
     for (int c = 0; c < kGradCodons; c++) out[kGradParams + c] = emis[c][lane];
-    out[kGradRaw - 1] = zrow;
 }
 
-// Per pair: Z (rows added in order), ln Z, and the factor weight / Z.
-__global__ void grad_z_kernel(GradReduceArgs a, double *factor) {
+// Per pair: ln Z and the factor weight / (Z's mantissa).
+__global__ void grad_zfactor_kernel(GradReduceArgs a, double *factor) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.npairs) return;
-    double z = 0.0;
-    for (int i = 0; i <= a.M; i++) z += a.raw[((size_t)p * (a.M + 1) + i) * kGradRaw + kGradRaw - 1];
-    a.lnZ[p] = log(z);
-    factor[p] = a.weight[p] / z;
+    const double zm = a.zInfo[2 * (size_t)p], ze = a.zInfo[2 * (size_t)p + 1];
+    a.lnZ[p] = log(zm) + ze * 0.693147180559945309417;  // ln 2
+    factor[p] = a.weight[p] / zm;
 }
 
 // counts[r][c] = sum over pairs, in order, of the pair's flow / Z * weight.
 __global__ void grad_reduce_kernel(GradReduceArgs a, const double *factor) {
 #pragma clang fp contract(off)
-    const int W = kGradRaw - 1;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (a.M + 1) * W) return;
-    const int r = id / W, c = id - r * W;
+    if (id >= (a.M + 1) * kGradRaw) return;
     double acc = 0.0;
-    for (int p = 0; p < a.npairs; p++) acc += a.raw[((size_t)p * (a.M + 1) + r) * kGradRaw + c] * factor[p];
+    for (int p = 0; p < a.npairs; p++) acc += a.raw[(size_t)p * (a.M + 1) * kGradRaw + id] * factor[p];
     a.counts[id] = acc;
 }
 
@@ -353,6 +520,12 @@ hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStre
     return hipGetLastError();
 }
 
+hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s) {
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(grad_z_kernel, dim3((npairs + 63) / 64), dim3(64), 0, s, a, npairs);
+    return hipGetLastError();
+}
+
 hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
     hipLaunchKernelGGL(grad_counts_kernel, dim3(npairs, (M + 1 + 63) / 64), dim3(64), 0, s, a);
@@ -360,8 +533,8 @@ hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t 
 }
 
 hipError_t launch_grad_reduce(const GradReduceArgs &a, double *factor, hipStream_t s) {
-    if (a.npairs > 0) hipLaunchKernelGGL(grad_z_kernel, dim3((a.npairs + 63) / 64), dim3(64), 0, s, a, factor);
-    const int cells = (a.M + 1) * (kGradRaw - 1);
+    if (a.npairs > 0) hipLaunchKernelGGL(grad_zfactor_kernel, dim3((a.npairs + 63) / 64), dim3(64), 0, s, a, factor);
+    const int cells = (a.M + 1) * kGradRaw;
     hipLaunchKernelGGL(grad_reduce_kernel, dim3((cells + 255) / 256), dim3(256), 0, s, a, (const double *)factor);
     return hipGetLastError();
 }

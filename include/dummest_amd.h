@@ -242,8 +242,9 @@ double dm_last_bw_ms(const dm_engine *e);
 /* Gradient of the summed forward log-likelihood of one profile over DNA
  * sequences, on the GPU (FP64 forward, backward and expected counts over the
  * reference's recurrences, dummest-dp-forward.hh:9-250 and
- * dummest-dp-backward.hh:444-615, without rescaling: a pair's ln Z must stay
- * below ~700 nats; sequences of 3..4096 letters). Per sequence s the total of
+ * dummest-dp-backward.hh:444-615, scaled by a power of two per 64-row strip
+ * and 64-column block, so any ln Z fits; sequences of 3..4096 letters). Per
+ * sequence s the total of
  * all local alignments Z_s = sum_{i,j} prefixSeed(j) W1(i,j) (ln_z[s]; may be
  * NULL), and counts = sum_s weights[s] (NULL = 1) times the expected uses of
  * each parameter: (M+1) x 76 doubles, per profile row the 11 Params of
@@ -251,7 +252,7 @@ double dm_last_bw_ms(const dm_engine *e);
  * frameshift rates) then the match emissions by codon (64 codons in ACGT
  * order, then codons holding another letter; d ln Z / d ln e of that codon).
  * The background and the null model stay fixed. dm_last_grad_ms: device time
- * of the DP and count kernels of the last call; dm_last_grad_cells: its cells
+ * of the DP, total and count kernels of the last call; dm_last_grad_cells: its cells
  * (sum of (M+1) n). "grad_budget" (dm_set_option) bounds the FP64 matrices of
  * one launch batch (default 8 GiB). */
 int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,

@@ -190,3 +190,71 @@ def test_autograd_function(engine):
     out.backward()
     assert out.item() == pytest.approx(2.5 * F)
     assert np.allclose(e.grad.numpy(), 2.5 * ge) and np.allclose(t.grad.numpy(), 2.5 * gt)
+
+
+def strong_case(rng, M, host_len, copies):
+    """Peaked emissions and the profile's most likely protein, reverse-translated
+    and embedded `copies` times without mutations: ln Z far past FP64's range."""
+    el = rng.normal(size=(M, 20)) * 5.0
+    tl = rng.normal(size=(M + 1, 7)) * 0.3 + np.array([5.0, 0, 0, 0.8, 0, 0.8, 0])
+    raw = np.zeros((M + 1, 29))
+    MtoI, MtoD, ItoI, DtoD, _ = G.transitions(tl)
+    raw[:, 0], raw[:, 1], raw[:, 2], raw[:, 3] = MtoI, ItoI, MtoD, DtoD
+    raw[:M, 4:24] = G._softmax(el)
+    bg = O.finalize(O.RawProfile("s", M, raw), dtype=np.float64).bg
+    hom = synth.reverse_translate(rng, el.argmax(1))
+    host = synth.random_dna(rng, host_len)
+    for k in range(copies):
+        at = (k + 1) * host_len // (copies + 1)
+        host = host[:at] + hom + host[at:]
+    return el, tl, bg, host
+
+
+def test_strong_homologs_beyond_fp64_range(engine):
+    # ln Z of about 1056 nats (FP64 ends at e^709): two exact copies of a
+    # 400-position profile's most likely protein in one sequence, one copy in
+    # another, and a random sequence; the strip/block exponents must carry the
+    # rise over the first copy, the fall after it and the second rise
+    rng = np.random.default_rng(7)
+    M = 400
+    el, tl, bg, two = strong_case(rng, M, 600, 2)
+    one = strong_case(np.random.default_rng(7), M, 900, 1)[3]
+    seqs = [two, one, synth.random_dna(rng, 1500)]
+    print("strong homologs: M", M, "lengths", [len(s) for s in seqs], flush=True)
+    L = G.SummedLogLikelihood(engine, seqs, bg)
+    F, ge, gt, lnz = L.evaluate(el, tl)
+    want = oracle_lnz(el, tl, bg, seqs)
+    print("ln Z", lnz, "oracle", want)
+    assert want[0] > 1000 and np.isfinite(lnz).all()
+    assert np.abs(lnz - want).max() < 2e-7 * max(len(s) for s in seqs) + 1e-5
+    # the two copies share the posterior: about one match use per row in all
+    v, pa = G.profile_from_logits(el, tl, bg)
+    v, pa, b = (np.ascontiguousarray(x.astype(np.float32)) for x in (v, pa, bg))
+    pid = engine.add_profile(M, v, pa, b)
+    prof = O.Profile("s", M, v, pa, b, [])
+    for s in seqs:
+        _, got = engine.grad_counts(pid, [s])
+        want_c, by_codon, _ = OF.matrix_flows(prof, s)
+        for k, w in want_c.items():
+            err = np.abs(got[:, OF.PARAM_INDEX[k]] - w).max()
+            print(f"  n {len(s)} {k}: max |GPU - oracle| = {err:.3g} of max {w.max():.3g}")
+            # the oracle's FP32 matrices over a 1,200-letter alignment
+            assert err <= 1e-3 * max(w.max(), 1e-6), k
+        assert np.abs(got[:, 11:] - by_codon).max() <= 1e-3 * max(by_codon.max(), 1e-6)
+    # finite differences of the GPU's own sum of ln Z at 16 coordinates
+    h = 1e-3
+    scale = max(np.abs(ge).max(), np.abs(gt).max())
+    worst = 0.0
+    for _ in range(16):
+        which = int(rng.integers(0, 2))
+        r = int(rng.integers(0, M if which == 0 else M + 1))
+        k = int(rng.integers(0, 20 if which == 0 else 7))
+        a, b2 = (el.copy(), el.copy()) if which == 0 else (tl.copy(), tl.copy())
+        a[r, k] += h
+        b2[r, k] -= h
+        fp = L.evaluate(a, tl)[0] if which == 0 else L.evaluate(el, a)[0]
+        fm = L.evaluate(b2, tl)[0] if which == 0 else L.evaluate(el, b2)[0]
+        g = ge[r, k] if which == 0 else gt[r, k]
+        worst = max(worst, abs((fp - fm) / (2 * h) - g))
+    print("max |GPU - FD| / max |grad| =", worst / scale)
+    assert worst <= 1e-4 * scale
