@@ -69,7 +69,7 @@ struct GradReduceArgs {
 hipError_t launch_grad_forward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s);
-hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s);
+hipError_t launch_grad_counts(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 // factor: npairs doubles of scratch (weight / Z mantissa per pair)
 hipError_t launch_grad_reduce(const GradReduceArgs &a, double *factor, hipStream_t s);
 

@@ -210,17 +210,21 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
                 dConv = 0;
             }
         };
+        // one outer iteration per 64 steps: the inner loop issues no loads, so
+        // its stores are never waited for
         const int T = n + 63;
-        int t = 0;
-        for (; t + 4 <= T; t += 4) {
-            if ((t & 63) == 0) boundary(t);
-            step(t);
-            step(t + 1);
-            step(t + 2);
-            step(t + 3);
+        for (int t0 = 0; t0 < T; t0 += 64) {
+            boundary(t0);
+            const int tEnd = min(t0 + 64, T);
+            int t = t0;
+            for (; t + 4 <= tEnd; t += 4) {
+                step(t);
+                step(t + 1);
+                step(t + 2);
+                step(t + 3);
+            }
+            for (; t < tEnd; t++) step(t);
         }
-        if ((t & 63) == 0) boundary(t);
-        for (; t < T; t++) step(t);
         __threadfence();  // the next strip reads this strip's hand-over values
     }
 }
@@ -366,17 +370,21 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 dConv = 0;
             }
         };
+        // one outer iteration per 64 steps: the inner loop issues no loads, so
+        // its stores are never waited for
         const int T = n + 63;
-        int t = 0;
-        for (; t + 4 <= T; t += 4) {
-            if ((t & 63) == 0) boundary(t);
-            step(t);
-            step(t + 1);
-            step(t + 2);
-            step(t + 3);
+        for (int t0 = 0; t0 < T; t0 += 64) {
+            boundary(t0);
+            const int tEnd = min(t0 + 64, T);
+            int t = t0;
+            for (; t + 4 <= tEnd; t += 4) {
+                step(t);
+                step(t + 1);
+                step(t + 2);
+                step(t + 3);
+            }
+            for (; t < tEnd; t++) step(t);
         }
-        if ((t & 63) == 0) boundary(t);
-        for (; t < T; t++) step(t);
         __threadfence();
     }
 }
@@ -410,79 +418,102 @@ __global__ void grad_z_kernel(GradArgs a, int npairs) {
     zi[1] = (double)ze;
 }
 
-// Per row (one lane): the flows of the row's parameters and codons over all
-// columns, each as its stored product times 2^(forward exponent + backward
-// exponent - Z's exponent).
+__device__ __forceinline__ double wave_sum(double v) {  // the same total in every lane
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Per pair (one wavefront), row by row: lane l takes columns 64c+l of each
+// 64-column block c, so every matrix read is a contiguous 512-byte run. The
+// flows of the row's parameters are summed per lane and then across the wave;
+// the match flows go into the row's 65 codon bins in LDS. Each flow is its
+// stored product times 2^(forward exponent + backward exponent - Z's exponent).
 __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
 #pragma clang fp contract(off)
-    __shared__ double emis[kGradCodons][64];
+    __shared__ double bins[kGradCodons];
+    __shared__ double sBg[32];
+    extern __shared__ uint8_t sCodes[];  // code(j+1), then the codon index at j+1, per column
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
-    const int i = 64 * blockIdx.y + lane;
-    for (int c = 0; c < kGradCodons; c++) emis[c][lane] = 0.0;
-    if (i > M) return;
     const double s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
-    const RowParams P = row_params(a, i);
-    const double a0sC = P.a0 * sC, a1s1 = P.a1 * s1, a2s2 = P.a2 * s2, b0sC = P.b0 * sC;
-    const double d1s2 = P.d1 * s2, d2s1 = P.d2 * s1, enterC = P.enter * sC;
-    const float *Er = a.emis + a.prof->emOff + (int64_t)i * kEmStride;
+    uint8_t *sCodon = sCodes + n;
     const Rec *rec = a.rec + PR.recOff;
     const uint8_t *codon = a.codon + PR.seqOff;
-    const bool hasBelow = i < M, hasAbove = i > 0;
-    const size_t off = a.cellOff[p], here = (size_t)i * n, below = hasBelow ? (size_t)(i + 1) * n : here,
-                 above = hasAbove ? (size_t)(i - 1) * n : here;
-    const double *w0 = a.w0 + off + here, *Z0 = a.Z0 + off + here, *Yup = a.Y + off + above;
-    const double *W1 = a.W1 + off + here, *W1d = a.W1 + off + below, *Ybd = a.Yb + off + below;
-    const double *Zbh = a.Zb + off + here;
-    const int ZE = (int)a.zInfo[2 * (size_t)a.rawIdx[p] + 1];
-    // exponent rows: forward strips of rows i and i-1, backward strips of rows i and i+1
-    const int *eF = a.expF + a.expOff[p], *eB = a.expB + a.expOff[p];
-    const int *eFh = eF + (int64_t)(i >> 6) * NB, *eFu = eF + (int64_t)((hasAbove ? i - 1 : i) >> 6) * NB;
-    const int *eBh = eB + (int64_t)((M - i) >> 6) * NB, *eBd = eB + (int64_t)((M - (hasBelow ? i + 1 : i)) >> 6) * NB;
-    int ef = 0, efu = 0, rb = 0, bh0 = 0, bh1 = 0, bd0 = 0, bd1 = 0;
-    double cnt[kGradParams] = {0.0};
-    for (int j = 0; j < n; j++) {
-        if ((j & 63) == 0) {
-            ef = eFh[j >> 6];
-            efu = eFu[j >> 6];
-        }
-        if (j == 0 || ((n - 1 - j) & 63) == 63) {  // backward block of column j, and the block right of it
-            rb = (n - 1 - j) >> 6;
-            bh0 = eBh[rb];
-            bd0 = eBd[rb];
-            bh1 = rb > 0 ? eBh[rb - 1] : bh0;
-            bd1 = rb > 0 ? eBd[rb - 1] : bd0;
-        }
-        auto inBlock = [&](int d) { return ((n - 1 - j - d) >> 6) == rb; };
-        const int eh1 = inBlock(1) ? bh0 : bh1, eh2 = inBlock(2) ? bh0 : bh1, eh3 = inBlock(3) ? bh0 : bh1;
-        const int ed1 = inBlock(1) ? bd0 : bd1, ed2 = inBlock(2) ? bd0 : bd1, ed3 = inBlock(3) ? bd0 : bd1;
-        const int code = (int)(rec[j].codes & 255);  // code(j+1)
-        const double w = w0[j];
-        const double e = Er[code], g = a.bg[code];
-        const double wd1 = (hasBelow && j + 1 < n) ? W1d[j + 1] : 0.0;
-        const double wd2 = (hasBelow && j + 2 < n) ? W1d[j + 2] : 0.0;
-        const double wd3 = (hasBelow && j + 3 < n) ? W1d[j + 3] : 0.0;
-        const double ybd = hasBelow ? Ybd[j] : 0.0;
-        const double wh1 = (j + 1 < n) ? W1[j + 1] : 0.0, wh2 = (j + 2 < n) ? W1[j + 2] : 0.0;
-        const double zb3 = (j + 3 < n) ? Zbh[j + 3] : 0.0;
-        const double yup = hasAbove ? Yup[j] : 0.0;
-        const int f = ef - ZE;
-        const double fm = ldexp(w * enterC * e * wd3, f + ed3);
-        cnt[10] += fm;
-        emis[(j + 1 < n) ? codon[j + 1] : kGradCodons - 1][lane] += fm;
-        cnt[7] += ldexp(w * d1s2 * wd2, f + ed2);
-        cnt[8] += ldexp(w * d2s1 * wd1, f + ed1);
-        cnt[6] += ldexp(w * P.d0 * ybd, f + bd0);
-        cnt[9] += ldexp(yup * P.eps * ybd, efu - ZE + bd0);
-        cnt[1] += ldexp(w * a1s1 * wh1, f + eh1);
-        cnt[2] += ldexp(w * a2s2 * wh2, f + eh2);
-        cnt[0] += ldexp(w * a0sC * g * zb3, f + eh3);
-        cnt[3] += ldexp(Z0[j] * b0sC * g * zb3, f + eh3);
+    for (int j = lane; j < n; j += 64) {
+        sCodes[j] = (uint8_t)(rec[j].codes & 255);
+        sCodon[j] = j + 1 < n ? codon[j + 1] : (uint8_t)(kGradCodons - 1);
     }
-    double *out = a.raw + ((size_t)a.rawIdx[p] * (M + 1) + i) * kGradRaw;
-    for (int k = 0; k < kGradParams; k++) out[k] = cnt[k];
-    for (int c = 0; c < kGradCodons; c++) out[kGradParams + c] = emis[c][lane];
+    if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
+    const size_t off = a.cellOff[p];
+    const double *w0 = a.w0 + off, *Z0 = a.Z0 + off, *Yf = a.Y + off;
+    const double *W1 = a.W1 + off, *Yb = a.Yb + off, *Zb = a.Zb + off;
+    const int ZE = (int)a.zInfo[2 * (size_t)a.rawIdx[p] + 1];
+    const int *eF = a.expF + a.expOff[p], *eB = a.expB + a.expOff[p];
+    double *outBase = a.raw + (size_t)a.rawIdx[p] * (M + 1) * kGradRaw;
+    for (int i = 0; i <= M; i++) {
+        const bool hasBelow = i < M, hasAbove = i > 0;
+        const RowParams P = row_params(a, i);
+        const double a0sC = P.a0 * sC, a1s1 = P.a1 * s1, a2s2 = P.a2 * s2, b0sC = P.b0 * sC;
+        const double d1s2 = P.d1 * s2, d2s1 = P.d2 * s1, enterC = P.enter * sC;
+        const float *Er = a.emis + a.prof->emOff + (int64_t)i * kEmStride;
+        // exponent rows: forward strips of rows i and i-1, backward strips of rows i and i+1
+        const int *eFh = eF + (int64_t)(i >> 6) * NB, *eFu = eF + (int64_t)((hasAbove ? i - 1 : i) >> 6) * NB;
+        const int *eBh = eB + (int64_t)((M - i) >> 6) * NB;
+        const int *eBd = eB + (int64_t)((M - (hasBelow ? i + 1 : i)) >> 6) * NB;
+        const double *w0r = w0 + (size_t)i * n, *Z0r = Z0 + (size_t)i * n, *W1r = W1 + (size_t)i * n;
+        const double *Zbr = Zb + (size_t)i * n;
+        const double *Yu = Yf + (size_t)(hasAbove ? i - 1 : i) * n;
+        const double *W1d = W1 + (size_t)(hasBelow ? i + 1 : i) * n, *Ybd = Yb + (size_t)(hasBelow ? i + 1 : i) * n;
+        bins[lane] = 0.0;
+        if (lane == 0) bins[64] = 0.0;
+        __syncthreads();
+        double cnt[kGradParams] = {0.0};
+        for (int c = 0; c < NB; c++) {
+            const int j = 64 * c + lane;
+            if (j < n) {
+                const int f = eFh[c] - ZE, fu = eFu[c] - ZE;
+                // backward exponent of column x (x >= j): its block from the right
+                auto ebh = [&](int x) { return eBh[(n - 1 - x) >> 6]; };
+                auto ebd = [&](int x) { return eBd[(n - 1 - x) >> 6]; };
+                const int code = sCodes[j];
+                const double w = w0r[j];
+                const double e = Er[code], g = sBg[code];
+                const bool r1 = j + 1 < n, r2 = j + 2 < n, r3 = j + 3 < n;
+                const double wd1 = (hasBelow && r1) ? W1d[j + 1] : 0.0;
+                const double wd2 = (hasBelow && r2) ? W1d[j + 2] : 0.0;
+                const double wd3 = (hasBelow && r3) ? W1d[j + 3] : 0.0;
+                const double ybd = hasBelow ? Ybd[j] : 0.0;
+                const double wh1 = r1 ? W1r[j + 1] : 0.0, wh2 = r2 ? W1r[j + 2] : 0.0;
+                const double zb3 = r3 ? Zbr[j + 3] : 0.0;
+                const double yup = hasAbove ? Yu[j] : 0.0;
+                const int x1 = r1 ? j + 1 : j, x2 = r2 ? j + 2 : j, x3 = r3 ? j + 3 : j;
+                const double fm = ldexp(w * enterC * e * wd3, f + ebd(x3));
+                cnt[10] += fm;
+                atomicAdd(&bins[sCodon[j]], fm);
+                cnt[7] += ldexp(w * d1s2 * wd2, f + ebd(x2));
+                cnt[8] += ldexp(w * d2s1 * wd1, f + ebd(x1));
+                const int e0 = ebd(j);
+                cnt[6] += ldexp(w * P.d0 * ybd, f + e0);
+                cnt[9] += ldexp(yup * P.eps * ybd, fu + e0);
+                cnt[1] += ldexp(w * a1s1 * wh1, f + ebh(x1));
+                cnt[2] += ldexp(w * a2s2 * wh2, f + ebh(x2));
+                const int e3 = ebh(x3);
+                cnt[0] += ldexp(w * a0sC * g * zb3, f + e3);
+                cnt[3] += ldexp(Z0r[j] * b0sC * g * zb3, f + e3);
+            }
+        }
+        double *out = outBase + (size_t)i * kGradRaw;
+#pragma unroll
+        for (int k = 0; k < kGradParams; k++) {
+            const double v = wave_sum(cnt[k]);
+            if (lane == 0) out[k] = v;
+        }
+        __syncthreads();
+        out[kGradParams + lane] = bins[lane];
+        if (lane == 0) out[kGradParams + 64] = bins[64];
+        __syncthreads();
+    }
 }
 
 // Per pair: ln Z and the factor weight / (Z's mantissa).
@@ -526,9 +557,9 @@ hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s) {
+hipError_t launch_grad_counts(const GradArgs &a, int npairs, int maxN, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_counts_kernel, dim3(npairs, (M + 1 + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(grad_counts_kernel, dim3(npairs), dim3(64), 2 * (size_t)maxN + 16, s, a);
     return hipGetLastError();
 }
 
