@@ -1637,7 +1637,7 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
         int64_t cells = 0, maxCells = 1, maxPairs = 1, maxBnd = 1, maxExp = 1;
         int64_t bnd = 0, nexp = 0;
         for (int32_t s = 0; s < nseq; s++) {
-            const int64_t c = gradCells(M, lengths[s]);
+            const int64_t c = gradSlots(M, lengths[s]);
             if (cells > 0 && (cells + c) * 6 * 8 > e->gradBudget) {
                 batch.push_back((size_t)s);
                 cells = 0;
@@ -1689,7 +1689,7 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
                 meta[np + k] = bo;
                 meta[2 * np + k] = eo;
                 rawIdx[k] = (int)C.order[k];
-                co += gradCells(M, n);
+                co += gradSlots(M, n);
                 bo += n;
                 eo += gradExps(M, n);
                 maxN = std::max(maxN, n);
@@ -1730,7 +1730,7 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
             HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
             HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
             HIPCHK(launch_grad_z(ga, np, e->stream));
-            HIPCHK(launch_grad_counts(ga, np, maxN, e->stream));
+            HIPCHK(launch_grad_counts(ga, np, M, e->stream));
             HIPCHK(hipEventRecord(e->ev1, e->stream));
             HIPCHK(hipEventSynchronize(e->ev1));
             float ms = 0.f;

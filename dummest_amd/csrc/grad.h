@@ -8,14 +8,19 @@
 
 namespace dm {
 
-// Per pair (profile row i = 0..M, DNA column j = 0..n-1) six FP64 matrices,
-// row-major with pitch n: the forward w0, its delete chain Y and insert chain
-// Z0, and the backward W1, Yb, Zb (dummest-dp-forward.hh:9-250,
-// dummest-dp-backward.hh:444-615). Values are stored scaled by a power of two
-// per 64-row strip and 64-column block (expF / expB: true value = stored x
-// 2^exponent), so any ln Z fits. Expected counts per row: the 11 parameters of
-// the row's Params (alpha'[3], beta'[3], delta'[3], eps', enter) and the
-// emission by codon (64 codons, 64 = a codon with a non-ACGT letter).
+// Per pair (profile row i = 0..M, DNA column j = 0..n-1) six FP64 matrices:
+// the forward w0, its delete chain Y and insert chain Z0, and the backward W1,
+// Yb, Zb (dummest-dp-forward.hh:9-250, dummest-dp-backward.hh:444-615). Both
+// passes run the rows in 64-row strips (strip s = rows 64s..64s+63), lane k of
+// a wavefront one row, one column behind its neighbour, and store each step's
+// 64 values contiguously: matrix[s][t][lane] with t = 0..n+62 (forward: row
+// 64s+lane, column t-lane; backward: row 64s+63-lane, column n-1-t+lane),
+// zeros where the cell lies outside the profile or the sequence. Values are
+// stored scaled by a power of two per strip and 64-column block (expF / expB:
+// true value = stored x 2^exponent), so any ln Z fits. Expected counts per
+// row: the 11 parameters of the row's Params (alpha'[3], beta'[3], delta'[3],
+// eps', enter) and the emission by codon (64 codons, 64 = a codon with a
+// non-ACGT letter).
 constexpr int kGradParams = 11;
 constexpr int kGradCodons = 65;
 constexpr int kGradRaw = kGradParams + kGradCodons;
@@ -24,6 +29,8 @@ constexpr int kGradMaxN = 4096;
 
 inline int64_t gradCells(int M, int n) { return (int64_t)(M + 1) * n; }
 inline int gradStrips(int M) { return (M + 64) / 64; }  // 64-row strips covering rows 0..M
+// stored values per matrix and pair: strips x steps x lanes
+inline int64_t gradSlots(int M, int n) { return (int64_t)gradStrips(M) * (n + 63) * 64; }
 inline int gradBlocks(int n) { return (n + 63) / 64; }  // 64-column blocks
 // exponents of one pair: forward and backward each strips x blocks
 inline int64_t gradExps(int M, int n) { return (int64_t)gradStrips(M) * gradBlocks(n); }
@@ -42,14 +49,13 @@ struct GradArgs {
     // per pair (n each, at the pair's bndOff): strip hand-over of the edge row
     // (forward: inflow and Y; backward: W1, Yb and the column sum of W1) and
     // the exponent each column was stored at; colSum: the column sums of W1
-    // over all rows (at the top strip's backward exponents)
+    // over all rows (at strip 0's backward exponents)
     double *bndA, *bndB, *bndC, *colSum;
     int *bndE;
     const int64_t *bndOff;
-    // per pair (strips x blocks at expOff): forward strips from row 0 down
-    // (strip s = rows 64s..64s+63, block c = columns 64c..64c+63), backward
-    // strips from row M up (strip s = rows M-64s down to M-64s-63, block c =
-    // columns n-1-64c down to n-64-64c)
+    // per pair (strips x blocks at expOff), strip s = rows 64s..64s+63 in both
+    // passes; forward block c = columns 64c..64c+63, backward block c =
+    // columns n-1-64c down to n-64-64c
     int *expF, *expB;
     const int64_t *expOff;
     double *zInfo;      // per slot: Z = zInfo[0] x 2^zInfo[1]
@@ -69,7 +75,7 @@ struct GradReduceArgs {
 hipError_t launch_grad_forward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s);
-hipError_t launch_grad_counts(const GradArgs &a, int npairs, int maxN, hipStream_t s);
+hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s);
 // factor: npairs doubles of scratch (weight / Z mantissa per pair)
 hipError_t launch_grad_reduce(const GradReduceArgs &a, double *factor, hipStream_t s);
 

@@ -111,11 +111,12 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
         sCode[j] = (uint8_t)((r.codes >> 8) & 255);  // code(j-2)
     }
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
+    const size_t SP = (size_t)(n + 63) * 64;  // stored values per strip
     double *w0 = a.w0 + a.cellOff[p], *Yf = a.Y + a.cellOff[p], *Z0 = a.Z0 + a.cellOff[p];
     double *hA = a.bndA + a.bndOff[p], *hB = a.bndB + a.bndOff[p];
     int *hE = a.bndE + a.bndOff[p];
     int *expF = a.expF + a.expOff[p];
-    for (int base = 0, strip = 0; base <= M; base += 64, strip++) {
+    for (int base = 0, strip = 0; base <= M; base += 64, strip++, w0 += SP, Yf += SP, Z0 += SP) {
         const int i = base + lane;
         const bool rowOk = i <= M;
         const RowParams P = row_params(a, rowOk ? i : 0);
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
             const double eA = lane_read(cA, t & 63), eB = lane_read(cB, t & 63);
             const int j = t - lane;
             const int El = (t & 63) >= lane ? Ecur : Eprev;  // the exponent of column j's block
-            double oC = lastC, oY = lastY;
+            double oC = lastC, oY = lastY, sw = 0.0, sy = 0.0, sz = 0.0;
             if (rowOk && j >= 0 && j < n) {
                 const double W0in = lane == 0 ? eA : inC, Yup = lane == 0 ? eB : inY;
                 const int code = sCode[j];
@@ -173,10 +174,9 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
                 const double y = P.d0 * w + P.eps * Yup;
                 oC = X + y + d1s2 * w2v + d2s1 * w1v;  // row i+1's inflow at column j
                 oY = y;
-                const size_t at = (size_t)i * n + j;
-                w0[at] = w;
-                Yf[at] = y;
-                Z0[at] = z0;
+                sw = w;
+                sy = y;
+                sz = z0;
                 if (lane == 63) {
                     hA[j] = oC;
                     hB[j] = oY;
@@ -191,6 +191,10 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
             }
             lastC = oC;
             lastY = oY;
+            const size_t at = (size_t)t * 64 + lane;  // one contiguous run per step
+            w0[at] = sw;
+            Yf[at] = sy;
+            Z0[at] = sz;
         };
         // at step 64c: the edge chunk of block c, then block c's exponent (no
         // block starts at or past column n: the lanes stay at the last one)
@@ -250,14 +254,19 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
         sCode[j] = (uint8_t)(r.codes & 255);  // code(j+1)
     }
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
-    double *W1 = a.W1 + a.cellOff[p], *Yb = a.Yb + a.cellOff[p], *Zb = a.Zb + a.cellOff[p];
+    const int S = (M + 64) / 64;
+    const size_t SP = (size_t)(n + 63) * 64;  // stored values per strip
     double *hA = a.bndA + a.bndOff[p], *hB = a.bndB + a.bndOff[p], *hC = a.bndC + a.bndOff[p];
     double *cs = a.colSum + a.bndOff[p];
     int *hE = a.bndE + a.bndOff[p];
     int *expB = a.expB + a.expOff[p];
-    for (int top = M, strip = 0; top >= 0; top -= 64, strip++) {
+    // strip s = rows 64s..64s+63 from the bottom strip up; lane 0 runs the strip's last row
+    for (int strip = S - 1; strip >= 0; strip--) {
+        const int top = 64 * strip + 63;
+        double *W1 = a.W1 + a.cellOff[p] + strip * SP, *Yb = a.Yb + a.cellOff[p] + strip * SP;
+        double *Zb = a.Zb + a.cellOff[p] + strip * SP;
         const int i = top - lane;
-        const bool rowOk = i >= 0;
+        const bool rowOk = i <= M;
         const RowParams P = row_params(a, rowOk ? i : 0);
         const double a0sC = P.a0 * sC, a1s1 = P.a1 * s1, a2s2 = P.a2 * s2, b0sC = P.b0 * sC;
         const double d1s2 = P.d1 * s2, d2s1 = P.d2 * s1, enterC = P.enter * sC;
@@ -265,9 +274,9 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
         stage_emissions(sE, a, M, lane, [&](int rr) { return top - rr; });
         __syncthreads();
         const double *Er = sE + lane * kNCode;
-        // row top+1 (the previous strip's last row): W1, Yb, the column sum and
+        // row top+1 (the strip below's first row): W1, Yb, the column sum and
         // their exponent at column j = n-1-t for step t, in chunks of 64 columns
-        const bool edge = top < M;
+        const bool edge = strip < S - 1;
         auto chunkA = [&](int c) { return edge ? hA[max(n - 1 - 64 * c - lane, 0)] : 0.0; };
         auto chunkB = [&](int c) { return edge ? hB[max(n - 1 - 64 * c - lane, 0)] : 0.0; };
         auto chunkC = [&](int c) { return edge ? hC[max(n - 1 - 64 * c - lane, 0)] : 0.0; };
@@ -310,7 +319,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
             const double eA = lane_read(cA, t & 63), eB = lane_read(cB, t & 63), eC = lane_read(cC, t & 63);
             const int j = n - 1 - t + lane;
             const int El = (t & 63) >= lane ? Ecur : Eprev;
-            double oW = lastW, oY = lastY, oS = lastS;
+            double oW = lastW, oY = lastY, oS = lastS, sw = 0.0, sy = 0.0, sz = 0.0;
             if (rowOk && j >= 0 && j < n) {
                 const double Wdn = lane == 0 ? eA : inW, Ynext = lane == 0 ? eB : inY, Sdn = lane == 0 ? eC : inS;
                 const int code = sCode[j];
@@ -325,10 +334,9 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 const double y = __fma_rn(P.eps, Ynext, w);
                 const double zb = __fma_rn(b0sC, zb3 * g, w);
                 const double sum = Sdn + w;
-                const size_t at = (size_t)i * n + j;
-                W1[at] = w;
-                Yb[at] = y;
-                Zb[at] = zb;
+                sw = w;
+                sy = y;
+                sz = zb;
                 if (lane == 63) {
                     hA[j] = w;
                     hB[j] = y;
@@ -351,6 +359,10 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
             lastW = oW;
             lastY = oY;
             lastS = oS;
+            const size_t at = (size_t)t * 64 + lane;  // one contiguous run per step
+            W1[at] = sw;
+            Yb[at] = sy;
+            Zb[at] = sz;
         };
         auto boundary = [&](int t) {
             if (t > 0) {
@@ -398,7 +410,7 @@ __global__ void grad_z_kernel(GradArgs a, int npairs) {
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
     const double *cs = a.colSum + a.bndOff[p];
-    const int *eb = a.expB + a.expOff[p] + (int64_t)(M / 64) * NB;  // the strip holding row 0
+    const int *eb = a.expB + a.expOff[p];  // strip 0 holds row 0
     const Rec *rec = a.rec + PR.recOff;
     double zm = 0.0;
     int ze = kNoExp;
@@ -418,101 +430,106 @@ __global__ void grad_z_kernel(GradArgs a, int npairs) {
     zi[1] = (double)ze;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {  // the same total in every lane
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-// Per pair (one wavefront), row by row: lane l takes columns 64c+l of each
-// 64-column block c, so every matrix read is a contiguous 512-byte run. The
-// flows of the row's parameters are summed per lane and then across the wave;
-// the match flows go into the row's 65 codon bins in LDS. Each flow is its
-// stored product times 2^(forward exponent + backward exponent - Z's exponent).
+// Per (pair, strip) one wavefront: lane k takes row 64s+k and walks the
+// forward's steps t = 0..n+62 (column j = t-k), so each read is one contiguous
+// run of the diagonal-major matrices: the forward's cell is at step t, and the
+// backward's cell of row 64s+k or 64s+k+1 at column j+d is at step n+62-t-d or
+// n+61-t-d, the same in every lane (the strip's edge lanes read the
+// neighbouring strip). Cells outside the sequence (the first and last 63
+// steps of a lane) contribute zero. Flows of the row's parameters in registers, its match
+// flows by codon in LDS; each flow is its stored product times
+// 2^(forward exponent + backward exponent - Z's exponent).
 __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
 #pragma clang fp contract(off)
-    __shared__ double bins[kGradCodons];
-    __shared__ double sBg[32];
-    extern __shared__ uint8_t sCodes[];  // code(j+1), then the codon index at j+1, per column
-    const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
+    __shared__ double bins[kGradCodons][64];
+    __shared__ int sExp[4][kGradMaxN / 64];  // forward strips s-1, s; backward strips s, s+1
+    const int p = a.pairBase + blockIdx.x, s = blockIdx.y, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
-    const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
+    const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6, S = (M + 64) / 64;
+    const int i = 64 * s + lane;
+    for (int c = 0; c < kGradCodons; c++) bins[c][lane] = 0.0;
+    {
+        const int *eF = a.expF + a.expOff[p], *eB = a.expB + a.expOff[p];
+        for (int c = lane; c < NB; c += 64) {
+            sExp[0][c] = s > 0 ? eF[(int64_t)(s - 1) * NB + c] : 0;
+            sExp[1][c] = eF[(int64_t)s * NB + c];
+            sExp[2][c] = eB[(int64_t)s * NB + c];
+            sExp[3][c] = s + 1 < S ? eB[(int64_t)(s + 1) * NB + c] : 0;
+        }
+    }
+    __syncthreads();
+    const bool rowOk = i <= M, hasBelow = i < M, hasAbove = i > 0;
     const double s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
-    uint8_t *sCodon = sCodes + n;
+    const RowParams P = row_params(a, rowOk ? i : 0);
+    const double a0sC = P.a0 * sC, a1s1 = P.a1 * s1, a2s2 = P.a2 * s2, b0sC = P.b0 * sC;
+    const double d1s2 = P.d1 * s2, d2s1 = P.d2 * s1, enterC = P.enter * sC;
+    const float *Er = a.emis + a.prof->emOff + (int64_t)(rowOk ? i : 0) * kEmStride;
     const Rec *rec = a.rec + PR.recOff;
     const uint8_t *codon = a.codon + PR.seqOff;
-    for (int j = lane; j < n; j += 64) {
-        sCodes[j] = (uint8_t)(rec[j].codes & 255);
-        sCodon[j] = j + 1 < n ? codon[j + 1] : (uint8_t)(kGradCodons - 1);
-    }
-    if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
+    const int64_t SP = (int64_t)(n + 63) * 64;
     const size_t off = a.cellOff[p];
     const double *w0 = a.w0 + off, *Z0 = a.Z0 + off, *Yf = a.Y + off;
     const double *W1 = a.W1 + off, *Yb = a.Yb + off, *Zb = a.Zb + off;
+    // offsets of column j = 0 (the forward's grow by 64 per column, the
+    // backward's fall by 64); a missing neighbour row reads row i and is zeroed
+    const int64_t fHere = s * SP + (int64_t)lane * 65;
+    const int64_t fAbove = !hasAbove ? fHere : lane > 0 ? fHere - 65 : (s - 1) * SP + 63 * 64 + 63;
+    const int64_t bHere = s * SP + (int64_t)(n - 1) * 64 + 63 * 65 - (int64_t)lane * 65;
+    const int64_t bBelow = !hasBelow ? bHere : lane < 63 ? bHere - 65 : (s + 1) * SP + (int64_t)(n - 1) * 64 + 63 * 64 + 63;
+    const int *eFh = sExp[1], *eFu = lane > 0 ? sExp[1] : sExp[0];
+    const int *eBh = sExp[2], *eBd = lane < 63 ? sExp[2] : sExp[3];
+    const double up = hasAbove ? 1.0 : 0.0, dn = hasBelow ? 1.0 : 0.0;
     const int ZE = (int)a.zInfo[2 * (size_t)a.rawIdx[p] + 1];
-    const int *eF = a.expF + a.expOff[p], *eB = a.expB + a.expOff[p];
-    double *outBase = a.raw + (size_t)a.rawIdx[p] * (M + 1) * kGradRaw;
-    for (int i = 0; i <= M; i++) {
-        const bool hasBelow = i < M, hasAbove = i > 0;
-        const RowParams P = row_params(a, i);
-        const double a0sC = P.a0 * sC, a1s1 = P.a1 * s1, a2s2 = P.a2 * s2, b0sC = P.b0 * sC;
-        const double d1s2 = P.d1 * s2, d2s1 = P.d2 * s1, enterC = P.enter * sC;
-        const float *Er = a.emis + a.prof->emOff + (int64_t)i * kEmStride;
-        // exponent rows: forward strips of rows i and i-1, backward strips of rows i and i+1
-        const int *eFh = eF + (int64_t)(i >> 6) * NB, *eFu = eF + (int64_t)((hasAbove ? i - 1 : i) >> 6) * NB;
-        const int *eBh = eB + (int64_t)((M - i) >> 6) * NB;
-        const int *eBd = eB + (int64_t)((M - (hasBelow ? i + 1 : i)) >> 6) * NB;
-        const double *w0r = w0 + (size_t)i * n, *Z0r = Z0 + (size_t)i * n, *W1r = W1 + (size_t)i * n;
-        const double *Zbr = Zb + (size_t)i * n;
-        const double *Yu = Yf + (size_t)(hasAbove ? i - 1 : i) * n;
-        const double *W1d = W1 + (size_t)(hasBelow ? i + 1 : i) * n, *Ybd = Yb + (size_t)(hasBelow ? i + 1 : i) * n;
-        bins[lane] = 0.0;
-        if (lane == 0) bins[64] = 0.0;
-        __syncthreads();
-        double cnt[kGradParams] = {0.0};
-        for (int c = 0; c < NB; c++) {
-            const int j = 64 * c + lane;
-            if (j < n) {
-                const int f = eFh[c] - ZE, fu = eFu[c] - ZE;
-                // backward exponent of column x (x >= j): its block from the right
-                auto ebh = [&](int x) { return eBh[(n - 1 - x) >> 6]; };
-                auto ebd = [&](int x) { return eBd[(n - 1 - x) >> 6]; };
-                const int code = sCodes[j];
-                const double w = w0r[j];
-                const double e = Er[code], g = sBg[code];
-                const bool r1 = j + 1 < n, r2 = j + 2 < n, r3 = j + 3 < n;
-                const double wd1 = (hasBelow && r1) ? W1d[j + 1] : 0.0;
-                const double wd2 = (hasBelow && r2) ? W1d[j + 2] : 0.0;
-                const double wd3 = (hasBelow && r3) ? W1d[j + 3] : 0.0;
-                const double ybd = hasBelow ? Ybd[j] : 0.0;
-                const double wh1 = r1 ? W1r[j + 1] : 0.0, wh2 = r2 ? W1r[j + 2] : 0.0;
-                const double zb3 = r3 ? Zbr[j + 3] : 0.0;
-                const double yup = hasAbove ? Yu[j] : 0.0;
-                const int x1 = r1 ? j + 1 : j, x2 = r2 ? j + 2 : j, x3 = r3 ? j + 3 : j;
-                const double fm = ldexp(w * enterC * e * wd3, f + ebd(x3));
-                cnt[10] += fm;
-                atomicAdd(&bins[sCodon[j]], fm);
-                cnt[7] += ldexp(w * d1s2 * wd2, f + ebd(x2));
-                cnt[8] += ldexp(w * d2s1 * wd1, f + ebd(x1));
-                const int e0 = ebd(j);
-                cnt[6] += ldexp(w * P.d0 * ybd, f + e0);
-                cnt[9] += ldexp(yup * P.eps * ybd, fu + e0);
-                cnt[1] += ldexp(w * a1s1 * wh1, f + ebh(x1));
-                cnt[2] += ldexp(w * a2s2 * wh2, f + ebh(x2));
-                const int e3 = ebh(x3);
-                cnt[0] += ldexp(w * a0sC * g * zb3, f + e3);
-                cnt[3] += ldexp(Z0r[j] * b0sC * g * zb3, f + e3);
-            }
+    double cnt[kGradParams] = {0.0};
+    if (rowOk) {
+        // step t: lane k at column t - k, so every lane's reads of one matrix
+        // form one contiguous run (cells outside the sequence read a clamped
+        // column and count zero)
+        auto step = [&](int t) {
+            const int jr = t - lane;
+            const double v = (jr >= 0 && jr < n) ? 1.0 : 0.0;
+            const int j = min(max(jr, 0), n - 1);
+            const int64_t fb = (int64_t)j * 64;
+            const int r1 = min(j + 1, n - 1), r2 = min(j + 2, n - 1), r3 = min(j + 3, n - 1);
+            const double m1 = j + 1 < n ? v : 0.0, m2 = j + 2 < n ? v : 0.0, m3 = j + 3 < n ? v : 0.0;
+            const int code = (int)(rec[j].codes & 255);  // code(j+1)
+            const int cod = codon[r1];
+            const double w = v * w0[fHere + fb], z0 = Z0[fHere + fb], yup = up * Yf[fAbove + fb];
+            const double wd1 = dn * m1 * W1[bBelow - (int64_t)r1 * 64];
+            const double wd2 = dn * m2 * W1[bBelow - (int64_t)r2 * 64];
+            const double wd3 = dn * m3 * W1[bBelow - (int64_t)r3 * 64];
+            const double ybd = dn * Yb[bBelow - fb];
+            const double wh1 = m1 * W1[bHere - (int64_t)r1 * 64], wh2 = m2 * W1[bHere - (int64_t)r2 * 64];
+            const double zb3 = m3 * Zb[bHere - (int64_t)r3 * 64];
+            const double e = Er[code], g = a.bg[code];
+            const int f = eFh[j >> 6] - ZE, fu = eFu[j >> 6] - ZE;
+            auto bx = [&](int x) { return (n - 1 - x) >> 6; };
+            const double fm = ldexp(w * enterC * e * wd3, f + eBd[bx(r3)]);
+            cnt[10] += fm;
+            bins[j + 1 < n ? cod : kGradCodons - 1][lane] += fm;
+            cnt[7] += ldexp(w * d1s2 * wd2, f + eBd[bx(r2)]);
+            cnt[8] += ldexp(w * d2s1 * wd1, f + eBd[bx(r1)]);
+            const int e0 = eBd[bx(j)];
+            cnt[6] += ldexp(w * P.d0 * ybd, f + e0);
+            cnt[9] += ldexp(v * yup * P.eps * ybd, fu + e0);
+            cnt[1] += ldexp(w * a1s1 * wh1, f + eBh[bx(r1)]);
+            cnt[2] += ldexp(w * a2s2 * wh2, f + eBh[bx(r2)]);
+            const int e3 = eBh[bx(r3)];
+            cnt[0] += ldexp(w * a0sC * g * zb3, f + e3);
+            cnt[3] += ldexp(v * z0 * b0sC * g * zb3, f + e3);
+        };
+        const int T = n + 63;
+        int t = 0;
+        for (; t + 4 <= T; t += 4) {
+            step(t);
+            step(t + 1);
+            step(t + 2);
+            step(t + 3);
         }
-        double *out = outBase + (size_t)i * kGradRaw;
-#pragma unroll
-        for (int k = 0; k < kGradParams; k++) {
-            const double v = wave_sum(cnt[k]);
-            if (lane == 0) out[k] = v;
-        }
-        __syncthreads();
-        out[kGradParams + lane] = bins[lane];
-        if (lane == 0) out[kGradParams + 64] = bins[64];
-        __syncthreads();
+        for (; t < T; t++) step(t);
+        double *out = a.raw + ((size_t)a.rawIdx[p] * (M + 1) + i) * kGradRaw;
+        for (int k = 0; k < kGradParams; k++) out[k] = cnt[k];
+        for (int c = 0; c < kGradCodons; c++) out[kGradParams + c] = bins[c][lane];
     }
 }
 
@@ -557,9 +574,9 @@ hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_grad_counts(const GradArgs &a, int npairs, int maxN, hipStream_t s) {
+hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_counts_kernel, dim3(npairs), dim3(64), 2 * (size_t)maxN + 16, s, a);
+    hipLaunchKernelGGL(grad_counts_kernel, dim3(npairs, (M + 64) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
