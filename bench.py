@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_CELL = 43          # fwd+bwd: backward 20 + forward 21 + anchor 2 (algorithmic)
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector with packed math / matrix)
 HBM_PEAK_GBS = 8000.0
+GRAD_BYTES_PER_CELL = 104    # gradient: 2 x 24 B of stored FP64 matrices written, 56 B read back
 
 
 def log(msg):
@@ -256,10 +257,15 @@ def main():
         F, ge, gt, lnz = L.evaluate(g3["emis_logits"], g3["trans_logits"])
         wall = time.perf_counter() - t
         gcells, gms = e.last_grad_cells, e.last_grad_ms
+        # stored-matrix traffic per cell: forward and backward each write three
+        # doubles, the count pass reads seven
+        gbytes = GRAD_BYTES_PER_CELL * gcells
         gradient = {"workload": f"d sum ln Z / d logits: one profile of {g3['M']} positions x "
                                 f"{len(g3['seqs'])} sequences of 1-3 kb (half frameshifted homologs)",
                     "cells": int(gcells), "device_ms": round(gms, 2), "wall_s": round(wall, 3),
                     "cells_per_s": float(f"{gcells / (gms * 1e-3):.4g}"), "dtype": "f64",
+                    "matrix_bytes_per_cell": GRAD_BYTES_PER_CELL,
+                    "matrix_GB_per_s": round(gbytes / (gms * 1e-3) / 1e9, 1), "hbm_peak_GB_per_s": HBM_PEAK_GBS,
                     "batch_budget_gib": args.grad_budget, "max_ln_Z": round(float(lnz.max()), 3),
                     "finite": bool(np.isfinite(lnz).all() and np.isfinite(ge).all() and np.isfinite(gt).all())}
         log(f"gradient: {gcells:.4g} cells, {gms:.1f} ms on the device, {wall:.2f} s in all, "
