@@ -1730,7 +1730,7 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
             HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
             HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
             HIPCHK(launch_grad_z(ga, np, e->stream));
-            HIPCHK(launch_grad_counts(ga, np, M, e->stream));
+            HIPCHK(launch_grad_counts(ga, np, M, maxN, e->stream));
             HIPCHK(hipEventRecord(e->ev1, e->stream));
             HIPCHK(hipEventSynchronize(e->ev1));
             float ms = 0.f;

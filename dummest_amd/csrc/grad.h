@@ -24,8 +24,9 @@ namespace dm {
 constexpr int kGradParams = 11;
 constexpr int kGradCodons = 65;
 constexpr int kGradRaw = kGradParams + kGradCodons;
-// the pair's per-column codes and seeds sit in LDS: sequences up to kGradMaxN
-constexpr int kGradMaxN = 4096;
+// longest sequence: its six matrices (48 B per cell and a 64-row strip) must
+// fit the matrix budget of one launch batch
+constexpr int kGradMaxN = 1 << 20;
 
 inline int64_t gradCells(int M, int n) { return (int64_t)(M + 1) * n; }
 inline int gradStrips(int M) { return (M + 64) / 64; }  // 64-row strips covering rows 0..M
@@ -75,7 +76,7 @@ struct GradReduceArgs {
 hipError_t launch_grad_forward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStream_t s);
 hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s);
-hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s);
+hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, int maxN, hipStream_t s);
 // factor: npairs doubles of scratch (weight / Z mantissa per pair)
 hipError_t launch_grad_reduce(const GradReduceArgs &a, double *factor, hipStream_t s);
 

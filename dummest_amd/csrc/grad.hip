@@ -66,6 +66,8 @@ __device__ __forceinline__ RowParams row_params(const GradArgs &a, int row) {
     return RowParams{pr[0], pr[1], pr[2], pr[3], pr[6], pr[7], pr[8], pr[9], pr[10]};
 }
 
+constexpr int kExpLds = 256;      // 64-column blocks whose exponents the count pass stages in LDS
+constexpr int kRing = 256;        // columns of seeds and codes a pass keeps in LDS (the lanes of a step span 64)
 constexpr int kNoExp = -1000000;  // binary exponent given to zero
 constexpr int kRaise = 64;        // a block whose largest value passes 2^kRaise gets a higher exponent
 constexpr int kLower = 256;       // one whose values all stay below 2^-kLower gets a lower one (not below 0)
@@ -98,18 +100,15 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
     __shared__ double sBg[32];
-    extern __shared__ float sCol[];  // prefixSeed per column, then code(j-2) per column (bytes)
+    // prefixSeed and code(j-2) of column j at slot j mod kRing: the lanes of a
+    // step span 64 columns, and every 64 steps the next 64 are written
+    __shared__ float sCol[kRing];
+    __shared__ uint8_t sCode[kRing];
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
     const double s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
-    uint8_t *sCode = reinterpret_cast<uint8_t *>(sCol + n);
     const Rec *rec = a.rec + PR.recOff;
-    for (int j = lane; j < n; j += 64) {
-        const Rec r = rec[j];
-        sCol[j] = r.ps;
-        sCode[j] = (uint8_t)((r.codes >> 8) & 255);  // code(j-2)
-    }
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
     const size_t SP = (size_t)(n + 63) * 64;  // stored values per strip
     double *w0 = a.w0 + a.cellOff[p], *Yf = a.Y + a.cellOff[p], *Z0 = a.Z0 + a.cellOff[p];
@@ -166,11 +165,11 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
             double oC = lastC, oY = lastY, sw = 0.0, sy = 0.0, sz = 0.0;
             if (rowOk && j >= 0 && j < n) {
                 const double W0in = lane == 0 ? eA : inC, Yup = lane == 0 ? eB : inY;
-                const int code = sCode[j];
+                const int code = sCode[j & (kRing - 1)];
                 const double e = Er[code], g = sBg[code];
                 const double X = enterC * e * w3v;
                 const double z0 = g * sC * (P.a0 * w3v + P.b0 * z3);
-                const double w = W0in + (z0 + a1s1 * w1v + a2s2 * w2v + ldexp((double)sCol[j], -El));
+                const double w = W0in + (z0 + a1s1 * w1v + a2s2 * w2v + ldexp((double)sCol[j & (kRing - 1)], -El));
                 const double y = P.d0 * w + P.eps * Yup;
                 oC = X + y + d1s2 * w2v + d2s1 * w1v;  // row i+1's inflow at column j
                 oY = y;
@@ -198,7 +197,14 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
         };
         // at step 64c: the edge chunk of block c, then block c's exponent (no
         // block starts at or past column n: the lanes stay at the last one)
+        Rec cur = rec[min(lane, n - 1)];  // the records of the next block to enter the ring
         auto boundary = [&](int t) {
+            {
+                const int jw = t + lane;  // block t/64's columns (past n: unused slots)
+                sCol[jw & (kRing - 1)] = cur.ps;
+                sCode[jw & (kRing - 1)] = (uint8_t)((cur.codes >> 8) & 255);  // code(j-2)
+                cur = rec[min(t + 64 + lane, n - 1)];
+            }
             if (t > 0) {
                 cA = nA;
                 cB = nB;
@@ -213,6 +219,7 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
                 Eprev = Ecur;
                 dConv = 0;
             }
+            __syncthreads();  // the ring's new block is visible to every lane
         };
         // one outer iteration per 64 steps: the inner loop issues no loads, so
         // its stores are never waited for
@@ -241,18 +248,13 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
     __shared__ double sBg[32];
-    extern __shared__ float sCol[];  // suffixSeed per column, then code(j+1) per column
+    __shared__ float sCol[kRing];  // suffixSeed and code(j+1) of column j at j mod kRing
+    __shared__ uint8_t sCode[kRing];
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
     const double s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
-    uint8_t *sCode = reinterpret_cast<uint8_t *>(sCol + n);
     const Rec *rec = a.rec + PR.recOff;
-    for (int j = lane; j < n; j += 64) {
-        const Rec r = rec[j];
-        sCol[j] = r.ss;
-        sCode[j] = (uint8_t)(r.codes & 255);  // code(j+1)
-    }
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
     const int S = (M + 64) / 64;
     const size_t SP = (size_t)(n + 63) * 64;  // stored values per strip
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
             double oW = lastW, oY = lastY, oS = lastS, sw = 0.0, sy = 0.0, sz = 0.0;
             if (rowOk && j >= 0 && j < n) {
                 const double Wdn = lane == 0 ? eA : inW, Ynext = lane == 0 ? eB : inY, Sdn = lane == 0 ? eC : inS;
-                const int code = sCode[j];
+                const int code = sCode[j & (kRing - 1)];
                 const double e = Er[code], g = sBg[code];
                 double w = r3 * e * enterC + Ynext * P.d0;
                 w = w + r2 * d1s2;
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 w = w + zb3 * g * a0sC;
                 w = w + c2 * a2s2;
                 w = w + c1 * a1s1;
-                w = w + ldexp((double)sCol[j], -El);
+                w = w + ldexp((double)sCol[j & (kRing - 1)], -El);
                 const double y = __fma_rn(P.eps, Ynext, w);
                 const double zb = __fma_rn(b0sC, zb3 * g, w);
                 const double sum = Sdn + w;
@@ -364,7 +366,14 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
             Yb[at] = sy;
             Zb[at] = sz;
         };
+        Rec cur = rec[max(n - 1 - lane, 0)];  // the records of the next block to enter the ring
         auto boundary = [&](int t) {
+            {
+                const int jw = n - 1 - t - lane;  // block t/64's columns, right to left (below 0: unused)
+                sCol[jw & (kRing - 1)] = cur.ss;
+                sCode[jw & (kRing - 1)] = (uint8_t)(cur.codes & 255);  // code(j+1)
+                cur = rec[max(n - 1 - t - 64 - lane, 0)];
+            }
             if (t > 0) {
                 cA = nA;
                 cB = nB;
@@ -381,6 +390,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 Eprev = Ecur;
                 dConv = 0;
             }
+            __syncthreads();  // the ring's new block is visible to every lane
         };
         // one outer iteration per 64 steps: the inner loop issues no loads, so
         // its stores are never waited for
@@ -439,22 +449,29 @@ __global__ void grad_z_kernel(GradArgs a, int npairs) {
 // steps of a lane) contribute zero. Flows of the row's parameters in registers, its match
 // flows by codon in LDS; each flow is its stored product times
 // 2^(forward exponent + backward exponent - Z's exponent).
+template <bool kLdsExp>
 __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double bins[kGradCodons][64];
-    __shared__ int sExp[4][kGradMaxN / 64];  // forward strips s-1, s; backward strips s, s+1
+    // exponent rows of forward strips s-1, s and backward strips s, s+1: in LDS
+    // when every sequence of the launch has at most 64 kExpLds letters (kLdsExp),
+    // read in place otherwise
+    __shared__ int sExp[4][kLdsExp ? kExpLds : 1];
     const int p = a.pairBase + blockIdx.x, s = blockIdx.y, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6, S = (M + 64) / 64;
     const int i = 64 * s + lane;
     for (int c = 0; c < kGradCodons; c++) bins[c][lane] = 0.0;
+    const int *expRow[4];
     {
         const int *eF = a.expF + a.expOff[p], *eB = a.expB + a.expOff[p];
-        for (int c = lane; c < NB; c += 64) {
-            sExp[0][c] = s > 0 ? eF[(int64_t)(s - 1) * NB + c] : 0;
-            sExp[1][c] = eF[(int64_t)s * NB + c];
-            sExp[2][c] = eB[(int64_t)s * NB + c];
-            sExp[3][c] = s + 1 < S ? eB[(int64_t)(s + 1) * NB + c] : 0;
+        expRow[0] = eF + (int64_t)(s > 0 ? s - 1 : s) * NB;  // read only for rows below 64s
+        expRow[1] = eF + (int64_t)s * NB;
+        expRow[2] = eB + (int64_t)s * NB;
+        expRow[3] = eB + (int64_t)(s + 1 < S ? s + 1 : s) * NB;  // read only for rows above 64s+63
+        if constexpr (kLdsExp) {
+            for (int r = 0; r < 4; r++)
+                for (int c = lane; c < NB; c += 64) sExp[r][c] = expRow[r][c];
         }
     }
     __syncthreads();
@@ -476,8 +493,10 @@ __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
     const int64_t fAbove = !hasAbove ? fHere : lane > 0 ? fHere - 65 : (s - 1) * SP + 63 * 64 + 63;
     const int64_t bHere = s * SP + (int64_t)(n - 1) * 64 + 63 * 65 - (int64_t)lane * 65;
     const int64_t bBelow = !hasBelow ? bHere : lane < 63 ? bHere - 65 : (s + 1) * SP + (int64_t)(n - 1) * 64 + 63 * 64 + 63;
-    const int *eFh = sExp[1], *eFu = lane > 0 ? sExp[1] : sExp[0];
-    const int *eBh = sExp[2], *eBd = lane < 63 ? sExp[2] : sExp[3];
+    const int *eFh = kLdsExp ? sExp[1] : expRow[1], *eFu = kLdsExp ? (lane > 0 ? sExp[1] : sExp[0])
+                                                                   : (lane > 0 ? expRow[1] : expRow[0]);
+    const int *eBh = kLdsExp ? sExp[2] : expRow[2], *eBd = kLdsExp ? (lane < 63 ? sExp[2] : sExp[3])
+                                                                   : (lane < 63 ? expRow[2] : expRow[3]);
     const double up = hasAbove ? 1.0 : 0.0, dn = hasBelow ? 1.0 : 0.0;
     const int ZE = (int)a.zInfo[2 * (size_t)a.rawIdx[p] + 1];
     double cnt[kGradParams] = {0.0};
@@ -554,17 +573,18 @@ __global__ void grad_reduce_kernel(GradReduceArgs a, const double *factor) {
 
 }  // namespace
 
-static size_t col_lds(int maxN) { return (size_t)maxN * sizeof(float) + (size_t)maxN + 16; }
 
 hipError_t launch_grad_forward(const GradArgs &a, int npairs, int maxN, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_forward_kernel, dim3(npairs), dim3(64), col_lds(maxN), s, a);
+    (void)maxN;
+    hipLaunchKernelGGL(grad_forward_kernel, dim3(npairs), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_backward_kernel, dim3(npairs), dim3(64), col_lds(maxN), s, a);
+    (void)maxN;
+    hipLaunchKernelGGL(grad_backward_kernel, dim3(npairs), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -574,9 +594,12 @@ hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, hipStream_t s) {
+hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, int maxN, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_counts_kernel, dim3(npairs, (M + 64) / 64), dim3(64), 0, s, a);
+    if (maxN <= 64 * kExpLds)
+        hipLaunchKernelGGL(grad_counts_kernel<true>, dim3(npairs, (M + 64) / 64), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(grad_counts_kernel<false>, dim3(npairs, (M + 64) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

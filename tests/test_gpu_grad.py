@@ -258,3 +258,49 @@ def test_strong_homologs_beyond_fp64_range(engine):
         worst = max(worst, abs((fp - fm) / (2 * h) - g))
     print("max |GPU - FD| / max |grad| =", worst / scale)
     assert worst <= 1e-4 * scale
+
+
+def test_long_sequences(engine):
+    # sequences far past one LDS window: 20,000 letters with two frameshifted
+    # homologs (313 column blocks: the count pass reads its exponents in place)
+    # and 9,000 letters with one; ln Z vs the oracle's FP64 total, counts vs the
+    # flows through the oracle's matrices
+    rng = np.random.default_rng(20)
+    M = 40
+    el, tl, bg, _ = make_case(rng, M, [10])
+    p = G._softmax(el)
+    seqs = []
+    for n, copies in ((20000, 2), (9000, 1)):
+        host = synth.random_dna(rng, n)
+        for c in range(copies):
+            prot = np.array([rng.choice(20, p=p[r]) for r in range(M)])
+            hom = synth.mutate_dna(rng, synth.reverse_translate(rng, prot), indel=0.01)
+            at = (c + 1) * n // (copies + 1)
+            host = host[:at] + hom + host[at + len(hom):]
+        seqs.append(host)
+    print("long sequences:", [len(s) for s in seqs], flush=True)
+    L = G.SummedLogLikelihood(engine, seqs, bg)
+    F, ge, gt, lnz = L.evaluate(el, tl)
+    want = oracle_lnz(el, tl, bg, seqs)
+    print("ln Z", lnz, "oracle", want)
+    assert np.abs(lnz - want).max() < 2e-7 * max(len(s) for s in seqs) + 1e-5
+    v, pa = G.profile_from_logits(el, tl, bg)
+    v, pa, b = (np.ascontiguousarray(x.astype(np.float32)) for x in (v, pa, bg))
+    pid = engine.add_profile(M, v, pa, b)
+    prof = O.Profile("l", M, v, pa, b, [])
+    for s in seqs:
+        _, got = engine.grad_counts(pid, [s])
+        assert np.allclose(got[:, 11:].sum(1), got[:, 10], rtol=1e-12, atol=1e-300)
+        want_c, by_codon, _ = OF.matrix_flows(prof, s)
+        for k, w in want_c.items():
+            err = np.abs(got[:, OF.PARAM_INDEX[k]] - w).max()
+            print(f"  n {len(s)} {k}: max |GPU - oracle| = {err:.3g} of max {w.max():.3g}")
+            assert err <= 1e-4 * max(w.max(), 1e-6), k
+        assert np.abs(got[:, 11:] - by_codon).max() <= 1e-4 * max(by_codon.max(), 1e-6)
+
+
+def test_longest_sequence_limit(engine):
+    v, pa = G.profile_from_logits(np.zeros((5, 20)), np.zeros((6, 7)), np.full(29, 0.05))
+    pid = engine.add_profile(5, v, pa, np.full(29, 0.05))
+    with pytest.raises(D.DummestError, match="letters"):
+        engine.grad_counts(pid, ["A" * ((1 << 20) + 1)])
