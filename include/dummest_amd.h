@@ -243,7 +243,7 @@ double dm_last_bw_ms(const dm_engine *e);
  * sequences, on the GPU (FP64 forward, backward and expected counts over the
  * reference's recurrences, dummest-dp-forward.hh:9-250 and
  * dummest-dp-backward.hh:444-615, scaled by a power of two per 64-row strip
- * and 64-column block, so any ln Z fits; sequences of 3..4096 letters). Per
+ * and 64-column block, so any ln Z fits; sequences of 3..2^20 letters). Per
  * sequence s the total of
  * all local alignments Z_s = sum_{i,j} prefixSeed(j) W1(i,j) (ln_z[s]; may be
  * NULL), and counts = sum_s weights[s] (NULL = 1) times the expected uses of
