@@ -411,20 +411,24 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
     }
 }
 
-// Per pair: Z = sum_j prefixSeed(j) colSum(j) as a mantissa and a binary
-// exponent (zInfo), the columns added left to right.
-__global__ void grad_z_kernel(GradArgs a, int npairs) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= npairs) return;
-    const int p = a.pairBase + k;
+__device__ __forceinline__ double wave_sum(double v) {  // the same total in every lane
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Per pair (one wavefront): Z = sum_j prefixSeed(j) colSum(j) as a mantissa
+// and a binary exponent (zInfo); lane l sums columns l, l+64, ... at its own
+// exponent, then the lanes meet at the largest.
+__global__ __launch_bounds__(64) void grad_z_kernel(GradArgs a) {
+    const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
-    const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
+    const int n = PR.n;
     const double *cs = a.colSum + a.bndOff[p];
     const int *eb = a.expB + a.expOff[p];  // strip 0 holds row 0
     const Rec *rec = a.rec + PR.recOff;
     double zm = 0.0;
     int ze = kNoExp;
-    for (int j = 0; j < n; j++) {
+    for (int j = lane; j < n; j += 64) {
         const double v = (double)rec[j].ps * cs[j];
         if (!(v > 0.0)) continue;
         const int e = eb[(n - 1 - j) >> 6];
@@ -435,9 +439,13 @@ __global__ void grad_z_kernel(GradArgs a, int npairs) {
             zm += ldexp(v, e - ze);
         }
     }
-    double *zi = a.zInfo + 2 * (size_t)a.rawIdx[p];
-    zi[0] = zm;
-    zi[1] = (double)ze;
+    const int zmax = wave_max(ze);
+    const double tot = wave_sum(ldexp(zm, ze - zmax));
+    if (lane == 0) {
+        double *zi = a.zInfo + 2 * (size_t)a.rawIdx[p];
+        zi[0] = tot;
+        zi[1] = (double)zmax;
+    }
 }
 
 // Per (pair, strip) one wavefront: lane k takes row 64s+k and walks the
@@ -590,7 +598,7 @@ hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStre
 
 hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_z_kernel, dim3((npairs + 63) / 64), dim3(64), 0, s, a, npairs);
+    hipLaunchKernelGGL(grad_z_kernel, dim3(npairs), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

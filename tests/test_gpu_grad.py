@@ -304,3 +304,35 @@ def test_longest_sequence_limit(engine):
     pid = engine.add_profile(5, v, pa, np.full(29, 0.05))
     with pytest.raises(D.DummestError, match="letters"):
         engine.grad_counts(pid, ["A" * ((1 << 20) + 1)])
+
+
+def test_unusual_letters_and_short_sequences(engine):
+    # lowercase letters are bases; N, R, Y make the codon '?' (code 22) and its
+    # match flow goes to the non-ACGT codon bin; sequences of 3-5 letters; an
+    # empty list gives zero counts; fewer than 3 letters is refused
+    rng = np.random.default_rng(33)
+    M = 12
+    el, tl, bg, seqs = make_case(rng, M, [90, 70])
+    s0 = seqs[0]
+    s0 = s0[:10].lower() + "N" + s0[11:40] + "RY" + s0[42:60].lower() + "NNN" + s0[63:]
+    seqs = [s0, seqs[1], "ACG", "TGGC", "atgaa"]
+    v, pa = G.profile_from_logits(el, tl, bg)
+    v, pa, b = (np.ascontiguousarray(x.astype(np.float32)) for x in (v, pa, bg))
+    pid = engine.add_profile(M, v, pa, b)
+    prof = O.Profile("u", M, v, pa, b, [])
+    for s in seqs:
+        print("letters case, n", len(s), flush=True)
+        lnz, got = engine.grad_counts(pid, [s])
+        want, by_codon, lnz_o = OF.matrix_flows(prof, s)
+        assert abs(lnz[0] - lnz_o) < 1e-4
+        for k, w in want.items():
+            assert np.abs(got[:, OF.PARAM_INDEX[k]] - w).max() <= 1e-4 * max(w.max(), 1e-6), (s, k)
+        assert np.abs(got[:, 11:] - by_codon).max() <= 1e-4 * max(by_codon.max(), 1e-6)
+        assert np.allclose(got[:, 11:].sum(1), got[:, 10], rtol=1e-12, atol=1e-300)
+    # the non-ACGT codon bin is used by the first sequence
+    _, got = engine.grad_counts(pid, [s0])
+    assert got[:, 11 + 64].sum() > 0
+    lnz, got = engine.grad_counts(pid, [])
+    assert len(lnz) == 0 and not got.any()
+    with pytest.raises(D.DummestError, match="at least 3"):
+        engine.grad_counts(pid, ["AC"])
