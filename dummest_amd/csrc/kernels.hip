@@ -685,8 +685,11 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             Rec rcur = brecAt(0, k);
             // strip kernels: the strip below's top row (W1, Y, exponent), also ahead
             Bnd gq4[kPF];
+            // (loaded in every strip so that the load is not conditional, which
+            // would make the compiler wait for all memory operations each step;
+            // used only when there is a strip below)
             auto ghostAt = [&](int tt) -> Bnd {
-                if (STR && below) return bIn[bidx(Tmine - 1 - tt - k, n)];
+                if (STR) return bIn[bidx(Tmine - 1 - tt - k, n)];
                 return Bnd{0.f, 0.f, 0};
             };
 #pragma unroll
@@ -716,13 +719,17 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 for (int q = 0; q < R; q++) eqN[q] = etk[(nx.codes & 255) * ROWS + q];
                 const float gqN = bgl[nx.codes & 255];
                 float gw = 0.f, gy = 0.f;
+                Bnd gcur{0.f, 0.f, 0};
+                if (STR) {
+                    gcur = gq4[PH % kPF];
+                    gq4[PH % kPF] = ghostAt(t + kPF);
+                }
                 if (STR && below) {
                     // last lane: the strip below's top row from the boundary buffer.
                     // When that row sits far above this strip's scale, raise the
                     // exponent first (before any neighbour value is read).
-                    const float rw = gq4[PH % kPF].v, ry = gq4[PH % kPF].y;
-                    const int ge = gq4[PH % kPF].e;
-                    gq4[PH % kPF] = ghostAt(t + kPF);
+                    const float rw = gcur.v, ry = gcur.y;
+                    const int ge = gcur.e;
                     if (MODE != kFilter) {
                         int need = 0;
                         if (k == G - 1 && (rw > 0.f || ry > 0.f)) {
@@ -932,8 +939,9 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
 
             // strip kernels: the strip above's last row (carried sum, Y, exponent), also ahead
             Bnd fq4[kPF];
+            // (loaded in every strip, used only when there is a strip above)
             auto fghostAt = [&](int tt) -> Bnd {
-                if (STR && above) return bF[bidx(tt - k, n)];
+                if (STR) return bF[bidx(tt - k, n)];
                 return Bnd{0.f, 0.f, 0};
             };
 #pragma unroll
@@ -967,12 +975,16 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 const int EbNext = (t < Tmine) ? Eq[SL] : kNoStep;
                 loadW1(t + kPF, Wq[SL], Eq[SL]);
                 float gn = 0.f, gy = 0.f;
+                Bnd fcurB{0.f, 0.f, 0};
+                if (STR) {
+                    fcurB = fq4[SL];
+                    fq4[SL] = fghostAt(t + kPF);
+                }
                 if (STR && above) {
                     // lane 0: the strip above's last row from the boundary buffer;
                     // raise this strip's exponent first when that row is far above it
-                    const float rn = fq4[SL].v, ry = fq4[SL].y;
-                    const int ge = fq4[SL].e;
-                    fq4[SL] = fghostAt(t + kPF);
+                    const float rn = fcurB.v, ry = fcurB.y;
+                    const int ge = fcurB.e;
                     int need = 0;
                     if (k == 0 && (rn > 0.f || ry > 0.f)) {
                         int ex;
