@@ -97,8 +97,8 @@ def main():
                     help="reads masked with tantan on the device in the sequence-masking figure (0: skip)")
     ap.add_argument("--grad-seqs", type=int, default=4096,
                     help="sequences in the summed-likelihood gradient figure (M = 300 profile; 0: skip)")
-    ap.add_argument("--grad-budget", type=int, default=64,
-                    help="GiB of FP64 matrices per gradient launch batch")
+    ap.add_argument("--grad-budget", type=int, default=160,
+                    help="GiB of FP64 matrices per gradient launch batch (160: the whole workload in one)")
     ap.add_argument("--tune-file", default="",
                     help="load the autotuned cost table from this file if it exists, else tune and save it there")
     ap.add_argument("--copy-probe", type=int, default=0,
