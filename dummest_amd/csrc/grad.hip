@@ -22,11 +22,13 @@
 // Layout: one wavefront per pair for forward and backward, profile rows 64 at
 // a time (lane k = one row, one column behind the row it depends on), the
 // values from that row arriving by DPP; the first row of a strip reads the
-// previous strip's last row from a hand-over buffer in 64-column chunks. The
-// pair's per-column codes and seeds sit in LDS with the strip's emission rows.
-// Every pass stores its matrices in full (row-major, pitch n); the count pass
-// gives each row a lane walking its row, with the codon-indexed emission sums
-// in LDS.
+// neighbouring strip's edge row from a hand-over buffer in 64-column chunks.
+// The strip's emission rows sit in LDS, the pair's per-column codes and seeds
+// in a 256-column LDS ring. Every pass stores its matrices in full,
+// diagonal-major (each step's 64 values contiguous). Z comes from the column
+// sums of W1; the count pass gives each (pair, strip) a wavefront whose lanes
+// walk the forward's anti-diagonals, so every read is a contiguous run, with
+// the codon-indexed emission sums in LDS.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
         const bool rowOk = i <= M;
         const RowParams P = row_params(a, rowOk ? i : 0);
         const double a1s1 = P.a1 * s1, a2s2 = P.a2 * s2, d1s2 = P.d1 * s2, d2s1 = P.d2 * s1, enterC = P.enter * sC;
-        __syncthreads();  // columns staged; the previous strip is done with sE
+        __syncthreads();  // the previous strip is done with sE and the column ring
         stage_emissions(sE, a, M, lane, [&](int rr) { return base + rr; });
         __syncthreads();
         const double *Er = sE + lane * kNCode;
@@ -240,9 +242,10 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
     }
 }
 
-// Backward: W1, Yb, Zb of every cell, strip by strip from row M up; lane k of
-// a strip runs row top-k at column n-1-t+k in step t. Blocks of 64 columns
-// from the right (expB) as in the forward. The column sums of W1 run down
+// Backward: W1, Yb, Zb of every cell, strip by strip from the strip holding
+// row M up (strip s = rows 64s..64s+63, as in the forward); lane k of a strip
+// runs row 64s+63-k at column n-1-t+k in step t. Blocks of 64 columns from the
+// right (expB) as in the forward. The column sums of W1 run down
 // the lanes with Yb and reach row 0 as sum_i W1(i, j) (colSum).
 __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
