@@ -114,6 +114,9 @@ def _declare(L):
     L.dm_last_grad_ms.restype = d
     L.dm_last_grad_cells.argtypes = [vp]
     L.dm_last_grad_cells.restype = i64
+    L.dm_grad_set_create.argtypes = [vp, i32, ctypes.c_char_p, _P(i64), _P(i32), _P(i32)]
+    L.dm_grad_set.argtypes = [vp, i32, i32, _P(d), _P(d), _P(d)]
+    L.dm_grad_set_destroy.argtypes = [vp, i32]
     L.dm_xxh3_128.argtypes = [ctypes.c_char_p, i64, _P(ctypes.c_uint64)]
     L.dm_calibration_key.argtypes = [ctypes.c_char_p, i32, _P(f), ctypes.c_char_p, i32, i32, i32, _P(d), d,
                                      ctypes.c_char_p]
@@ -492,6 +495,31 @@ class Engine:
                                   None if w is None else _ptr(w, ctypes.c_double), _ptr(lnz, ctypes.c_double),
                                   _ptr(counts, ctypes.c_double)), "dm_grad")
         return lnz[:len(seqs)], counts
+
+    def grad_set(self, seqs) -> int:
+        """Register DNA sequences for repeated gradient evaluations (see
+        dm_grad_set_create); returns the set id."""
+        data, offs, lens = self._dna_batch(seqs)
+        out = ctypes.c_int32()
+        self._check(lib().dm_grad_set_create(self._h, len(seqs), data, _ptr(offs, ctypes.c_int64),
+                                             _ptr(lens, ctypes.c_int32), ctypes.byref(out)), "dm_grad_set_create")
+        self._grad_set_sizes = getattr(self, "_grad_set_sizes", {})
+        self._grad_set_sizes[out.value] = len(seqs)
+        return out.value
+
+    def grad_set_counts(self, prof: int, set_id: int, weights=None):
+        """grad_counts over a registered set (bit-identical results)."""
+        n = self._grad_set_sizes[set_id]
+        M = self.profile_length(prof)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        lnz = np.zeros(max(1, n), np.float64)
+        counts = np.zeros((M + 1, GRAD_WIDTH), np.float64)
+        self._check(lib().dm_grad_set(self._h, prof, set_id, None if w is None else _ptr(w, ctypes.c_double),
+                                      _ptr(lnz, ctypes.c_double), _ptr(counts, ctypes.c_double)), "dm_grad_set")
+        return lnz[:n], counts
+
+    def grad_set_destroy(self, set_id: int):
+        self._check(lib().dm_grad_set_destroy(self._h, set_id), "dm_grad_set_destroy")
 
     @property
     def last_grad_ms(self) -> float:

@@ -97,6 +97,14 @@ struct dm_engine {
     // gradient of the summed log-likelihood (dm_grad)
     int64_t gradBudget = (int64_t)8 << 30;  // bytes of FP64 matrices per launch batch
     DevBuf dGradCodon, dGradMat, dGradBnd, dGradMeta, dGradRaw, dGradW, dGradOut, dGradExp;
+    // registered sequence sets (dm_grad_set_create): consecutive sequences of
+    // the store and their codon indices on the device
+    struct GradSet {
+        int32_t s0 = -1, n = 0;
+        size_t codes0 = 0;
+        DevBuf dCodon;
+    };
+    std::vector<GradSet> gradSets;
     double lastGradMs = 0;
     int64_t lastGradCells = 0;
 
@@ -799,6 +807,7 @@ void dm_engine_destroy(dm_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     e->dBnd.release();
+    for (auto &g : e->gradSets) g.dCodon.release();
     for (DevBuf *b : {&e->dProfs, &e->dParams, &e->dEmis, &e->dBg, &e->dCodes, &e->dPairs, &e->dRec,
                       &e->dNull, &e->dUnits, &e->dCounter, &e->dWbuf, &e->dW0buf, &e->dEbuf, &e->dFbuf,
                       &e->dOut, &e->dTanBases, &e->dTanProbs, &e->dTanScales, &e->dTanMeta, &e->dTanB2f,
@@ -1565,33 +1574,16 @@ double dm_last_grad_ms(const dm_engine *e) { return e ? e->lastGradMs : 0.0; }
 int64_t dm_last_grad_cells(const dm_engine *e) { return e ? e->lastGradCells : 0; }
 
 // ------------------------------------- gradient of the summed log-likelihood
-int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
-            const double *weights, double *lnZ, double *counts) {
-    if (!e || nseq < 0 || !counts || (nseq > 0 && (!dna || !offsets || !lengths))) return DM_EINVAL;
-    if (prof < 0 || prof >= (int)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
-    for (int32_t s = 0; s < nseq; s++) {
-        if (offsets[s] < 0 || lengths[s] < 3)
-            return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
-        if (lengths[s] > kGradMaxN)
-            return fail(e, DM_EUNSUP, "the gradient takes sequences of up to " + std::to_string(kGradMaxN) + " letters");
-    }
-    HIPCHK(hipSetDevice(e->device));
-    int rc = syncProfiles(e);
-    if (rc) return rc;
-    const int M = e->profiles[prof].M;
-    const int WR = kGradRaw;
-    if (nseq == 0) {
-        std::fill(counts, counts + (size_t)(M + 1) * WR, 0.0);
-        return DM_OK;
-    }
-    // The sequences go into the engine's store for the duration of the call:
-    // decoded codes for the null model and the DP, codon indices for the
-    // emission counts (16 b0 + 4 b1 + b2 over ACGT, 64 otherwise).
-    const size_t codes0 = e->codes.size(), seqs0 = e->seqLen.size();
+// Appends DNA sequences to the engine's store (decoded codes for the null model
+// and the DP) and returns their codon indices for the emission counts (16 b0 +
+// 4 b1 + b2 over ACGT, 64 otherwise), one byte per letter.
+static void appendGradSequences(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offsets,
+                                const int32_t *lengths, std::vector<uint8_t> &codon) {
+    const size_t codes0 = e->codes.size();
     int64_t total = 0;
     for (int32_t s = 0; s < nseq; s++) total += lengths[s];
     e->codes.resize(codes0 + (size_t)total);
-    std::vector<uint8_t> codon((size_t)total);
+    codon.assign((size_t)total, 64);
     auto base4 = [](char c) -> int {
         switch (c) {
         case 'A': case 'a': return 0;
@@ -1601,163 +1593,245 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
         default: return -1;
         }
     };
-    std::vector<int32_t> pp(nseq, prof), ss(nseq);
     int64_t at = 0;
     for (int32_t s = 0; s < nseq; s++) {
         const char *d = dna + offsets[s];
         const int n = lengths[s];
         decodeDna(d, n, e->codes.data() + codes0 + at);
-        for (int j = 0; j < n; j++) {
-            int c = 64;
-            if (j + 2 < n) {
-                const int x = base4(d[j]), y = base4(d[j + 1]), z = base4(d[j + 2]);
-                if (x >= 0 && y >= 0 && z >= 0) c = 16 * x + 4 * y + z;
-            }
-            codon[at + j] = (uint8_t)c;
+        for (int j = 0; j + 2 < n; j++) {
+            const int x = base4(d[j]), y = base4(d[j + 1]), z = base4(d[j + 2]);
+            if (x >= 0 && y >= 0 && z >= 0) codon[at + j] = (uint8_t)(16 * x + 4 * y + z);
         }
         e->seqOff.push_back((int64_t)(codes0 + at));
         e->seqLen.push_back(n);
-        ss[s] = (int32_t)(seqs0 + s);
         at += n;
     }
     e->seqDirty = true;
-    auto restore = [&]() {
-        e->codes.resize(codes0);
-        e->seqOff.resize(seqs0);
-        e->seqLen.resize(seqs0);
-        e->seqDirty = true;
-    };
-    auto run = [&]() -> int {
-        int r = syncSequences(e);
-        if (r) return r;
-        HIPCHK(e->dGradCodon.ensure((size_t)total));
-        HIPCHK(hipMemcpyAsync(e->dGradCodon.p, codon.data(), (size_t)total, hipMemcpyHostToDevice, e->stream));
-        // batches whose six matrices fit the budget
-        std::vector<size_t> batch{0};
-        int64_t cells = 0, maxCells = 1, maxPairs = 1, maxBnd = 1, maxExp = 1;
-        int64_t bnd = 0, nexp = 0;
-        for (int32_t s = 0; s < nseq; s++) {
-            const int64_t c = gradSlots(M, lengths[s]);
-            if (cells > 0 && (cells + c) * 6 * 8 > e->gradBudget) {
-                batch.push_back((size_t)s);
-                cells = 0;
-                bnd = 0;
-                nexp = 0;
-            }
-            cells += c;
-            bnd += lengths[s];
-            nexp += gradExps(M, lengths[s]);
-            maxCells = std::max(maxCells, cells);
-            maxBnd = std::max(maxBnd, bnd);
-            maxExp = std::max(maxExp, nexp);
-            maxPairs = std::max<int64_t>(maxPairs, (int64_t)s + 1 - (int64_t)batch.back());
+}
+
+static int checkGradLengths(dm_engine *e, int32_t nseq, const int64_t *offsets, const int32_t *lengths) {
+    for (int32_t s = 0; s < nseq; s++) {
+        if (offsets[s] < 0 || lengths[s] < 3)
+            return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
+        if (lengths[s] > kGradMaxN)
+            return fail(e, DM_EUNSUP, "the gradient takes sequences of up to " + std::to_string(kGradMaxN) + " letters");
+    }
+    return DM_OK;
+}
+
+// The gradient over the nseq consecutive sequences of the store from id s0
+// (codes contiguous from codes0), their codon indices on the device at dCodon.
+static int gradRun(dm_engine *e, int32_t prof, int32_t nseq, int32_t s0, size_t codes0, const uint8_t *dCodon,
+                   const double *weights, double *lnZ, double *counts) {
+    const int M = e->profiles[prof].M;
+    const int WR = kGradRaw;
+    std::vector<int32_t> lengthsV(nseq), pp(nseq, prof), ss(nseq);
+    for (int32_t s = 0; s < nseq; s++) {
+        ss[s] = s0 + s;
+        lengthsV[s] = e->seqLen[(size_t)(s0 + s)];
+    }
+    const int32_t *lengths = lengthsV.data();
+    int r = syncSequences(e);
+    if (r) return r;
+    // batches whose six matrices fit the budget
+    std::vector<size_t> batch{0};
+    int64_t cells = 0, maxCells = 1, maxPairs = 1, maxBnd = 1, maxExp = 1;
+    int64_t bnd = 0, nexp = 0;
+    for (int32_t s = 0; s < nseq; s++) {
+        const int64_t c = gradSlots(M, lengths[s]);
+        if (cells > 0 && (cells + c) * 6 * 8 > e->gradBudget) {
+            batch.push_back((size_t)s);
+            cells = 0;
+            bnd = 0;
+            nexp = 0;
         }
-        batch.push_back((size_t)nseq);
-        HIPCHK(e->dGradMat.ensure(sizeof(double) * 6 * (size_t)maxCells));
-        // hand-over: three doubles and an exponent per column, and the column sums
-        HIPCHK(e->dGradBnd.ensure((sizeof(double) * 4 + sizeof(int)) * (size_t)maxBnd));
-        HIPCHK(e->dGradExp.ensure(sizeof(int) * 2 * (size_t)maxExp));
-        HIPCHK(e->dGradMeta.ensure((sizeof(int64_t) * 3 + sizeof(int)) * (size_t)maxPairs));
-        HIPCHK(e->dGradRaw.ensure(sizeof(double) * (size_t)nseq * (M + 1) * kGradRaw));
-        // per sequence: weight, ln Z, factor, Z as mantissa and exponent
-        HIPCHK(e->dGradW.ensure(sizeof(double) * 5 * (size_t)nseq));
-        HIPCHK(e->dGradOut.ensure(sizeof(double) * (size_t)(M + 1) * WR));
-        std::vector<double> w(nseq, 1.0);
-        if (weights) std::copy(weights, weights + nseq, w.begin());
-        HIPCHK(hipMemcpyAsync(e->dGradW.p, w.data(), sizeof(double) * nseq, hipMemcpyHostToDevice, e->stream));
-        std::vector<int64_t> sorted(nseq);
-        std::iota(sorted.begin(), sorted.end(), 0);
-        const std::vector<int> cfgOf(nseq, 0);  // only the pair records and the null model are used
-        double *mat = e->dGradMat.as<double>();
-        const int64_t C6 = maxCells;
-        if (!e->ev0) HIPCHK(hipEventCreate(&e->ev0));
-        if (!e->ev1) HIPCHK(hipEventCreate(&e->ev1));
-        float msAll = 0.f;
-        for (size_t b = 0; b + 1 < batch.size(); b++) {
-            Chunk C;
-            buildChunk(e, kScore, sorted, cfgOf, pp.data(), ss.data(), batch[b], batch[b + 1], C);
-            int r2 = uploadChunk(e, C);  // pair records and the null model
-            if (r2) return r2;
-            const int np = (int)C.pairs.size();
-            std::vector<int64_t> meta(3 * (size_t)np);  // cell, hand-over and exponent offsets
-            std::vector<int> rawIdx(np);
-            int64_t co = 0, bo = 0, eo = 0;
-            int maxN = 3;
-            for (int k = 0; k < np; k++) {
-                const int n = C.pairs[k].n;
-                meta[k] = co;
-                meta[np + k] = bo;
-                meta[2 * np + k] = eo;
-                rawIdx[k] = (int)C.order[k];
-                co += gradSlots(M, n);
-                bo += n;
-                eo += gradExps(M, n);
-                maxN = std::max(maxN, n);
-            }
-            int64_t *dm = e->dGradMeta.as<int64_t>();
-            int *dIdx = (int *)(dm + 3 * np);
-            HIPCHK(hipMemcpyAsync(dm, meta.data(), sizeof(int64_t) * 3 * np, hipMemcpyHostToDevice, e->stream));
-            HIPCHK(hipMemcpyAsync(dIdx, rawIdx.data(), sizeof(int) * np, hipMemcpyHostToDevice, e->stream));
-            GradArgs ga{};
-            ga.prof = e->dProfs.as<ProfInfo>() + prof;
-            ga.params = e->dParams.as<float>();
-            ga.emis = e->dEmis.as<float>();
-            ga.bg = e->dBg.as<float>() + (int64_t)prof * 32;
-            ga.pairs = e->dPairs.as<PairInfo>();
-            ga.rec = e->dRec.as<Rec>();
-            ga.codon = e->dGradCodon.as<uint8_t>() - codes0;  // indexed by the pairs' seqOff
-            ga.pairBase = 0;
-            ga.cellOff = dm;
-            ga.bndOff = dm + np;
-            ga.expOff = dm + 2 * np;
-            ga.rawIdx = dIdx;
-            ga.w0 = mat;
-            ga.Y = mat + C6;
-            ga.Z0 = mat + 2 * C6;
-            ga.W1 = mat + 3 * C6;
-            ga.Yb = mat + 4 * C6;
-            ga.Zb = mat + 5 * C6;
-            ga.bndA = e->dGradBnd.as<double>();
-            ga.bndB = ga.bndA + maxBnd;
-            ga.bndC = ga.bndA + 2 * maxBnd;
-            ga.colSum = ga.bndA + 3 * maxBnd;
-            ga.bndE = (int *)(ga.bndA + 4 * maxBnd);
-            ga.expF = e->dGradExp.as<int>();
-            ga.expB = ga.expF + maxExp;
-            ga.zInfo = e->dGradW.as<double>() + 3 * (size_t)nseq;
-            ga.raw = e->dGradRaw.as<double>();
-            HIPCHK(hipEventRecord(e->ev0, e->stream));
-            HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
-            HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
-            HIPCHK(launch_grad_z(ga, np, e->stream));
-            HIPCHK(launch_grad_counts(ga, np, M, maxN, e->stream));
-            HIPCHK(hipEventRecord(e->ev1, e->stream));
-            HIPCHK(hipEventSynchronize(e->ev1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-            msAll += ms;
+        cells += c;
+        bnd += lengths[s];
+        nexp += gradExps(M, lengths[s]);
+        maxCells = std::max(maxCells, cells);
+        maxBnd = std::max(maxBnd, bnd);
+        maxExp = std::max(maxExp, nexp);
+        maxPairs = std::max<int64_t>(maxPairs, (int64_t)s + 1 - (int64_t)batch.back());
+    }
+    batch.push_back((size_t)nseq);
+    HIPCHK(e->dGradMat.ensure(sizeof(double) * 6 * (size_t)maxCells));
+    // hand-over: three doubles and an exponent per column, and the column sums
+    HIPCHK(e->dGradBnd.ensure((sizeof(double) * 4 + sizeof(int)) * (size_t)maxBnd));
+    HIPCHK(e->dGradExp.ensure(sizeof(int) * 2 * (size_t)maxExp));
+    HIPCHK(e->dGradMeta.ensure((sizeof(int64_t) * 3 + sizeof(int)) * (size_t)maxPairs));
+    HIPCHK(e->dGradRaw.ensure(sizeof(double) * (size_t)nseq * (M + 1) * kGradRaw));
+    // per sequence: weight, ln Z, factor, Z as mantissa and exponent
+    HIPCHK(e->dGradW.ensure(sizeof(double) * 5 * (size_t)nseq));
+    HIPCHK(e->dGradOut.ensure(sizeof(double) * (size_t)(M + 1) * WR));
+    std::vector<double> w(nseq, 1.0);
+    if (weights) std::copy(weights, weights + nseq, w.begin());
+    HIPCHK(hipMemcpyAsync(e->dGradW.p, w.data(), sizeof(double) * nseq, hipMemcpyHostToDevice, e->stream));
+    std::vector<int64_t> sorted(nseq);
+    std::iota(sorted.begin(), sorted.end(), 0);
+    const std::vector<int> cfgOf(nseq, 0);  // only the pair records and the null model are used
+    double *mat = e->dGradMat.as<double>();
+    const int64_t C6 = maxCells;
+    if (!e->ev0) HIPCHK(hipEventCreate(&e->ev0));
+    if (!e->ev1) HIPCHK(hipEventCreate(&e->ev1));
+    float msAll = 0.f;
+    for (size_t b = 0; b + 1 < batch.size(); b++) {
+        Chunk C;
+        buildChunk(e, kScore, sorted, cfgOf, pp.data(), ss.data(), batch[b], batch[b + 1], C);
+        int r2 = uploadChunk(e, C);  // pair records and the null model
+        if (r2) return r2;
+        const int np = (int)C.pairs.size();
+        std::vector<int64_t> meta(3 * (size_t)np);  // cell, hand-over and exponent offsets
+        std::vector<int> rawIdx(np);
+        int64_t co = 0, bo = 0, eo = 0;
+        int maxN = 3;
+        for (int k = 0; k < np; k++) {
+            const int n = C.pairs[k].n;
+            meta[k] = co;
+            meta[np + k] = bo;
+            meta[2 * np + k] = eo;
+            rawIdx[k] = (int)C.order[k];
+            co += gradSlots(M, n);
+            bo += n;
+            eo += gradExps(M, n);
+            maxN = std::max(maxN, n);
         }
-        GradReduceArgs ra{};
-        ra.M = M;
-        ra.npairs = nseq;
-        ra.raw = e->dGradRaw.as<double>();
-        ra.zInfo = e->dGradW.as<double>() + 3 * (size_t)nseq;
-        ra.weight = e->dGradW.as<double>();
-        ra.lnZ = e->dGradW.as<double>() + nseq;
-        ra.counts = e->dGradOut.as<double>();
-        HIPCHK(launch_grad_reduce(ra, e->dGradW.as<double>() + 2 * (size_t)nseq, e->stream));
-        HIPCHK(hipMemcpyAsync(counts, ra.counts, sizeof(double) * (size_t)(M + 1) * WR, hipMemcpyDeviceToHost,
-                              e->stream));
-        if (lnZ) HIPCHK(hipMemcpyAsync(lnZ, ra.lnZ, sizeof(double) * nseq, hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
-        e->lastGradMs = msAll;
-        e->lastGradCells = 0;
-        for (int32_t s = 0; s < nseq; s++) e->lastGradCells += gradCells(M, lengths[s]);
+        int64_t *dm = e->dGradMeta.as<int64_t>();
+        int *dIdx = (int *)(dm + 3 * np);
+        HIPCHK(hipMemcpyAsync(dm, meta.data(), sizeof(int64_t) * 3 * np, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(dIdx, rawIdx.data(), sizeof(int) * np, hipMemcpyHostToDevice, e->stream));
+        GradArgs ga{};
+        ga.prof = e->dProfs.as<ProfInfo>() + prof;
+        ga.params = e->dParams.as<float>();
+        ga.emis = e->dEmis.as<float>();
+        ga.bg = e->dBg.as<float>() + (int64_t)prof * 32;
+        ga.pairs = e->dPairs.as<PairInfo>();
+        ga.rec = e->dRec.as<Rec>();
+        ga.codon = dCodon - codes0;  // indexed by the pairs' seqOff
+        ga.pairBase = 0;
+        ga.cellOff = dm;
+        ga.bndOff = dm + np;
+        ga.expOff = dm + 2 * np;
+        ga.rawIdx = dIdx;
+        ga.w0 = mat;
+        ga.Y = mat + C6;
+        ga.Z0 = mat + 2 * C6;
+        ga.W1 = mat + 3 * C6;
+        ga.Yb = mat + 4 * C6;
+        ga.Zb = mat + 5 * C6;
+        ga.bndA = e->dGradBnd.as<double>();
+        ga.bndB = ga.bndA + maxBnd;
+        ga.bndC = ga.bndA + 2 * maxBnd;
+        ga.colSum = ga.bndA + 3 * maxBnd;
+        ga.bndE = (int *)(ga.bndA + 4 * maxBnd);
+        ga.expF = e->dGradExp.as<int>();
+        ga.expB = ga.expF + maxExp;
+        ga.zInfo = e->dGradW.as<double>() + 3 * (size_t)nseq;
+        ga.raw = e->dGradRaw.as<double>();
+        HIPCHK(hipEventRecord(e->ev0, e->stream));
+        HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
+        HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
+        HIPCHK(launch_grad_z(ga, np, e->stream));
+        HIPCHK(launch_grad_counts(ga, np, M, maxN, e->stream));
+        HIPCHK(hipEventRecord(e->ev1, e->stream));
+        HIPCHK(hipEventSynchronize(e->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        msAll += ms;
+    }
+    GradReduceArgs ra{};
+    ra.M = M;
+    ra.npairs = nseq;
+    ra.raw = e->dGradRaw.as<double>();
+    ra.zInfo = e->dGradW.as<double>() + 3 * (size_t)nseq;
+    ra.weight = e->dGradW.as<double>();
+    ra.lnZ = e->dGradW.as<double>() + nseq;
+    ra.counts = e->dGradOut.as<double>();
+    HIPCHK(launch_grad_reduce(ra, e->dGradW.as<double>() + 2 * (size_t)nseq, e->stream));
+    HIPCHK(hipMemcpyAsync(counts, ra.counts, sizeof(double) * (size_t)(M + 1) * WR, hipMemcpyDeviceToHost,
+                          e->stream));
+    if (lnZ) HIPCHK(hipMemcpyAsync(lnZ, ra.lnZ, sizeof(double) * nseq, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->lastGradMs = msAll;
+    e->lastGradCells = 0;
+    for (int32_t s = 0; s < nseq; s++) e->lastGradCells += gradCells(M, lengths[s]);
+    return DM_OK;
+}
+
+int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+            const double *weights, double *lnZ, double *counts) {
+    if (!e || nseq < 0 || !counts || (nseq > 0 && (!dna || !offsets || !lengths))) return DM_EINVAL;
+    if (prof < 0 || prof >= (int)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
+    int rc = checkGradLengths(e, nseq, offsets, lengths);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->device));
+    rc = syncProfiles(e);
+    if (rc) return rc;
+    const int M = e->profiles[prof].M;
+    if (nseq == 0) {
+        std::fill(counts, counts + (size_t)(M + 1) * kGradRaw, 0.0);
         return DM_OK;
-    };
-    rc = run();
-    restore();
+    }
+    // the sequences stay in the engine's store for the duration of the call
+    const size_t codes0 = e->codes.size(), seqs0 = e->seqLen.size();
+    std::vector<uint8_t> codon;
+    appendGradSequences(e, nseq, dna, offsets, lengths, codon);
+    rc = [&]() -> int {
+        HIPCHK(e->dGradCodon.ensure(codon.size()));
+        HIPCHK(hipMemcpyAsync(e->dGradCodon.p, codon.data(), codon.size(), hipMemcpyHostToDevice, e->stream));
+        return gradRun(e, prof, nseq, (int32_t)seqs0, codes0, e->dGradCodon.as<uint8_t>(), weights, lnZ, counts);
+    }();
+    e->codes.resize(codes0);
+    e->seqOff.resize(seqs0);
+    e->seqLen.resize(seqs0);
+    e->seqDirty = true;
     return rc;
+}
+
+int dm_grad_set_create(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+                       int32_t *set) {
+    if (!e || nseq < 0 || !set || (nseq > 0 && (!dna || !offsets || !lengths))) return DM_EINVAL;
+    int rc = checkGradLengths(e, nseq, offsets, lengths);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->device));
+    dm_engine::GradSet g;
+    g.s0 = (int32_t)e->seqLen.size();
+    g.n = nseq;
+    g.codes0 = e->codes.size();
+    std::vector<uint8_t> codon;
+    appendGradSequences(e, nseq, dna, offsets, lengths, codon);
+    if (!codon.empty()) {
+        HIPCHK(g.dCodon.ensure(codon.size()));
+        HIPCHK(hipMemcpy(g.dCodon.p, codon.data(), codon.size(), hipMemcpyHostToDevice));
+    }
+    e->gradSets.push_back(g);
+    *set = (int32_t)e->gradSets.size() - 1;
+    return DM_OK;
+}
+
+int dm_grad_set(dm_engine *e, int32_t prof, int32_t set, const double *weights, double *lnZ, double *counts) {
+    if (!e || !counts) return DM_EINVAL;
+    if (prof < 0 || prof >= (int)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
+    if (set < 0 || set >= (int)e->gradSets.size() || e->gradSets[set].s0 < 0)
+        return fail(e, DM_EINVAL, "unknown sequence set");
+    HIPCHK(hipSetDevice(e->device));
+    int rc = syncProfiles(e);
+    if (rc) return rc;
+    const auto &g = e->gradSets[set];
+    if (g.n == 0) {
+        std::fill(counts, counts + (size_t)(e->profiles[prof].M + 1) * kGradRaw, 0.0);
+        return DM_OK;
+    }
+    return gradRun(e, prof, g.n, g.s0, g.codes0, g.dCodon.as<uint8_t>(), weights, lnZ, counts);
+}
+
+int dm_grad_set_destroy(dm_engine *e, int32_t set) {
+    if (!e) return DM_EINVAL;
+    if (set < 0 || set >= (int)e->gradSets.size() || e->gradSets[set].s0 < 0)
+        return fail(e, DM_EINVAL, "unknown sequence set");
+    HIPCHK(hipSetDevice(e->device));
+    e->gradSets[set].dCodon.release();
+    e->gradSets[set].s0 = -1;  // the sequences stay in the store (other ids keep their meaning)
+    return DM_OK;
 }
 
 // ------------------------------------------------ profile builder: Baum-Welch

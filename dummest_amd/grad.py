@@ -121,6 +121,7 @@ class SummedLogLikelihood:
         self.engine, self.seqs, self.bg = engine, list(seqs), np.asarray(bg, np.float64)
         self.rates, self.weights = tuple(rates), weights
         self.pid = None  # the engine's slot for the current profile, reused by every evaluation
+        self.set_id = None  # the engine's registered copy of the sequences
 
     def evaluate(self, emis_logits, trans_logits):
         """(F, d F / d emission logits, d F / d transition logits, per-sequence ln Z)."""
@@ -131,7 +132,9 @@ class SummedLogLikelihood:
             self.pid = self.engine.add_profile(M, values, params, self.bg)
         else:
             self.engine.set_profile(self.pid, M, values, params, self.bg)
-        lnz, counts = self.engine.grad_counts(self.pid, self.seqs, self.weights)
+        if self.set_id is None:  # decoded and uploaded once for every evaluation
+            self.set_id = self.engine.grad_set(self.seqs)
+        lnz, counts = self.engine.grad_set_counts(self.pid, self.set_id, self.weights)
         ge, gt, _ = logit_grads(counts, emis_logits, trans_logits, self.rates)
         w = np.ones(len(lnz)) if self.weights is None else np.asarray(self.weights, np.float64)
         return float((w * lnz).sum()), ge, gt, lnz

@@ -259,6 +259,17 @@ int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int
             const double *weights, double *ln_z, double *counts);
 double dm_last_grad_ms(const dm_engine *e);
 int64_t dm_last_grad_cells(const dm_engine *e);
+/* The same gradient over a registered sequence set, for callers that evaluate
+ * it many times over the same sequences (gradient descent on a profile):
+ * dm_grad_set_create decodes the sequences once, keeps them in the engine's
+ * store and their codon indices on the device, and writes the set's id;
+ * dm_grad_set computes what dm_grad computes for them (bit-identical results);
+ * dm_grad_set_destroy frees the set's device memory. No counterpart in the
+ * reference (its search never forms the total likelihood). */
+int dm_grad_set_create(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
+                       int32_t *set);
+int dm_grad_set(dm_engine *e, int32_t prof, int32_t set, const double *weights, double *ln_z, double *counts);
+int dm_grad_set_destroy(dm_engine *e, int32_t set);
 
 /* Autotuner: times every kernel configuration (G lanes per pair, R rows per
  * lane, and the row-strip kernel) on this device with synthetic pairs of

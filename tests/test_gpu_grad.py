@@ -336,3 +336,30 @@ def test_unusual_letters_and_short_sequences(engine):
     assert len(lnz) == 0 and not got.any()
     with pytest.raises(D.DummestError, match="at least 3"):
         engine.grad_counts(pid, ["AC"])
+
+
+def test_registered_set_is_bit_identical(engine):
+    # dm_grad_set over registered sequences gives dm_grad's numbers bit for
+    # bit, with weights, across profile updates, interleaved with dm_grad calls
+    rng = np.random.default_rng(44)
+    M = 30
+    el, tl, bg, seqs = make_case(rng, M, [120, 333, 64, 250])
+    v, pa = G.profile_from_logits(el, tl, bg)
+    pid = engine.add_profile(M, v, pa, bg)
+    sid = engine.grad_set(seqs)
+    lnz_a, c_a = engine.grad_counts(pid, seqs)
+    lnz_b, c_b = engine.grad_set_counts(pid, sid)
+    assert np.array_equal(lnz_a, lnz_b) and np.array_equal(c_a, c_b)
+    w = [1.0, 0.25, 3.0, 0.0]
+    assert np.array_equal(engine.grad_counts(pid, seqs, w)[1], engine.grad_set_counts(pid, sid, w)[1])
+    v2, pa2 = G.profile_from_logits(el * 0.5, tl, bg)
+    engine.set_profile(pid, M, v2, pa2, bg)
+    lnz_c, c_c = engine.grad_set_counts(pid, sid)
+    lnz_d, c_d = engine.grad_counts(pid, seqs)
+    assert np.array_equal(lnz_c, lnz_d) and np.array_equal(c_c, c_d) and not np.array_equal(c_c, c_a)
+    empty = engine.grad_set([])
+    lz, ce = engine.grad_set_counts(pid, empty)
+    assert len(lz) == 0 and not ce.any()
+    engine.grad_set_destroy(sid)
+    with pytest.raises(D.DummestError, match="unknown sequence set"):
+        engine.grad_set_counts(pid, sid)
