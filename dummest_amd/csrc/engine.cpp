@@ -97,6 +97,7 @@ struct dm_engine {
     // gradient of the summed log-likelihood (dm_grad)
     int64_t gradBudget = (int64_t)8 << 30;  // bytes of FP64 matrices per launch batch
     DevBuf dGradCodon, dGradMat, dGradBnd, dGradMeta, dGradRaw, dGradW, dGradOut, dGradExp;
+    std::vector<ProfInfo> profInfo;  // device offsets of each profile's tables (last full upload)
     // registered sequence sets (dm_grad_set_create): consecutive sequences of
     // the store and their codon indices on the device
     struct GradSet {
@@ -154,6 +155,7 @@ static int syncProfiles(dm_engine *e) {
         HIPCHK(hipMemcpy(e->dEmis.p, emis.data(), sizeof(float) * emis.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->dBg.p, bg.data(), sizeof(float) * bg.size(), hipMemcpyHostToDevice));
     }
+    e->profInfo = info;
     e->profDirty = false;
     return DM_OK;
 }
@@ -905,13 +907,29 @@ int dm_set_profile(dm_engine *e, int32_t id, int32_t M, const float *values, con
     if (!e || M < 2 || !values || !params || !bg) return DM_EINVAL;
     if (id < 0 || id >= (int)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
     HostProfile &p = e->profiles[id];
+    const bool inPlace = !e->profDirty && p.M == M && id < (int)e->profInfo.size();
     p.M = M;
     p.values.assign(values, values + (size_t)(M + 1) * kWidth);
     p.params.assign(params, params + (size_t)(M + 2) * 11);
     p.bg.assign(bg, bg + kWidth);
     p.consensus.assign(M, 0);
     p.masked = 0;
-    e->profDirty = true;
+    if (!inPlace) {
+        e->profDirty = true;
+        return DM_OK;
+    }
+    // same length, tables in sync: rewrite this profile's slices only (a
+    // gradient loop updates one profile per evaluation among many)
+    HIPCHK(hipSetDevice(e->device));
+    const ProfInfo &pi = e->profInfo[id];
+    std::vector<float> emis((size_t)(M + 1) * kEmStride, 0.f), bgv(32, 0.f);
+    for (int i = 0; i <= M; i++)
+        for (int c = 0; c < kNCode; c++) emis[(size_t)i * kEmStride + c] = p.values[(size_t)i * kWidth + 4 + c];
+    for (int c = 0; c < kNCode; c++) bgv[c] = p.bg[4 + c];
+    HIPCHK(hipMemcpy(e->dParams.as<float>() + pi.parOff, p.params.data(), sizeof(float) * p.params.size(),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->dEmis.as<float>() + pi.emOff, emis.data(), sizeof(float) * emis.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->dBg.as<float>() + (int64_t)id * 32, bgv.data(), sizeof(float) * 32, hipMemcpyHostToDevice));
     return DM_OK;
 }
 

@@ -353,10 +353,13 @@ def test_registered_set_is_bit_identical(engine):
     w = [1.0, 0.25, 3.0, 0.0]
     assert np.array_equal(engine.grad_counts(pid, seqs, w)[1], engine.grad_set_counts(pid, sid, w)[1])
     v2, pa2 = G.profile_from_logits(el * 0.5, tl, bg)
-    engine.set_profile(pid, M, v2, pa2, bg)
+    engine.set_profile(pid, M, v2, pa2, bg)  # same length: rewritten in place on the device
     lnz_c, c_c = engine.grad_set_counts(pid, sid)
     lnz_d, c_d = engine.grad_counts(pid, seqs)
     assert np.array_equal(lnz_c, lnz_d) and np.array_equal(c_c, c_d) and not np.array_equal(c_c, c_a)
+    pid2 = engine.add_profile(M, v2, pa2, bg)  # the same profile through a full upload
+    lnz_e, c_e = engine.grad_set_counts(pid2, sid)
+    assert np.array_equal(lnz_e, lnz_c) and np.array_equal(c_e, c_c)
     empty = engine.grad_set([])
     lz, ce = engine.grad_set_counts(pid, empty)
     assert len(lz) == 0 and not ce.any()
