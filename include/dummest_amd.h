@@ -66,8 +66,10 @@ int dm_add_profile(dm_engine *e, int32_t M, const float *values, const float *pa
                    int32_t *id);
 
 /* Replace profile `id` with new finalized arrays (as dm_add_profile; the
- * length may change). For callers that re-evaluate a profile many times, such
- * as a training loop over dm_grad; no counterpart in the reference. */
+ * length may change; with the same length and the device tables in sync only
+ * this profile's tables are rewritten). For callers that re-evaluate a profile
+ * many times, such as a training loop over dm_grad; no counterpart in the
+ * reference. */
 int dm_set_profile(dm_engine *e, int32_t id, int32_t M, const float *values, const float *params, const float *bg);
 
 /* Copy out a profile's finalized arrays (sizes as in dm_add_profile). */
