@@ -27,12 +27,19 @@ LIB_PATH = os.path.join(HERE, "libdummest_amd.so")
 WIDTH = 29
 ALPHABET = "ACDEFGHIKLMNPQRSTVWYUO?*"
 DEFAULT_RATES = (0.0171, 0.0018, 0.0328, 0.0083, 0.0005, 0.046875)
+# status codes of the C ABI (include/dummest_amd.h)
+DM_OK, DM_EINVAL, DM_EHIP, DM_EFORMAT, DM_EUNSUP, DM_EIO, DM_ERANGE = 0, -1, -2, -3, -4, -5, -6
 
 _lib = None
 
 
 class DummestError(RuntimeError):
-    pass
+    """A failing call of the C ABI; ``code`` is its DM_E* status (None when
+    the failure did not come from a status code)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def lib():
@@ -338,7 +345,7 @@ class Engine:
     def _check(self, rc, what):
         if rc:
             msg = lib().dm_engine_error(self._h).decode(errors="replace")
-            raise DummestError(f"{what} failed ({rc}): {msg}")
+            raise DummestError(f"{what} failed ({rc}): {msg}", rc)
 
     # -- configuration ------------------------------------------------------
     def set_rates(self, rates=DEFAULT_RATES):
@@ -579,13 +586,18 @@ class Engine:
 
     def find_similarities(self, prof: int, seq: int, min_prob_ratio: float, hit_cap=4096, seg_cap=1 << 20):
         """Hits of one pair: list of (lnratio, m, e, anchor1, anchor2, [(start1, start2, length)])."""
-        hits = np.zeros((hit_cap, 6), np.float64)
-        segs = np.zeros((seg_cap, 3), np.int32)
         nh, ns = ctypes.c_int32(), ctypes.c_int32()
         i32 = ctypes.c_int32
-        self._check(lib().dm_pair_hits(self._h, prof, seq, float(min_prob_ratio), _ptr(hits, ctypes.c_double),
-                                       hit_cap, _ptr(segs, i32), seg_cap, ctypes.byref(nh), ctypes.byref(ns)),
-                    "dm_pair_hits")
+        while True:
+            hits = np.zeros((hit_cap, 6), np.float64)
+            segs = np.zeros((seg_cap, 3), np.int32)
+            rc = lib().dm_pair_hits(self._h, prof, seq, float(min_prob_ratio), _ptr(hits, ctypes.c_double),
+                                    hit_cap, _ptr(segs, i32), seg_cap, ctypes.byref(nh), ctypes.byref(ns))
+            if rc != DM_ERANGE:
+                break
+            # the buffers were too small: the call reported the sizes it needs
+            hit_cap, seg_cap = max(hit_cap, nh.value), max(seg_cap, ns.value)
+        self._check(rc, "dm_pair_hits")
         out = []
         k = 0
         for h in hits[:nh.value]:

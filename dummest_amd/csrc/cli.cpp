@@ -799,13 +799,20 @@ int main(int argc, char *argv[]) {
                 if (less) continue;
                 ++nGated;
                 int32_t nh = 0, ns = 0;
-                r = dm_pair_hits(E, pp[k], ss[k], (double)ratio, hits.data(), 4096, segs.data(), 1 << 20, &nh, &ns);
+                r = dm_pair_hits(E, pp[k], ss[k], (double)ratio, hits.data(), (int32_t)(hits.size() / 6), segs.data(),
+                                 (int32_t)(segs.size() / 3), &nh, &ns);
+                if (r == DM_ERANGE) {  // more hits than the buffers hold: grow them to the reported need
+                    hits.resize(6 * std::max<size_t>(hits.size() / 6, (size_t)nh));
+                    segs.resize(3 * std::max<size_t>(segs.size() / 3, (size_t)ns));
+                    r = dm_pair_hits(E, pp[k], ss[k], (double)ratio, hits.data(), (int32_t)(hits.size() / 6),
+                                     segs.data(), (int32_t)(segs.size() / 3), &nh, &ns);
+                }
                 if (r) return engineFail(E, "dm_pair_hits", r);
                 const Target &T = targets[reqTarget[rq[k]]];
                 const HostProfile &P = profiles[pi];
                 const SeqState &S = T.rec;
                 int so = 0;
-                for (int h = 0; h < std::min(nh, 4096); h++) {
+                for (int h = 0; h < nh; h++) {
                     const double lnr = hits[6 * h];
                     const int a1 = (int)hits[6 * h + 3], a2 = (int)hits[6 * h + 4], nseg = (int)hits[6 * h + 5];
                     const int32_t *sg = &segs[3 * so];

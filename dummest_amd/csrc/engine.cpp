@@ -67,6 +67,7 @@ struct dm_engine {
     std::vector<int64_t> seqOff;
     std::vector<int32_t> seqLen;
     bool seqDirty = true;
+    int64_t seqEpoch = 0;  // dm_clear_sequences count: sequence ids of an earlier epoch are gone
     DevBuf dCodes;
     DevBuf dPairs, dRec, dNull, dUnits, dCounter, dWbuf, dW0buf, dEbuf, dFbuf, dOut;
     std::vector<int> cfgG, cfgR;
@@ -108,6 +109,15 @@ struct dm_engine {
     std::vector<GradSet> gradSets;
     double lastGradMs = 0;
     int64_t lastGradCells = 0;
+    // hits of the last dm_pair_hits call, kept so that a call repeated with
+    // larger buffers after DM_ERANGE does not run the pair again
+    struct HitCache {
+        int32_t prof = -1, seq = -1;
+        double ratio = 0;
+        int64_t seqEpoch = -1, profGen = -1;
+        std::vector<Hit> hits;
+    } hitCache;
+    int64_t profGen = 0;  // profile edits (a cached hit list of an older generation is stale)
 
     float profileMaskThr() const { return (maskBits & 1) ? maskThr : -1.f; }
 
@@ -413,17 +423,29 @@ struct dm_plan {
     void *hostOut = nullptr;  // pinned staging for results
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
+    // what the plan was built against: the sequence epoch (its sequence ids)
+    // and the length of every profile it uses (configurations, scratch sizes)
+    int64_t seqEpoch = 0;
+    std::vector<std::pair<int32_t, int32_t>> profM;
+    // what a captured graph holds by value: the device tables, the position of
+    // the zero-code byte after the codes, the background frameshift rates
+    const void *capTabs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t capCodes = 0;
+    float capFs1 = 0.f, capFs2 = 0.f;
+    void dropGraph() {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graphExec = nullptr;
+        graph = nullptr;
+    }
     void release() {
         for (auto &c : chunks) {
             c.pairs.release();
             c.units.release();
         }
         for (DevBuf *b : {&out, &rec, &nullScratch, &counter, &wbuf, &ebuf, &bnd}) b->release();
-        if (graphExec) (void)hipGraphExecDestroy(graphExec);
-        if (graph) (void)hipGraphDestroy(graph);
+        dropGraph();
         if (hostOut) (void)hipHostFree(hostOut);
-        graphExec = nullptr;
-        graph = nullptr;
         hostOut = nullptr;
     }
 };
@@ -442,6 +464,8 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
     P.wantBwd = wantBwd;
     P.npairs = npairs;
     P.cells = 0;
+    P.seqEpoch = e->seqEpoch;
+    std::vector<char> used(nprof, 0);
     for (int64_t i = 0; i < npairs; i++) {
         if (prof[i] < 0 || prof[i] >= nprof || seq[i] < 0 || seq[i] >= nseq)
             return fail(e, DM_EINVAL, "pair references an unknown profile or sequence");
@@ -449,6 +473,10 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         if (n < 3) return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
         cfgOf[i] = chooseConfig(e, M, n, e->plain);
         P.cells += (int64_t)(M + 1) * n;
+        if (!used[prof[i]]) {
+            used[prof[i]] = 1;
+            P.profM.push_back({prof[i], M});
+        }
     }
     std::vector<int64_t> sorted(npairs);
     std::iota(sorted.begin(), sorted.end(), 0);
@@ -644,10 +672,57 @@ static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *sc
     return DM_OK;
 }
 
+// The device tables a graph holds by value, as they are now.
+static void captureKey(const dm_engine *e, const void *tabs[5], size_t &codes, float &fs1, float &fs2) {
+    tabs[0] = e->dProfs.p;
+    tabs[1] = e->dParams.p;
+    tabs[2] = e->dEmis.p;
+    tabs[3] = e->dBg.p;
+    tabs[4] = e->dCodes.p;
+    codes = e->codes.size();
+    fs1 = e->bgfs1();
+    fs2 = e->bgfs2();
+}
+
+static int capturePlan(dm_engine *e, dm_plan &P);
+
+// Before a plan runs (the engine's tables in sync): a plan whose sequences were
+// cleared, or one of whose profiles changed length, no longer describes its
+// pairs (DM_EINVAL); otherwise the launches take the current table addresses
+// (an upload after the plan was built may have moved them), and a captured
+// graph recorded against other tables or rates is captured again.
+static int refreshPlan(dm_engine *e, dm_plan &P) {
+    if (P.seqEpoch != e->seqEpoch)
+        return fail(e, DM_EINVAL, "the engine's sequences were cleared after the plan was created");
+    for (const auto &pm : P.profM)
+        if (pm.first >= (int32_t)e->profiles.size() || e->profiles[pm.first].M != pm.second)
+            return fail(e, DM_EINVAL, "a profile of the plan changed length after the plan was created");
+    for (DevChunk &D : P.chunks)
+        for (Launch &L : D.launches) {
+            L.args.profs = e->dProfs.as<ProfInfo>();
+            L.args.params = e->dParams.as<float>();
+            L.args.emis = e->dEmis.as<float>();
+            L.args.bg = e->dBg.as<float>();
+        }
+    if (P.graphExec) {
+        const void *tabs[5];
+        size_t codes;
+        float fs1, fs2;
+        captureKey(e, tabs, codes, fs1, fs2);
+        if (std::memcmp(tabs, P.capTabs, sizeof(tabs)) != 0 || codes != P.capCodes || fs1 != P.capFs1 ||
+            fs2 != P.capFs2) {
+            P.dropGraph();
+            return capturePlan(e, P);
+        }
+    }
+    return DM_OK;
+}
+
 // Capture a plan's launches (all chunks, in stream order) into a hipGraph;
 // later executions replay the graph.
 static int capturePlan(dm_engine *e, dm_plan &P) {
     if (P.graphExec) return DM_OK;
+    captureKey(e, P.capTabs, P.capCodes, P.capFs1, P.capFs2);
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = DM_OK;
     for (DevChunk &D : P.chunks) {
@@ -842,6 +917,7 @@ int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, 
     if (count) *count = (int32_t)ps.size();
     for (auto &p : ps) e->profiles.push_back(std::move(p));
     e->profDirty = true;
+    e->profGen++;
     return DM_OK;
 }
 
@@ -885,6 +961,7 @@ int dm_add_raw_profiles(dm_engine *e, int32_t nprof, const int32_t *M, const flo
         e->profiles.push_back(std::move(p));
     }
     e->profDirty = true;
+    e->profGen++;
     return DM_OK;
 }
 
@@ -900,6 +977,7 @@ int dm_add_profile(dm_engine *e, int32_t M, const float *values, const float *pa
     if (id) *id = (int32_t)e->profiles.size();
     e->profiles.push_back(std::move(p));
     e->profDirty = true;
+    e->profGen++;
     return DM_OK;
 }
 
@@ -914,10 +992,11 @@ int dm_set_profile(dm_engine *e, int32_t id, int32_t M, const float *values, con
     p.bg.assign(bg, bg + kWidth);
     p.consensus.assign(M, 0);
     p.masked = 0;
-    if (!inPlace) {
-        e->profDirty = true;
-        return DM_OK;
-    }
+    e->profGen++;
+    // the device tables count as stale until every copy below has succeeded: a
+    // failed copy leaves them dirty, so the next call re-uploads them all
+    e->profDirty = true;
+    if (!inPlace) return DM_OK;
     // same length, tables in sync: rewrite this profile's slices only (a
     // gradient loop updates one profile per evaluation among many)
     HIPCHK(hipSetDevice(e->device));
@@ -930,6 +1009,7 @@ int dm_set_profile(dm_engine *e, int32_t id, int32_t M, const float *values, con
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->dEmis.as<float>() + pi.emOff, emis.data(), sizeof(float) * emis.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->dBg.as<float>() + (int64_t)id * 32, bgv.data(), sizeof(float) * 32, hipMemcpyHostToDevice));
+    e->profDirty = false;
     return DM_OK;
 }
 
@@ -1093,6 +1173,13 @@ int dm_clear_sequences(dm_engine *e) {
     e->seqOff.clear();
     e->seqLen.clear();
     e->seqDirty = true;
+    e->seqEpoch++;
+    // registered gradient sets live in the store: they go with it
+    if (!e->gradSets.empty()) (void)hipSetDevice(e->device);
+    for (auto &g : e->gradSets) {
+        g.dCodon.release();
+        g.s0 = -1;
+    }
     return DM_OK;
 }
 
@@ -1199,7 +1286,11 @@ int dm_post_process(int32_t M, const float *values, const float *params, const f
 
 int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio, double *hits, int32_t hit_cap,
                  int32_t *segs, int32_t seg_cap, int32_t *nhits, int32_t *nsegs) {
-    if (!e || !hits || !segs || !nhits || !nsegs) return DM_EINVAL;
+    if (!e || !hits || !segs || !nhits || !nsegs || hit_cap < 0 || seg_cap < 0) return DM_EINVAL;
+    auto &hc = e->hitCache;
+    if (hc.prof == prof && hc.seq == seq && hc.ratio == min_prob_ratio && hc.seqEpoch == e->seqEpoch &&
+        hc.profGen == e->profGen)
+        return writeHits(e, hc.hits, hits, hit_cap, segs, seg_cap, nhits, nsegs);
     HIPCHK(hipSetDevice(e->device));
     PairDump D;
     int rc = dumpPair(e, prof, seq, D);
@@ -1226,16 +1317,30 @@ int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio,
     in.sC = D.sC;
     in.bgfs1 = e->bgfs1();
     in.bgfs2 = e->bgfs2();
-    std::vector<Hit> out;
-    postProcess(P, in, (float)min_prob_ratio, out);
-    return writeHits(e, out, hits, hit_cap, segs, seg_cap, nhits, nsegs);
+    hc.hits.clear();
+    postProcess(P, in, (float)min_prob_ratio, hc.hits);
+    hc.prof = prof;
+    hc.seq = seq;
+    hc.ratio = min_prob_ratio;
+    hc.seqEpoch = e->seqEpoch;
+    hc.profGen = e->profGen;
+    return writeHits(e, hc.hits, hits, hit_cap, segs, seg_cap, nhits, nsegs);
 }
 
+// Hits and their segments into the caller's arrays. When they do not fit,
+// *nhits / *nsegs receive the counts needed and nothing is written (DM_ERANGE).
 static int writeHits(dm_engine *e, const std::vector<Hit> &out, double *hits, int32_t hit_cap, int32_t *segs,
                      int32_t seg_cap, int32_t *nhits, int32_t *nsegs) {
+    int64_t needH = (int64_t)out.size(), needS = 0;
+    for (const Hit &h : out) needS += (int64_t)h.segs.size();
+    if (needH > hit_cap || needS > seg_cap) {
+        *nhits = (int32_t)std::min<int64_t>(needH, INT32_MAX);
+        *nsegs = (int32_t)std::min<int64_t>(needS, INT32_MAX);
+        return fail(e, DM_ERANGE, "hit buffers too small: " + std::to_string(needH) + " hits and " +
+                                      std::to_string(needS) + " segments needed");
+    }
     int32_t nh = 0, ns = 0;
     for (const Hit &h : out) {
-        if (nh >= hit_cap) return fail(e, DM_EINVAL, "hit capacity exceeded");
         double *o = hits + 6 * nh;
         o[0] = h.lnratio;
         o[1] = h.pair.m;
@@ -1244,7 +1349,6 @@ static int writeHits(dm_engine *e, const std::vector<Hit> &out, double *hits, in
         o[4] = h.anchor2;
         o[5] = (double)h.segs.size();
         for (const Seg &s : h.segs) {
-            if (ns >= seg_cap) return fail(e, DM_EINVAL, "segment capacity exceeded");
             segs[3 * ns] = s.start1;
             segs[3 * ns + 1] = s.start2;
             segs[3 * ns + 2] = s.length;
@@ -1279,6 +1383,8 @@ int dm_plan_execute(dm_engine *e, dm_plan *P, float *best, float *mid_m, int32_t
     if (rc) return rc;
     rc = syncSequences(e);
     if (rc) return rc;
+    rc = refreshPlan(e, *P);
+    if (rc) return rc;
     if (P->mode == kFilter) {
         if (P->npairs > 0 && !best) return DM_EINVAL;
         return executePlan(e, *P, best, nullptr);
@@ -1304,6 +1410,8 @@ int dm_plan_capture(dm_engine *e, dm_plan *P) {
     int rc = syncProfiles(e);
     if (rc) return rc;
     rc = syncSequences(e);
+    if (rc) return rc;
+    rc = refreshPlan(e, *P);
     if (rc) return rc;
     return capturePlan(e, *P);
 }
@@ -1835,6 +1943,9 @@ int dm_grad_set(dm_engine *e, int32_t prof, int32_t set, const double *weights, 
     int rc = syncProfiles(e);
     if (rc) return rc;
     const auto &g = e->gradSets[set];
+    if ((size_t)g.s0 + (size_t)g.n > e->seqLen.size() ||
+        (g.n > 0 && (size_t)e->seqOff[(size_t)g.s0] != g.codes0) || g.codes0 > e->codes.size())
+        return fail(e, DM_EINVAL, "the sequence set's sequences are no longer in the engine's store");
     if (g.n == 0) {
         std::fill(counts, counts + (size_t)(e->profiles[prof].M + 1) * kGradRaw, 0.0);
         return DM_OK;

@@ -104,6 +104,57 @@ def make_profile(rng, L: int, name: str, ndesc=50, anc=None):
     return "\n".join(lines) + "\n", anc, descs
 
 
+def simulate_msa(rng, L: int = 200, ndesc: int = 50, name: str = "sim", sub=0.30, indel=0.02):
+    """Stockholm text of `ndesc` descendants of a random ancestor of L amino
+    acids (uniform over the 20), each with `sub` uniform off-diagonal
+    substitutions and `indel` indels of geometric length (p = 0.5), aligned to
+    the ancestor: one column per ancestral position (a gap where it was
+    deleted) and, before each, as many insertion columns as the longest
+    insertion there (gaps in the descendants without one). The columns of
+    ancestral positions are filled by most descendants and become the
+    profile's positions when dummest-build keeps columns with a residue
+    fraction of at least 0.5.
+    Returns (text, ancestor as letter indices, descendants as letter indices)."""
+    anc = rng.integers(0, 20, size=L)
+    descs = [evolve(rng, anc, sub=sub, indel=indel) for _ in range(ndesc)]
+    # insertion runs before ancestral position i (i = L: after the last)
+    ins_before = np.zeros(L + 1, np.int64)
+    per_desc = []
+    for seq, al, _, _ in descs:
+        runs = {}
+        cur = []
+        nxt = 0
+        cols = {}
+        for aa, a in zip(seq, al):
+            if a < 0:
+                cur.append(int(aa))
+            else:
+                if cur:
+                    runs[int(a)] = cur
+                    cur = []
+                cols[int(a)] = int(aa)
+                nxt = int(a) + 1
+        if cur:
+            runs[L if nxt >= L else nxt] = runs.get(L if nxt >= L else nxt, []) + cur
+        for i, r in runs.items():
+            ins_before[i] = max(ins_before[i], len(r))
+        per_desc.append((runs, cols))
+    rows = []
+    for runs, cols in per_desc:
+        out = []
+        for i in range(L + 1):
+            r = runs.get(i, [])
+            out.append("".join(AA[x].lower() for x in r) + "." * (ins_before[i] - len(r)))
+            if i < L:
+                out.append(AA[cols[i]] if i in cols else "-")
+        rows.append("".join(out))
+    lines = ["# STOCKHOLM 1.0", "", f"#=GF ID {name}", ""]
+    for k, r in enumerate(rows):
+        lines.append(f"{name}_{k:03d} {r}")
+    lines.append("//")
+    return "\n".join(lines) + "\n", anc, [d[0] for d in descs]
+
+
 def reverse_translate(rng, prot: np.ndarray) -> str:
     return "".join(CODONS[AA[a]][rng.integers(len(CODONS[AA[a]]))] for a in prot)
 
@@ -212,14 +263,16 @@ def mutate_np(rng, dna: np.ndarray, indel=0.01, sub=0.0) -> np.ndarray:
     return res
 
 
-def workload_c4(seed=1004, n_profiles=5000, n_reads=100000, pair_rate=0.004, max_len=2000):
+def workload_c4(seed=1004, n_profiles=5000, n_reads=100000, pair_rate=0.004, max_len=2000, source_pairs=False):
     """Uneven reads against many profiles (the fwd+bwd-at-scale configuration).
 
     profiles: lengths lognormal (median 150, sigma 0.8, clipped to [30, 2000]
     and here also to `max_len`); reads: uniform 150-300 bp, half from a
     profile's ancestor (reverse-translated, then 1% 1-nt indels and 0.5%
     substitutions), half random (GC 0.5); pairs: each (read, profile) kept
-    with probability `pair_rate`.
+    with probability `pair_rate`. source_pairs adds, after those, the pair of
+    every homologous read with the profile it came from (reduced-size parity
+    runs, where the random mask alone would pair few reads with their source).
     Returns dict(Ms, raws, reads (uint8 DNA arrays), prof, seq)."""
     rng = np.random.default_rng(seed)
     Ms = np.minimum(lognormal_lengths(rng, n_profiles), max_len)
@@ -247,6 +300,10 @@ def workload_c4(seed=1004, n_profiles=5000, n_reads=100000, pair_rate=0.004, max
     per = rng.binomial(n_profiles, pair_rate, size=n_reads)
     seq = np.repeat(np.arange(n_reads, dtype=np.int32), per)
     prof = rng.integers(0, n_profiles, size=len(seq)).astype(np.int32)
+    if source_pairs:
+        h = np.nonzero(homolog)[0].astype(np.int32)
+        seq = np.concatenate([seq, h])
+        prof = np.concatenate([prof, src[h].astype(np.int32)])
     return dict(Ms=Ms, raws=raws, reads=reads, prof=prof, seq=seq, homolog=homolog, src=src)
 
 

@@ -25,6 +25,7 @@ extern "C" {
 #define DM_EFORMAT -3  /* unreadable profile or sequence data */
 #define DM_EUNSUP -4   /* input outside what the kernels cover */
 #define DM_EIO -5      /* a file could not be written */
+#define DM_ERANGE -6   /* output buffers too small; the call reports the sizes it needs */
 
 typedef struct dm_engine dm_engine;
 
@@ -84,6 +85,9 @@ int dm_decode(const char *dna, int64_t n, uint8_t *codes);
  * Replaces the per-request decoded vectors (SequenceData, dummest-core.hh:277-283). */
 int dm_add_sequences(dm_engine *e, int32_t nseq, const uint8_t *codes, const int64_t *offsets,
                      const int32_t *lengths, int32_t *first_id);
+/* Empty the sequence store (ids restart at 0). Plans built on the old
+ * sequences and registered gradient sets (dm_grad_set_create) become invalid:
+ * executing such a plan or evaluating such a set returns DM_EINVAL. */
 int dm_clear_sequences(dm_engine *e);
 
 /* Masking of simple regions with tantan (dummest.cc -m and --tantan-threshold,
@@ -131,9 +135,13 @@ int dm_pair_null(dm_engine *e, int32_t prof, int32_t seq, float *prefix_seed, fl
 
 /* Local hits of one pair after the gate (forwardPass epilogue + traceback,
  * dummest-dp-forward.hh:252-545, dummest-traceback.hh:6-118). min_prob_ratio as
- * in SequenceRequest (dummest-core.hh:285-288). Writes up to hit_cap hits as
+ * in SequenceRequest (dummest-core.hh:285-288). Writes the hits as
  * (lnratio, mantissa, exponent, anchor1, anchor2, nseg) and their segments as
- * (start1, start2, length); *nhits / *nsegs receive the counts. */
+ * (start1, start2, length); *nhits / *nsegs receive the counts. When more than
+ * hit_cap hits or seg_cap segments are found, nothing is written, *nhits /
+ * *nsegs receive the sizes needed and the call returns DM_ERANGE; the same
+ * call with larger buffers then returns the kept result without running the
+ * pair again (the reference has no per-pair hit limit). */
 int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio, double *hits, int32_t hit_cap,
                  int32_t *segs, int32_t seg_cap, int32_t *nhits, int32_t *nsegs);
 
@@ -150,8 +158,14 @@ int dm_plan_execute(dm_engine *e, dm_plan *plan, float *best, float *mid_m, int3
                     int32_t *bwd_e);
 void dm_plan_destroy(dm_engine *e, dm_plan *plan);
 /* Capture a plan's launches into a hipGraph; later dm_plan_execute calls
- * replay the graph. The engine's profiles and sequences must not change while
- * a captured plan is in use. */
+ * replay the graph.
+ * A plan stays valid while its profiles keep their lengths and the sequence
+ * store is not cleared: profiles and sequences may be added, and profiles
+ * rewritten at the same length (dm_set_profile), between executions; a plan
+ * then reads the current tables (a captured graph recorded against tables
+ * that have since moved, or against other rates, is captured again). After
+ * dm_clear_sequences, or when a profile of the plan changed length,
+ * dm_plan_execute and dm_plan_capture return DM_EINVAL. */
 int dm_plan_capture(dm_engine *e, dm_plan *plan);
 
 /* Per kernel configuration of the last execution: G (lanes per pair), R (rows
