@@ -785,18 +785,17 @@ int main(int argc, char *argv[]) {
                 const int pi = reqProfRatio[rq[k]].first;
                 const float ratio = reqProfRatio[rq[k]].second;
                 if (ratio < 0) continue;  // calibration-style lanes report no alignments
-                // gate: best anchored score >= minProbRatio (dummest-dp-forward.hh:263-271)
-                ES minExp{0.f, 0};
-                {
-                    int ex = 0;
-                    const double m = std::frexp((double)ratio, &ex);
-                    if (ratio != 0) minExp = ES{(float)m, ex};
+                // gate: best anchored score >= minProbRatio (dummest-dp-forward.hh:263-271).
+                // The score pass rounds differently from the reference (fused
+                // multiply-adds, another summation order), so here a pair is
+                // dropped only when it is clearly below the threshold; the
+                // post-gate step (dm_pair_hits) works on matrices that equal the
+                // reference's and applies the exact gate itself.
+                if (ratio > 0) {
+                    if (mm[k] == 0.f) continue;
+                    const double lnGot = std::log((double)mm[k]) + (double)me[k] * 0.6931471805599453;
+                    if (lnGot < std::log((double)ratio) - 1e-5) continue;
                 }
-                const ES got{mm[k], me[k]};
-                const bool less = got.m == 0.f ? minExp.m != 0.f
-                                               : (minExp.m == 0.f ? false
-                                                  : (got.e != minExp.e ? got.e < minExp.e : got.m < minExp.m));
-                if (less) continue;
                 ++nGated;
                 int32_t nh = 0, ns = 0;
                 r = dm_pair_hits(E, pp[k], ss[k], (double)ratio, hits.data(), (int32_t)(hits.size() / 6), segs.data(),

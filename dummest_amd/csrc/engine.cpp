@@ -130,11 +130,16 @@ static int fail(dm_engine *e, int code, const std::string &msg) {
     return code;
 }
 
+// A failing HIP call also clears the runtime's last error: launches report
+// hipGetLastError(), which would otherwise hand the same error (say, an
+// allocation that did not fit) to the engine's next, unrelated launch.
 #define HIPCHK(expr)                                                                     \
     do {                                                                                 \
         hipError_t _r = (expr);                                                          \
-        if (_r != hipSuccess)                                                            \
+        if (_r != hipSuccess) {                                                          \
+            (void)hipGetLastError();                                                     \
             return fail(e, DM_EHIP, std::string(#expr) + ": " + hipGetErrorString(_r)); \
+        }                                                                                \
     } while (0)
 
 // ---------------------------------------------------------------------------
@@ -1673,19 +1678,22 @@ int dm_copy_probe(dm_engine *e, int64_t bytes, double *ms) {
     if (!e || bytes < 16 || (bytes & 15) || !ms) return DM_EINVAL;
     HIPCHK(hipSetDevice(e->device));
     DevBuf src, dst;
-    HIPCHK(src.ensure((size_t)bytes));
-    HIPCHK(dst.ensure((size_t)bytes));
-    HIPCHK(launch_copy_probe(src.as<float>(), dst.as<float>(), bytes / 16, e->stream));  // warm-up
-    HIPCHK(hipEventRecord(e->ev0, e->stream));
-    HIPCHK(launch_copy_probe(src.as<float>(), dst.as<float>(), bytes / 16, e->stream));
-    HIPCHK(hipEventRecord(e->ev1, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    float t = 0;
-    HIPCHK(hipEventElapsedTime(&t, e->ev0, e->ev1));
-    *ms = t;
+    const int rc = [&]() -> int {
+        HIPCHK(src.ensure((size_t)bytes));
+        HIPCHK(dst.ensure((size_t)bytes));
+        HIPCHK(launch_copy_probe(src.as<float>(), dst.as<float>(), bytes / 16, e->stream));  // warm-up
+        HIPCHK(hipEventRecord(e->ev0, e->stream));
+        HIPCHK(launch_copy_probe(src.as<float>(), dst.as<float>(), bytes / 16, e->stream));
+        HIPCHK(hipEventRecord(e->ev1, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, e->ev0, e->ev1));
+        *ms = t;
+        return DM_OK;
+    }();
     src.release();
     dst.release();
-    return DM_OK;
+    return rc;
 }
 
 #ifndef DM_BUILD_ID

@@ -422,6 +422,23 @@ template <int G> __device__ __forceinline__ Rec rec_from_above(const Rec &r) {
     return o;
 }
 
+// Single-rounding products and sums that the compiler may not fuse into
+// multiply-adds: the matrices handed to the post-gate steps (dump mode)
+// evaluate every cell's expression in the reference's order of operations
+// (dummest-dp-backward.hh:523-548, dummest-dp-forward.hh:123,158-165,218-222),
+// fusing only where it calls fma itself, so that each cell equals the
+// reference's FP32 value (the per-pair power-of-two scaling, different from
+// the reference's per-row rescaling, is exact) and the hits traced through
+// near-equal anchors and max-plus ties come out the same.
+__device__ __forceinline__ float mul_nc(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_nc(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
 // Exact running maximum of float values whose power-of-two offset changes
 // rarely: cur holds the max at offset off; (sm, se) the exact max of all
 // values retired at earlier offsets.
@@ -439,13 +456,16 @@ struct ExpMax {
             if (es_less(ES{sm, se}, c)) { sm = c.m; se = c.e; }
         }
     }
+    // The kept maximum seeds the running max at the new offset when it is a
+    // float there; otherwise the running max starts at 0 (retire() compares
+    // exactly with (sm, se), so nothing is lost). Clamping a larger maximum to
+    // FLT_MAX would be wrong: FLT_MAX * 2^off exceeds sm * 2^se when se - off
+    // is 128, and would then be retired as the maximum.
     __device__ __forceinline__ void rebase(int newOff) {
         retire();
         off = newOff;
-        if (sm == 0.f) { cur = 0.f; return; }
         const int d = se - newOff;
-        cur = d > 127 ? FLT_MAX : (d < -150 ? 0.f : ldexpf(sm, d));
-        if (!(cur <= FLT_MAX)) cur = FLT_MAX;
+        cur = (sm == 0.f || d > 127 || d < -126) ? 0.f : ldexpf(sm, d);
     }
 };
 
@@ -465,12 +485,13 @@ struct ExpMaxD {
             if (es_less(ES{sm, se}, c)) { sm = c.m; se = c.e; }
         }
     }
+    // As ExpMax::rebase: a kept maximum outside the double range at the new
+    // offset leaves the running max at 0 (never DBL_MAX, which could exceed it).
     __device__ __forceinline__ void rebase(int newOff) {
         retire();
         off = newOff;
-        if (sm == 0.f) { cur = 0.0; return; }
         const int d = se - newOff;
-        cur = d > 1000 ? DBL_MAX : (d < -1000 ? 0.0 : ldexp((double)sm, d));
+        cur = (sm == 0.f || d > 1000 || d < -1000) ? 0.0 : ldexp((double)sm, d);
     }
 };
 
@@ -765,15 +786,28 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
 #pragma unroll
                 for (int q = R - 1; q >= 0; q--) {
                     const float e = eq[q];
-                    const float eE = e * enterC[q];
                     const float zg = Zr[S3][q] * rcg;
-                    float w = fmaf(bw3, eE, q < nValid ? ssr : 0.f);
-                    w = fmaf(bw2, d1s2[q], w);
-                    w = fmaf(bw1, d2s1[q], w);
-                    w = fmaf(zg, a0sC[q], w);
-                    w = fmaf(H[S2][q], a2s2[q], w);
-                    w = fmaf(H[S1][q], a1s1[q], w);
-                    w = fmaf(bY, d0[q], w);
+                    float w;
+                    if constexpr (MODE == kDump) {
+                        // n3 e enter + Y d0 + n2 d1 + n1 d2 + Z g a0 + c2 a2 + c1 a1 + seed, left to right
+                        w = mul_nc(mul_nc(bw3, e), enterC[q]);
+                        w = add_nc(w, mul_nc(bY, d0[q]));
+                        w = add_nc(w, mul_nc(bw2, d1s2[q]));
+                        w = add_nc(w, mul_nc(bw1, d2s1[q]));
+                        w = add_nc(w, mul_nc(zg, a0sC[q]));
+                        w = add_nc(w, mul_nc(H[S2][q], a2s2[q]));
+                        w = add_nc(w, mul_nc(H[S1][q], a1s1[q]));
+                        w = add_nc(w, q < nValid ? ssr : 0.f);
+                    } else {
+                        const float eE = e * enterC[q];
+                        w = fmaf(bw3, eE, q < nValid ? ssr : 0.f);
+                        w = fmaf(bw2, d1s2[q], w);
+                        w = fmaf(bw1, d2s1[q], w);
+                        w = fmaf(zg, a0sC[q], w);
+                        w = fmaf(H[S2][q], a2s2[q], w);
+                        w = fmaf(H[S1][q], a1s1[q], w);
+                        w = fmaf(bY, d0[q], w);
+                    }
                     const float y = fmaf(eps[q], bY, w);
                     const float z = fmaf(b0sC[q], zg, w);
                     if (MODE == kFilter) fbest = fmaxf(fbest, w * psp);
@@ -1015,15 +1049,27 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 for (int q = 0; q < R; q++) {
                     const float w1 = Wf[S1][q], w2 = Wf[S2][q], w3 = Wf[S3][q];
                     const float e = eqf[q];
-                    const float X = enterC[q] * e * w3;
-                    const float Z0 = gsC * fmaf(b0[q], Zf[S3][q], a0[q] * w3);
-                    float w0 = fmaf(a1s1[q], w1, Z0);
-                    w0 = fmaf(a2s2[q], w2, w0);
-                    w0 = Nin + (w0 + psr);
-                    const float y = fmaf(d0[q], w0, eps[q] * Ya);
-                    float nn = X + y;
-                    nn = fmaf(d1s2[q], w2, nn);
-                    nn = fmaf(d2s1[q], w1, nn);
+                    float X, Z0, w0, y, nn;
+                    if constexpr (MODE == kDump) {
+                        // X = enter e w3; Z0 = g sC (a0 w3 + b0 Z0'); w0 = W0in + (Z0 + a1 w1 + a2 w2 + seed);
+                        // Y = d0 w0 + eps Y'; next row's W0in = X + Y + d1 w2 + d2 w1, left to right
+                        X = mul_nc(mul_nc(enterC[q], e), w3);
+                        Z0 = mul_nc(gsC, add_nc(mul_nc(a0[q], w3), mul_nc(b0[q], Zf[S3][q])));
+                        w0 = add_nc(add_nc(add_nc(Z0, mul_nc(a1s1[q], w1)), mul_nc(a2s2[q], w2)), psr);
+                        w0 = add_nc(Nin, w0);
+                        y = add_nc(mul_nc(d0[q], w0), mul_nc(eps[q], Ya));
+                        nn = add_nc(add_nc(add_nc(X, y), mul_nc(d1s2[q], w2)), mul_nc(d2s1[q], w1));
+                    } else {
+                        X = enterC[q] * e * w3;
+                        Z0 = gsC * fmaf(b0[q], Zf[S3][q], a0[q] * w3);
+                        w0 = fmaf(a1s1[q], w1, Z0);
+                        w0 = fmaf(a2s2[q], w2, w0);
+                        w0 = Nin + (w0 + psr);
+                        y = fmaf(d0[q], w0, eps[q] * Ya);
+                        nn = X + y;
+                        nn = fmaf(d1s2[q], w2, nn);
+                        nn = fmaf(d2s1[q], w1, nn);
+                    }
                     if (MODE == kScore) midMax.cur = fmaxf(midMax.cur, w0 * W1v[q]);
                     if (MODE == kDump && inD && active) w0d[(int64_t)D * ROWS + k * R + q] = w0;
                     smax = fmaxf(smax, fmaxf(w0, y));

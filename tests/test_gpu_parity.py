@@ -89,6 +89,52 @@ def test_pair_matrices(engine, L, n):
         assert err.max() < TOL_NATS, (err.max(), np.unravel_index(np.argmax(np.where(live, np.abs(gl - ol), 0)), gl.shape))
 
 
+def test_dump_matrices_equal_oracle(engine):
+    """The matrices of the post-gate step evaluate every cell in the
+    reference's order of operations: each live cell of w0 and W1 (within e^-80
+    of the largest) equals the oracle's FP32 value exactly, under single-strip
+    and row-strip kernels, for a strong hit that rescales many times (the
+    medium fixture against the herpes profile, M = 1359) and for synthetic
+    profiles of 100-700 positions with embedded homologs."""
+    import gzip
+    from tests.common import GOLDEN, read_fasta
+    table = engine.config_table()
+    cases = []
+    with tempfile.NamedTemporaryFile("wb", suffix=".hmm", delete=False) as f:
+        f.write(gzip.open(os.path.join(GOLDEN, "herpes.hmm.gz")).read())
+        path = f.name
+    try:
+        hp = engine.read_hmm(path)[0]
+    finally:
+        os.unlink(path)
+    M, v, p, b = engine.get_profile(hp)
+    hc = D.decode(read_fasta(os.path.join(GOLDEN, "medium.fa"))[0][1][1500:2700])
+    cases.append((hp, O.Profile("herpes", M, v, p, b, []), hc, engine.add_sequences([hc])[0]))
+    for L, n in ((100, 700), (300, 1100), (700, 800)):
+        pid, prof, codes, sids = load_case(engine, 1200 + L, L, [n], homolog_every=1)
+        cases.append((pid, prof, codes[0], sids[0]))
+    try:
+        for pid, prof, c, s in cases:
+            (ow0m, ow0e), (oW1m, oW1e), _, _ = O.matrices(prof, c)
+            top = max(D.es_log(ow0m, ow0e).max(), D.es_log(oW1m, oW1e).max())
+            for idx, (G, R, strips, _) in enumerate(table):
+                if (G, R, strips) not in ((16, 8, False), (32, 6, False), (64, 16, False), (32, 6, True),
+                                          (64, 4, True), (64, 16, True)):
+                    continue
+                if not strips and G * R < prof.M + 1:
+                    continue
+                print(f"dump exact M={prof.M} n={len(c)} config {G},{R},{strips}", flush=True)
+                engine.set_option("config", idx)
+                (gw0m, gw0e), (gW1m, gW1e) = engine.pair_matrices(pid, s, len(c))
+                for gm, ge, om, oe in ((gw0m, gw0e, ow0m, ow0e), (gW1m, gW1e, oW1m, oW1e)):
+                    live = D.es_log(om, oe) > top - 80.0
+                    assert live.sum() > 100
+                    bad = live & ((gm != om) | (ge != oe))
+                    assert not bad.any(), (G, R, strips, int(bad.sum()), np.argwhere(bad)[:3])
+    finally:
+        engine.set_option("config", -1)
+
+
 def reported(hits):
     """Hits that become records: the reference skips empty alignments at emit
     (dummest-search.hh:36-41)."""

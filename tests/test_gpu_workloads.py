@@ -105,6 +105,36 @@ def test_uneven_reads_batched_plan(reads_case):
     c["mid_scores"] = (mm, me)
 
 
+def test_strong_hits_every_strip_configuration(reads_case):
+    """Homologous reads of the profiles longer than 512 rows (strong hits, ln
+    ratio above 100) under every row-strip kernel: each strip's forward
+    starts a new anchor offset far below the hit found in an earlier strip
+    (158 nats sits 128 binary orders above it in this workload), which once
+    let a clamped running maximum replace the true one."""
+    c = reads_case
+    e, w = c["e"], c["w"]
+    ks = [k for k in range(len(c["prof"]))
+          if int(w["Ms"][w["prof"][k]]) > 512 and w["homolog"][w["seq"][k]] and w["src"][w["seq"][k]] == w["prof"][k]]
+    assert len(ks) >= 3
+    prof, seq = c["prof"][ks], c["seq"][ks]
+    mid = []
+    for k in ks:
+        r = O.search(c["profs"][int(w["prof"][k])], c["codes"][int(w["seq"][k])], -2.0, need_align=False)
+        mid.append(ln(*r.global_best))
+    assert max(mid) > 100
+    table = e.config_table()
+    try:
+        for idx, (G, R, strips, _) in enumerate(table):
+            if not strips:
+                continue
+            e.set_option("config", idx)
+            mm, me, _, _ = e.fwdbwd_scores(prof, seq, want_bwd=False)
+            for j in range(len(ks)):
+                assert abs(ln(mm[j], me[j]) - mid[j]) < TOL_NATS, (G, R, ks[j], ln(mm[j], me[j]), mid[j])
+    finally:
+        e.set_option("config", -1)
+
+
 def test_uneven_reads_hits(reads_case):
     """Every pair past the gate (best anchored score >= e^20, about the
     threshold an E-value of 1e-3 sets for these sizes): hits identical to the
