@@ -72,6 +72,8 @@ def _declare(L):
     L.or_fwdbwd_batch_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), _p(u8), _p(ctypes.c_int64), _p(i), i,
                                       _p(f), _p(f), i]
     L.or_gumbel.argtypes = [_p(d), i, d, _p(d)]
+    L.or_set_mutation.argtypes = [i]
+    L.or_set_mutation.restype = None
     L.or_calib_sequence_f32.argtypes = [_p(f), _p(d), i, i, i, ctypes.c_char_p]
     for name in ("add", "badd", "mul", "bmul"):
         getattr(L, "or_es_" + name).argtypes = [f, i, f, i, _p(f), _p(i)]
@@ -356,6 +358,15 @@ def ref_cache_read(path: str):
     L.ref_cache_read(path.encode(), n, keys, stride, _ptr(vals, ctypes.c_double))
     raw = keys.raw
     return {raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode(): vals[i].copy() for i in range(n)}
+
+
+def set_mutation(m: int) -> None:
+    """Plant a known mistake in the oracle's DP (0 = none), for tests that
+    show their check fails on it: 1 backward 1-nt/2-nt insertion scales
+    swapped, 2 forward match emission read one column late, 3 filter match
+    step scaled as two letters, 4 backward codon-insertion background read one
+    column late."""
+    lib().or_set_mutation(int(m))
 
 
 def decode(dna: str | bytes) -> np.ndarray:

@@ -67,6 +67,12 @@ struct FtzScope {
     ~FtzScope() { _mm_setcsr(old); }
 };
 
+// Planted mistakes for the tests that show a check can fail (0 = none):
+// 1 backward 1-nt/2-nt insertion scales swapped, 2 forward match emission read
+// one column late, 3 filter match step scaled as two letters, 4 backward codon
+// insertion background read one column late. Set only by or_set_mutation.
+static int g_mutation = 0;
+
 // dummest-score.hh:62-67
 static inline ES normalizeScore(double m, int e) {
     if (m == 0.0) return ES{0.0f, 0};
@@ -701,7 +707,7 @@ static F filterScore(const Prof<F> &P, const uint8_t *codes, int n, const Rates<
     for (int i = P.M; i >= 0; i--) {
         const F *pr = P.par(i);
         const F *em = P.emis(i);
-        const F enterC = pr[10] * sC, d0 = pr[6], d1 = pr[7] * s2, d2 = pr[8] * s1;
+        const F enterC = pr[10] * (g_mutation == 3 ? s2 : sC), d0 = pr[6], d1 = pr[7] * s2, d2 = pr[8] * s1;
         const F a0 = pr[0] * sC, a1 = pr[1] * s1, a2 = pr[2] * s2, b0 = pr[3] * sC;
         const F eps = pr[9], rsc = scale;
         F Z[4] = {0, 0, 0, 0};
@@ -968,7 +974,9 @@ static void search(Work<F> &Wk, std::vector<Sim> &out, const Prof<F> &P, const u
         const F *pr = P.par(i);
         const F *em = P.emis(i);
         const F enterC = pr[10] * sC, d0 = pr[6], d1 = pr[7] * s2, d2 = pr[8] * s1;
-        const F a0 = pr[0] * sC, a1 = pr[1] * s1, a2 = pr[2] * s2, b0 = pr[3] * sC;
+        const F a0 = pr[0] * sC, a1 = pr[1] * (g_mutation == 1 ? s2 : s1), a2 = pr[2] * (g_mutation == 1 ? s1 : s2),
+                b0 = pr[3] * sC;
+        const int goff = (g_mutation == 4) ? 1 : 0;
         F Z[4] = {0, 0, 0, 0};
         const F eps = pr[9];
         F rsc = scale * rowScaleOf(Wk.bcnt[i]);
@@ -979,7 +987,7 @@ static void search(Work<F> &Wk, std::vector<Sim> &out, const Prof<F> &P, const u
         for (int j = W - 1; j >= 0; j--) {
             int r0 = j & 3, r3 = (j + 3) & 3;
             F e = em[code(j + 1)];
-            F g = bgAt(j + 1);
+            F g = bgAt(j + 1 + goff);
             F nAt = *Wk.w1(i + 1, j);
             F w = n3 * e * enterC + Ynext[j] * d0 + n2 * d1 + n1 * d2 + Z[r3] * g * a0 + c2 * a2 + c1 * a1 +
                   N.suffixSeed[j] * rsc;
@@ -1042,11 +1050,12 @@ static void search(Work<F> &Wk, std::vector<Sim> &out, const Prof<F> &P, const u
         F rowMax = 0;
         F Z0[4] = {0, 0, 0, 0}, Z1[4] = {0, 0, 0, 0}, Z2[4] = {0, 0, 0, 0};
         const bool hasNext = (i + 1 <= M);
+        const int eoff = (g_mutation == 2) ? 1 : 0;
         F ws[3] = {0, 0, 0};
         for (int j = 0; j < W; j++) {
             F w1v = ws[0], w2v = ws[1], w3v = ws[2];
             int r0 = j & 3, r3 = (j - 3) & 3;
-            F e = em[code(j - 2)];
+            F e = em[code(j - 2 + eoff)];
             F g = bgAt(j - 2);
             F Xij = enterC * e * w3v;
             if (needAlign) {
@@ -1463,6 +1472,8 @@ static void totalsImpl(int M, const F *values, const F *params, const F *bg, con
 }
 
 extern "C" {
+
+void or_set_mutation(int m) { g_mutation = m; }
 
 int or_finalize_f32(int M, float *values, float *params, float *bg, const double *opts, int bgType,
                     const float *tantan, double thr) {
