@@ -69,8 +69,9 @@ def _declare(L):
     L.dm_engine_error.restype = ctypes.c_char_p
     L.dm_set_rates.argtypes = [vp, _P(d)]
     L.dm_read_hmm.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, _P(i32), _P(i32)]
-    L.dm_parse_hmm_text.argtypes = [ctypes.c_char_p, ctypes.c_int, _P(d), d, i32, _P(i32), _P(f), _P(f), _P(f),
-                                    _P(i32), _P(i32)]
+    L.dm_parse_hmm_text.argtypes = [ctypes.c_char_p, ctypes.c_int, _P(d), d, d, ctypes.c_int, i32, _P(i32), _P(f),
+                                    _P(f), _P(f), _P(i32), _P(i32)]
+    L.dm_set_profile_filter.argtypes = [vp, d, ctypes.c_int]
     L.dm_add_raw_profiles.argtypes = [vp, i32, _P(i32), _P(f), ctypes.c_int, _P(i32)]
     L.dm_add_profile.argtypes = [vp, i32, _P(f), _P(f), _P(f), _P(i32)]
     L.dm_set_profile.argtypes = [vp, i32, i32, _P(f), _P(f), _P(f)]
@@ -148,17 +149,19 @@ def decode(dna) -> np.ndarray:
 
 
 def parse_hmm_text(text: str, index: int = 0, bg_type: str = "G", rates=DEFAULT_RATES, mask_threshold=None,
-                   with_masked=False):
+                   with_masked=False, filter_std_dev=0.0, keep_nonvarying=False):
     """Host-side readProfiles + finalizeProfile of one profile of an HMMER3/f text
     (no device needed): returns (M, values, params, bg, count), plus the number
     of masked positions when with_masked. mask_threshold turns on profile
-    masking (-m 1)."""
+    masking (-m 1); filter_std_dev the Gaussian letter filter (-d, with
+    keep_nonvarying -D)."""
     r = np.array(rates, np.float64)
     M, cnt, nm = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     t = text.encode()
     thr = -1.0 if mask_threshold is None else float(mask_threshold)
-    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), thr, index, ctypes.byref(M), None, None,
-                                 None, ctypes.byref(cnt), None)
+    flt = (float(filter_std_dev), int(bool(keep_nonvarying)))
+    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), thr, *flt, index, ctypes.byref(M), None,
+                                 None, None, ctypes.byref(cnt), None)
     if rc:
         raise DummestError(f"dm_parse_hmm_text failed ({rc})")
     m = M.value
@@ -166,8 +169,8 @@ def parse_hmm_text(text: str, index: int = 0, bg_type: str = "G", rates=DEFAULT_
     p = np.zeros((m + 2, 11), np.float32)
     b = np.zeros(WIDTH, np.float32)
     f = ctypes.c_float
-    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), thr, index, ctypes.byref(M), _ptr(v, f),
-                                 _ptr(p, f), _ptr(b, f), ctypes.byref(cnt), ctypes.byref(nm))
+    rc = lib().dm_parse_hmm_text(t, ord(bg_type), _ptr(r, ctypes.c_double), thr, *flt, index, ctypes.byref(M),
+                                 _ptr(v, f), _ptr(p, f), _ptr(b, f), ctypes.byref(cnt), ctypes.byref(nm))
     if rc:
         raise DummestError(f"dm_parse_hmm_text failed ({rc})")
     if with_masked:
@@ -431,6 +434,12 @@ class Engine:
     def set_masking(self, mask: int, threshold: float = 0.5):
         """-m (1 profiles, 2 sequences, 3 both) and --tantan-threshold."""
         self._check(lib().dm_set_masking(self._h, int(mask), float(threshold)), "dm_set_masking")
+
+    def set_profile_filter(self, std_dev: float = 0.0, keep_nonvarying: bool = False):
+        """Gaussian letter filter (-d, or -D with keep_nonvarying) of the profiles
+        read or added afterwards; 0 turns it off."""
+        self._check(lib().dm_set_profile_filter(self._h, float(std_dev), int(bool(keep_nonvarying))),
+                    "dm_set_profile_filter")
 
     def profile_masked(self, pid: int) -> int:
         """Positions of profile pid masked by tantan."""

@@ -24,8 +24,8 @@
 // file and keys as the reference's, so a profile calibrated once is looked up
 // instead of calibrated again (unless -S); only the misses go to the GPU.
 //
-// Not available here (the command fails with a message rather than giving
-// different results): the Gaussian profile filter (-d/-D).
+// The Gaussian profile filter (-d/-D) is applied on the host when the
+// profiles are read (filterLetterProbabilities, dummest-profile.hh:64-103).
 #include <getopt.h>
 #include <unistd.h>
 
@@ -330,6 +330,7 @@ int main(int argc, char *argv[]) {
         o.threads = hc ? (int)hc : 1;
     }
     double filterStdDev = 0;
+    bool keepNonvarying = false;
     const char help[] =
         "usage: dummest-amd profiles.hmm [sequences.fa]\n"
         "\n"
@@ -345,6 +346,9 @@ int main(int argc, char *argv[]) {
         "  -N N, --total-length N  override total search space size (default: sum of input lengths)\n"
         "  -s S, --strand S  accepted for compatibility; each record is searched as given\n"
         "  -m M, --mask M    mask simple regions of: 0=neither, 1=profile, 2=sequence, 3=both (default: 3)\n"
+        "  -d D, --dev D     standard deviation of a Gaussian high-pass filter on position-specific\n"
+        "                    letter probabilities\n"
+        "  -D D, --Dev D     same as above, but keep the non-varying component\n"
         "  -t T, --trials T  random sequences for calibration (default: 1000)\n"
         "  -l L, --length L  length of each random sequence (default: 5000)\n"
         "  -b B, --border B  border added to each random sequence (default: 100)\n"
@@ -421,8 +425,10 @@ int main(int argc, char *argv[]) {
             break;
         case 'd':
         case 'D':
+            // too low: a coarse discretized Gaussian; too high: slow (dummest.cc:185-195)
             filterStdDev = std::strtod(optarg, nullptr);
             if (filterStdDev < 2 || filterStdDev > 1000) return badOpt();
+            keepNonvarying = (c == 'D');
             break;
         case 't':
             o.trials = intFromText(optarg);
@@ -477,14 +483,15 @@ int main(int argc, char *argv[]) {
         std::cerr << help;
         return 1;
     }
-    if (filterStdDev > 0) return die("the Gaussian profile filter (-d/-D) is not available in this build");
+    if (filterStdDev > 0) o.mask &= 2;  // filtering turns off profile masking (dummest.cc:284-285)
     if (o.border > INT32_MAX - 64 - o.length) return die("sequence + border is too big");
 
     // ------------------------------------------------------------- profiles
     std::vector<HostProfile> profiles;
     {
         std::string msg;
-        if (!readHmmFile(argv[optind], o.bgType, o.rates, profiles, msg, (o.mask & 1) ? (float)o.tantan : -1.f))
+        if (!readHmmFile(argv[optind], o.bgType, o.rates, profiles, msg, (o.mask & 1) ? (float)o.tantan : -1.f,
+                         filterStdDev, keepNonvarying))
             return die(msg.empty() ? "can't read the profile data" : msg);
     }
     dm_engine *E = nullptr;
@@ -501,6 +508,8 @@ int main(int argc, char *argv[]) {
 
     std::cout << "# DUMMEST-AMD 1\n";
     std::cout << "# Bytes per floating-point number: 4\n";
+    if (filterStdDev > 0)
+        std::cout << "# Filtering position-specific letter probabilities: std dev " << filterStdDev << "\n";
     std::cout << "# Background letter probabilities: "
               << (o.bgType == 'A' ? "arithmetic mean" : o.bgType == 'G' ? "geometric mean" : "median")
               << " of foreground probabilities\n";

@@ -26,13 +26,16 @@ struct HostProfile {
 
 // readProfiles + finalizeProfile (dummest-profile.hh:105-329). maskThr >= 0
 // turns on profile masking (-m 1) at that tantan threshold; negative = off.
+// filterSd > 0 applies the Gaussian letter filter of -d (keepNonvarying: -D)
+// instead of masking (dummest-profile.hh:64-103, 119-121).
 bool readHmmFile(const char *path, int bgType, const double rates[6], std::vector<HostProfile> &out,
-                 std::string &msg, float maskThr = -1.f);
+                 std::string &msg, float maskThr = -1.f, double filterSd = 0, bool keepNonvarying = false);
 bool readHmmText(const std::string &text, int bgType, const double rates[6], std::vector<HostProfile> &out,
-                 std::string &msg, float maskThr = -1.f);
+                 std::string &msg, float maskThr = -1.f, double filterSd = 0, bool keepNonvarying = false);
 
 // finalizeProfile on values as read (probabilities, (M+1) x 29, reader order).
-bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr = -1.f);
+bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr = -1.f,
+                 double filterSd = 0, bool keepNonvarying = false);
 
 // tantan as DUMMEST calls it (tantan-wrapper.hh:13-64, gap-free HMM): the
 // likelihood ratios (BLOSUM62 at lambda 0.324 for proteins, +1/-1 for DNA

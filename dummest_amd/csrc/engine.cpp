@@ -86,6 +86,8 @@ struct dm_engine {
     double lastTantanMs = 0;  // device time of the last dm_tantan_dna kernel
     int maskBits = 0;       // -m: 1 = profiles, 2 = sequences
     float maskThr = 0.5f;   // --tantan-threshold (TANTAN_MASK_THRESHOLD, a Float)
+    double filterSd = 0;    // -d / -D: Gaussian letter filter of profiles read afterwards (0 = off)
+    bool keepNonvarying = false;
     DevBuf dTanBases, dTanProbs, dTanScales, dTanMeta, dTanB2f;
     // profile builder's Baum-Welch (dm_bw_prepare / dm_bw_estep)
     int bwL = -1, bwA = 0, bwN = 0;
@@ -264,9 +266,16 @@ static void layoutNullScratch(Chunk &C) {
     C.nullScratch = std::max<int64_t>(off, 64);
 }
 
-// Build the queue for caller pairs [sorted positions b, e).
+// Build the queue for caller pairs [sorted positions b, e). Sequence lengths
+// and code offsets come from the engine's store, or from lens / offs (indexed
+// by the pairs' sequence numbers) when given.
 static void buildChunk(const dm_engine *E, int mode, const std::vector<int64_t> &sorted, const std::vector<int> &cfgOf,
-                       const int32_t *prof, const int32_t *seq, size_t b, size_t en, Chunk &C) {
+                       const int32_t *prof, const int32_t *seq, size_t b, size_t en, Chunk &C,
+                       const int32_t *lens = nullptr, const int64_t *offs = nullptr) {
+    if (!lens) {
+        lens = E->seqLen.data();
+        offs = E->seqOff.data();
+    }
     const int ncfg = (int)E->cfgG.size();
     C = Chunk();
     C.cfgFirstUnit.assign(ncfg, 0);
@@ -294,7 +303,7 @@ static void buildChunk(const dm_engine *E, int mode, const std::vector<int64_t> 
         auto &us = unitsByCfg[cfg];
         const int G = E->cfgG[cfg];
         const int WG = dp_wg(G, E->cfgR[cfg], E->cfgStrips[cfg] != 0, mode);
-        auto lenOf = [&](const std::vector<int64_t> &x) { return x.empty() ? -1 : E->seqLen[seq[x[0]]]; };
+        auto lenOf = [&](const std::vector<int64_t> &x) { return x.empty() ? -1 : lens[seq[x[0]]]; };
         if (WG == 1) {
             // longest first (units hold length-sorted pairs, so pair 0 is the longest)
             std::stable_sort(us.begin(), us.end(), [&](const std::vector<int64_t> &x, const std::vector<int64_t> &y) {
@@ -342,8 +351,8 @@ static void buildChunk(const dm_engine *E, int mode, const std::vector<int64_t> 
                 PairInfo pi{};
                 pi.prof = prof[pidx];
                 pi.seq = seq[pidx];
-                pi.n = E->seqLen[pi.seq];
-                pi.seqOff = E->seqOff[pi.seq];
+                pi.n = lens[pi.seq];
+                pi.seqOff = offs[pi.seq];
                 pi.recOff = C.records + kRPad;
                 C.records += recSpan(pi.n);
                 U.maxN = std::max(U.maxN, pi.n);
@@ -917,7 +926,8 @@ int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, 
     if (!e || !path) return DM_EINVAL;
     std::vector<HostProfile> ps;
     std::string msg;
-    if (!readHmmFile(path, bg_type, e->rates, ps, msg, e->profileMaskThr())) return fail(e, DM_EFORMAT, msg);
+    if (!readHmmFile(path, bg_type, e->rates, ps, msg, e->profileMaskThr(), e->filterSd, e->keepNonvarying))
+        return fail(e, DM_EFORMAT, msg);
     if (first_id) *first_id = (int32_t)e->profiles.size();
     if (count) *count = (int32_t)ps.size();
     for (auto &p : ps) e->profiles.push_back(std::move(p));
@@ -926,12 +936,15 @@ int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, 
     return DM_OK;
 }
 
-int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], double mask_threshold, int32_t index,
-                      int32_t *M, float *values, float *params, float *bg, int32_t *count, int32_t *masked) {
+int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], double mask_threshold,
+                      double filter_std_dev, int keep_nonvarying, int32_t index, int32_t *M, float *values,
+                      float *params, float *bg, int32_t *count, int32_t *masked) {
     if (!text || !rates || mask_threshold > 1) return DM_EINVAL;
+    if (filter_std_dev != 0 && !(filter_std_dev >= 2 && filter_std_dev <= 1000)) return DM_EINVAL;
     std::vector<HostProfile> ps;
     std::string msg;
-    if (!readHmmText(text, bg_type, rates, ps, msg, mask_threshold >= 0 ? (float)mask_threshold : -1.f))
+    if (!readHmmText(text, bg_type, rates, ps, msg, mask_threshold >= 0 ? (float)mask_threshold : -1.f,
+                     filter_std_dev, keep_nonvarying != 0))
         return DM_EFORMAT;
     if (count) *count = (int32_t)ps.size();
     if (index < 0 || index >= (int)ps.size()) return ps.empty() ? DM_EFORMAT : DM_OK;
@@ -962,7 +975,8 @@ int dm_add_raw_profiles(dm_engine *e, int32_t nprof, const int32_t *M, const flo
             p.consensus[i] = (char)(std::max_element(r, r + 20) - r);
         }
         std::string msg;
-        if (!finalizeRaw(p, bg_type, e->rates, msg, e->profileMaskThr())) return fail(e, DM_EFORMAT, msg);
+        if (!finalizeRaw(p, bg_type, e->rates, msg, e->profileMaskThr(), e->filterSd, e->keepNonvarying))
+            return fail(e, DM_EFORMAT, msg);
         e->profiles.push_back(std::move(p));
     }
     e->profDirty = true;
@@ -1040,6 +1054,15 @@ int dm_set_masking(dm_engine *e, int mask, double threshold) {
     if (!e || mask < 0 || mask > 3 || !(threshold >= 0 && threshold <= 1)) return DM_EINVAL;
     e->maskBits = mask;
     e->maskThr = (float)threshold;
+    return DM_OK;
+}
+
+int dm_set_profile_filter(dm_engine *e, double std_dev, int keep_nonvarying) {
+    if (!e) return DM_EINVAL;
+    if (std_dev != 0 && !(std_dev >= 2 && std_dev <= 1000))
+        return fail(e, DM_EINVAL, "the filter's standard deviation must lie in [2, 1000] (dummest.cc:185-195)");
+    e->filterSd = std_dev;
+    e->keepNonvarying = keep_nonvarying != 0;
     return DM_OK;
 }
 

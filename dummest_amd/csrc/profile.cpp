@@ -5,7 +5,8 @@
 // as probabilities exp(-x) ("*" = 0, probFromText dummest-profile.hh:14-19).
 // Row r holds node r's transitions and node r+1's emissions.
 // Transform: finalizeProfile (dummest-profile.hh:105-233), with the profile
-// masking of -m 1 (tantan on the consensus) when a threshold is given. The
+// masking of -m 1 (tantan on the consensus) when a threshold is given, or the
+// Gaussian letter filter of -d/-D when a standard deviation is. The
 // arithmetic keeps the reference's Float/double mix (Float = float for the
 // FP32 engine): rate products are formed in float, the per-row Params in
 // double and narrowed to float.
@@ -60,13 +61,69 @@ double backgroundMean(const float *v, int length, int step, int type, const floa
     return med[n / 2];
 }
 
-bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr) {
+// The Gaussian letter filter of -d/-D (filterLetterProbabilities,
+// dummest-profile.hh:64-103): for each letter, the log probability less the
+// position's mean log probability over the 20 letters is a signal along the
+// profile; its Gaussian-smoothed version (standard deviation sd positions,
+// tails cut at 8 sd, the profile taken as circular) is subtracted, which
+// keeps the part that varies from position to position. keepNonvarying (-D)
+// adds the signal's mean back. Each position is then renormalized. Doubles
+// throughout, results narrowed to float, in the reference's order of
+// operations (its profiles must come out bit for bit); the Gaussian weights
+// are tabulated once per call, the same doubles the reference recomputes.
+void gaussianLetterFilter(float *lp, int length, int step, double sd, bool keepNonvarying) {
+    const int A = 20;
+    const double inv2var = 0.5 / (sd * sd);
+    const double norm = sd * 2.5066282746310005;  // sd * sqrt(2 pi)
+    const int reach = (int)std::ceil(sd * 8);
+    std::vector<double> gw(2 * (size_t)reach + 1);
+    for (int d = -reach; d <= reach; ++d) gw[d + reach] = std::exp(-1.0 * d * d * inv2var);
+    std::vector<double> posMeanLog(length), sig(length);
+    for (int i = 0; i < length; ++i) {
+        double prod = 1;
+        for (int k = 0; k < A; ++k) prod *= lp[(size_t)i * step + k];
+        posMeanLog[i] = std::log(prod) / A;
+    }
+    for (int k = 0; k < A; ++k) {
+        for (int i = 0; i < length; ++i) sig[i] = std::log((double)lp[(size_t)i * step + k]) - posMeanLog[i];
+        double level = 0;
+        if (keepNonvarying) {
+            for (int i = 0; i < length; ++i) level += sig[i];
+            level /= length;
+        }
+        for (int i = 0; i < length; ++i) {
+            double smooth = 0;
+            for (int d = -reach; d <= reach; ++d) {
+                int x = (i + d) % length;
+                if (x < 0) x += length;
+                smooth += sig[x] * gw[d + reach];
+            }
+            smooth /= norm;
+            lp[(size_t)i * step + k] = (float)std::exp(sig[i] - smooth + level);
+        }
+    }
+    for (int i = 0; i < length; ++i) {
+        float *row = lp + (size_t)i * step;
+        double total = 0;
+        for (int k = 0; k < A; ++k) total += row[k];
+        for (int k = 0; k < A; ++k) row[k] = (float)(row[k] / total);
+    }
+}
+
+bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr, double filterSd,
+                  bool keepNonvarying) {
     const float ins1 = (float)rates[0], ins2 = (float)rates[1], del1 = (float)rates[2], del2 = (float)rates[3];
     const float stop = (float)rates[4], bgStop = (float)rates[5];
     const int M = p.M;
     float *v = p.values.data();
     float *end = v + (size_t)kWidth * M;
     if (end[3] <= 0) end[3] = (float)backgroundMean(v + kWidth + 3, M - 1, kWidth, 'G');
+    // The Gaussian filter takes the place of profile masking
+    // (dummest-profile.hh:119-126; dummest.cc also drops -m's profile bit).
+    if (filterSd > 0) {
+        gaussianLetterFilter(v + 4, M, kWidth, filterSd, keepNonvarying);
+        maskThr = -1.f;
+    }
     // Profile masking (-m 1, dummest-profile.hh:122-125): tantan repeat
     // probabilities of the consensus (the first most probable letter of each
     // node); positions at or above the threshold leave the background means
@@ -151,7 +208,7 @@ bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string
 }  // namespace
 
 bool readHmmText(const std::string &text, int bgType, const double rates[6], std::vector<HostProfile> &out,
-                 std::string &msg, float maskThr) {
+                 std::string &msg, float maskThr, double filterSd, bool keepNonvarying) {
     std::istringstream in(text);
     std::string line;
     int state = 0;
@@ -188,7 +245,7 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
                 if (cur.M < 2) return bad("profile shorter than 2 positions");
                 vals.insert(vals.end(), kWidth - 4, 0.f);
                 cur.values = vals;
-                if (!finalizeImpl(cur, bgType, rates, msg, maskThr)) return false;
+                if (!finalizeImpl(cur, bgType, rates, msg, maskThr, filterSd, keepNonvarying)) return false;
                 out.push_back(std::move(cur));
                 cur = HostProfile();
                 vals.clear();
@@ -219,12 +276,13 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
     return true;
 }
 
-bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr) {
-    return finalizeImpl(p, bgType, rates, msg, maskThr);
+bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr, double filterSd,
+                 bool keepNonvarying) {
+    return finalizeImpl(p, bgType, rates, msg, maskThr, filterSd, keepNonvarying);
 }
 
 bool readHmmFile(const char *path, int bgType, const double rates[6], std::vector<HostProfile> &out, std::string &msg,
-                 float maskThr) {
+                 float maskThr, double filterSd, bool keepNonvarying) {
     std::ifstream f(path, std::ios::binary);
     if (!f) {
         msg = std::string("can't open file: ") + path;
@@ -232,7 +290,7 @@ bool readHmmFile(const char *path, int bgType, const double rates[6], std::vecto
     }
     std::stringstream ss;
     ss << f.rdbuf();
-    return readHmmText(ss.str(), bgType, rates, out, msg, maskThr);
+    return readHmmText(ss.str(), bgType, rates, out, msg, maskThr, filterSd, keepNonvarying);
 }
 
 // Codon table in TCAG order; non-ACGT letters make the codon '?'.

@@ -50,9 +50,11 @@ int dm_read_hmm(dm_engine *e, const char *path, int bg_type, int32_t *first_id, 
  * writes profile number `index` (M, values (M+1)x29, params (M+2)x11, bg 29;
  * pass NULL arrays to query M), the number of profiles in the text and the
  * positions masked. mask_threshold >= 0 turns on profile masking (-m 1) at that
- * tantan threshold; negative leaves it off. */
-int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], double mask_threshold, int32_t index,
-                      int32_t *M, float *values, float *params, float *bg, int32_t *count, int32_t *masked);
+ * tantan threshold; negative leaves it off. filter_std_dev > 0 applies the
+ * Gaussian letter filter (as dm_set_profile_filter) instead of masking. */
+int dm_parse_hmm_text(const char *text, int bg_type, const double rates[6], double mask_threshold,
+                      double filter_std_dev, int keep_nonvarying, int32_t index, int32_t *M, float *values,
+                      float *params, float *bg, int32_t *count, int32_t *masked);
 
 /* Add profiles from values as the reader stores them (probabilities, (M+1)x29
  * per profile, concatenated: transitions m->i i->i m->d d->d then node i+1's
@@ -96,6 +98,15 @@ int dm_clear_sequences(dm_engine *e);
  * dm_add_dna is masked (makeMaskedSequence, dummest-sequence.hh:206-236).
  * threshold in [0, 1] (default 0.5). */
 int dm_set_masking(dm_engine *e, int mask, double threshold);
+/* Gaussian letter filter of the profiles read or added afterwards (-d and
+ * -D, dummest.cc:185-195; filterLetterProbabilities, dummest-profile.hh:64-103,
+ * applied by finalizeProfile at :119-121): each letter's log probability less
+ * the position's mean log probability, minus its Gaussian smoothing along the
+ * profile (std_dev positions, circular), renormalized per position;
+ * keep_nonvarying (-D) keeps the mean level. std_dev in [2, 1000], or 0 to turn
+ * it off. While it is on, profile masking (mask bit 1) is not applied, as the
+ * reference turns it off (dummest.cc:284-285). */
+int dm_set_profile_filter(dm_engine *e, double std_dev, int keep_nonvarying);
 /* Profile positions masked by tantan ("# Positions masked by tantan", dummest.cc:389-395);
  * -1 for an invalid id. */
 int dm_profile_masked(const dm_engine *e, int32_t id);

@@ -49,8 +49,8 @@ def _p(t):
 
 def _declare(L):
     f, d, i, u8, c = ctypes.c_float, ctypes.c_double, ctypes.c_int, ctypes.c_uint8, ctypes.c_char
-    L.or_finalize_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), i, _p(f), d]
-    L.or_finalize_f64.argtypes = [i, _p(d), _p(d), _p(d), _p(d), i, _p(f), d]
+    L.or_finalize_f32.argtypes = [i, _p(f), _p(f), _p(f), _p(d), i, _p(f), d, d, i]
+    L.or_finalize_f64.argtypes = [i, _p(d), _p(d), _p(d), _p(d), i, _p(f), d, d, i]
     L.or_tantan.argtypes = [_p(u8), i, i, _p(f)]
     L.or_blosum62.argtypes = [_p(ctypes.c_int8)]
     L.or_mask_dna.argtypes = [ctypes.c_char_p, i, d, ctypes.c_char_p, _p(f)]
@@ -199,9 +199,12 @@ class Profile:
     masked: int = 0      # positions masked by tantan (profile masking)
 
 
-def finalize(rp: RawProfile, dtype=np.float32, opts=DEFAULT_OPTS, bg_type="G", mask_threshold=None) -> Profile:
+def finalize(rp: RawProfile, dtype=np.float32, opts=DEFAULT_OPTS, bg_type="G", mask_threshold=None,
+             filter_std_dev=0.0, keep_nonvarying=False) -> Profile:
     """finalizeProfile (dummest-profile.hh:105-233); mask_threshold turns on
-    the profile masking of -m 1 (tantan on the consensus, :122-125)."""
+    the profile masking of -m 1 (tantan on the consensus, :122-125),
+    filter_std_dev the Gaussian letter filter of -d / -D (:64-103, 119-121),
+    which takes masking's place."""
     dt = np.dtype(dtype)
     vals = np.ascontiguousarray(rp.raw.astype(dt))
     params = np.zeros((rp.length + 2, 11), dtype=dt)
@@ -210,11 +213,11 @@ def finalize(rp: RawProfile, dtype=np.float32, opts=DEFAULT_OPTS, bg_type="G", m
     fn = lib().or_finalize_f32 if dt == np.float32 else lib().or_finalize_f64
     t = _ct(dt)
     rep = None
-    if mask_threshold is not None:
+    if mask_threshold is not None and not filter_std_dev:
         rep = tantan(np.array(rp.consensus, np.uint8), True)
     rp_ptr = _ptr(rep, ctypes.c_float) if rep is not None else None
     ok = fn(rp.length, _ptr(vals, t), _ptr(params, t), _ptr(bg, t), _ptr(o, ctypes.c_double), ord(bg_type),
-            rp_ptr, float(mask_threshold or 0.0))
+            rp_ptr, float(mask_threshold or 0.0), float(filter_std_dev), int(bool(keep_nonvarying)))
     if not ok:
         raise ValueError("finalizeProfile rejected the profile")
     prof = Profile(rp.name, rp.length, vals, params, bg, list(rp.consensus))

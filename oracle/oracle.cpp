@@ -313,9 +313,51 @@ static double myMean(const F *values, int length, int step, int meanType, const 
     return med[n / 2];
 }
 
+// filterLetterProbabilities, dummest-profile.hh:64-103 (with normalize and
+// meanOfLogs, :20-35, and mean, dummest-core.hh:335-340), for the 20 letters
+// of `length` rows `step` apart. Doubles, results narrowed to F.
+template <class F>
+static void filterLetterProbabilities(F *letterProbs, int length, int step, double stdDev, bool keepNonvaryingTerm) {
+    const double sqrt2pi = 2.5066282746310005;
+    const double inv2var = 0.5 / (stdDev * stdDev);
+    const int gaussianLimit = (int)std::ceil(stdDev * 8);
+    const int alphabetSize = 20;
+    std::vector<double> meanLogProbs(length), values(length);
+    for (int i = 0; i < length; ++i) {
+        double m = 1;
+        for (int k = 0; k < alphabetSize; ++k) m *= letterProbs[i * step + k];
+        meanLogProbs[i] = std::log(m) / alphabetSize;
+    }
+    for (int k = 0; k < alphabetSize; ++k) {
+        for (int i = 0; i < length; ++i) values[i] = std::log((double)letterProbs[i * step + k]) - meanLogProbs[i];
+        double addItBack = 0.0;
+        if (keepNonvaryingTerm) {
+            double t = 0;
+            for (int i = 0; i < length; ++i) t += values[i];
+            addItBack = t / length;
+        }
+        for (int i = 0; i < length; ++i) {
+            double sum = 0;
+            for (int j = -gaussianLimit; j <= gaussianLimit; ++j) {
+                int x = (i + j) % length;  // circular profile
+                if (x < 0) x += length;
+                sum += values[x] * std::exp(-1.0 * j * j * inv2var);
+            }
+            sum /= stdDev * sqrt2pi;
+            letterProbs[i * step + k] = (F)std::exp(values[i] - sum + addItBack);
+        }
+    }
+    for (int i = 0; i < length; ++i) {
+        F *x = letterProbs + i * step;
+        double total = 0;
+        for (int k = 0; k < alphabetSize; ++k) total += x[k];
+        for (int k = 0; k < alphabetSize; ++k) x[k] /= total;
+    }
+}
+
 template <class F>
 static int finalizeProfile(int M, F *values, F *params, F *bgOut, const Rates<F> &R, int bgType,
-                           const float *tantan = nullptr, F thr = 0) {
+                           const float *tantan = nullptr, F thr = 0, double filterSd = 0, bool keepNonvarying = false) {
     // tantan: repeat probabilities of the consensus (profile masking, -m 1),
     // computed after the final epsilon (dummest-profile.hh:116-126); positions
     // at or above thr leave the background means and emit like the background
@@ -323,6 +365,11 @@ static int finalizeProfile(int M, F *values, F *params, F *bgOut, const Rates<F>
     const int W = kWidth;
     F *end = values + W * M;
     if (end[3] <= 0) end[3] = (F)myMean<F>(values + W + 3, M - 1, W, 'G');
+    // dummest-profile.hh:119-126: the filter, when on, replaces profile masking
+    if (filterSd > 0) {
+        filterLetterProbabilities<F>(values + 4, M, W, filterSd, keepNonvarying);
+        tantan = nullptr;
+    }
     double sumOfMeans = 0;
     for (int k = 4; k < 24; ++k) {
         double mean = myMean<F>(values + k, M, W, bgType, tantan, thr);
@@ -1476,16 +1523,16 @@ extern "C" {
 void or_set_mutation(int m) { g_mutation = m; }
 
 int or_finalize_f32(int M, float *values, float *params, float *bg, const double *opts, int bgType,
-                    const float *tantan, double thr) {
+                    const float *tantan, double thr, double filterSd, int keepNonvarying) {
     Rates<float> R;
     R.init(opts);
-    return finalizeProfile<float>(M, values, params, bg, R, bgType, tantan, (float)thr);
+    return finalizeProfile<float>(M, values, params, bg, R, bgType, tantan, (float)thr, filterSd, keepNonvarying != 0);
 }
 int or_finalize_f64(int M, double *values, double *params, double *bg, const double *opts, int bgType,
-                    const float *tantan, double thr) {
+                    const float *tantan, double thr, double filterSd, int keepNonvarying) {
     Rates<double> R;
     R.init(opts);
-    return finalizeProfile<double>(M, values, params, bg, R, bgType, tantan, thr);
+    return finalizeProfile<double>(M, values, params, bg, R, bgType, tantan, thr, filterSd, keepNonvarying != 0);
 }
 
 void or_tantan(const uint8_t *seq, int n, int isProtein, float *out) { tantanProbs(seq, n, isProtein != 0, out); }

@@ -30,10 +30,15 @@ def test_bad_option_values_fail(args, tmp_path):
     assert "usage" in r.stderr or "bad option" in r.stderr.lower() or "invalid" in r.stderr.lower()
 
 
-def test_gaussian_filter_is_refused_not_ignored(tmp_path):
-    # the Gaussian profile filter is not restated: it must fail loudly
-    r = run("-d", "5", str(tmp_path / "x.hmm"))
-    assert r.returncode == 1 and "-d" in r.stderr
+@pytest.mark.parametrize("opt", ["-d", "-D"])
+def test_gaussian_filter_options(opt, tmp_path):
+    # standard deviations outside [2, 1000] are bad options (dummest.cc:185-195);
+    # a valid one passes option checking (the run then stops at the missing file)
+    for bad in ("1.9", "1001"):
+        r = run(opt, bad, str(tmp_path / "x.hmm"))
+        assert r.returncode == 1 and "can't open" not in r.stderr
+    r = run(opt, "5", str(tmp_path / "x.hmm"))
+    assert r.returncode == 1 and "can't open" in r.stderr
 
 
 @pytest.mark.parametrize("mask", ["0", "1", "2", "3"])

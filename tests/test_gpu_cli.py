@@ -153,11 +153,12 @@ def key(r):
             r["plen"], r["slen"], r["r1"], r["r2"])
 
 
-@pytest.mark.parametrize("mask", [0, 3])
-def test_cli_records_match_oracle(tmp_path, mask):
+@pytest.mark.parametrize("mask,filt", [(0, None), (3, None), (3, ("-d", 4.0)), (0, ("-D", 6.0))])
+def test_cli_records_match_oracle(tmp_path, mask, filt):
     """-m 0, and -m 3 (tantan on the profile and on every contig, masked
-    letters decoding as '?' and printing in lowercase)."""
-    print(f"cli end-to-end -m {mask}", flush=True)
+    letters decoding as '?' and printing in lowercase); the Gaussian letter
+    filter -d (which turns profile masking off, dummest.cc:284-285) and -D."""
+    print(f"cli end-to-end -m {mask} filter {filt}", flush=True)
     rng = np.random.default_rng(77)
     text, prof, dnas = synthetic_case(701, 120, [900, 1500, 700, 2000, 1200, 800], homolog_every=2)
     # a contig break ('X' is not a sequence letter) and IUPAC letters inside a homolog record
@@ -166,6 +167,11 @@ def test_cli_records_match_oracle(tmp_path, mask):
     # microsatellites (masked under -m 2) in homolog and non-homolog records
     for k, at in ((0, 450), (1, 200), (4, 600)):
         dnas[k] = dnas[k][:at] + "CA" * 30 + dnas[k][at + 60:]
+    extra = []
+    if filt:
+        extra = [filt[0], str(filt[1])]
+        mask &= 2  # the filter replaces profile masking
+        prof = O.finalize(O.parse_hmm(text)[0], np.float32, filter_std_dev=filt[1], keep_nonvarying=filt[0] == "-D")
     if mask & 1:
         prof = O.finalize(O.parse_hmm(text)[0], np.float32, mask_threshold=0.5)
     fasta = []
@@ -179,9 +185,13 @@ def test_cli_records_match_oracle(tmp_path, mask):
     (tmp_path / "p.hmm").write_text(text)
     (tmp_path / "s.fa").write_text(fasta_text)
     evalue = 1.0
-    r = subprocess.run([CLI, "-m", str(mask), "-t", str(TRIALS), "-l", str(LENGTH), "-b", str(BORDER), "-e", str(evalue),
-                        str(tmp_path / "p.hmm"), str(tmp_path / "s.fa")], capture_output=True, text=True, timeout=300)
+    mflag = str(mask | 1) if filt and mask == 2 else str(mask)  # -m 3 given with -d: profile bit dropped
+    r = subprocess.run([CLI, "-m", mflag, "-t", str(TRIALS), "-l", str(LENGTH), "-b", str(BORDER), "-e", str(evalue)]
+                       + extra + [str(tmp_path / "p.hmm"), str(tmp_path / "s.fa")], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stderr
+    if filt:
+        assert f"# Filtering position-specific letter probabilities: std dev {filt[1]:g}" in r.stdout
     got = parse_records(r.stdout)
 
     # oracle: calibration
