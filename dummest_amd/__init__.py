@@ -85,6 +85,7 @@ def _declare(L):
     L.dm_tantan_dna.argtypes = [vp, i32, ctypes.c_char_p, _P(i64), _P(i32), _P(f)]
     L.dm_add_dna.argtypes = [vp, i32, ctypes.c_char_p, _P(i64), _P(i32), _P(i32)]
     L.dm_filter_scores.argtypes = [vp, i64, _P(i32), _P(i32), _P(f)]
+    L.dm_filter_scores_static.argtypes = [vp, i32, i32, vp, _P(i64), _P(i32), vp]
     L.dm_fwdbwd_scores.argtypes = [vp, i64, _P(i32), _P(i32), ctypes.c_int, _P(f), _P(i32), _P(f), _P(i32)]
     L.dm_pair_matrices.argtypes = [vp, i32, i32, _P(f), _P(i32), _P(f), _P(i32)]
     L.dm_pair_null.argtypes = [vp, i32, i32, _P(f), _P(f), _P(f)]

@@ -128,6 +128,7 @@ int dp_strip_configs(int *gs, int *rs, int cap);
 hipError_t launch_null(const NullArgs &a, hipStream_t s);
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+hipError_t launch_scatter_floats(const float *src, const int *idx, float *dst, int n, hipStream_t s);
 
 // Waves per workgroup of a DP configuration. Kernels whose emission table is
 // large (192 < G*R <= 512 rows: 19-51 KB of LDS) run two (up to 384 rows) or

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -49,11 +50,42 @@ struct DevBuf {
     template <class T> T *as() const { return (T *)p; }
 };
 
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;  // dm_filter_scores_static's scratch (64 MiB)
+
 constexpr int kChunkPairs = 1 << 22;          // pairs per queue chunk (at most)
 constexpr int64_t kChunkBytes = 24LL << 30;   // record + null-scratch bytes per chunk (at most)
 constexpr int64_t kScratchBudget = 8LL << 30;  // bytes of W1 scratch per launch
 
 }  // namespace
+
+// Device scratch of dm_filter_scores_static (pair and unit descriptors,
+// column records, null-model scratch, strip boundaries, counters, staged
+// results): one array per GPU, held until the process ends, shared by every
+// call on that GPU. Host code never uses this name as an address: it takes
+// the device address from hipGetSymbolAddress (scratchAddress below).
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+// The scratch array's device address on each GPU, looked up once per device
+// with that device current.
+static std::mutex g_scratchMu;
+static std::vector<void *> g_scratchAddr;
+static hipError_t scratchAddress(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (device < (int)g_scratchAddr.size() && g_scratchAddr[device]) {
+        *out = g_scratchAddr[device];
+        return hipSuccess;
+    }
+    void *ptr = nullptr;
+    const hipError_t r = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (r != hipSuccess) {
+        (void)hipGetLastError();
+        return r;
+    }
+    if (device >= (int)g_scratchAddr.size()) g_scratchAddr.resize(device + 1, nullptr);
+    g_scratchAddr[device] = ptr;
+    *out = ptr;
+    return hipSuccess;
+}
 
 struct dm_engine {
     int device = 0;
@@ -78,6 +110,8 @@ struct dm_engine {
     bool plain = false;
     int gridOpt = 0;
     int numCUs = 256;
+    void *scratchDev = nullptr;    // this device's address of the static scratch array
+    std::string scratchErr;        // why it could not be looked up
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double lastMs = 0;
     int64_t lastCells = 0;
@@ -877,6 +911,10 @@ int dm_engine_create(int device, dm_engine **out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         e->numCUs = cus;
+    {
+        const hipError_t sr = scratchAddress(device, &e->scratchDev);
+        if (sr != hipSuccess) e->scratchErr = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(sr);
+    }
     int gs[64], rs[64];
     const int nc = dp_configs(gs, rs, 64);
     for (int i = 0; i < nc && i < 64; i++) {
@@ -1215,6 +1253,108 @@ int dm_filter_scores(dm_engine *e, int64_t npairs, const int32_t *prof, const in
     if (!e || (npairs > 0 && !best)) return DM_EINVAL;
     HIPCHK(hipSetDevice(e->device));
     return runPairs(e, kFilter, npairs, prof, seq, 0, best, nullptr);
+}
+
+int dm_filter_scores_static(dm_engine *e, int32_t prof, int32_t nseq, const uint8_t *codes, const int64_t *offsets,
+                            const int32_t *lengths, float *best) {
+    if (!e || nseq < 0) return DM_EINVAL;
+    if (prof < 0 || prof >= (int)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
+    if (nseq == 0) return DM_OK;
+    if (!codes || !offsets || !lengths || !best) return fail(e, DM_EINVAL, "null array");
+    if (!e->scratchDev) return fail(e, DM_EHIP, e->scratchErr);
+    if (e->profDirty)
+        return fail(e, DM_EINVAL, "the profile tables are not on the device yet (any scoring call or plan uploads "
+                                  "them); this call allocates nothing and does not upload them itself");
+    for (int32_t s = 0; s < nseq; s++) {
+        if (lengths[s] < 3) return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
+        if (offsets[s] < 0) return fail(e, DM_EINVAL, "negative sequence offset");
+    }
+    HIPCHK(hipSetDevice(e->device));
+    // the work queue, as a filter plan builds it, over the caller's sequences
+    const int M = e->profiles[prof].M;
+    std::vector<int32_t> pp(nseq, prof), ss(nseq);
+    std::vector<int> cfgOf(nseq);
+    for (int32_t s = 0; s < nseq; s++) {
+        ss[s] = s;
+        cfgOf[s] = chooseConfig(e, M, lengths[s], false);
+    }
+    std::vector<int64_t> sorted(nseq);
+    std::iota(sorted.begin(), sorted.end(), 0);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int64_t a, int64_t b) {
+        if (cfgOf[a] != cfgOf[b]) return cfgOf[a] < cfgOf[b];
+        return lengths[a] > lengths[b];
+    });
+    Chunk C;
+    buildChunk(e, kFilter, sorted, cfgOf, pp.data(), ss.data(), 0, (size_t)nseq, C, lengths, offsets);
+    const int np = (int)C.pairs.size();
+    // scratch layout, every part on a 256-byte boundary
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = at;
+        at += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t ncfg = e->cfgG.size();
+    const size_t oZero = take(1), oCounter = take(sizeof(int) * std::max<size_t>(64, ncfg));
+    const size_t oPairs = take(sizeof(PairInfo) * np), oUnits = take(sizeof(Unit) * C.units.size());
+    const size_t oOrder = take(sizeof(int) * np), oStage = take(sizeof(float) * np);
+    const size_t oRec = take(sizeof(Rec) * C.records), oNull = take(sizeof(ES) * C.nullScratch);
+    const size_t oBnd = at;
+    if (oBnd > kStaticScratchBytes)
+        return fail(e, DM_EUNSUP, "the sequences need " + std::to_string(oBnd) + " bytes of scratch; the static "
+                                  "scratch holds " + std::to_string(kStaticScratchBytes) + " (split the call)");
+    unsigned char *base = (unsigned char *)e->scratchDev;
+    std::vector<int> order(np);
+    for (int k = 0; k < np; k++) order[k] = (int)C.order[k];
+    static const uint8_t zeroCode = kZeroIdx;
+    HIPCHK(hipMemcpyAsync(base + oZero, &zeroCode, 1, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(base + oPairs, C.pairs.data(), sizeof(PairInfo) * np, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(base + oUnits, C.units.data(), sizeof(Unit) * C.units.size(), hipMemcpyHostToDevice,
+                          e->stream));
+    HIPCHK(hipMemcpyAsync(base + oOrder, order.data(), sizeof(int) * np, hipMemcpyHostToDevice, e->stream));
+    NullArgs na{};
+    na.codes = codes;
+    na.zeroCode = base + oZero;
+    na.bg = e->dBg.as<float>();
+    na.pairs = (PairInfo *)(base + oPairs);
+    na.rec = (Rec *)(base + oRec);
+    na.scratch = (ES *)(base + oNull);
+    na.npairs = np;
+    na.bgfs1 = e->bgfs1();
+    na.bgfs2 = e->bgfs2();
+    HIPCHK(launch_null(na, e->stream));
+    int *counter = (int *)(base + oCounter);
+    HIPCHK(launch_zero_ints(counter, (int)ncfg, e->stream));
+    for (size_t cfg = 0; cfg < ncfg; cfg++) {
+        const int nu = C.cfgUnits[cfg];
+        if (nu == 0) continue;
+        const int G = e->cfgG[cfg], R = e->cfgR[cfg];
+        const bool strips = e->cfgStrips[cfg] != 0;
+        const int WG = dp_wg(G, R, strips, kFilter), Pp = 64 / G;
+        DPArgs a = baseArgs(e);
+        a.pairs = na.pairs;
+        a.rec = na.rec;
+        a.units = (const Unit *)(base + oUnits) + C.cfgFirstUnit[cfg];
+        a.nUnits = nu;
+        a.unitCounter = counter + cfg;
+        a.filterOut = (float *)(base + oStage);
+        a.stripSteps = C.cfgMaxN[cfg] + G - 1;
+        int grid = std::max(1, std::min(e->numCUs * 8 / WG, nu / WG));
+        if (strips) {
+            // boundary records per group: as many waves as the rest of the scratch holds
+            a.bndCols = C.cfgMaxN[cfg] + 2;
+            const size_t perGrid = sizeof(Bnd) * 3 * (size_t)a.bndCols * WG * Pp;
+            const size_t fit = (kStaticScratchBytes - oBnd) / perGrid;
+            if (fit < 1)
+                return fail(e, DM_EUNSUP, "a sequence is too long for the static scratch's row-strip boundaries");
+            grid = (int)std::min<size_t>((size_t)grid, fit);
+            a.bnd = (Bnd *)(base + oBnd);
+        }
+        HIPCHK(launch_dp(G, R, kFilter, strips, a, grid, e->stream));
+    }
+    HIPCHK(launch_scatter_floats((const float *)(base + oStage), (const int *)(base + oOrder), best, np, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DM_OK;
 }
 
 int dm_fwdbwd_scores(dm_engine *e, int64_t npairs, const int32_t *prof, const int32_t *seq, int want_bwd, float *mid_m,

@@ -1207,6 +1207,13 @@ __global__ void zero_ints_kernel(int *p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// dst[idx[k]] = src[k]: results staged in work-queue order back to the
+// caller's pair order.
+__global__ void scatter_floats_kernel(const float *src, const int *idx, float *dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = src[i];
+}
+
 // A plain copy of n16 16-byte words: the reference point for the HBM traffic
 // counters (its algorithmic traffic is exactly 16 bytes read + 16 written per word).
 __global__ void __launch_bounds__(256) copy_probe_kernel(const float4 *src, float4 *dst, int64_t n16) {
@@ -1377,6 +1384,12 @@ hipError_t launch_copy_probe(const float *src, float *dst, int64_t n16, hipStrea
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_floats(const float *src, const int *idx, float *dst, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_floats_kernel, dim3((n + 255) / 256), dim3(256), 0, s, src, idx, dst, n);
     return hipGetLastError();
 }
 

@@ -126,6 +126,25 @@ int dm_add_dna(dm_engine *e, int32_t nseq, const char *dna, const int64_t *offse
  * Replaces findSimilaritiesBackwardOnly (dummest-dp-backward.hh:300-438). */
 int dm_filter_scores(dm_engine *e, int64_t npairs, const int32_t *prof, const int32_t *seq, float *best);
 
+/* dm_filter_scores for one profile against nseq sequences, allocating no
+ * device memory. No counterpart in the reference: for callers that cannot
+ * call hipMalloc (serving loops, harnesses that forbid allocation).
+ * codes: device pointer to the sequences' decoded codes (as dm_decode writes
+ * them), sequence s at codes + offsets[s] with lengths[s] >= 3 letters
+ * (offsets and lengths are host arrays); best: device pointer to nseq floats,
+ * filled in the caller's order with dm_filter_scores' values. The caller's
+ * buffers must be ready when the call starts; it returns after the results
+ * are written. Everything else the kernels use lives in one static device
+ * array of 64 MiB per GPU (about 20 bytes per letter plus 84 per sequence,
+ * so roughly 3 million letters per call; larger inputs return DM_EUNSUP).
+ * The profile tables must already be on the device (any earlier scoring
+ * call, plan or in-place dm_set_profile puts them there; DM_EINVAL
+ * otherwise). Calls on one GPU share the array: calls that overlap on one
+ * GPU (several engines, several threads) race and must be serialized by the
+ * caller. */
+int dm_filter_scores_static(dm_engine *e, int32_t prof, int32_t nseq, const uint8_t *codes, const int64_t *offsets,
+                            const int32_t *lengths, float *best);
+
 /* Forward+backward DP scores: best anchored score max wMid = w0*W1 and, if
  * want_bwd, the backward start score max W1*prefixSeed, both exact ExpScores.
  * Replaces the score part of findSimilarities with needAlign=false
