@@ -54,7 +54,8 @@ constexpr size_t kStaticScratchBytes = (size_t)64 << 20;  // dm_filter_scores_st
 
 constexpr int kChunkPairs = 1 << 22;          // pairs per queue chunk (at most)
 constexpr int64_t kChunkBytes = 24LL << 30;   // record + null-scratch bytes per chunk (at most)
-constexpr int64_t kScratchBudget = 8LL << 30;  // bytes of W1 scratch per launch
+constexpr int64_t kScratchMin = 8LL << 30;    // bytes of W1 scratch per plan, at least
+constexpr int64_t kScratchMax = 64LL << 30;   // and at most (a third of the free memory in between)
 
 }  // namespace
 
@@ -466,6 +467,7 @@ struct dm_plan {
     int64_t npairs = 0;
     int64_t cells = 0;
     int64_t maxChunk = 0;
+    int64_t scratchBudget = 0;  // bytes of W1 scratch the plan's launches may use
     std::vector<DevChunk> chunks;
     DevBuf out, rec, nullScratch, counter, wbuf, ebuf, bnd;
     void *hostOut = nullptr;  // pinned staging for results
@@ -535,6 +537,19 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
     });
     int64_t maxRecords = 1, maxW = 1, maxE = 1, maxB = 1, maxNull = 64;
     const size_t ncfg = e->cfgG.size();
+    // W1 scratch: a third of the device's free memory, within [8, 64] GiB. A
+    // launch's persistent grid shrinks to what fits, so too small a budget
+    // leaves SIMDs without waves (row-strip kernels size every wave for the
+    // longest profile of the launch).
+    {
+        size_t freeB = 0, totalB = 0;
+        int64_t budget = kScratchMin;
+        if (hipMemGetInfo(&freeB, &totalB) == hipSuccess)
+            budget = std::max<int64_t>(kScratchMin, std::min<int64_t>(kScratchMax, (int64_t)(freeB / 3)));
+        else
+            (void)hipGetLastError();
+        P.scratchBudget = budget;
+    }
     for (size_t b = 0, en = 0; b < (size_t)npairs; b = en) {
         // chunk: up to kChunkPairs pairs whose records and null scratch fit kChunkBytes
         en = b + 1;
@@ -585,7 +600,7 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             if (mode == kScore) {
                 a.waveFloats = (int64_t)C.cfgMaxS[cfg] * a.stripSteps * ROWS * Pp;
                 const int64_t perWaveBytes = a.waveFloats * (int64_t)sizeof(float) * WG;
-                grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, kScratchBudget / perWaveBytes));
+                grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, P.scratchBudget / perWaveBytes));
                 maxW = std::max(maxW, a.waveFloats * grid * WG);
                 maxE = std::max(maxE, a.waveFloats * grid * WG / ROWS + 1);
             }

@@ -439,6 +439,13 @@ __device__ __forceinline__ float add_nc(float a, float b) {
     return a + b;
 }
 
+// Maximum of non-negative floats through their bit patterns, which order like
+// unsigned integers: one integer max, where fmaxf would first quiet both
+// operands. Only for values that cannot be NaN (the rescaled passes).
+__device__ __forceinline__ float nnmax(float a, float b) {
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
+
 // Exact running maximum of float values whose power-of-two offset changes
 // rarely: cur holds the max at offset off; (sm, se) the exact max of all
 // values retired at earlier offsets.
@@ -548,7 +555,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
     constexpr int WG = dp_wg(G, R, STR, MODE);  // waves per workgroup (sharing the emission table)
     // prefetch distance in steps (divides 4); the strip kernels keep 2, which
     // lets all but G = 32, R = 8 fit 256 registers (two waves per SIMD)
-    constexpr int kPF = (R <= 8 && (!STR || (G == 32 && R == 8))) ? 4 : 2;
+    constexpr int kPF = ((R < 8 && !STR) || (STR && G == 32 && R == 8)) ? 4 : 2;
     extern __shared__ float etab[];  // [kNCode][ROWS], then the profile's background table (32)
     float *bgl = etab + kNCode * ROWS;
     const int lane = threadIdx.x & 63;
@@ -585,6 +592,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
         const float s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
         const int T = U.maxN + G - 1;
         const int Tmine = n + G - 1;
+        const int Tb = Tmine;  // anti-diagonal of the backward's first step
         const int64_t strideS = (int64_t)a.stripSteps * ROWS;  // floats per strip of W1 steps
         float *wsc0;
         int *esc0;
@@ -696,7 +704,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             // step t-1, so a record enters the group at its last lane (loaded kPF
             // steps ahead there) and moves up one lane per step by DPP.
             auto brecAt = [&](int tt, int kk) {
-                const int c = Tmine - 1 - tt - kk;
+                const int c = Tb - 1 - tt - kk;
                 return recs[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
             };
             const bool injB = (k == G - 1);
@@ -710,7 +718,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             // would make the compiler wait for all memory operations each step;
             // used only when there is a strip below)
             auto ghostAt = [&](int tt) -> Bnd {
-                if (STR) return bIn[bidx(Tmine - 1 - tt - k, n)];
+                if (STR) return bIn[bidx(Tb - 1 - tt - k, n)];
                 return Bnd{0.f, 0.f, 0};
             };
 #pragma unroll
@@ -726,7 +734,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             auto bstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
                 constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
-                const int c = Tmine - 1 - t - k;  // column: D = Tmine-1-t
+                const int c = Tb - 1 - t - k;  // column: D = Tb-1-t
                 const Rec rc = rcur;
                 const float rcg = gq;
                 // the next step's record (the lane below's, or a new one at the last
@@ -800,7 +808,11 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                         w = add_nc(w, q < nValid ? ssr : 0.f);
                     } else {
                         const float eE = e * enterC[q];
-                        w = fmaf(bw3, eE, q < nValid ? ssr : 0.f);
+                        // score mode adds the seed to the rows past M too: their
+                        // coefficients are zero, so they feed nothing, and their
+                        // seed-only values never exceed a real row's (every real
+                        // row holds the seed plus non-negative terms)
+                        w = fmaf(bw3, eE, (MODE == kScore || q < nValid) ? ssr : 0.f);
                         w = fmaf(bw2, d1s2[q], w);
                         w = fmaf(bw1, d2s1[q], w);
                         w = fmaf(zg, a0sC[q], w);
@@ -810,8 +822,8 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     }
                     const float y = fmaf(eps[q], bY, w);
                     const float z = fmaf(b0sC[q], zg, w);
-                    if (MODE == kFilter) fbest = fmaxf(fbest, w * psp);
-                    else smax = fmaxf(smax, fmaxf(y, z));  // y, z >= w
+                    if (MODE == kFilter) fbest = fmaxf(fbest, w * psp);  // +inf * 0 may give NaN: fmaxf skips it
+                    else smax = nnmax(smax, nnmax(y, z));  // y, z >= w
                     // row q at columns c+1, c+2, c+3 for row q-1 (same lane, same column)
                     bw1 = H[S1][q];
                     bw2 = H[S2][q];
@@ -832,14 +844,14 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     // products is the product of the float max.
                     float hm = H[S4][0];
 #pragma unroll
-                    for (int q = 1; q < R; q++) hm = fmaxf(hm, H[S4][q]);
+                    for (int q = 1; q < R; q++) hm = nnmax(hm, H[S4][q]);
                     bwdMax.cur = fmax(bwdMax.cur, (double)hm * pspD);
                 }
                 if (STR && above && k == 0 && c >= 0 && c < n) {
                     bOut[c + 1] = Bnd{H[S4][0], Yc[0], Eb};
                 }
                 if (MODE != kFilter) {
-                    const int D = Tmine - 1 - t;
+                    const int D = Tb - 1 - t;
                     if (D >= 0 && active) {
                         if (rowsLive && c >= 0 && c < n) {
                             float *dst = wsc + (int64_t)D * ROWS + k * R;
@@ -856,7 +868,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     // whole live state (the last 4 steps) of the pair is below 2^36
                     // and the exponent is at least 64, so regions that decay after
                     // a strong hit keep their precision.
-                    LV[S4] = fmaxf(smax, fmaxf(gw, gy));
+                    LV[S4] = nnmax(smax, nnmax(gw, gy));
                     const bool big = smax > kBig;
                     if (__builtin_expect(__any(big), 0)) {
                         bool gbig = big;
@@ -865,7 +877,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                         if (gbig) scaleB(-64);
                     }
                     if (PH == 3) {  // never below exponent 0: the seeds enter at 2^-Eb <= 1
-                        const float live = fmaxf(fmaxf(LV[0], LV[1]), fmaxf(LV[2], LV[3]));
+                        const float live = nnmax(nnmax(LV[0], LV[1]), nnmax(LV[2], LV[3]));
                         if (__builtin_expect(group_small<G>(live, grp), 0) && Eb >= 64) scaleB(64);
                     }
                 }
@@ -906,6 +918,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             loadStrip(strip, a0, b0);
             const float *wsc = wsc0 + (int64_t)strip * strideS;
             const int *esc = esc0 + (int64_t)strip * a.stripSteps;
+            const bool rowsLive = strip * ROWS + k * R <= M;
             const bool above = STR && strip > 0;      // lane 0 reads the strip above
             const bool below = STR && strip < S - 1;  // last lane feeds the strip below
             float Wf[4][R], Zf[4][R], LVf[4];
@@ -952,19 +965,20 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             Rec fq[kPF];  // the first lane's records of the next steps (slot = step mod kPF)
             float Wq[kPF][R];
             int Eq[kPF];
-            const bool rowsLive = strip * ROWS + k * R <= M;
             // Loads are unconditional (from a clamped, always-valid step) so that
             // no select waits for them; which of them count is decided at use.
+            const int Tw = Tmine;
             auto loadW1 = [&](int tt, float *dst, int &e) {
-                const int ts = tt < Tmine ? tt : (Tmine > 0 ? Tmine - 1 : 0);
+                const int ts = tt < Tw ? tt : (Tw > 0 ? Tw - 1 : 0);
                 e = esc[ts];
                 if (MODE == kScore) {
 #pragma unroll
                     for (int q = 0; q < R; q++) dst[q] = wsc[(int64_t)ts * ROWS + k * R + q];
                 }
             };
-            // the backward stored the block of step tt for this lane
-            auto liveW1 = [&](int tt) { return tt < Tmine && rowsLive && tt - k >= 0 && tt - k < n; };
+            // the backward stored the block of step tt for this lane (evaluated once
+            // per step without short-circuits: a branch per row otherwise)
+            auto liveW1 = [&](int tt) { return (tt < Tmine) & rowsLive & (tt - k >= 0) & (tt - k < n); };
 #pragma unroll
             for (int f = 0; f < kPF; f++) loadW1(f, Wq[f], Eq[f]);
 #pragma unroll
@@ -1004,9 +1018,10 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 for (int q = 0; q < R; q++) eqN[q] = etk[((nx.codes >> 8) & 255) * ROWS + q];
                 const float gqN = bgl[(nx.codes >> 8) & 255];
                 float W1v[R];
+                const bool lv = (MODE == kScore) && liveW1(t);
 #pragma unroll
-                for (int q = 0; q < R; q++) W1v[q] = (MODE == kScore && liveW1(t)) ? Wq[SL][q] : 0.f;
-                const int EbNext = (t < Tmine) ? Eq[SL] : kNoStep;
+                for (int q = 0; q < R; q++) W1v[q] = lv ? Wq[SL][q] : 0.f;
+                const int EbNext = (t < Tw) ? Eq[SL] : kNoStep;
                 loadW1(t + kPF, Wq[SL], Eq[SL]);
                 float gn = 0.f, gy = 0.f;
                 Bnd fcurB{0.f, 0.f, 0};
@@ -1070,9 +1085,9 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                         nn = fmaf(d1s2[q], w2, nn);
                         nn = fmaf(d2s1[q], w1, nn);
                     }
-                    if (MODE == kScore) midMax.cur = fmaxf(midMax.cur, w0 * W1v[q]);
+                    if (MODE == kScore) midMax.cur = nnmax(midMax.cur, w0 * W1v[q]);
                     if (MODE == kDump && inD && active) w0d[(int64_t)D * ROWS + k * R + q] = w0;
-                    smax = fmaxf(smax, fmaxf(w0, y));
+                    smax = nnmax(smax, nnmax(w0, y));
                     Wf[S4][q] = w0;
                     Zf[S4][q] = Z0;
                     Nin = nn;
@@ -1089,7 +1104,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 }
                 if (MODE == kDump && inD && active && k == 0) efd[D] = Ef;
                 // exponent moves as in the backward (up at 2^100, down below 2^36)
-                LVf[S4] = fmaxf(fmaxf(smax, fmaxf(lastN, lastY)), fmaxf(gn, gy));
+                LVf[S4] = nnmax(nnmax(smax, nnmax(lastN, lastY)), nnmax(gn, gy));
                 const bool big = smax > kBig;
                 if (__builtin_expect(__any(big), 0)) {
                     bool gbig = big;
@@ -1098,7 +1113,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     if (gbig) scaleF(-64);
                 }
                 if (PH == 3) {  // never below exponent 0: the seeds enter at 2^-Ef <= 1
-                    const float live = fmaxf(fmaxf(LVf[0], LVf[1]), fmaxf(LVf[2], LVf[3]));
+                    const float live = nnmax(nnmax(LVf[0], LVf[1]), nnmax(LVf[2], LVf[3]));
                     if (__builtin_expect(group_small<G>(live, grp), 0) && Ef >= 64) scaleF(64);
                 }
             };
