@@ -623,15 +623,46 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
 
         // Per-strip setup: LDS emission rows and per-row coefficients
         // (dummest-dp-backward.hh:481-491, dummest-dp-forward.hh:72-80).
-        auto loadStrip = [&](int strip, float *a0v, float *b0v) {
+        // fill: load the strip's emission table (the forward's first strip is the
+        // backward's last, whose table is still in LDS).
+        auto loadStrip = [&](int strip, float *a0v, float *b0v, bool fill) {
             const int rb = strip * ROWS;
-            __syncthreads();
-            for (int idx = (int)threadIdx.x; idx < kNCode * ROWS; idx += 64 * WG) {
-                const int c = idx / ROWS, r = rb + idx - c * ROWS;
-                etab[idx] = (r <= M) ? a.emis[PI.emOff + (int64_t)r * kEmStride + c] : 0.f;
+            if (fill) {
+                __syncthreads();  // the previous table is no longer read
+                // Thread x fills rows x, x + 64 WG, ...: a row's 25 values come with
+                // seven 16-byte loads (the HBM rows hold 32 floats), and every load
+                // of a batch of rows is issued before the first LDS write, so the
+                // fill waits on memory once per batch rather than once per value.
+                constexpr int kPer = (ROWS + 64 * WG - 1) / (64 * WG);
+                constexpr int kBatch = (R >= 12) ? 1 : 2;
+#pragma unroll 1
+                for (int i0 = 0; i0 < kPer; i0 += kBatch) {
+                    float4 v[kBatch][7];
+#pragma unroll
+                    for (int i = 0; i < kBatch; i++) {
+                        const int rr = (int)threadIdx.x + (i0 + i) * 64 * WG;
+                        const int r = min(rb + rr, M);  // rows past M read row M, stored as 0
+                        const float4 *src = (const float4 *)(a.emis + PI.emOff + (int64_t)r * kEmStride);
+#pragma unroll
+                        for (int w = 0; w < 7; w++) v[i][w] = src[w];
+                    }
+#pragma unroll
+                    for (int i = 0; i < kBatch; i++) {
+                        const int rr = (int)threadIdx.x + (i0 + i) * 64 * WG;
+                        if (i0 + i < kPer && rr < ROWS) {
+                            const bool live = rb + rr <= M;
+#pragma unroll
+                            for (int c = 0; c < kNCode; c++) {
+                                const float4 q4 = v[i][c >> 2];
+                                const float x = (c & 3) == 0 ? q4.x : (c & 3) == 1 ? q4.y : (c & 3) == 2 ? q4.z : q4.w;
+                                etab[c * ROWS + rr] = live ? x : 0.f;
+                            }
+                        }
+                    }
+                }
+                if (threadIdx.x < 32) bgl[threadIdx.x] = a.bg[(int64_t)U.prof * 32 + threadIdx.x];
+                __syncthreads();
             }
-            if (threadIdx.x < 32) bgl[threadIdx.x] = a.bg[(int64_t)U.prof * 32 + threadIdx.x];
-            __syncthreads();
 #pragma unroll
             for (int q = 0; q < R; q++) {
                 const int r = rb + k * R + q;
@@ -652,7 +683,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
         // ------------------------------------------------------------- backward
         for (int strip = S - 1; strip >= 0; strip--) {
             float a0sC[R], b0sC[R];
-            loadStrip(strip, a0sC, b0sC);
+            loadStrip(strip, a0sC, b0sC, true);
             // rows of this lane that exist (r <= M): a prefix of its R rows; the
             // seed enters only there
             const int nValid = active ? max(0, min(R, M - (strip * ROWS + k * R) + 1)) : 0;
@@ -915,7 +946,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
         midMax.init(kPre);
         for (int strip = 0; strip < S; strip++) {
             float a0[R], b0[R];
-            loadStrip(strip, a0, b0);
+            loadStrip(strip, a0, b0, strip > 0);
             const float *wsc = wsc0 + (int64_t)strip * strideS;
             const int *esc = esc0 + (int64_t)strip * a.stripSteps;
             const bool rowsLive = strip * ROWS + k * R <= M;
