@@ -123,6 +123,8 @@ def _declare(L):
     L.dm_last_grad_ms.restype = d
     L.dm_last_grad_cells.argtypes = [vp]
     L.dm_last_grad_cells.restype = i64
+    L.dm_last_grad_segments.argtypes = [vp]
+    L.dm_last_grad_segments.restype = i64
     L.dm_grad_set_create.argtypes = [vp, i32, ctypes.c_char_p, _P(i64), _P(i32), _P(i32)]
     L.dm_grad_set.argtypes = [vp, i32, i32, _P(d), _P(d), _P(d)]
     L.dm_grad_set_destroy.argtypes = [vp, i32]
@@ -545,6 +547,11 @@ class Engine:
     @property
     def last_grad_cells(self) -> int:
         return lib().dm_last_grad_cells(self._h)
+
+    @property
+    def last_grad_segments(self) -> int:
+        """Strip segments the last gradient call recomputed from checkpoint rows."""
+        return lib().dm_last_grad_segments(self._h)
 
     # -- scoring ------------------------------------------------------------
     @staticmethod

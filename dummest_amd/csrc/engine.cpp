@@ -146,6 +146,7 @@ struct dm_engine {
     std::vector<GradSet> gradSets;
     double lastGradMs = 0;
     int64_t lastGradCells = 0;
+    int64_t lastGradSegments = 0;  // strip segments recomputed from edge rows by the last gradient call
     // hits of the last dm_pair_hits call, kept so that a call repeated with
     // larger buffers after DM_ERANGE does not run the pair again
     struct HitCache {
@@ -1884,6 +1885,7 @@ double dm_last_tantan_ms(const dm_engine *e) { return e ? e->lastTantanMs : 0.0;
 double dm_last_bw_ms(const dm_engine *e) { return e ? e->lastBwMs : 0.0; }
 double dm_last_grad_ms(const dm_engine *e) { return e ? e->lastGradMs : 0.0; }
 int64_t dm_last_grad_cells(const dm_engine *e) { return e ? e->lastGradCells : 0; }
+int64_t dm_last_grad_segments(const dm_engine *e) { return e ? e->lastGradSegments : 0; }
 
 // ------------------------------------- gradient of the summed log-likelihood
 // Appends DNA sequences to the engine's store (decoded codes for the null model
@@ -1945,30 +1947,52 @@ static int gradRun(dm_engine *e, int32_t prof, int32_t nseq, int32_t s0, size_t 
     const int32_t *lengths = lengthsV.data();
     int r = syncSequences(e);
     if (r) return r;
-    // batches whose six matrices fit the budget
-    std::vector<size_t> batch{0};
-    int64_t cells = 0, maxCells = 1, maxPairs = 1, maxBnd = 1, maxExp = 1;
-    int64_t bnd = 0, nexp = 0;
-    for (int32_t s = 0; s < nseq; s++) {
-        const int64_t c = gradSlots(M, lengths[s]);
-        if (cells > 0 && (cells + c) * 6 * 8 > e->gradBudget) {
-            batch.push_back((size_t)s);
-            cells = 0;
-            bnd = 0;
-            nexp = 0;
+    // Batches of consecutive sequences whose six matrices fit the budget
+    // together. A sequence whose matrices alone pass it forms a batch of its
+    // own, run in segments of seg strips (at least one) recomputed from the
+    // strips' edge rows.
+    struct Batch {
+        size_t b, e;
+        int seg;  // strips per segment; S = the matrices held in full
+    };
+    const int S = gradStrips(M);
+    std::vector<Batch> batches;
+    int64_t maxCells = 1, maxPairs = 1, maxBnd = 1, maxExp = 1;
+    {
+        int64_t cells = 0, bnd = 0, nexp = 0;
+        size_t first = 0;
+        auto close = [&](size_t end) {
+            if (end > first) batches.push_back({first, end, S});
+            first = end;
+            cells = bnd = nexp = 0;
+        };
+        for (int32_t s = 0; s < nseq; s++) {
+            const int n = lengths[s];
+            const int64_t c = gradSlots(M, n), perStrip = (int64_t)(n + 63) * 64;
+            if (c * 6 * 8 > e->gradBudget) {
+                close((size_t)s);
+                const int seg = (int)std::clamp<int64_t>(e->gradBudget / (6 * 8 * perStrip), 1, S);
+                batches.push_back({(size_t)s, (size_t)s + 1, seg});
+                first = (size_t)s + 1;
+                maxCells = std::max(maxCells, seg * perStrip);
+                maxBnd = std::max(maxBnd, (int64_t)S * n);
+                maxExp = std::max(maxExp, gradExps(M, n));
+                continue;
+            }
+            if (cells > 0 && (cells + c) * 6 * 8 > e->gradBudget) close((size_t)s);
+            cells += c;
+            bnd += (int64_t)S * n;
+            nexp += gradExps(M, n);
+            maxCells = std::max(maxCells, cells);
+            maxBnd = std::max(maxBnd, bnd);
+            maxExp = std::max(maxExp, nexp);
+            maxPairs = std::max<int64_t>(maxPairs, (int64_t)s + 1 - (int64_t)first);
         }
-        cells += c;
-        bnd += lengths[s];
-        nexp += gradExps(M, lengths[s]);
-        maxCells = std::max(maxCells, cells);
-        maxBnd = std::max(maxBnd, bnd);
-        maxExp = std::max(maxExp, nexp);
-        maxPairs = std::max<int64_t>(maxPairs, (int64_t)s + 1 - (int64_t)batch.back());
+        close((size_t)nseq);
     }
-    batch.push_back((size_t)nseq);
     HIPCHK(e->dGradMat.ensure(sizeof(double) * 6 * (size_t)maxCells));
-    // hand-over: three doubles and an exponent per column, and the column sums
-    HIPCHK(e->dGradBnd.ensure((sizeof(double) * 4 + sizeof(int)) * (size_t)maxBnd));
+    // edge rows: five doubles and two exponents per strip and column
+    HIPCHK(e->dGradBnd.ensure((sizeof(double) * 5 + sizeof(int) * 2) * (size_t)maxBnd));
     HIPCHK(e->dGradExp.ensure(sizeof(int) * 2 * (size_t)maxExp));
     HIPCHK(e->dGradMeta.ensure((sizeof(int64_t) * 3 + sizeof(int)) * (size_t)maxPairs));
     HIPCHK(e->dGradRaw.ensure(sizeof(double) * (size_t)nseq * (M + 1) * kGradRaw));
@@ -1986,13 +2010,14 @@ static int gradRun(dm_engine *e, int32_t prof, int32_t nseq, int32_t s0, size_t 
     if (!e->ev0) HIPCHK(hipEventCreate(&e->ev0));
     if (!e->ev1) HIPCHK(hipEventCreate(&e->ev1));
     float msAll = 0.f;
-    for (size_t b = 0; b + 1 < batch.size(); b++) {
+    int64_t segments = 0;
+    for (const Batch &bt : batches) {
         Chunk C;
-        buildChunk(e, kScore, sorted, cfgOf, pp.data(), ss.data(), batch[b], batch[b + 1], C);
+        buildChunk(e, kScore, sorted, cfgOf, pp.data(), ss.data(), bt.b, bt.e, C);
         int r2 = uploadChunk(e, C);  // pair records and the null model
         if (r2) return r2;
         const int np = (int)C.pairs.size();
-        std::vector<int64_t> meta(3 * (size_t)np);  // cell, hand-over and exponent offsets
+        std::vector<int64_t> meta(3 * (size_t)np);  // cell, edge and exponent offsets
         std::vector<int> rawIdx(np);
         int64_t co = 0, bo = 0, eo = 0;
         int maxN = 3;
@@ -2003,7 +2028,7 @@ static int gradRun(dm_engine *e, int32_t prof, int32_t nseq, int32_t s0, size_t 
             meta[2 * np + k] = eo;
             rawIdx[k] = (int)C.order[k];
             co += gradSlots(M, n);
-            bo += n;
+            bo += (int64_t)S * n;
             eo += gradExps(M, n);
             maxN = std::max(maxN, n);
         }
@@ -2030,20 +2055,39 @@ static int gradRun(dm_engine *e, int32_t prof, int32_t nseq, int32_t s0, size_t 
         ga.W1 = mat + 3 * C6;
         ga.Yb = mat + 4 * C6;
         ga.Zb = mat + 5 * C6;
-        ga.bndA = e->dGradBnd.as<double>();
-        ga.bndB = ga.bndA + maxBnd;
-        ga.bndC = ga.bndA + 2 * maxBnd;
-        ga.colSum = ga.bndA + 3 * maxBnd;
-        ga.bndE = (int *)(ga.bndA + 4 * maxBnd);
+        ga.fwdA = e->dGradBnd.as<double>();
+        ga.fwdB = ga.fwdA + maxBnd;
+        ga.bwdA = ga.fwdA + 2 * maxBnd;
+        ga.bwdB = ga.fwdA + 3 * maxBnd;
+        ga.bwdC = ga.fwdA + 4 * maxBnd;
+        ga.fwdE = (int *)(ga.fwdA + 5 * maxBnd);
+        ga.bwdE = ga.fwdE + maxBnd;
         ga.expF = e->dGradExp.as<int>();
         ga.expB = ga.expF + maxExp;
         ga.zInfo = e->dGradW.as<double>() + 3 * (size_t)nseq;
         ga.raw = e->dGradRaw.as<double>();
         HIPCHK(hipEventRecord(e->ev0, e->stream));
+        ga.stripLo = 0;
+        ga.stripHi = S - 1;
+        ga.store = bt.seg >= S;
+        // both passes over every strip (the matrices, or the edge rows alone), Z
         HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
         HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
         HIPCHK(launch_grad_z(ga, np, e->stream));
-        HIPCHK(launch_grad_counts(ga, np, M, maxN, e->stream));
+        if (bt.seg >= S) {
+            HIPCHK(launch_grad_counts(ga, np, M, maxN, e->stream));
+        } else {
+            // per segment: its strips again from their edge rows, with the matrices, then counted
+            ga.store = 1;
+            for (int lo = 0; lo < S; lo += bt.seg) {
+                ga.stripLo = lo;
+                ga.stripHi = std::min(lo + bt.seg, S) - 1;
+                HIPCHK(launch_grad_forward(ga, np, maxN, e->stream));
+                HIPCHK(launch_grad_backward(ga, np, maxN, e->stream));
+                HIPCHK(launch_grad_counts(ga, np, M, maxN, e->stream));
+                segments++;
+            }
+        }
         HIPCHK(hipEventRecord(e->ev1, e->stream));
         HIPCHK(hipEventSynchronize(e->ev1));
         float ms = 0.f;
@@ -2064,6 +2108,7 @@ static int gradRun(dm_engine *e, int32_t prof, int32_t nseq, int32_t s0, size_t 
     if (lnZ) HIPCHK(hipMemcpyAsync(lnZ, ra.lnZ, sizeof(double) * nseq, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     e->lastGradMs = msAll;
+    e->lastGradSegments = segments;
     e->lastGradCells = 0;
     for (int32_t s = 0; s < nseq; s++) e->lastGradCells += gradCells(M, lengths[s]);
     return DM_OK;

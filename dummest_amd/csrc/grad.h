@@ -21,12 +21,19 @@ namespace dm {
 // row: the 11 parameters of the row's Params (alpha'[3], beta'[3], delta'[3],
 // eps', enter) and the emission by codon (64 codons, 64 = a codon with a
 // non-ACGT letter).
+//
+// Checkpoints: every strip keeps the edge row it hands to its neighbour strip
+// (n values per strip and quantity, 1/64 of a matrix strip), so any run of
+// strips can be computed again from its edges alone, with the same values. A
+// pair whose matrices do not fit the budget runs both passes once without
+// storing matrices (the edges only), then per segment of strips the two
+// passes again with the matrices and the count pass: O((M/64) n) edge memory
+// plus one segment's matrices in place of O(M n).
 constexpr int kGradParams = 11;
 constexpr int kGradCodons = 65;
 constexpr int kGradRaw = kGradParams + kGradCodons;
-// longest sequence: its six matrices (48 B per cell and a 64-row strip) must
-// fit the matrix budget of one launch batch
-constexpr int kGradMaxN = 1 << 20;
+// longest sequence: one 64-row strip of its matrices takes 3 KiB per letter
+constexpr int kGradMaxN = 1 << 28;
 
 inline int64_t gradCells(int M, int n) { return (int64_t)(M + 1) * n; }
 inline int gradStrips(int M) { return (M + 64) / 64; }  // 64-row strips covering rows 0..M
@@ -46,13 +53,19 @@ struct GradArgs {
     const uint8_t *codon;  // per pair: codon index of each column (at pairs[p].seqOff)
     int pairBase;          // first pair of this launch
     const int64_t *cellOff;  // per pair, relative to the matrix bases
+    // strips stripLo..stripHi of each pair, strip s at cellOff[p] + (s -
+    // stripLo) x (n+63) x 64; the forward and backward launches run those
+    // strips and store them when store != 0, the count launch reads them
     double *w0, *Y, *Z0, *W1, *Yb, *Zb;
-    // per pair (n each, at the pair's bndOff): strip hand-over of the edge row
-    // (forward: inflow and Y; backward: W1, Yb and the column sum of W1) and
-    // the exponent each column was stored at; colSum: the column sums of W1
-    // over all rows (at strip 0's backward exponents)
-    double *bndA, *bndB, *bndC, *colSum;
-    int *bndE;
+    int stripLo, stripHi, store;
+    // per pair and strip (n values at bndOff[p] + s x n), the edge row a strip
+    // hands on: forward (row 64s+63, to strip s+1) the inflow into the next row
+    // (fwdA), Y (fwdB) and the exponent each column is stored at (fwdE);
+    // backward (row 64s, to strip s-1) W1 (bwdA), Yb (bwdB), the column sum of
+    // W1 over rows 64s..M (bwdC) and the exponent (bwdE). Strip 0's bwdC holds
+    // the column sums over all rows.
+    double *fwdA, *fwdB, *bwdA, *bwdB, *bwdC;
+    int *fwdE, *bwdE;
     const int64_t *bndOff;
     // per pair (strips x blocks at expOff), strip s = rows 64s..64s+63 in both
     // passes; forward block c = columns 64c..64c+63, backward block c =

@@ -22,13 +22,14 @@
 // Layout: one wavefront per pair for forward and backward, profile rows 64 at
 // a time (lane k = one row, one column behind the row it depends on), the
 // values from that row arriving by DPP; the first row of a strip reads the
-// neighbouring strip's edge row from a hand-over buffer in 64-column chunks.
+// neighbouring strip's edge row, which every strip keeps, in 64-column chunks.
 // The strip's emission rows sit in LDS, the pair's per-column codes and seeds
-// in a 256-column LDS ring. Every pass stores its matrices in full,
-// diagonal-major (each step's 64 values contiguous). Z comes from the column
-// sums of W1; the count pass gives each (pair, strip) a wavefront whose lanes
-// walk the forward's anti-diagonals, so every read is a contiguous run, with
-// the codon-indexed emission sums in LDS.
+// in a 256-column LDS ring. A pass stores the matrices of the strips it runs
+// diagonal-major (each step's 64 values contiguous), or only their edge rows
+// (the checkpoints from which a run of strips is computed again). Z comes
+// from the column sums of W1; the count pass gives each (pair, strip) a
+// wavefront whose lanes walk the forward's anti-diagonals, so every read is a
+// contiguous run, with the codon-indexed emission sums in LDS.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -94,10 +95,12 @@ __device__ __forceinline__ int next_block_exp(int E, int m) {
     return E;
 }
 
-// Forward: w0, Y, Z0 of every cell, strip by strip from row 0 down. Lane k of
-// a strip runs row base+k at column t-k in step t. Each 64-column block of
-// the strip has one exponent (expF), chosen when lane 0 enters the block; a
-// lane converts its three columns of history when it enters the block itself.
+// Forward: w0, Y, Z0 of strips stripLo..stripHi (stored when kStore), strip by
+// strip down, from the edge row of strip stripLo-1. Lane k of a strip runs row
+// base+k at column t-k in step t. Each 64-column block of the strip has one
+// exponent (expF), chosen when lane 0 enters the block; a lane converts its
+// three columns of history when it enters the block itself.
+template <bool kStore>
 __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
@@ -114,11 +117,16 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
     const size_t SP = (size_t)(n + 63) * 64;  // stored values per strip
     double *w0 = a.w0 + a.cellOff[p], *Yf = a.Y + a.cellOff[p], *Z0 = a.Z0 + a.cellOff[p];
-    double *hA = a.bndA + a.bndOff[p], *hB = a.bndB + a.bndOff[p];
-    int *hE = a.bndE + a.bndOff[p];
     int *expF = a.expF + a.expOff[p];
-    for (int base = 0, strip = 0; base <= M; base += 64, strip++, w0 += SP, Yf += SP, Z0 += SP) {
-        const int i = base + lane;
+    for (int strip = a.stripLo; strip <= a.stripHi; strip++, w0 += SP, Yf += SP, Z0 += SP) {
+        const int base = 64 * strip, i = base + lane;
+        // the edge row this strip reads (strip-1's) and the one it writes
+        const int64_t inAt = a.bndOff[p] + (int64_t)(strip > 0 ? strip - 1 : 0) * n;
+        const int64_t outAt = a.bndOff[p] + (int64_t)strip * n;
+        const double *hA = a.fwdA + inAt, *hB = a.fwdB + inAt;
+        const int *hE = a.fwdE + inAt;
+        double *oA = a.fwdA + outAt, *oB = a.fwdB + outAt;
+        int *oE = a.fwdE + outAt;
         const bool rowOk = i <= M;
         const RowParams P = row_params(a, rowOk ? i : 0);
         const double a1s1 = P.a1 * s1, a2s2 = P.a2 * s2, d1s2 = P.d1 * s2, d2s1 = P.d2 * s1, enterC = P.enter * sC;
@@ -149,7 +157,7 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
                 cA = ldexp(cA, cE - Ecur);
                 cB = ldexp(cB, cE - Ecur);
             }
-            if (lane == 0) expF[strip * NB + c] = Ecur;
+            if (lane == 0) expF[(int64_t)strip * NB + c] = Ecur;
         };
         auto step = [&](int t) {
             if (dConv != 0 && lane == (t & 63)) {  // this lane's first column of the newest block
@@ -179,9 +187,9 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
                 sy = y;
                 sz = z0;
                 if (lane == 63) {
-                    hA[j] = oC;
-                    hB[j] = oY;
-                    hE[j] = El;
+                    oA[j] = oC;
+                    oB[j] = oY;
+                    oE[j] = El;
                 }
                 w3v = w2v;
                 w2v = w1v;
@@ -192,10 +200,12 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
             }
             lastC = oC;
             lastY = oY;
-            const size_t at = (size_t)t * 64 + lane;  // one contiguous run per step
-            w0[at] = sw;
-            Yf[at] = sy;
-            Z0[at] = sz;
+            if constexpr (kStore) {
+                const size_t at = (size_t)t * 64 + lane;  // one contiguous run per step
+                w0[at] = sw;
+                Yf[at] = sy;
+                Z0[at] = sz;
+            }
         };
         // at step 64c: the edge chunk of block c, then block c's exponent (no
         // block starts at or past column n: the lanes stay at the last one)
@@ -242,11 +252,13 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
     }
 }
 
-// Backward: W1, Yb, Zb of every cell, strip by strip from the strip holding
-// row M up (strip s = rows 64s..64s+63, as in the forward); lane k of a strip
-// runs row 64s+63-k at column n-1-t+k in step t. Blocks of 64 columns from the
-// right (expB) as in the forward. The column sums of W1 run down
-// the lanes with Yb and reach row 0 as sum_i W1(i, j) (colSum).
+// Backward: W1, Yb, Zb of strips stripHi..stripLo (stored when kStore), strip
+// by strip up (strip s = rows 64s..64s+63, as in the forward), from the edge
+// row of strip stripHi+1; lane k of a strip runs row 64s+63-k at column
+// n-1-t+k in step t. Blocks of 64 columns from the right (expB) as in the
+// forward. The column sums of W1 run along the lanes with Yb and leave each
+// strip in its edge row (bwdC), strip 0's as sum_i W1(i, j).
+template <bool kStore>
 __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
@@ -261,15 +273,19 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
     if (lane < 32) sBg[lane] = lane < kNCode ? (double)a.bg[lane] : 0.0;
     const int S = (M + 64) / 64;
     const size_t SP = (size_t)(n + 63) * 64;  // stored values per strip
-    double *hA = a.bndA + a.bndOff[p], *hB = a.bndB + a.bndOff[p], *hC = a.bndC + a.bndOff[p];
-    double *cs = a.colSum + a.bndOff[p];
-    int *hE = a.bndE + a.bndOff[p];
     int *expB = a.expB + a.expOff[p];
     // strip s = rows 64s..64s+63 from the bottom strip up; lane 0 runs the strip's last row
-    for (int strip = S - 1; strip >= 0; strip--) {
+    for (int strip = a.stripHi; strip >= a.stripLo; strip--) {
         const int top = 64 * strip + 63;
-        double *W1 = a.W1 + a.cellOff[p] + strip * SP, *Yb = a.Yb + a.cellOff[p] + strip * SP;
-        double *Zb = a.Zb + a.cellOff[p] + strip * SP;
+        const size_t at0 = a.cellOff[p] + (size_t)(strip - a.stripLo) * SP;
+        double *W1 = a.W1 + at0, *Yb = a.Yb + at0, *Zb = a.Zb + at0;
+        // the edge row this strip reads (strip+1's) and the one it writes
+        const int64_t inAt = a.bndOff[p] + (int64_t)(strip + 1 < S ? strip + 1 : strip) * n;
+        const int64_t outAt = a.bndOff[p] + (int64_t)strip * n;
+        const double *hA = a.bwdA + inAt, *hB = a.bwdB + inAt, *hC = a.bwdC + inAt;
+        const int *hE = a.bwdE + inAt;
+        double *oA = a.bwdA + outAt, *oB = a.bwdB + outAt, *oC = a.bwdC + outAt;
+        int *oE = a.bwdE + outAt;
         const int i = top - lane;
         const bool rowOk = i <= M;
         const RowParams P = row_params(a, rowOk ? i : 0);
@@ -307,7 +323,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 cB = ldexp(cB, cE - Ecur);
                 cC = ldexp(cC, cE - Ecur);
             }
-            if (lane == 0) expB[strip * NB + c] = Ecur;
+            if (lane == 0) expB[(int64_t)strip * NB + c] = Ecur;
         };
         auto step = [&](int t) {
             if (dConv != 0 && lane == (t & 63)) {  // this lane's first column of the newest block
@@ -343,12 +359,11 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 sy = y;
                 sz = zb;
                 if (lane == 63) {
-                    hA[j] = w;
-                    hB[j] = y;
-                    hC[j] = sum;
-                    hE[j] = El;
+                    oA[j] = w;
+                    oB[j] = y;
+                    oC[j] = sum;
+                    oE[j] = El;
                 }
-                if (i == 0) cs[j] = sum;
                 r3 = r2;
                 r2 = r1;
                 r1 = Wdn;
@@ -364,10 +379,12 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
             lastW = oW;
             lastY = oY;
             lastS = oS;
-            const size_t at = (size_t)t * 64 + lane;  // one contiguous run per step
-            W1[at] = sw;
-            Yb[at] = sy;
-            Zb[at] = sz;
+            if constexpr (kStore) {
+                const size_t at = (size_t)t * 64 + lane;  // one contiguous run per step
+                W1[at] = sw;
+                Yb[at] = sy;
+                Zb[at] = sz;
+            }
         };
         Rec cur = rec[max(n - 1 - lane, 0)];  // the records of the next block to enter the ring
         auto boundary = [&](int t) {
@@ -426,7 +443,7 @@ __global__ __launch_bounds__(64) void grad_z_kernel(GradArgs a) {
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n;
-    const double *cs = a.colSum + a.bndOff[p];
+    const double *cs = a.bwdC + a.bndOff[p];  // strip 0's edge: the column sums over all rows
     const int *eb = a.expB + a.expOff[p];  // strip 0 holds row 0
     const Rec *rec = a.rec + PR.recOff;
     double zm = 0.0;
@@ -451,15 +468,16 @@ __global__ __launch_bounds__(64) void grad_z_kernel(GradArgs a) {
     }
 }
 
-// Per (pair, strip) one wavefront: lane k takes row 64s+k and walks the
-// forward's steps t = 0..n+62 (column j = t-k), so each read is one contiguous
-// run of the diagonal-major matrices: the forward's cell is at step t, and the
-// backward's cell of row 64s+k or 64s+k+1 at column j+d is at step n+62-t-d or
-// n+61-t-d, the same in every lane (the strip's edge lanes read the
-// neighbouring strip). Cells outside the sequence (the first and last 63
-// steps of a lane) contribute zero. Flows of the row's parameters in registers, its match
-// flows by codon in LDS; each flow is its stored product times
-// 2^(forward exponent + backward exponent - Z's exponent).
+// Per (pair, strip s of stripLo..stripHi) one wavefront: lane k takes row
+// 64s+k and walks the forward's steps t = 0..n+62 (column j = t-k), so each
+// read is one contiguous run of the diagonal-major matrices: the forward's
+// cell is at step t, and the backward's cell of row 64s+k or 64s+k+1 at column
+// j+d is at step n+62-t-d or n+61-t-d, the same in every lane (the strip's
+// edge lanes read the neighbouring strips' edge rows). Cells outside the
+// sequence (the first and last 63 steps of a lane) contribute zero. Flows of
+// the row's parameters in registers, its match flows by codon in LDS; each
+// flow is its stored product times 2^(forward exponent + backward exponent -
+// Z's exponent).
 template <bool kLdsExp>
 __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
 #pragma clang fp contract(off)
@@ -468,7 +486,7 @@ __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
     // when every sequence of the launch has at most 64 kExpLds letters (kLdsExp),
     // read in place otherwise
     __shared__ int sExp[4][kLdsExp ? kExpLds : 1];
-    const int p = a.pairBase + blockIdx.x, s = blockIdx.y, lane = threadIdx.x;
+    const int p = a.pairBase + blockIdx.x, s = a.stripLo + blockIdx.y, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6, S = (M + 64) / 64;
     const int i = 64 * s + lane;
@@ -495,15 +513,33 @@ __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
     const Rec *rec = a.rec + PR.recOff;
     const uint8_t *codon = a.codon + PR.seqOff;
     const int64_t SP = (int64_t)(n + 63) * 64;
-    const size_t off = a.cellOff[p];
+    const size_t off = a.cellOff[p] + (size_t)(s - a.stripLo) * SP;
     const double *w0 = a.w0 + off, *Z0 = a.Z0 + off, *Yf = a.Y + off;
     const double *W1 = a.W1 + off, *Yb = a.Yb + off, *Zb = a.Zb + off;
     // offsets of column j = 0 (the forward's grow by 64 per column, the
-    // backward's fall by 64); a missing neighbour row reads row i and is zeroed
-    const int64_t fHere = s * SP + (int64_t)lane * 65;
-    const int64_t fAbove = !hasAbove ? fHere : lane > 0 ? fHere - 65 : (s - 1) * SP + 63 * 64 + 63;
-    const int64_t bHere = s * SP + (int64_t)(n - 1) * 64 + 63 * 65 - (int64_t)lane * 65;
-    const int64_t bBelow = !hasBelow ? bHere : lane < 63 ? bHere - 65 : (s + 1) * SP + (int64_t)(n - 1) * 64 + 63 * 64 + 63;
+    // backward's fall by 64)
+    const int64_t fHere = (int64_t)lane * 65;
+    const int64_t bHere = (int64_t)(n - 1) * 64 + 63 * 65 - (int64_t)lane * 65;
+    // the neighbour rows: Y of row i-1 and W1, Yb of row i+1 at column c are
+    // yU[c yUs], wD[c wDs], ybD[c wDs]; inside the strip the matrices, at its
+    // edges the neighbour strips' edge rows (one value per column); a missing
+    // neighbour row reads row i and is zeroed
+    const double *yU = Yf + fHere - 65, *wD = W1 + bHere - 65, *ybD = Yb + bHere - 65;
+    int64_t yUs = 64, wDs = -64;
+    if (!hasAbove) {
+        yU = Yf + fHere;
+    } else if (lane == 0) {
+        yU = a.fwdB + a.bndOff[p] + (int64_t)(s - 1) * n;
+        yUs = 1;
+    }
+    if (!hasBelow) {
+        wD = W1 + bHere;
+        ybD = Yb + bHere;
+    } else if (lane == 63) {
+        wD = a.bwdA + a.bndOff[p] + (int64_t)(s + 1) * n;
+        ybD = a.bwdB + a.bndOff[p] + (int64_t)(s + 1) * n;
+        wDs = 1;
+    }
     const int *eFh = kLdsExp ? sExp[1] : expRow[1], *eFu = kLdsExp ? (lane > 0 ? sExp[1] : sExp[0])
                                                                    : (lane > 0 ? expRow[1] : expRow[0]);
     const int *eBh = kLdsExp ? sExp[2] : expRow[2], *eBd = kLdsExp ? (lane < 63 ? sExp[2] : sExp[3])
@@ -524,11 +560,11 @@ __global__ __launch_bounds__(64) void grad_counts_kernel(GradArgs a) {
             const double m1 = j + 1 < n ? v : 0.0, m2 = j + 2 < n ? v : 0.0, m3 = j + 3 < n ? v : 0.0;
             const int code = (int)(rec[j].codes & 255);  // code(j+1)
             const int cod = codon[r1];
-            const double w = v * w0[fHere + fb], z0 = Z0[fHere + fb], yup = up * Yf[fAbove + fb];
-            const double wd1 = dn * m1 * W1[bBelow - (int64_t)r1 * 64];
-            const double wd2 = dn * m2 * W1[bBelow - (int64_t)r2 * 64];
-            const double wd3 = dn * m3 * W1[bBelow - (int64_t)r3 * 64];
-            const double ybd = dn * Yb[bBelow - fb];
+            const double w = v * w0[fHere + fb], z0 = Z0[fHere + fb], yup = up * yU[(int64_t)j * yUs];
+            const double wd1 = dn * m1 * wD[(int64_t)r1 * wDs];
+            const double wd2 = dn * m2 * wD[(int64_t)r2 * wDs];
+            const double wd3 = dn * m3 * wD[(int64_t)r3 * wDs];
+            const double ybd = dn * ybD[(int64_t)j * wDs];
             const double wh1 = m1 * W1[bHere - (int64_t)r1 * 64], wh2 = m2 * W1[bHere - (int64_t)r2 * 64];
             const double zb3 = m3 * Zb[bHere - (int64_t)r3 * 64];
             const double e = Er[code], g = a.bg[code];
@@ -588,14 +624,20 @@ __global__ void grad_reduce_kernel(GradReduceArgs a, const double *factor) {
 hipError_t launch_grad_forward(const GradArgs &a, int npairs, int maxN, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
     (void)maxN;
-    hipLaunchKernelGGL(grad_forward_kernel, dim3(npairs), dim3(64), 0, s, a);
+    if (a.store)
+        hipLaunchKernelGGL(grad_forward_kernel<true>, dim3(npairs), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(grad_forward_kernel<false>, dim3(npairs), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_grad_backward(const GradArgs &a, int npairs, int maxN, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
     (void)maxN;
-    hipLaunchKernelGGL(grad_backward_kernel, dim3(npairs), dim3(64), 0, s, a);
+    if (a.store)
+        hipLaunchKernelGGL(grad_backward_kernel<true>, dim3(npairs), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(grad_backward_kernel<false>, dim3(npairs), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -607,10 +649,12 @@ hipError_t launch_grad_z(const GradArgs &a, int npairs, hipStream_t s) {
 
 hipError_t launch_grad_counts(const GradArgs &a, int npairs, int M, int maxN, hipStream_t s) {
     if (npairs <= 0) return hipSuccess;
+    (void)M;
+    const dim3 grid(npairs, a.stripHi - a.stripLo + 1);
     if (maxN <= 64 * kExpLds)
-        hipLaunchKernelGGL(grad_counts_kernel<true>, dim3(npairs, (M + 64) / 64), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(grad_counts_kernel<true>, grid, dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL(grad_counts_kernel<false>, dim3(npairs, (M + 64) / 64), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(grad_counts_kernel<false>, grid, dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

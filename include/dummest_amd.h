@@ -289,7 +289,7 @@ double dm_last_bw_ms(const dm_engine *e);
  * sequences, on the GPU (FP64 forward, backward and expected counts over the
  * reference's recurrences, dummest-dp-forward.hh:9-250 and
  * dummest-dp-backward.hh:444-615, scaled by a power of two per 64-row strip
- * and 64-column block, so any ln Z fits; sequences of 3..2^20 letters). Per
+ * and 64-column block, so any ln Z fits; sequences of 3..2^28 letters). Per
  * sequence s the total of
  * all local alignments Z_s = sum_{i,j} prefixSeed(j) W1(i,j) (ln_z[s]; may be
  * NULL), and counts = sum_s weights[s] (NULL = 1) times the expected uses of
@@ -300,11 +300,19 @@ double dm_last_bw_ms(const dm_engine *e);
  * The background and the null model stay fixed. dm_last_grad_ms: device time
  * of the DP, total and count kernels of the last call; dm_last_grad_cells: its cells
  * (sum of (M+1) n). "grad_budget" (dm_set_option) bounds the FP64 matrices of
- * one launch batch (default 8 GiB). */
+ * one launch batch (default 8 GiB; 48 B per cell). A sequence whose matrices
+ * alone pass it is checkpointed: both passes run once keeping only the edge
+ * row of every 64-row strip (48 B per strip and letter), then each segment of
+ * strips that fits the budget (at least one strip, 3 KiB per letter) is
+ * computed again from its edge rows and counted; the results are the same bit
+ * for bit. dm_last_grad_segments: the segments so recomputed by the last call
+ * (0 when every sequence was held in full). Replaces the reference's full
+ * O(M n) matrix per lane batch (dummest-dp-backward.hh:252). */
 int dm_grad(dm_engine *e, int32_t prof, int32_t nseq, const char *dna, const int64_t *offsets, const int32_t *lengths,
             const double *weights, double *ln_z, double *counts);
 double dm_last_grad_ms(const dm_engine *e);
 int64_t dm_last_grad_cells(const dm_engine *e);
+int64_t dm_last_grad_segments(const dm_engine *e);
 /* The same gradient over a registered sequence set, for callers that evaluate
  * it many times over the same sequences (gradient descent on a profile):
  * dm_grad_set_create decodes the sequences once, keeps them in the engine's

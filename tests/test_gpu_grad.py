@@ -303,7 +303,88 @@ def test_longest_sequence_limit(engine):
     v, pa = G.profile_from_logits(np.zeros((5, 20)), np.zeros((6, 7)), np.full(29, 0.05))
     pid = engine.add_profile(5, v, pa, np.full(29, 0.05))
     with pytest.raises(D.DummestError, match="letters"):
-        engine.grad_counts(pid, ["A" * ((1 << 20) + 1)])
+        engine.grad_counts(pid, ["A" * ((1 << 28) + 1)])
+
+
+def _grad_with_budget(engine, pid, seqs, budget, weights=None):
+    engine.set_option("grad_budget", budget)
+    try:
+        lnz, c = engine.grad_counts(pid, seqs, weights=weights)
+        return lnz, c, engine.last_grad_segments
+    finally:
+        engine.set_option("grad_budget", 8 << 30)
+
+
+def test_checkpointed_segments_bit_identical(engine):
+    """Sequences whose matrices pass the budget are checkpointed (the edge row
+    of every 64-row strip kept, each segment of strips computed again): ln Z
+    and the counts equal the full-matrix path's bit for bit, at segments of
+    one, two and three strips and with a sequence held in full beside them;
+    the full path's are checked against the oracle's flows in
+    test_counts_match_oracle_matrices and here for the longest sequence."""
+    rng = np.random.default_rng(71)
+    M = 200  # 4 strips
+    lens = [3000, 1500, 700]
+    el, tl, bg, seqs = make_case(rng, M, lens, homolog_every=1)
+    v, pa = G.profile_from_logits(el, tl, bg)
+    v, pa, b = (np.ascontiguousarray(x.astype(np.float32)) for x in (v, pa, bg))
+    pid = engine.add_profile(M, v, pa, b)
+    wts = [1.0, 0.25, 3.0]
+    print("checkpointed gradient: full path", flush=True)
+    lnz, c, seg = _grad_with_budget(engine, pid, seqs, 8 << 30, wts)
+    assert seg == 0
+    strip = lambda n: 48 * (n + 63) * 64  # bytes of one strip's six matrices
+    # budget of two strips of the longest: it runs in 2 segments of 2 strips,
+    # the 1500-letter one in segments of 3 and 1, the shortest in full
+    cases = [(2 * strip(3000), 4), (1, 12), (3 * strip(3000), 2)]
+    for budget, want_seg in cases:
+        print("checkpointed gradient: budget", budget, flush=True)
+        lnz2, c2, seg = _grad_with_budget(engine, pid, seqs, budget, wts)
+        assert seg == want_seg, (budget, seg)
+        np.testing.assert_array_equal(lnz2, lnz)
+        np.testing.assert_array_equal(c2, c)
+    prof = O.Profile("k", M, v, pa, b, [])
+    _, got, seg = _grad_with_budget(engine, pid, seqs[:1], 1)
+    assert seg == 4
+    want, by_codon, lnz_o = OF.matrix_flows(prof, seqs[0])
+    for k, w in want.items():
+        assert np.abs(got[:, OF.PARAM_INDEX[k]] - w).max() <= 1e-4 * max(w.max(), 1e-6), k
+    assert np.abs(got[:, 11:] - by_codon).max() <= 1e-4 * max(by_codon.max(), 1e-6)
+
+
+def test_sequence_past_a_million_letters(engine):
+    """A 1.1-million-letter contig (past 2^20 letters) with three embedded
+    homologs: held in full (6.8 GB of matrices) and checkpointed one strip at
+    a time, the same numbers bit for bit; ln Z against the oracle's FP64
+    total."""
+    rng = np.random.default_rng(72)
+    M = 70  # 2 strips
+    n = 1_100_000
+    el, tl, bg, _ = make_case(rng, M, [10])
+    p = G._softmax(el)
+    host = synth.random_dna(rng, n)
+    for c in range(3):
+        prot = np.array([rng.choice(20, p=p[r]) for r in range(M)])
+        hom = synth.mutate_dna(rng, synth.reverse_translate(rng, prot), indel=0.01)
+        at = (c + 1) * n // 4
+        host = host[:at] + hom + host[at + len(hom):]
+    assert len(host) == n
+    v, pa = G.profile_from_logits(el, tl, bg)
+    v, pa, b = (np.ascontiguousarray(x.astype(np.float32)) for x in (v, pa, bg))
+    pid = engine.add_profile(M, v, pa, b)
+    print("contig of", n, "letters: full matrices", flush=True)
+    lnz, c, seg = _grad_with_budget(engine, pid, [host], 8 << 30)
+    assert seg == 0
+    print("contig of", n, "letters: checkpointed", flush=True)
+    lnz2, c2, seg = _grad_with_budget(engine, pid, [host], 4 << 30)
+    assert seg == 2
+    np.testing.assert_array_equal(lnz2, lnz)
+    np.testing.assert_array_equal(c2, c)
+    assert np.allclose(c[:, 11:].sum(1), c[:, 10], rtol=1e-12, atol=1e-300)
+    want = oracle_lnz(el, tl, bg, [host])
+    print("ln Z", lnz, "oracle", want, flush=True)
+    assert abs(lnz[0] - want[0]) < 2e-7 * n + 1e-5
+    assert lnz[0] > 20.0  # the homologs are found
 
 
 def test_unusual_letters_and_short_sequences(engine):
