@@ -9,8 +9,9 @@ sequences and the work queue already resident on the device.
 
 Reported beside it, in the same run:
   * the plain per-pair form (one DP launch per pair) on a sample of the pairs;
-  * the CPU oracle (the repo's C++ restatement of the reference engine) on a
-    bounded sample, threaded over the host cores;
+  * the lane-batched SIMD CPU restatement of the same pass (one sequence per
+    AVX-512 / AVX2 lane, length-sorted lane batches, -O3) on a bounded random
+    sample of the pairs, threaded over the usable host cores (at most 16);
   * the roofline of the dominant DP kernel: 43 algorithmic FP32 flops per
     forward+backward cell against the 157.3 TFLOP/s FP32 peak;
   * the calibration sweep, sequence masking, and the gradient of the summed
@@ -40,29 +41,46 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, pairs_idx, budget_s=15.0, threads=None):
-    """Oracle fwd+bwd scores on pairs grouped by profile until the budget is spent."""
+def cpu_threads():
+    """The host cores this process may use, at most 16 (one GPU's share of the
+    host's cores)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(w, order, gpu_mid, budget_s=15.0, threads=None):
+    """The lane-batched SIMD CPU restatement of the same forward+backward score
+    pass (oracle/simd_baseline.cpp: one sequence per AVX-512 / AVX2 lane,
+    length-sorted lane batches, all usable host cores) on a random sample of
+    the workload's pairs sized to about budget_s seconds; its mid scores are
+    checked against the GPU's for the same pairs."""
     import oracle as O
-    threads = threads or min(16, os.cpu_count() or 1)
-    by_prof = {}
-    for k in pairs_idx:
-        by_prof.setdefault(int(w["prof"][k]), []).append(int(w["seq"][k]))
-    cells = 0
-    done = 0
-    elapsed = 0.0
-    for p, seqs in by_prof.items():
-        prof = w["oracle_profiles"][p]
-        codes = [w["codes"][s] for s in seqs]
-        t = time.perf_counter()
-        O.fwdbwd_batch(prof, codes, threads=threads)
-        elapsed += time.perf_counter() - t
-        cells += sum((prof.M + 1) * len(c) for c in codes)
-        done += len(seqs)
-        if elapsed > budget_s:
-            break
-    return dict(value=cells / elapsed, unit="cells/s", cores=threads, kind="port",
-                sample=f"{done} pairs of the same workload ({cells:.3g} cells, {elapsed:.1f} s), "
-                       f"oracle fwd+bwd scores (FP32, exact ExpScore anchors), {threads} threads")
+    from dummest_amd import es_log
+    threads = threads or cpu_threads()
+    profs = [w["oracle_profiles"][k] for k in range(len(w["Ms"]))]
+    pp, ps = np.asarray(w["prof"]), np.asarray(w["seq"])
+    pair_cells = (np.asarray(w["Ms"])[pp] + 1) * np.array([len(c) for c in w["codes"]])[ps]
+    # a first sample of ~1% of the pairs measures the rate that sizes the main one
+    cal = order[: max(256, len(order) // 100)]
+    _, _, sec, lanes = O.simd_fwdbwd_pairs(profs, w["codes"], pp[cal], ps[cal], threads=threads, want_bwd=False)
+    rate = pair_cells[cal].sum() / sec
+    k = int(np.searchsorted(np.cumsum(pair_cells[order]), rate * budget_s)) + 1
+    sel = order[:min(k, len(order))]
+    mid, _, sec, lanes = O.simd_fwdbwd_pairs(profs, w["codes"], pp[sel], ps[sel], threads=threads, want_bwd=False)
+    cells = int(pair_cells[sel].sum())
+    a = es_log(mid[:, 0], mid[:, 1].astype(np.int32))
+    b = es_log(gpu_mid[0][sel], gpu_mid[1][sel])
+    dev = float(np.max(np.where(a == b, 0.0, np.abs(a - b))))
+    isa = "AVX-512" if lanes == 16 else "AVX2"
+    return dict(value=cells / sec, unit="cells/s", cores=threads, kind="port",
+                sample=f"{len(sel)} random pairs of the same workload ({cells:.3g} cells, {sec:.1f} s): "
+                       f"lane-batched SIMD restatement ({isa}, {lanes} sequences per vector, length-sorted "
+                       f"lane batches, -O3), {threads} threads on {O.cpu_model()} (nproc {os.cpu_count()}); "
+                       f"max |mid score - GPU| {dev:.2e} nats",
+                max_abs_dnats_vs_gpu=dev)
 
 
 def traffic_of(kernel, cells_per_launch, build):
@@ -277,7 +295,7 @@ def main():
         for k, p in enumerate(pids):
             Mk, v, pp, b = e.get_profile(p)
             w["oracle_profiles"][k] = O.Profile(f"p{k}", Mk, v, pp, b, [])
-        cpu = cpu_baseline(w, rng.permutation(len(prof)), budget_s=args.cpu_budget)
+        cpu = cpu_baseline(w, rng.permutation(len(prof)), (mm, me), budget_s=args.cpu_budget)
         log(f"cpu: {cpu['value']:.4g} cells/s ({cpu['cores']} threads)")
         cpu["value"] = float(f"{cpu['value']:.4g}")
 

@@ -570,3 +570,87 @@ def es_batch_scale(a, f):
 
 def es_promote(v, cum):
     return _es2(lib().or_es_promote, v, cum)
+
+
+# -- the lane-batched SIMD CPU baseline (simd_baseline.cpp; measurement only) --
+_simd = {}
+_SIMD_FLAGS_V4 = ("avx512f", "avx512dq", "avx512bw", "avx512vl")
+
+
+def cpu_flags() -> set:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def simd_variant(prefer: str | None = None) -> str:
+    """'v4' (AVX-512, 16 lanes) where the host has it, else 'v3' (AVX2, 8 lanes)."""
+    if prefer:
+        return prefer
+    return "v4" if all(f in cpu_flags() for f in _SIMD_FLAGS_V4) else "v3"
+
+
+def simd_lib(variant: str | None = None):
+    v = simd_variant(variant)
+    if v not in _simd:
+        path = os.path.join(HERE, "_build", f"libsimd_{v}.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        i, f, d = ctypes.c_int, ctypes.c_float, ctypes.c_double
+        L.sb_lanes.restype = i
+        L.sb_fwdbwd_pairs.argtypes = [i, _p(i), _p(ctypes.c_int64), _p(ctypes.c_int64), _p(f), _p(f), _p(f), _p(d),
+                                      _p(ctypes.c_uint8), _p(ctypes.c_int64), _p(i), i, _p(i), _p(i), _p(f), _p(f), i, i]
+        L.sb_fwdbwd_pairs.restype = d
+        _simd[v] = L
+    return _simd[v]
+
+
+def simd_fwdbwd_pairs(profiles: list, seqs: list, pair_prof, pair_seq, opts=DEFAULT_OPTS, threads=None,
+                      variant: str | None = None, want_bwd=True):
+    """Forward+backward scores of (profile, sequence) pairs by the SIMD
+    baseline: (mid (npairs, 2), bwd (npairs, 2), seconds of batched work,
+    lanes). profiles: Profile objects; seqs: code arrays; pair_prof /
+    pair_seq: indices into them."""
+    L = simd_lib(variant)
+    Ms = np.array([p.M for p in profiles], np.int32)
+    vals = [np.ascontiguousarray(p.values, np.float32).ravel() for p in profiles]
+    pars = [np.ascontiguousarray(p.params, np.float32).ravel() for p in profiles]
+    val_off = np.zeros(len(profiles), np.int64)
+    par_off = np.zeros(len(profiles), np.int64)
+    if len(profiles) > 1:
+        val_off[1:] = np.cumsum([v.size for v in vals[:-1]])
+        par_off[1:] = np.cumsum([q.size for q in pars[:-1]])
+    values = np.concatenate(vals) if vals else np.zeros(1, np.float32)
+    params = np.concatenate(pars) if pars else np.zeros(1, np.float32)
+    bgs = np.ascontiguousarray(np.concatenate([np.asarray(p.bg, np.float32)[:29] for p in profiles]))
+    codes, offs, lens = pack(seqs)
+    pp = np.ascontiguousarray(pair_prof, np.int32)
+    ps = np.ascontiguousarray(pair_seq, np.int32)
+    mid = np.zeros((len(pp), 2), np.float32)
+    bwd = np.zeros((len(pp), 2), np.float32)
+    o = np.array(opts, np.float64)
+    f = ctypes.c_float
+    i = ctypes.c_int
+    sec = L.sb_fwdbwd_pairs(len(profiles), _ptr(Ms, i), _ptr(val_off, ctypes.c_int64), _ptr(par_off, ctypes.c_int64),
+                            _ptr(values, f), _ptr(params, f), _ptr(bgs, f), _ptr(o, ctypes.c_double),
+                            _ptr(codes if codes.size else np.zeros(1, np.uint8), ctypes.c_uint8),
+                            _ptr(offs, ctypes.c_int64), _ptr(lens, i), len(pp), _ptr(pp, i), _ptr(ps, i),
+                            _ptr(mid, f), _ptr(bwd, f), int(bool(want_bwd)), int(threads or os.cpu_count() or 1))
+    return mid, bwd, sec, L.sb_lanes()
