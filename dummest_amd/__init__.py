@@ -90,6 +90,10 @@ def _declare(L):
     L.dm_pair_matrices.argtypes = [vp, i32, i32, _P(f), _P(i32), _P(f), _P(i32)]
     L.dm_pair_null.argtypes = [vp, i32, i32, _P(f), _P(f), _P(f)]
     L.dm_pair_hits.argtypes = [vp, i32, i32, d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
+    L.dm_pairs_hits.argtypes = [vp, i64, _P(i32), _P(i32), _P(d), _P(d), i64, _P(i32), i64, _P(i64), _P(i64),
+                                _P(i64)]
+    L.dm_last_hits_ms.argtypes = [vp]
+    L.dm_last_hits_ms.restype = d
     L.dm_plan_create.argtypes = [vp, ctypes.c_int, i64, _P(i32), _P(i32), ctypes.c_int, _P(vp)]
     L.dm_plan_execute.argtypes = [vp, vp, _P(f), _P(f), _P(i32), _P(f), _P(i32)]
     L.dm_plan_capture.argtypes = [vp, vp]
@@ -600,6 +604,40 @@ class Engine:
         f = ctypes.c_float
         self._check(lib().dm_pair_null(self._h, prof, seq, _ptr(ps, f), _ptr(ss, f), _ptr(sc, f)), "dm_pair_null")
         return ps, ss, sc
+
+    def find_similarities_batch(self, prof, seq, min_prob_ratio, hit_cap=4096, seg_cap=1 << 16):
+        """Hits of many pairs in one call (dm_pairs_hits): per pair, the list
+        find_similarities returns for it. min_prob_ratio: one value or one per pair."""
+        p, s = self._pairs(prof, seq)
+        n = len(p)
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(min_prob_ratio, np.float64), (n,)))
+        first = np.zeros(n + 1, np.int64)
+        nh, ns = ctypes.c_int64(), ctypes.c_int64()
+        i32, i64 = ctypes.c_int32, ctypes.c_int64
+        while True:
+            hits = np.zeros((max(1, hit_cap), 6), np.float64)
+            segs = np.zeros((max(1, seg_cap), 3), np.int32)
+            rc = lib().dm_pairs_hits(self._h, n, _ptr(p, i32), _ptr(s, i32), _ptr(r, ctypes.c_double),
+                                     _ptr(hits, ctypes.c_double), hit_cap, _ptr(segs, i32), seg_cap,
+                                     _ptr(first, i64), ctypes.byref(nh), ctypes.byref(ns))
+            if rc != DM_ERANGE:
+                break
+            hit_cap, seg_cap = max(hit_cap, nh.value), max(seg_cap, ns.value)
+        self._check(rc, "dm_pairs_hits")
+        out, k = [], 0
+        for q in range(n):
+            lst = []
+            for h in hits[first[q]:first[q + 1]]:
+                c = int(h[5])
+                lst.append((float(h[0]), float(h[1]), int(h[2]), int(h[3]), int(h[4]),
+                            [tuple(int(x) for x in segs[k + g]) for g in range(c)]))
+                k += c
+            out.append(lst)
+        return out
+
+    @property
+    def last_hits_ms(self) -> float:
+        return lib().dm_last_hits_ms(self._h)
 
     def find_similarities(self, prof: int, seq: int, min_prob_ratio: float, hit_cap=4096, seg_cap=1 << 20):
         """Hits of one pair: list of (lnratio, m, e, anchor1, anchor2, [(start1, start2, length)])."""

@@ -30,6 +30,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -322,7 +323,15 @@ int engineFail(dm_engine *e, const char *what, int rc) {
 
 }  // namespace
 
+// DM_TIMING set in the environment: the run's wall time and the part spent
+// in the post-gate step (dm_pairs_hits), on stderr at the end.
+static double wallNow() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+static double hitsWall = 0, hitsDevice = 0;
+
 int main(int argc, char *argv[]) {
+    const double runStart = wallNow();
     Options o;
     std::copy(kDefaults, kDefaults + 6, o.rates);
     {
@@ -788,39 +797,55 @@ int main(int argc, char *argv[]) {
                                  be.data());
             if (r) return engineFail(E, "dm_fwdbwd_scores", r);
             nFullPairs += (int64_t)pp.size();
-            std::vector<double> hits(6 * 4096);
-            std::vector<int32_t> segs(3 * (1 << 20));
+            // gate: best anchored score >= minProbRatio (dummest-dp-forward.hh:263-271).
+            // The score pass rounds differently from the reference (fused
+            // multiply-adds, another summation order), so here a pair is
+            // dropped only when it is clearly below the threshold; the
+            // post-gate step (dm_pairs_hits) works on matrices that equal the
+            // reference's and applies the exact gate itself.
+            std::vector<int32_t> gp, gs;
+            std::vector<double> gr;
+            std::vector<size_t> gk;
             for (size_t k = 0; k < pp.size(); k++) {
-                const int pi = reqProfRatio[rq[k]].first;
                 const float ratio = reqProfRatio[rq[k]].second;
                 if (ratio < 0) continue;  // calibration-style lanes report no alignments
-                // gate: best anchored score >= minProbRatio (dummest-dp-forward.hh:263-271).
-                // The score pass rounds differently from the reference (fused
-                // multiply-adds, another summation order), so here a pair is
-                // dropped only when it is clearly below the threshold; the
-                // post-gate step (dm_pair_hits) works on matrices that equal the
-                // reference's and applies the exact gate itself.
                 if (ratio > 0) {
                     if (mm[k] == 0.f) continue;
                     const double lnGot = std::log((double)mm[k]) + (double)me[k] * 0.6931471805599453;
                     if (lnGot < std::log((double)ratio) - 1e-5) continue;
                 }
-                ++nGated;
-                int32_t nh = 0, ns = 0;
-                r = dm_pair_hits(E, pp[k], ss[k], (double)ratio, hits.data(), (int32_t)(hits.size() / 6), segs.data(),
-                                 (int32_t)(segs.size() / 3), &nh, &ns);
-                if (r == DM_ERANGE) {  // more hits than the buffers hold: grow them to the reported need
-                    hits.resize(6 * std::max<size_t>(hits.size() / 6, (size_t)nh));
-                    segs.resize(3 * std::max<size_t>(segs.size() / 3, (size_t)ns));
-                    r = dm_pair_hits(E, pp[k], ss[k], (double)ratio, hits.data(), (int32_t)(hits.size() / 6),
-                                     segs.data(), (int32_t)(segs.size() / 3), &nh, &ns);
-                }
-                if (r) return engineFail(E, "dm_pair_hits", r);
+                gp.push_back(pp[k]);
+                gs.push_back(ss[k]);
+                gr.push_back((double)ratio);
+                gk.push_back(k);
+            }
+            nGated += (int64_t)gk.size();
+            // the hits of every gated pair in one batched call
+            std::vector<double> hits(6 * std::max<size_t>(4096, 4 * gk.size()));
+            std::vector<int32_t> segs(3 * std::max<size_t>(1 << 20, 64 * gk.size()));
+            std::vector<int64_t> first(gk.size() + 1, 0);
+            int64_t nh = 0, ns = 0;
+            const double hitsStart = wallNow();
+            r = dm_pairs_hits(E, (int64_t)gk.size(), gp.data(), gs.data(), gr.data(), hits.data(),
+                              (int64_t)(hits.size() / 6), segs.data(), (int64_t)(segs.size() / 3), first.data(), &nh,
+                              &ns);
+            if (r == DM_ERANGE) {  // more hits than the buffers hold: grow them to the reported need
+                hits.resize(6 * (size_t)nh);
+                segs.resize(3 * (size_t)ns);
+                r = dm_pairs_hits(E, (int64_t)gk.size(), gp.data(), gs.data(), gr.data(), hits.data(), nh,
+                                  segs.data(), ns, first.data(), &nh, &ns);
+            }
+            if (r) return engineFail(E, "dm_pairs_hits", r);
+            hitsWall += wallNow() - hitsStart;
+            hitsDevice += dm_last_hits_ms(E) * 1e-3;
+            int64_t so = 0;
+            for (size_t gi = 0; gi < gk.size(); gi++) {
+                const size_t k = gk[gi];
+                const int pi = reqProfRatio[rq[k]].first;
                 const Target &T = targets[reqTarget[rq[k]]];
                 const HostProfile &P = profiles[pi];
                 const SeqState &S = T.rec;
-                int so = 0;
-                for (int h = 0; h < nh; h++) {
+                for (int64_t h = first[gi]; h < first[gi + 1]; h++) {
                     const double lnr = hits[6 * h];
                     const int a1 = (int)hits[6 * h + 3], a2 = (int)hits[6 * h + 4], nseg = (int)hits[6 * h + 5];
                     const int32_t *sg = &segs[3 * so];
@@ -915,5 +940,10 @@ int main(int argc, char *argv[]) {
     std::cout << "# Filter stats: filter_batches=" << nFilterPairs << " discarded_empty=" << (nFilterPairs - nFullPairs)
               << " max_dp_width=" << maxWidth << "\n";
     if (!spool.empty()) std::remove(spool.c_str());
+    if (std::getenv("DM_TIMING")) {
+        const double total = wallNow() - runStart;
+        std::fprintf(stderr, "timing: run %.3f s, post-gate hits %.3f s (%.1f%%; %.3f s on the device) for %lld gated pairs\n",
+                     total, hitsWall, 100.0 * hitsWall / total, hitsDevice, (long long)nGated);
+    }
     return 0;
 }

@@ -74,6 +74,20 @@ struct PostInput {
     float bgfs1 = 0, bgfs2 = 0;
 };
 
+// An alignment anchor: its best anchored score and cell (profile row a1,
+// column a2).
+struct Anchor {
+    ES pair;
+    int a1, a2;
+};
+
+// The anchors of a pair in the order the traceback tries them
+// (dummest-dp-forward.hh:459-500): every column's best cell, ranked highest
+// first by std::sort over all columns (whose order of tied anchors the
+// results depend on), those at or above minProbRatio kept. For minProbRatio <
+// 0, only the first of the best anchors.
+void rankAnchors(const ES *colBest, const int *colRow, int n, float minProbRatio, std::vector<Anchor> &out);
+
 // A stored FP32 DP value at a power-of-two offset as an ExpScore (0 for
 // non-positive values, +inf saturates as promoteRescaledScore does).
 ES esFromStored(float v, int off);

@@ -10,8 +10,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -22,6 +24,7 @@
 #include "bw.h"
 #include "dm_layout.h"
 #include "grad.h"
+#include "postgate.h"
 #include "dm_host.h"
 #include "../../include/dummest_amd.h"
 
@@ -44,6 +47,28 @@ struct DevBuf {
     }
     void release() {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+// Pinned host staging, grown on demand.
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max(bytes, (size_t)4096);
+        hipError_t r = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (r == hipSuccess) cap = want;
+        return r;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -155,6 +180,18 @@ struct dm_engine {
         int64_t seqEpoch = -1, profGen = -1;
         std::vector<Hit> hits;
     } hitCache;
+    // hits of the last dm_pairs_hits call (the same, for a whole pair list)
+    struct HitsCache {
+        std::vector<int32_t> prof, seq;
+        std::vector<double> ratio;
+        int64_t seqEpoch = -1, profGen = -1;
+        std::vector<std::vector<Hit>> hits;
+    } hitsCache;
+    // batched post-gate path (dm_pairs_hits): dump matrices, tables, walk output
+    DevBuf dPgW0, dPgW1, dPgEf, dPgEb, dPgPairs, dPgRes, dPgMat, dPgCol, dPgInt, dPgSeg, dPgHit, dPgCand;
+    HostBuf hPgCand, hPgHit, hPgSeg;  // their host staging
+    double lastHitsMs = 0;    // device time of the last dm_pairs_hits call
+    int64_t lastHitsPairs = 0;
     int64_t profGen = 0;  // profile edits (a cached hit list of an older generation is stale)
 
     float profileMaskThr() const { return (maskBits & 1) ? maskThr : -1.f; }
@@ -324,7 +361,9 @@ static void buildChunk(const dm_engine *E, int mode, const std::vector<int64_t> 
     while (i < en) {
         const int64_t p0 = sorted[i];
         const int cfg = cfgOf[p0];
-        const int P = 64 / E->cfgG[cfg];
+        // a dump unit holds one pair: the wave's scale exponents then follow
+        // that pair alone, as when it is dumped on its own
+        const int P = mode == kDump ? 1 : 64 / E->cfgG[cfg];
         std::vector<int64_t> u;
         u.push_back(p0);
         size_t j = i + 1;
@@ -958,8 +997,11 @@ void dm_engine_destroy(dm_engine *e) {
                       &e->dOut, &e->dTanBases, &e->dTanProbs, &e->dTanScales, &e->dTanMeta, &e->dTanB2f,
                       &e->dBwLetters, &e->dBwMeta, &e->dBwTables, &e->dBwMat, &e->dBwRaw,
                       &e->dBwSeqW, &e->dBwOut, &e->dGradCodon, &e->dGradMat, &e->dGradBnd, &e->dGradMeta,
-                      &e->dGradRaw, &e->dGradW, &e->dGradOut, &e->dGradExp})
+                      &e->dGradRaw, &e->dGradW, &e->dGradOut, &e->dGradExp, &e->dPgW0, &e->dPgW1,
+                      &e->dPgEf, &e->dPgEb, &e->dPgPairs, &e->dPgRes, &e->dPgMat, &e->dPgCol, &e->dPgInt,
+                      &e->dPgSeg, &e->dPgHit, &e->dPgCand})
         b->release();
+    for (HostBuf *b : {&e->hPgCand, &e->hPgHit, &e->hPgSeg}) b->release();
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1510,6 +1552,360 @@ int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio,
     hc.profGen = e->profGen;
     return writeHits(e, hc.hits, hits, hit_cap, segs, seg_cap, nhits, nsegs);
 }
+
+// The post-gate path of many pairs at once (postgate.hip): per batch of pairs
+// whose device buffers fit the budget, the dump-mode DP of every pair (one
+// launch per kernel configuration), the per-column pass, flank totals and MEA
+// tables on the device; the anchors ranked on the host (rankAnchors); the
+// tracebacks on the device. The same numbers as dm_pair_hits pair by pair.
+static int pairsHitsRun(dm_engine *e, int64_t npairs, const int32_t *prof, const int32_t *seq, const double *ratio,
+                        std::vector<std::vector<Hit>> &out) {
+    out.assign((size_t)npairs, {});
+    const int nprof = (int)e->profiles.size(), nseq = (int)e->seqLen.size();
+    for (int64_t k = 0; k < npairs; k++) {
+        if (prof[k] < 0 || prof[k] >= nprof || seq[k] < 0 || seq[k] >= nseq)
+            return fail(e, DM_EINVAL, "pair references an unknown profile or sequence");
+        if (e->seqLen[seq[k]] < 3) return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
+    }
+    int rc = syncProfiles(e);
+    if (rc) return rc;
+    rc = syncSequences(e);
+    if (rc) return rc;
+    const int ncfg = (int)e->cfgG.size();
+    std::vector<int> cfgOf((size_t)npairs);
+    std::vector<int64_t> need((size_t)npairs);  // device bytes of a pair (dump, tables, columns, walk)
+    for (int64_t k = 0; k < npairs; k++) {
+        const int M = e->profiles[prof[k]].M, n = e->seqLen[seq[k]];
+        const int cfg = chooseConfig(e, M, n, false);
+        cfgOf[k] = cfg;
+        const int64_t ROWS = (int64_t)e->cfgG[cfg] * e->cfgR[cfg], S = stripsOf(e, cfg, M);
+        const int64_t steps = S * (n + e->cfgG[cfg] - 1);
+        need[k] = steps * ROWS * 8 + steps * 8 + 3 * (int64_t)(M + 2) * (n + 7) * 8 + (int64_t)kPostCols * (n + 8) * 8 +
+                  (2 * (int64_t)(n + 8) + M + 2) * 4 + (3 * (int64_t)n + 3 * (M + 4)) * 12 + (int64_t)n * 40;
+    }
+    int64_t budget = (int64_t)2 << 30;
+    {
+        size_t freeB = 0, totalB = 0;
+        if (hipMemGetInfo(&freeB, &totalB) == hipSuccess)
+            budget = std::max<int64_t>(budget, std::min<int64_t>((int64_t)32 << 30, (int64_t)(freeB / 4)));
+        else
+            (void)hipGetLastError();
+    }
+    if (!e->ev0) HIPCHK(hipEventCreate(&e->ev0));
+    if (!e->ev1) HIPCHK(hipEventCreate(&e->ev1));
+    double ms = 0;
+    // DM_TIMING in the environment: wall time of the phases on stderr
+    const bool timing = std::getenv("DM_TIMING") != nullptr;
+    double tPrep = 0, tTables = 0, tRank = 0, tWalk = 0, tOut = 0;
+    auto now = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+    double t0 = now();
+    for (int64_t b = 0, en = 0; b < npairs; b = en) {
+        int64_t bytes = 0;
+        for (en = b; en < npairs; en++) {
+            if (en > b && bytes + need[en] > budget) break;
+            bytes += need[en];
+        }
+        // the batch sorted as a plan sorts its pairs (configuration, profile, longest first)
+        std::vector<int64_t> sorted;
+        for (int64_t k = b; k < en; k++) sorted.push_back(k);
+        std::stable_sort(sorted.begin(), sorted.end(), [&](int64_t x, int64_t y) {
+            if (cfgOf[x] != cfgOf[y]) return cfgOf[x] < cfgOf[y];
+            if (prof[x] != prof[y]) return prof[x] < prof[y];
+            return e->seqLen[seq[x]] > e->seqLen[seq[y]];
+        });
+        Chunk C;
+        buildChunk(e, kDump, sorted, cfgOf, prof, seq, 0, sorted.size(), C);
+        const int np = (int)C.pairs.size();
+        // dump regions: per configuration, each pair S x T x ROWS values (T = the
+        // configuration's longest sequence + G - 1) and S x T exponents
+        std::vector<int64_t> wBase(ncfg, 0), eBase(ncfg, 0), cfgT(ncfg, 0);
+        std::vector<PostPair> pp(np);
+        int64_t wTot = 0, eTot = 0, matTot = 0, colTot = 0, intTot = 0, segTot = 0, hitTot = 0, maxMat = 1;
+        int maxN = 3, maxM = 1;
+        for (int cfg = 0; cfg < ncfg; cfg++) {
+            if (C.cfgUnits[cfg] == 0) continue;
+            const int G = e->cfgG[cfg], R = e->cfgR[cfg], ROWS = G * R;
+            cfgT[cfg] = C.cfgMaxN[cfg] + G - 1;
+            wTot = (wTot + 63) / 64 * 64;
+            wBase[cfg] = wTot;
+            eBase[cfg] = eTot;
+            int64_t off = 0;
+            for (int u = C.cfgFirstUnit[cfg]; u < C.cfgFirstUnit[cfg] + C.cfgUnits[cfg]; u++) {
+                for (int q = 0; q < C.units[u].count; q++) {
+                    const int c = C.units[u].first + q;
+                    PairInfo &pi = C.pairs[c];
+                    const int M = e->profiles[pi.prof].M;
+                    pi.bufOff = off;
+                    PostPair &P = pp[c];
+                    P.M = M;
+                    P.n = pi.n;
+                    P.G = G;
+                    P.R = R;
+                    P.T = (int)cfgT[cfg];
+                    P.prof = pi.prof;
+                    P.pairIdx = c;
+                    P.minRatio = (float)ratio[C.order[c]];
+                    P.wOff = wBase[cfg] + off;
+                    P.eOff = eBase[cfg] + off / ROWS;
+                    off += (int64_t)stripsOf(e, cfg, M) * cfgT[cfg] * ROWS;
+                }
+            }
+            wTot += off;
+            eTot += off / ROWS;
+        }
+        for (int c = 0; c < np; c++) {
+            PostPair &P = pp[c];
+            P.matOff = matTot;
+            P.colOff = colTot;
+            P.intOff = intTot;
+            P.segCap = 2 * P.n + 2 * (P.M + 4);
+            P.segOff = segTot;
+            P.hitOff = hitTot;
+            P.candOff = hitTot;
+            P.ncand = 0;
+            const int64_t mc = 3 * (int64_t)(P.M + 2) * (P.n + 7);
+            matTot += mc;
+            maxMat = std::max(maxMat, mc);
+            colTot += (int64_t)kPostCols * (P.n + 8);
+            intTot += 2 * (int64_t)(P.n + 8) + P.M + 2;
+            segTot += P.segCap;
+            hitTot += P.n;
+            maxN = std::max(maxN, P.n);
+            maxM = std::max(maxM, P.M);
+        }
+        HIPCHK(e->dPgW0.ensure(sizeof(float) * (size_t)std::max<int64_t>(wTot, 64)));
+        HIPCHK(e->dPgW1.ensure(sizeof(float) * (size_t)std::max<int64_t>(wTot, 64)));
+        HIPCHK(e->dPgEf.ensure(sizeof(int) * (size_t)std::max<int64_t>(eTot, 64)));
+        HIPCHK(e->dPgEb.ensure(sizeof(int) * (size_t)std::max<int64_t>(eTot, 64)));
+        HIPCHK(e->dPgPairs.ensure(sizeof(PostPair) * (size_t)np));
+        HIPCHK(e->dPgRes.ensure(sizeof(PostRes) * (size_t)np));
+        HIPCHK(e->dPgMat.ensure(sizeof(ES) * (size_t)matTot));
+        HIPCHK(e->dPgCol.ensure(sizeof(ES) * (size_t)colTot));
+        HIPCHK(e->dPgInt.ensure(sizeof(int) * (size_t)intTot));
+        HIPCHK(e->dPgSeg.ensure(sizeof(PostSeg) * (size_t)segTot));
+        HIPCHK(e->dPgHit.ensure(sizeof(PostHit) * (size_t)hitTot));
+        HIPCHK(e->dPgCand.ensure(sizeof(PostCand) * (size_t)hitTot));
+        rc = uploadChunk(e, C);  // pair records (with the dump offsets) and the null model
+        if (rc) return rc;
+        tPrep += now() - t0;
+        t0 = now();
+        HIPCHK(hipEventRecord(e->ev0, e->stream));
+        HIPCHK(launch_zero_ints(e->dCounter.as<int>(), ncfg, e->stream));
+        for (int cfg = 0; cfg < ncfg; cfg++) {
+            const int nu = C.cfgUnits[cfg];
+            if (nu == 0) continue;
+            const int G = e->cfgG[cfg], R = e->cfgR[cfg];
+            DPArgs a = baseArgs(e);
+            a.units = e->dUnits.as<Unit>() + C.cfgFirstUnit[cfg];
+            a.nUnits = nu;
+            a.unitCounter = e->dCounter.as<int>() + cfg;
+            a.wbuf = e->dPgW1.as<float>() + wBase[cfg];
+            a.w0buf = e->dPgW0.as<float>() + wBase[cfg];
+            a.ebuf = e->dPgEb.as<int>() + eBase[cfg];
+            a.fbuf = e->dPgEf.as<int>() + eBase[cfg];
+            a.stripSteps = cfgT[cfg];
+            const int grid = std::max(1, std::min(e->numCUs * 8, nu));
+            const bool strips = e->cfgStrips[cfg] != 0;
+            if (strips) {
+                a.bndCols = C.cfgMaxN[cfg] + 2;
+                HIPCHK(e->dBnd.ensure((size_t)(3 * a.bndCols * (64 / G)) * grid * sizeof(Bnd)));
+                a.bnd = e->dBnd.as<Bnd>();
+            }
+            HIPCHK(launch_dp(G, R, kDump, strips, a, grid, e->stream));
+        }
+        HIPCHK(hipMemcpyAsync(e->dPgPairs.p, pp.data(), sizeof(PostPair) * np, hipMemcpyHostToDevice, e->stream));
+        PostArgs pa{};
+        pa.npairs = np;
+        pa.pp = e->dPgPairs.as<PostPair>();
+        pa.res = e->dPgRes.as<PostRes>();
+        pa.profs = e->dProfs.as<ProfInfo>();
+        pa.params = e->dParams.as<float>();
+        pa.emis = e->dEmis.as<float>();
+        pa.bg = e->dBg.as<float>();
+        pa.pairs = e->dPairs.as<PairInfo>();
+        pa.rec = e->dRec.as<Rec>();
+        pa.codes = e->dCodes.as<uint8_t>();
+        pa.w0buf = e->dPgW0.as<float>();
+        pa.wbuf = e->dPgW1.as<float>();
+        pa.fbuf = e->dPgEf.as<int>();
+        pa.ebuf = e->dPgEb.as<int>();
+        pa.mat = e->dPgMat.as<ES>();
+        pa.col = e->dPgCol.as<ES>();
+        pa.ints = e->dPgInt.as<int>();
+        pa.segs = e->dPgSeg.as<PostSeg>();
+        pa.hits = e->dPgHit.as<PostHit>();
+        pa.cands = e->dPgCand.as<PostCand>();
+        pa.bgfs1 = e->bgfs1();
+        pa.bgfs2 = e->bgfs2();
+        HIPCHK(launch_post_tables(pa, maxN, maxM, maxMat, e->stream));
+        // each column's best cell, to the host for ranking
+        std::vector<PostRes> res(np);
+        HIPCHK(e->hPgCand.ensure(sizeof(PostCand) * (size_t)hitTot));
+        PostCand *cb = e->hPgCand.as<PostCand>();
+        HIPCHK(hipMemcpyAsync(res.data(), pa.res, sizeof(PostRes) * np, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(cb, pa.cands, sizeof(PostCand) * hitTot, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipEventRecord(e->ev1, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        float ms1 = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms1, e->ev0, e->ev1));
+        ms += ms1;
+        tTables += now() - t0;
+        t0 = now();
+        // anchors ranked per pair (independent), on a pool of host threads
+        std::vector<std::vector<Hit>> bestOnly(np);
+        {
+            const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            auto rankRange = [&](int c0, int c1) {
+                std::vector<ES> colBest;
+                std::vector<int> colRow;
+                std::vector<Anchor> ranked;
+                for (int c = c0; c < c1; c++) {
+                    PostPair &P = pp[c];
+                    const PostCand *cc = cb + P.candOff;
+                    colBest.resize(P.n);
+                    colRow.resize(P.n);
+                    for (int j = 0; j < P.n; j++) {
+                        colBest[j] = ES{cc[j].m, cc[j].e};
+                        colRow[j] = cc[j].a1;
+                    }
+                    if (P.minRatio < 0) {  // the best anchor, no alignment
+                        rankAnchors(colBest.data(), colRow.data(), P.n, P.minRatio, ranked);
+                        Hit h;
+                        h.pair = ranked.front().pair;
+                        h.lnratio = h.pair.m == 0.f ? -INFINITY
+                                                    : std::log(h.pair.m) + (double)h.pair.e * 0.6931471805599453;
+                        h.anchor1 = ranked.front().a1;
+                        h.anchor2 = ranked.front().a2;
+                        bestOnly[c].push_back(h);
+                        continue;
+                    }
+                    if (!res[c].live) continue;  // below the gate
+                    rankAnchors(colBest.data(), colRow.data(), P.n, P.minRatio, ranked);
+                    P.ncand = (int)ranked.size();
+                    for (size_t k = 0; k < ranked.size(); k++)
+                        cb[P.candOff + k] = PostCand{ranked[k].pair.m, ranked[k].pair.e, ranked[k].a1, ranked[k].a2};
+                }
+            };
+            if (nthr == 1 || np < 64) {
+                rankRange(0, np);
+            } else {
+                std::vector<std::thread> pool;
+                for (int w = 0; w < nthr; w++) pool.emplace_back(rankRange, (int)((int64_t)np * w / nthr),
+                                                                 (int)((int64_t)np * (w + 1) / nthr));
+                for (auto &th : pool) th.join();
+            }
+        }
+        bool walk = false;
+        for (int c = 0; c < np; c++) {
+            if (!bestOnly[c].empty()) out[C.order[c]] = std::move(bestOnly[c]);
+            if (pp[c].ncand) walk = true;
+        }
+        tRank += now() - t0;
+        t0 = now();
+        if (!walk) continue;
+        HIPCHK(hipEventRecord(e->ev0, e->stream));
+        HIPCHK(hipMemcpyAsync(e->dPgPairs.p, pp.data(), sizeof(PostPair) * np, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(e->dPgCand.p, cb, sizeof(PostCand) * hitTot, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(launch_post_walk(pa, e->stream));
+        HIPCHK(e->hPgHit.ensure(sizeof(PostHit) * (size_t)hitTot));
+        HIPCHK(e->hPgSeg.ensure(sizeof(PostSeg) * (size_t)segTot));
+        PostHit *hb = e->hPgHit.as<PostHit>();
+        PostSeg *sb = e->hPgSeg.as<PostSeg>();
+        HIPCHK(hipMemcpyAsync(res.data(), pa.res, sizeof(PostRes) * np, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(hb, pa.hits, sizeof(PostHit) * hitTot, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(sb, pa.segs, sizeof(PostSeg) * segTot, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipEventRecord(e->ev1, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipEventElapsedTime(&ms1, e->ev0, e->ev1));
+        ms += ms1;
+        tWalk += now() - t0;
+        t0 = now();
+        for (int c = 0; c < np; c++) {
+            const PostPair &P = pp[c];
+            if (P.ncand == 0) continue;
+            if (res[c].overflow) return fail(e, DM_EUNSUP, "post-gate segment buffer overflow");
+            for (int h = 0; h < res[c].nhits; h++) {
+                const PostHit &ph = hb[P.hitOff + h];
+                Hit H;
+                H.pair = ES{ph.m, ph.e};
+                H.lnratio = std::log(ph.m) + (double)ph.e * 0.6931471805599453;
+                H.anchor1 = ph.a1;
+                H.anchor2 = ph.a2;
+                for (int g = 0; g < ph.nseg; g++) {
+                    const PostSeg &q = sb[P.segOff + ph.seg0 + g];
+                    H.segs.push_back(Seg{q.start1, q.start2, q.length});
+                }
+                out[C.order[c]].push_back(std::move(H));
+            }
+        }
+        tOut += now() - t0;
+        t0 = now();
+    }
+    if (timing)
+        std::fprintf(stderr, "dm_pairs_hits: %lld pairs: prepare %.3f s, dumps+tables %.3f s, ranking %.3f s, "
+                             "tracebacks %.3f s, output %.3f s; device %.1f ms\n",
+                     (long long)npairs, tPrep, tTables, tRank, tWalk, tOut, ms);
+    e->lastHitsMs = ms;
+    e->lastHitsPairs = npairs;
+    return DM_OK;
+}
+
+int dm_pairs_hits(dm_engine *e, int64_t npairs, const int32_t *prof, const int32_t *seq, const double *min_prob_ratio,
+                  double *hits, int64_t hit_cap, int32_t *segs, int64_t seg_cap, int64_t *first_hit, int64_t *nhits,
+                  int64_t *nsegs) {
+    if (!e || npairs < 0 || hit_cap < 0 || seg_cap < 0 || !first_hit || !nhits || !nsegs ||
+        (npairs > 0 && (!prof || !seq || !min_prob_ratio)) || (hit_cap > 0 && !hits) || (seg_cap > 0 && !segs))
+        return DM_EINVAL;
+    auto &hc = e->hitsCache;
+    const bool cached = hc.seqEpoch == e->seqEpoch && hc.profGen == e->profGen && (int64_t)hc.prof.size() == npairs &&
+                        std::equal(hc.prof.begin(), hc.prof.end(), prof) && std::equal(hc.seq.begin(), hc.seq.end(), seq) &&
+                        std::equal(hc.ratio.begin(), hc.ratio.end(), min_prob_ratio);
+    if (!cached) {
+        HIPCHK(hipSetDevice(e->device));
+        hc.seqEpoch = -1;
+        int rc = pairsHitsRun(e, npairs, prof, seq, min_prob_ratio, hc.hits);
+        if (rc) return rc;
+        hc.prof.assign(prof, prof + npairs);
+        hc.seq.assign(seq, seq + npairs);
+        hc.ratio.assign(min_prob_ratio, min_prob_ratio + npairs);
+        hc.seqEpoch = e->seqEpoch;
+        hc.profGen = e->profGen;
+    }
+    int64_t needH = 0, needS = 0;
+    for (const auto &v : hc.hits) {
+        needH += (int64_t)v.size();
+        for (const Hit &h : v) needS += (int64_t)h.segs.size();
+    }
+    *nhits = needH;
+    *nsegs = needS;
+    if (needH > hit_cap || needS > seg_cap)
+        return fail(e, DM_ERANGE, "hit buffers too small: " + std::to_string(needH) + " hits and " +
+                                      std::to_string(needS) + " segments needed");
+    int64_t nh = 0, ns = 0;
+    for (int64_t k = 0; k < npairs; k++) {
+        first_hit[k] = nh;
+        for (const Hit &h : hc.hits[(size_t)k]) {
+            double *o = hits + 6 * nh;
+            o[0] = h.lnratio;
+            o[1] = h.pair.m;
+            o[2] = h.pair.e;
+            o[3] = h.anchor1;
+            o[4] = h.anchor2;
+            o[5] = (double)h.segs.size();
+            for (const Seg &g : h.segs) {
+                segs[3 * ns] = g.start1;
+                segs[3 * ns + 1] = g.start2;
+                segs[3 * ns + 2] = g.length;
+                ns++;
+            }
+            nh++;
+        }
+    }
+    first_hit[npairs] = nh;
+    return DM_OK;
+}
+
+double dm_last_hits_ms(const dm_engine *e) { return e ? e->lastHitsMs : 0.0; }
 
 // Hits and their segments into the caller's arrays. When they do not fit,
 // *nhits / *nsegs receive the counts needed and nothing is written (DM_ERANGE).

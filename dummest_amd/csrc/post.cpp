@@ -116,6 +116,50 @@ struct Mat {  // ExpScore matrix with a front pad, zero outside the written rang
 
 }  // namespace
 
+void rankAnchors(const ES *colBest, const int *colRow, int n, float minProbRatio, std::vector<Anchor> &out) {
+    struct Cand {
+        ES pair;
+        int a1, a2;
+        bool operator>(const Cand &o) const { return less(o.pair, pair); }
+        bool operator<(const Cand &o) const { return less(pair, o.pair); }
+    };
+    out.clear();
+    std::vector<Cand> opt(n, Cand{ES{0.f, 0}, 0, 0});
+    ES bestOverall{0.f, 0};
+    for (int j = 0; j < n; j++)
+        if (colBest[j].m != 0.f) {
+            if (less(bestOverall, colBest[j])) bestOverall = colBest[j];
+            opt[j] = Cand{colBest[j], colRow[j], j};
+        }
+    if (minProbRatio < 0) {
+        const Cand sel = *std::max_element(opt.begin(), opt.end());
+        out.push_back(Anchor{sel.pair, sel.a1, sel.a2});
+        return;
+    }
+    const ES minExp = norm((double)minProbRatio, 0);
+    if (bestOverall.m == 0.f || less(bestOverall, minExp)) return;
+    // Highest first, the anchors at or above the threshold are a prefix of the
+    // sorted columns. When no two of them tie, that prefix is the same however
+    // it is sorted, so only those are sorted; a tie among them falls back to
+    // std::sort over every column, whose order of tied elements is the
+    // reference's.
+    std::vector<Cand> top;
+    for (const Cand &c : opt)
+        if (c.pair.m != 0.f && !less(c.pair, minExp)) top.push_back(c);
+    std::sort(top.begin(), top.end(), std::greater<>());
+    bool tie = false;
+    for (size_t k = 1; k < top.size() && !tie; k++) tie = !less(top[k].pair, top[k - 1].pair);
+    if (tie) {
+        std::sort(opt.begin(), opt.end(), std::greater<>());
+        top.clear();
+        for (const Cand &c : opt) {
+            if (c.pair.m == 0.f || less(c.pair, minExp)) break;
+            top.push_back(c);
+        }
+    }
+    for (const Cand &c : top) out.push_back(Anchor{c.pair, c.a1, c.a2});
+}
+
 void postProcess(const HostProfile &P, const PostInput &in, float minProbRatio, std::vector<Hit> &out) {
     const int M = in.M, n = in.n;
     auto code = [&](int j) -> int { return (j >= 0 && j < n) ? (int)in.codes[j] : kZeroIdx; };
@@ -289,21 +333,10 @@ void postProcess(const HostProfile &P, const PostInput &in, float minProbRatio, 
         }
     }
     // anchors and traceback, dummest-dp-forward.hh:459-545
-    struct Cand {
-        ES pair;
-        int a1, a2;
-        bool operator>(const Cand &o) const { return less(o.pair, pair); }
-        bool operator<(const Cand &o) const { return less(pair, o.pair); }
-    };
-    std::vector<Cand> opt(n, Cand{ES{0.f, 0}, 0, 0});
-    ES bestOverall{0.f, 0};
-    for (int j = 0; j < n; j++)
-        if (colBest[j].m != 0.f) {
-            if (less(bestOverall, colBest[j])) bestOverall = colBest[j];
-            opt[j] = Cand{colBest[j], colRow[j], j};
-        }
+    std::vector<Anchor> ranked;
+    rankAnchors(colBest.data(), colRow.data(), n, minProbRatio, ranked);
     if (minProbRatio < 0) {
-        const Cand sel = *std::max_element(opt.begin(), opt.end());
+        const Anchor &sel = ranked.front();
         Hit h;
         h.pair = sel.pair;
         h.lnratio = lnOf(sel.pair);
@@ -312,12 +345,10 @@ void postProcess(const HostProfile &P, const PostInput &in, float minProbRatio, 
         out.push_back(h);
         return;
     }
-    if (bestOverall.m == 0.f || less(bestOverall, minExp)) return;
-    std::sort(opt.begin(), opt.end(), std::greater<>());
     std::vector<bool> aligned(n, false);
     const int cols = W + 4, bound = W;
-    for (const Cand &c : opt) {
-        if (c.pair.m == 0.f || less(c.pair, minExp) || aligned[c.a2]) continue;
+    for (const Anchor &c : ranked) {
+        if (aligned[c.a2]) continue;
         Hit h;
         h.pair = c.pair;
         h.lnratio = lnOf(c.pair);

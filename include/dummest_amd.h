@@ -175,6 +175,22 @@ int dm_pair_null(dm_engine *e, int32_t prof, int32_t seq, float *prefix_seed, fl
 int dm_pair_hits(dm_engine *e, int32_t prof, int32_t seq, double min_prob_ratio, double *hits, int32_t hit_cap,
                  int32_t *segs, int32_t seg_cap, int32_t *nhits, int32_t *nsegs);
 
+/* The hits of many pairs in one call, as the reference forms them for every
+ * lane of a lane batch after its forward pass (findFinalSimilaritiesBatched,
+ * dummest-search.hh:76-285, with dummest-dp-forward.hh:252-545): every pair's
+ * DP matrices, posterior match matrix, flank totals, MEA tables and
+ * tracebacks on the device, batched, with no per-pair synchronization; the
+ * same hits as dm_pair_hits pair by pair. Pair k's hits are rows
+ * first_hit[k] .. first_hit[k+1]-1 of hits (first_hit: npairs + 1 entries;
+ * rows as in dm_pair_hits), their segments consecutive in segs in hit order.
+ * *nhits / *nsegs receive the totals; DM_ERANGE when they pass hit_cap /
+ * seg_cap (nothing written; the same call with larger buffers returns the
+ * kept result). dm_last_hits_ms: device time of the last run. */
+int dm_pairs_hits(dm_engine *e, int64_t npairs, const int32_t *prof, const int32_t *seq, const double *min_prob_ratio,
+                  double *hits, int64_t hit_cap, int32_t *segs, int64_t seg_cap, int64_t *first_hit, int64_t *nhits,
+                  int64_t *nsegs);
+double dm_last_hits_ms(const dm_engine *e);
+
 /* Plans: the work queue of a pair list (configuration choice, length sort,
  * units, device-resident pair descriptors) built once, then executed any
  * number of times with only kernel launches on the device. mode 0 = filter
