@@ -1577,6 +1577,7 @@ static int pairsHitsRun(dm_engine *e, int64_t npairs, const int32_t *prof, const
     for (int64_t k = 0; k < npairs; k++) {
         const int M = e->profiles[prof[k]].M, n = e->seqLen[seq[k]];
         const int cfg = chooseConfig(e, M, n, false);
+        if (!post_rows_supported(e->cfgR[cfg])) return fail(e, DM_EUNSUP, "post-gate pass: unsupported rows per lane");
         cfgOf[k] = cfg;
         const int64_t ROWS = (int64_t)e->cfgG[cfg] * e->cfgR[cfg], S = stripsOf(e, cfg, M);
         const int64_t steps = S * (n + e->cfgG[cfg] - 1);

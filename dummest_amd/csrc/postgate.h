@@ -74,6 +74,8 @@ struct PostArgs {
 // zero the tables, then the per-column pass (anchors, column totals, X), the
 // flank totals, the MEA tables, all before the host ranks the candidates
 hipError_t launch_post_tables(const PostArgs &a, int maxN, int maxM, int64_t matCells, hipStream_t s);
+// rows per lane the per-column pass is built for
+bool post_rows_supported(int R);
 // the tracebacks of the ranked candidates
 hipError_t launch_post_walk(const PostArgs &a, hipStream_t s);
 

@@ -169,16 +169,19 @@ __device__ __forceinline__ unsigned long long wave_max_key(unsigned long long v)
 // w0 W1 and the column's best (first row on ties), the column totals of w0
 // and W1 (fwdSide, revSide), and X(i,j) = (enter e(code(j-2)) w0(i,j-3)) W1(i+1,j)
 // (post.cpp; dummest-dp-forward.hh:126-206). Row i of the dump is (strip s,
-// lane k, slot q) with i = s G R + k R + q, walked by counters: the cells of
-// one (s, k) at column j are R consecutive floats of step s T + j + k. The
-// four values a row needs are loaded a row ahead.
+// lane k, slot q) with i = s G R + k R + q: the R rows of one (s, k) at
+// column j are R consecutive floats of step s T + j + k, loaded together (the
+// dump's steps lie ROWS floats apart, so a thread reads a whole chunk per R
+// rows instead of one value per row). One launch per rows-per-lane R; pairs
+// of another R return at once.
+template <int R>
 __global__ __launch_bounds__(256) void post_columns_kernel(PostArgs a) {
 #pragma clang fp contract(off)
     const int p = blockIdx.y;
     const PostPair &P = a.pp[p];
-    if ((int)(blockIdx.x * blockDim.x) >= P.n) return;  // the whole block is past the sequence
+    if (P.R != R || (int)(blockIdx.x * blockDim.x) >= P.n) return;
     const PairView V(P, a);
-    const int M = P.M, n = P.n, G = P.G, R = P.R, T = P.T, ROWS = G * R;
+    const int M = P.M, n = P.n, G = P.G, T = P.T, ROWS = G * R;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const float sC = a.pairs[P.pairIdx].sC;
     const ProfInfo PI = a.profs[P.prof];
@@ -190,66 +193,78 @@ __global__ __launch_bounds__(256) void post_columns_kernel(PostArgs a) {
         const int cj = V.code(j - 2);
         const bool x3 = j >= 3;
         ES *X = V.X();
-        // raw values of row i: w0(i,j), W1(i,j), w0(i,j-3), W1(i+1,j) and their exponents
-        struct Raw {
-            float a, b, c, d;
-            int ea, eb, ec, ed;
+        // the chunk of group (s, k): w0 and W1 at column j, w0 at column j-3
+        float A[R], B[R], C[R];
+        int ea = 0, eb = 0, ec = 0;
+        auto loadGroup = [&](int s, int k, float *Aa, float *Bb, float *Cc, int &a_, int &b_, int &c_) {
+            const int64_t st = (int64_t)s * T + j + k;
+            const float *pa = w0b + st * ROWS + k * R, *pb = w1b + st * ROWS + k * R;
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                Aa[q] = pa[q];
+                Bb[q] = pb[q];
+            }
+            a_ = efb[st];
+            b_ = ebb[st];
+            if (x3) {
+                const float *pc = w0b + (st - 3) * ROWS + k * R;
+#pragma unroll
+                for (int q = 0; q < R; q++) Cc[q] = pc[q];
+                c_ = efb[st - 3];
+            }
         };
-        int k = 0, q = 0, s = 0;
-        auto next = [&](int &k_, int &q_, int &s_) {
-            if (++q_ == R) {
-                q_ = 0;
-                if (++k_ == G) {
-                    k_ = 0;
-                    ++s_;
+        loadGroup(0, 0, A, B, C, ea, eb, ec);
+        int i = 0;
+        for (int s = 0; i <= M; s++) {
+            for (int k = 0; k < G && i <= M; k++) {
+                // the next group's chunk, loaded ahead (its first W1 is row i+1 of this group's last row)
+                float A2[R], B2[R], C2[R];
+                int ea2 = 0, eb2 = 0, ec2 = 0;
+                const bool more = i + R <= M;
+                if (more) {
+                    int s2 = s, k2 = k + 1;
+                    if (k2 == G) {
+                        k2 = 0;
+                        ++s2;
+                    }
+                    loadGroup(s2, k2, A2, B2, C2, ea2, eb2, ec2);
+                }
+#pragma unroll
+                for (int q = 0; q < R; q++) {
+                    if (i + q > M) break;
+                    const int row = i + q;
+                    const ES wa = es_stored(A[q], ea), wb = es_stored(B[q], eb);
+                    const ES wm = es_bmul(wa, wb);
+                    if (es_less(gb, wm)) gb = wm;
+                    if (es_less(best, wm)) {
+                        best = wm;
+                        bestRow = row;
+                    }
+                    fs = es_add(fs, wa);
+                    rs = es_add(rs, wb);
+                    if (row < M && x3) {
+                        const ES w3 = es_stored(C[q], ec);
+                        if (w3.m != 0.f) {
+                            const ES wd = q + 1 < R ? es_stored(B[q + 1], eb) : es_stored(B2[0], eb2);
+                            const float enterC = a.params[PI.parOff + (int64_t)row * 11 + 10] * sC;
+                            const float xf = (enterC * a.emis[PI.emOff + (int64_t)row * kEmStride + cj]) * w3.m;
+                            X[V.at(row, j)] = es_bmul(es_stored(xf, w3.e), wd);
+                        }
+                    }
+                }
+                i += R;
+                if (more) {
+#pragma unroll
+                    for (int q = 0; q < R; q++) {
+                        A[q] = A2[q];
+                        B[q] = B2[q];
+                        C[q] = C2[q];
+                    }
+                    ea = ea2;
+                    eb = eb2;
+                    ec = ec2;
                 }
             }
-        };
-        auto load = [&](int i, int k_, int q_, int s_) {
-            Raw r{0.f, 0.f, 0.f, 0.f, 0, 0, 0, 0};
-            const int64_t st = (int64_t)s_ * T + j + k_;
-            const int off = k_ * R + q_;
-            r.a = w0b[st * ROWS + off];
-            r.ea = efb[st];
-            r.b = w1b[st * ROWS + off];
-            r.eb = ebb[st];
-            if (i < M && x3) {
-                r.c = w0b[(st - 3) * ROWS + off];
-                r.ec = efb[st - 3];
-                int k1 = k_, q1 = q_, s1 = s_;
-                next(k1, q1, s1);
-                const int64_t st1 = (int64_t)s1 * T + j + k1;
-                r.d = w1b[st1 * ROWS + k1 * R + q1];
-                r.ed = ebb[st1];
-            }
-            return r;
-        };
-        Raw cur = load(0, 0, 0, 0);
-        for (int i = 0; i <= M; i++) {
-            int kn = k, qn = q, sn = s;
-            next(kn, qn, sn);
-            const Raw nxt = i < M ? load(i + 1, kn, qn, sn) : cur;
-            const ES wa = es_stored(cur.a, cur.ea), wb = es_stored(cur.b, cur.eb);
-            const ES wm = es_bmul(wa, wb);
-            if (es_less(gb, wm)) gb = wm;
-            if (es_less(best, wm)) {
-                best = wm;
-                bestRow = i;
-            }
-            fs = es_add(fs, wa);
-            rs = es_add(rs, wb);
-            if (i < M && x3) {
-                const ES w3 = es_stored(cur.c, cur.ec);
-                if (w3.m != 0.f) {
-                    const float enterC = a.params[PI.parOff + (int64_t)i * 11 + 10] * sC;
-                    const float xf = (enterC * a.emis[PI.emOff + (int64_t)i * kEmStride + cj]) * w3.m;
-                    X[V.at(i, j)] = es_bmul(es_stored(xf, w3.e), es_stored(cur.d, cur.ed));
-                }
-            }
-            cur = nxt;
-            k = kn;
-            q = qn;
-            s = sn;
         }
         V.col(kColBest)[j] = best;
         V.colRow()[j] = bestRow;
@@ -766,7 +781,17 @@ hipError_t launch_post_tables(const PostArgs &a, int maxN, int maxM, int64_t max
     if (a.npairs <= 0) return hipSuccess;
     const int initBlocks = (int)std::min<int64_t>(1024, (maxCells + 255) / 256);
     hipLaunchKernelGGL(post_init_kernel, dim3(std::max(1, initBlocks), a.npairs), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(post_columns_kernel, dim3((maxN + 255) / 256, a.npairs), dim3(256), 0, s, a);
+    const dim3 cgrid((maxN + 255) / 256, a.npairs);
+    hipLaunchKernelGGL(post_columns_kernel<1>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<2>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<3>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<4>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<5>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<6>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<7>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<8>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<12>, cgrid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(post_columns_kernel<16>, cgrid, dim3(256), 0, s, a);
     hipLaunchKernelGGL(post_rowmax_kernel, dim3(a.npairs), dim3(256), 0, s, a);
     (void)maxM;
     hipLaunchKernelGGL(post_flanks_kernel, dim3(a.npairs), dim3(192), 0, s, a);
@@ -774,6 +799,8 @@ hipError_t launch_post_tables(const PostArgs &a, int maxN, int maxM, int64_t max
     if (maxN > kMeaShort) hipLaunchKernelGGL(post_mea_kernel<4>, dim3(a.npairs, 2), dim3(256), 0, s, a);
     return hipGetLastError();
 }
+
+bool post_rows_supported(int R) { return (R >= 1 && R <= 8) || R == 12 || R == 16; }
 
 hipError_t launch_post_walk(const PostArgs &a, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
