@@ -615,6 +615,14 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             for (int c = k; c < 3 * nb; c += G) bB[c] = Bnd{0.f, 0.f, 0};
         }
 
+        // Score mode's W1 block of a step (G R floats per group): lane k's rows
+        // in chunks of four, chunk j holding rows 4j..4j+3 of every lane of the
+        // group side by side (G x 4 floats, the last chunk narrower), so that
+        // each wide store or load instruction covers one contiguous run.
+        auto w1Off = [&](int kk, int q) {
+            const int j = q >> 2, wj = (R - 4 * j < 4) ? R - 4 * j : 4;
+            return G * 4 * j + kk * wj + (q & 3);
+        };
         float enterC[R], d0[R], d1s2[R], d2s1[R], a1s1[R], a2s2[R], eps[R];
         float fbest = 0.f;  // filter mode: max W1 * prefixSeed
         ExpMaxD bwdMax;  // max of W1 * prefixSeed, at offset Eb
@@ -748,8 +756,12 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             // (loaded in every strip so that the load is not conditional, which
             // would make the compiler wait for all memory operations each step;
             // used only when there is a strip below)
+            // Only the last lane uses the value; score and filter modes load its
+            // column in every lane of the group (one address per group, so the
+            // load moves one record rather than G), dump mode each lane's own
+            // column (whose values also enter the group's scale-up test).
             auto ghostAt = [&](int tt) -> Bnd {
-                if (STR) return bIn[bidx(Tb - 1 - tt - k, n)];
+                if (STR) return bIn[bidx(Tb - 1 - tt - (MODE == kDump ? k : G - 1), n)];
                 return Bnd{0.f, 0.f, 0};
             };
 #pragma unroll
@@ -762,8 +774,42 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             // lanes whose rows all lie past row M store and load no W1
             const bool rowsLive = strip * ROWS + k * R <= M;
 
-            auto bstep = [&](auto PHc, int t) {
+            // W1 of step tt (ring slot sl) and its exponent into the scratch.
+            // Score mode stores every lane's block of every step of the pair
+            // (anti-diagonals 0..Tb-1, inside the group's own scratch slot) with
+            // no test of rows or columns: the forward then reads only values
+            // this pair wrote, all finite, and the cells outside the matrix
+            // cannot raise the anchor maximum (columns outside the sequence meet
+            // a zero on one side, rows past M hold seed-only values that never
+            // exceed row M's at the same column). Dump mode stores the live
+            // cells of active pairs only (its buffers are the pairs' own).
+            auto storeW1 = [&](int tt, int sl) {
+                const int D = Tb - 1 - tt;
+                if (MODE == kScore) {
+                    if (D >= 0) {
+                        float *dst = wsc + (int64_t)D * ROWS;
+#pragma unroll
+                        for (int q = 0; q < R; q++) dst[w1Off(k, q)] = H[sl][q];
+                        esc[D] = Eb;  // every lane of the group writes the same value
+                    }
+                } else {
+                    const int c = D - k;
+                    if (D >= 0 && active) {
+                        if (rowsLive && c >= 0 && c < n) {
+                            float *dst = wsc + (int64_t)D * ROWS + k * R;
+#pragma unroll
+                            for (int q = 0; q < R; q++) dst[q] = H[sl][q];
+                        }
+                        if (k == 0) esc[D] = Eb;
+                    }
+                }
+            };
+
+            // DEFER (score mode, main loop): the step's W1 is stored at the start
+            // of the next four-step iteration rather than at once (see below)
+            auto bstep = [&](auto PHc, auto DEFc, int t) {
                 constexpr int PH = decltype(PHc)::value;
+                constexpr bool DEFER = decltype(DEFc)::value;
                 constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
                 const int c = Tb - 1 - t - k;  // column: D = Tb-1-t
                 const Rec rc = rcur;
@@ -881,18 +927,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 if (STR && above && k == 0 && c >= 0 && c < n) {
                     bOut[c + 1] = Bnd{H[S4][0], Yc[0], Eb};
                 }
-                if (MODE != kFilter) {
-                    const int D = Tb - 1 - t;
-                    if (D >= 0 && active) {
-                        if (rowsLive && c >= 0 && c < n) {
-                            float *dst = wsc + (int64_t)D * ROWS + k * R;
-#pragma unroll
-                            for (int q = 0; q < R; q++)
-                                dst[q] = (MODE == kScore) ? H[S4][q] * 7.888609052e-31f : H[S4][q];
-                        }
-                        if (k == 0) esc[D] = Eb;
-                    }
-                }
+                if (MODE == kDump || (MODE == kScore && !DEFER)) storeW1(t, S4);
                 if (MODE != kFilter) {
                     // One exponent per pair per step: lower the scale by 2^-64 when
                     // a value passes 2^100; every 4 steps, raise it by 2^64 when the
@@ -915,14 +950,27 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             };
             int t = 0;
             for (; t + 4 <= T; t += 4) {
-                bstep(std::integral_constant<int, 0>{}, t);
-                bstep(std::integral_constant<int, 1>{}, t + 1);
-                bstep(std::integral_constant<int, 2>{}, t + 2);
-                bstep(std::integral_constant<int, 3>{}, t + 3);
+                // Score mode stores an iteration's four W1 blocks (still in the
+                // ring, in the units of the current exponent) when the next one
+                // starts: the loop's waits for prefetched loads also wait for the
+                // stores issued before those loads, and stores issued a whole
+                // iteration earlier have mostly completed by then.
+                if (MODE == kScore && t > 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) storeW1(t - 4 + j, j);
+                }
+                bstep(std::integral_constant<int, 0>{}, std::bool_constant<MODE == kScore>{}, t);
+                bstep(std::integral_constant<int, 1>{}, std::bool_constant<MODE == kScore>{}, t + 1);
+                bstep(std::integral_constant<int, 2>{}, std::bool_constant<MODE == kScore>{}, t + 2);
+                bstep(std::integral_constant<int, 3>{}, std::bool_constant<MODE == kScore>{}, t + 3);
             }
-            if (t < T) bstep(std::integral_constant<int, 0>{}, t++);
-            if (t < T) bstep(std::integral_constant<int, 1>{}, t++);
-            if (t < T) bstep(std::integral_constant<int, 2>{}, t++);
+            if (MODE == kScore && t > 0) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) storeW1(t - 4 + j, j);
+            }
+            if (t < T) bstep(std::integral_constant<int, 0>{}, std::false_type{}, t++);
+            if (t < T) bstep(std::integral_constant<int, 1>{}, std::false_type{}, t++);
+            if (t < T) bstep(std::integral_constant<int, 2>{}, std::false_type{}, t++);
             if (STR) __syncthreads();  // boundary written above is read by the next strip
             if (a.wantBwd && MODE != kFilter) bwdMax.rebase(0);  // next strip starts at exponent 0
         }
@@ -949,7 +997,6 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             loadStrip(strip, a0, b0, strip > 0);
             const float *wsc = wsc0 + (int64_t)strip * strideS;
             const int *esc = esc0 + (int64_t)strip * a.stripSteps;
-            const bool rowsLive = strip * ROWS + k * R <= M;
             const bool above = STR && strip > 0;      // lane 0 reads the strip above
             const bool below = STR && strip < S - 1;  // last lane feeds the strip below
             float Wf[4][R], Zf[4][R], LVf[4];
@@ -1004,12 +1051,9 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 e = esc[ts];
                 if (MODE == kScore) {
 #pragma unroll
-                    for (int q = 0; q < R; q++) dst[q] = wsc[(int64_t)ts * ROWS + k * R + q];
+                    for (int q = 0; q < R; q++) dst[q] = wsc[(int64_t)ts * ROWS + w1Off(k, q)];
                 }
             };
-            // the backward stored the block of step tt for this lane (evaluated once
-            // per step without short-circuits: a branch per row otherwise)
-            auto liveW1 = [&](int tt) { return (tt < Tmine) & rowsLive & (tt - k >= 0) & (tt - k < n); };
 #pragma unroll
             for (int f = 0; f < kPF; f++) loadW1(f, Wq[f], Eq[f]);
 #pragma unroll
@@ -1019,8 +1063,9 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             // strip kernels: the strip above's last row (carried sum, Y, exponent), also ahead
             Bnd fq4[kPF];
             // (loaded in every strip, used only when there is a strip above)
+            // (used by the first lane; one address per group outside dump mode)
             auto fghostAt = [&](int tt) -> Bnd {
-                if (STR) return bF[bidx(tt - k, n)];
+                if (STR) return bF[bidx(tt - (MODE == kDump ? k : 0), n)];
                 return Bnd{0.f, 0.f, 0};
             };
 #pragma unroll
@@ -1048,10 +1093,13 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
 #pragma unroll
                 for (int q = 0; q < R; q++) eqN[q] = etk[((nx.codes >> 8) & 255) * ROWS + q];
                 const float gqN = bgl[(nx.codes >> 8) & 255];
+                // the stored block of this step scaled by 2^-100 (so that w0 W1
+                // cannot overflow), or zero past the pair's last step (the loads
+                // there repeat its last block)
                 float W1v[R];
-                const bool lv = (MODE == kScore) && liveW1(t);
+                const float w1f = (MODE == kScore && t < Tmine) ? 7.888609052e-31f : 0.f;
 #pragma unroll
-                for (int q = 0; q < R; q++) W1v[q] = lv ? Wq[SL][q] : 0.f;
+                for (int q = 0; q < R; q++) W1v[q] = Wq[SL][q] * w1f;
                 const int EbNext = (t < Tw) ? Eq[SL] : kNoStep;
                 loadW1(t + kPF, Wq[SL], Eq[SL]);
                 float gn = 0.f, gy = 0.f;
