@@ -114,6 +114,7 @@ struct DPArgs {
     int *fbuf;            // dump mode: per-step forward exponents
     int64_t waveFloats;   // score mode: scratch floats per wave
     int64_t stripSteps;   // steps per row strip in wbuf/ebuf (>= maxN + G - 1)
+    int64_t escGroup;     // score mode: ints of ebuf per group (strips of stripSteps + 8, see kernels.hip)
     struct Bnd *bnd;      // strip kernels: per-group boundary records (3 x bndCols)
     int64_t bndCols;      // >= maxN + 2
     float *filterOut;     // filter mode: best W1*prefixSeed per pair

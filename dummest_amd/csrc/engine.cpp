@@ -639,10 +639,12 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             grid = std::min(grid, nu / WG);
             if (mode == kScore) {
                 a.waveFloats = (int64_t)C.cfgMaxS[cfg] * a.stripSteps * ROWS * Pp;
+                // exponents: per group and strip, stripSteps entries between 4 guard ints on each side
+                a.escGroup = (int64_t)C.cfgMaxS[cfg] * (a.stripSteps + 8);
                 const int64_t perWaveBytes = a.waveFloats * (int64_t)sizeof(float) * WG;
                 grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, P.scratchBudget / perWaveBytes));
                 maxW = std::max(maxW, a.waveFloats * grid * WG);
-                maxE = std::max(maxE, a.waveFloats * grid * WG / ROWS + 1);
+                maxE = std::max(maxE, a.escGroup * Pp * grid * WG);
             }
             if (L.strips) {
                 a.bndCols = C.cfgMaxN[cfg] + 2;

@@ -538,6 +538,18 @@ template <int G> __device__ __forceinline__ int group_lane(int v, int grp, int a
     return r;
 }
 constexpr float kStep = 5.421011e-20f;  // 2^-64
+constexpr int kEscGuard = 4;            // score mode: guard ints around a strip's exponents
+
+// Four exponents moved with one 16-byte access at a 4-byte-aligned address.
+struct __attribute__((aligned(4))) Int4u {
+    int v0, v1, v2, v3;
+    template <int I> __device__ __forceinline__ int at() const {
+        if constexpr (I == 0) return v0;
+        else if constexpr (I == 1) return v1;
+        else if constexpr (I == 2) return v2;
+        else return v3;
+    }
+};
 constexpr int kPre = 100;               // bulk-mode prescale of stored W1 (2^-100)
 
 // Ghost reads across row strips (STR kernels, G = 64, one pair per wave):
@@ -545,12 +557,13 @@ constexpr int kPre = 100;               // bulk-mode prescale of stored W1 (2^-1
 // step that produced it; index c+1 for columns c in [-1, n], zero outside [0, n).
 __device__ __forceinline__ int bidx(int c, int n) { return c < -1 ? 0 : (c > n ? n + 1 : c + 1); }
 
-// Two waves per SIMD for the R = 8 single-strip kernels: they fit 256
-// registers without spilling, and the second wave hides latency (about 8-13%
-// faster per step than the unconstrained allocation, which takes one wave).
+// Two waves per SIMD for the R = 7 and R = 8 single-strip kernels and the
+// strip kernels up to R = 6: they fit 256 registers without spilling, and the
+// second wave hides latency (8-23% faster per step than the unconstrained
+// allocation, which takes one wave).
 template <int G, int R, int MODE, bool STR>
 __global__ void __launch_bounds__(64 * dp_wg(G, R, STR, MODE))
-__attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs a) {
+__attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <= 6)) ? 2 : 1))) dp_kernel(DPArgs a) {
     constexpr int ROWS = G * R;
     constexpr int WG = dp_wg(G, R, STR, MODE);  // waves per workgroup (sharing the emission table)
     // prefetch distance in steps (divides 4); the strip kernels keep 2, which
@@ -602,8 +615,11 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
         } else {
             const int64_t off = (int64_t)wslot * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G));
             wsc0 = a.wbuf + off;
-            esc0 = a.ebuf + off / ROWS;
+            // per strip stripSteps + 8 ints: 4 guard ints on either side of
+            // anti-diagonals 0..stripSteps-1 (exponents move four at a time)
+            esc0 = a.ebuf + ((int64_t)wslot * (64 / G) + grp) * a.escGroup + kEscGuard;
         }
+        const int64_t escStrip = (MODE == kDump) ? a.stripSteps : a.stripSteps + 2 * kEscGuard;
         // strip boundary buffers (per group), one (value, Y, exponent) record per
         // column, n + 2 entries each: two parities for the backward, one for the forward
         Bnd *bB = nullptr, *bF = nullptr;
@@ -701,7 +717,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 b0sC[q] *= sC;
             }
             float *wsc = wsc0 + (int64_t)strip * strideS;
-            int *esc = esc0 + (int64_t)strip * a.stripSteps;
+            int *esc = esc0 + (int64_t)strip * escStrip;
             const bool below = STR && strip < S - 1;  // a strip below feeds lane G-1
             const bool above = STR && strip > 0;      // lane 0 feeds the strip above
             const int par = strip & 1;
@@ -783,14 +799,16 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             // a zero on one side, rows past M hold seed-only values that never
             // exceed row M's at the same column). Dump mode stores the live
             // cells of active pairs only (its buffers are the pairs' own).
-            auto storeW1 = [&](int tt, int sl) {
+            // Score mode writes the exponents of a four-step iteration with one
+            // store (storeE4) when its W1 blocks are stored late, else one per step.
+            auto storeW1 = [&](int tt, int sl, bool withExp) {
                 const int D = Tb - 1 - tt;
                 if (MODE == kScore) {
                     if (D >= 0) {
                         float *dst = wsc + (int64_t)D * ROWS;
 #pragma unroll
                         for (int q = 0; q < R; q++) dst[w1Off(k, q)] = H[sl][q];
-                        esc[D] = Eb;  // every lane of the group writes the same value
+                        if (withExp) esc[D] = Eb;  // every lane of the group writes the same value
                     }
                 } else {
                     const int c = D - k;
@@ -803,6 +821,15 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                         if (k == 0) esc[D] = Eb;
                     }
                 }
+            };
+
+            // The exponent of the four steps tt0..tt0+3 (anti-diagonals D0-3..D0,
+            // D0 = Tb-1-tt0), all the current one: their W1 blocks are stored in
+            // its units. The guard ints before anti-diagonal 0 take the entries
+            // of a last iteration that runs past it.
+            auto storeE4 = [&](int tt0) {
+                const int D0 = Tb - 1 - tt0;
+                if (D0 >= 0) *(Int4u *)(esc + D0 - 3) = Int4u{Eb, Eb, Eb, Eb};
             };
 
             // DEFER (score mode, main loop): the step's W1 is stored at the start
@@ -927,7 +954,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 if (STR && above && k == 0 && c >= 0 && c < n) {
                     bOut[c + 1] = Bnd{H[S4][0], Yc[0], Eb};
                 }
-                if (MODE == kDump || (MODE == kScore && !DEFER)) storeW1(t, S4);
+                if (MODE == kDump || (MODE == kScore && !DEFER)) storeW1(t, S4, true);
                 if (MODE != kFilter) {
                     // One exponent per pair per step: lower the scale by 2^-64 when
                     // a value passes 2^100; every 4 steps, raise it by 2^64 when the
@@ -957,7 +984,8 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 // iteration earlier have mostly completed by then.
                 if (MODE == kScore && t > 0) {
 #pragma unroll
-                    for (int j = 0; j < 4; j++) storeW1(t - 4 + j, j);
+                    for (int j = 0; j < 4; j++) storeW1(t - 4 + j, j, false);
+                    storeE4(t - 4);
                 }
                 bstep(std::integral_constant<int, 0>{}, std::bool_constant<MODE == kScore>{}, t);
                 bstep(std::integral_constant<int, 1>{}, std::bool_constant<MODE == kScore>{}, t + 1);
@@ -966,7 +994,8 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             }
             if (MODE == kScore && t > 0) {
 #pragma unroll
-                for (int j = 0; j < 4; j++) storeW1(t - 4 + j, j);
+                for (int j = 0; j < 4; j++) storeW1(t - 4 + j, j, false);
+                storeE4(t - 4);
             }
             if (t < T) bstep(std::integral_constant<int, 0>{}, std::false_type{}, t++);
             if (t < T) bstep(std::integral_constant<int, 1>{}, std::false_type{}, t++);
@@ -996,7 +1025,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             float a0[R], b0[R];
             loadStrip(strip, a0, b0, strip > 0);
             const float *wsc = wsc0 + (int64_t)strip * strideS;
-            const int *esc = esc0 + (int64_t)strip * a.stripSteps;
+            const int *esc = esc0 + (int64_t)strip * escStrip;
             const bool above = STR && strip > 0;      // lane 0 reads the strip above
             const bool below = STR && strip < S - 1;  // last lane feeds the strip below
             float Wf[4][R], Zf[4][R], LVf[4];
@@ -1010,6 +1039,7 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             int Ef = 0;
             float rf = 1.f;
             int EbPrev = 0;
+            bool eChg = false;  // score mode: the current block has an exponent change
             if (MODE == kScore) midMax.rebase(Ef + EbPrev + kPre);
             float *w0d = (MODE == kDump) ? a.w0buf + PR.bufOff + (int64_t)strip * strideS : nullptr;
             int *efd = (MODE == kDump) ? a.fbuf + PR.bufOff / ROWS + (int64_t)strip * a.stripSteps : nullptr;
@@ -1048,12 +1078,19 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
             const int Tw = Tmine;
             auto loadW1 = [&](int tt, float *dst, int &e) {
                 const int ts = tt < Tw ? tt : (Tw > 0 ? Tw - 1 : 0);
-                e = esc[ts];
                 if (MODE == kScore) {
 #pragma unroll
                     for (int q = 0; q < R; q++) dst[q] = wsc[(int64_t)ts * ROWS + w1Off(k, q)];
+                    e = 0;
+                } else {
+                    e = esc[ts];
                 }
             };
+            // score mode: the exponents of the four-step block starting at tb,
+            // loaded a block ahead (entries past the pair's last step are unused;
+            // the guard ints after the strip keep the load in bounds)
+            auto loadE4 = [&](int tb) { return *(const Int4u *)(esc + (tb < Tw ? tb : 0)); };
+            Int4u e4cur = loadE4(0), e4nxt = loadE4(4);
 #pragma unroll
             for (int f = 0; f < kPF; f++) loadW1(f, Wq[f], Eq[f]);
 #pragma unroll
@@ -1100,7 +1137,18 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                 const float w1f = (MODE == kScore && t < Tmine) ? 7.888609052e-31f : 0.f;
 #pragma unroll
                 for (int q = 0; q < R; q++) W1v[q] = Wq[SL][q] * w1f;
-                const int EbNext = (t < Tw) ? Eq[SL] : kNoStep;
+                const int EbNext = (t < Tw) ? ((MODE == kScore) ? e4cur.at<PH>() : Eq[SL]) : kNoStep;
+                if (MODE == kScore && PH == 0) {
+                    // one test per four-step block: does any exponent of the block
+                    // (of this pair's steps) differ from the current one
+                    const bool d = ((t < Tw) & (e4cur.v0 != EbPrev)) | ((t + 1 < Tw) & (e4cur.v1 != EbPrev)) |
+                                   ((t + 2 < Tw) & (e4cur.v2 != EbPrev)) | ((t + 3 < Tw) & (e4cur.v3 != EbPrev));
+                    eChg = __any(d);
+                }
+                if (MODE == kScore && PH == 3) {
+                    e4cur = e4nxt;
+                    e4nxt = loadE4(t + 5);
+                }
                 loadW1(t + kPF, Wq[SL], Eq[SL]);
                 float gn = 0.f, gy = 0.f;
                 Bnd fcurB{0.f, 0.f, 0};
@@ -1131,11 +1179,11 @@ __attribute__((amdgpu_waves_per_eu((!STR && R == 8) ? 2 : 1))) dp_kernel(DPArgs 
                     Ya = gy;
                 }
                 const bool inD = D < Tmine;
-                const int EbD = (EbNext != kNoStep) ? EbNext : EbPrev;
-                if (MODE == kScore && __builtin_expect(__any(EbD != EbPrev), 0)) {
-                    if (EbD != EbPrev) midMax.rebase(Ef + EbD + kPre);
+                if (MODE != kScore || __builtin_expect(eChg, 0)) {
+                    const int EbD = (EbNext != kNoStep) ? EbNext : EbPrev;
+                    if (MODE == kScore && EbD != EbPrev) midMax.rebase(Ef + EbD + kPre);
+                    EbPrev = EbD;
                 }
-                EbPrev = EbD;
                 const float gsC = rcg * sC;
                 const float psr = rc.ps * rf;
                 float smax = 0.f;
