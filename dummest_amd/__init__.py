@@ -107,6 +107,7 @@ def _declare(L):
                                   _P(f), _P(d), d, _P(d), i32, _P(i32), i32, _P(i32), _P(i32)]
     L.dm_set_option.argtypes = [vp, ctypes.c_char_p, i64]
     L.dm_calibrate.argtypes = [vp, i32, _P(i32), i32, i32, i32, _P(d), _P(d), _P(d)]
+    L.dm_calibrate_anchors.argtypes = [vp, i32, _P(i32), i32, i32, i32, _P(i32), _P(d)]
     L.dm_autotune.argtypes = [vp, i32, _P(d), i32]
     L.dm_set_config_costs.argtypes = [vp, i32, _P(d)]
     L.dm_config_table.argtypes = [vp, i32, _P(i32), _P(i32), _P(i32), _P(d)]
@@ -689,6 +690,21 @@ class Engine:
         out = [{"mid": dict(zip(GUMBEL_KEYS, f[:8].tolist())), "filter": dict(zip(GUMBEL_KEYS, f[8:].tolist()))}
                for f in fits]
         return mid, flt, out
+
+    def calibrate_anchors(self, prof, seq_len=1000, border=100, trials=1000):
+        """The per-trial table of estimateK's verbose output
+        (dummest-calibration.hh:511-519) for the same trials as calibrate():
+        (anchors, mid_scores): anchors (nprof x trials x 2) = profile position
+        and sequence position of each trial's best mid-anchored score, and that
+        score in nats (nprof x trials)."""
+        p = np.ascontiguousarray(np.atleast_1d(prof), np.int32)
+        n = len(p)
+        anchors = np.zeros((n, trials, 2), np.int32)
+        mid = np.zeros((n, trials), np.float64)
+        self._check(lib().dm_calibrate_anchors(self._h, n, _ptr(p, ctypes.c_int32), int(seq_len), int(border),
+                                               int(trials), _ptr(anchors, ctypes.c_int32),
+                                               _ptr(mid, ctypes.c_double)), "dm_calibrate_anchors")
+        return anchors, mid
 
     def autotune(self, seq_len=300):
         """Time every kernel configuration on the device; later plans pick the

@@ -574,11 +574,22 @@ int main(int argc, char *argv[]) {
         midS.resize((size_t)nc * o.trials);
         fltS.resize((size_t)nc * o.trials);
     }
+    const int printVerbosity = (nargs < 2) * 2 + (o.evalue <= 0);
+    // verbose calibration prints each trial's best anchor (dummest-calibration.hh:511-519)
+    std::vector<int32_t> trialAnchor;
+    std::vector<double> trialMid;
     if (nc > 0) {
         if (o.trials < 2) return die("calibration needs at least 2 trials");
         rc = dm_calibrate(E, nc, calPids.data(), o.length, o.border, o.trials, o.scoresFile ? midS.data() : nullptr,
                           o.scoresFile ? fltS.data() : nullptr, fits.data());
         if (rc) return engineFail(E, "dm_calibrate", rc);
+        if (printVerbosity > 1) {
+            trialAnchor.resize((size_t)nc * o.trials * 2);
+            trialMid.resize((size_t)nc * o.trials);
+            rc = dm_calibrate_anchors(E, nc, calPids.data(), o.length, o.border, o.trials, trialAnchor.data(),
+                                      trialMid.data());
+            if (rc) return engineFail(E, "dm_calibrate_anchors", rc);
+        }
     }
     std::ofstream scores;
     if (o.scoresFile) {
@@ -586,7 +597,6 @@ int main(int argc, char *argv[]) {
         if (!scores) return die("can't open scores file");
         scores << "mode\tprofile_name\ttrial\tanchor_type\tln_score\n";
     }
-    const int printVerbosity = (nargs < 2) * 2 + (o.evalue <= 0);
     // The reference's end- and start-anchored samples are all -inf
     // (dummest-dp-forward.hh:536-543); their fits are those of such a sample.
     double endFits[8];
@@ -634,6 +644,22 @@ int main(int argc, char *argv[]) {
             for (int j = 0; j < 20; j++) sum += (float)(p.bg[4 + j] * ncod[j]);
             sum += (float)o.rates[5];
             std::cout << "# sum is " << sum << std::endl;
+            if (printVerbosity > 1) {
+                // the trial table, in trial order; the end- and start-anchored
+                // entries repeat the mid anchor with score -inf (dummest-dp-forward.hh:536-543)
+                std::cout << "#trial\tend-anchored\t\tstart-anchored\t\tmid-anchored\n"
+                             "#\tprofPos\tseqPos\tscore\tprofPos\tseqPos\tscore\tprofPos\tseqPos\tscore"
+                          << std::endl;
+                const std::string ninf = scoreStr(-INFINITY);
+                for (int t = 0; t < o.trials; t++) {
+                    const size_t k = (size_t)calK * o.trials + t;
+                    const int a1 = trialAnchor[2 * k], a2 = trialAnchor[2 * k + 1];
+                    std::cout << t + 1 << "\t" << a1 << "\t" << a2 << "\t" << ninf << "\t" << a1 << "\t" << a2 << "\t"
+                              << ninf << "\t" << a1 << "\t" << a2 << "\t" << scoreStr(natsToLog2(trialMid[k]))
+                              << "\n";
+                }
+                std::cout.flush();
+            }
             gCache.announce();  // with -S the store is the first use
             gCache.store(keys[i], c);
         }

@@ -2045,21 +2045,27 @@ int dm_set_option(dm_engine *e, const char *name, int64_t value) {
     return DM_OK;
 }
 
-int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border, int32_t trials,
-                 double *mid_scores, double *flt_scores, double *fits) {
-    if (!e || nprof < 1 || !prof || !fits) return DM_EINVAL;
+// The calibration trials of nprof profiles (trial t of a profile seeded with
+// 5489 + t, dummest-calibration.hh:436-482), decoded into the engine's
+// sequence store after its current sequences: pair i = (profile i / trials,
+// sequence seqs0 + i). dropTrials removes them again.
+static int checkCalibArgs(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border,
+                          int32_t trials) {
     if (seq_len < 1 || border < 0 || trials < 2 || (int64_t)seq_len + border < 3 ||
         (int64_t)seq_len + border > (1 << 30))
         return fail(e, DM_EINVAL, "calibration needs seq_len >= 1, border >= 0, trials >= 2");
     for (int32_t k = 0; k < nprof; k++)
         if (prof[k] < 0 || prof[k] >= (int32_t)e->profiles.size()) return fail(e, DM_EINVAL, "unknown profile");
-    HIPCHK(hipSetDevice(e->device));
-    // The trial sequences go into the engine's store for the duration of the
-    // call and are removed afterwards.
+    return DM_OK;
+}
+
+static void addTrials(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border,
+                      int32_t trials, std::vector<int32_t> &pp, std::vector<int32_t> &ss) {
     const size_t codes0 = e->codes.size(), seqs0 = e->seqLen.size();
     const int len = seq_len + border;
     const int64_t npairs = (int64_t)nprof * trials;
-    std::vector<int32_t> pp(npairs), ss(npairs);
+    pp.assign(npairs, 0);
+    ss.assign(npairs, 0);
     e->codes.resize(codes0 + (size_t)npairs * len);
     for (int64_t i = 0; i < npairs; i++) {
         pp[i] = prof[i / trials];
@@ -2069,27 +2075,45 @@ int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_l
     }
     // the trials are independent (each seeds its own generator): host threads
     // generate and decode them into their slots
-    {
-        const unsigned hc = std::thread::hardware_concurrency();
-        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(hc ? hc : 1, (npairs + 63) / 64));
-        std::vector<std::thread> pool;
-        for (int th = 0; th < nth; th++)
-            pool.emplace_back([&, th]() {
-                std::vector<char> dna(len);
-                for (int64_t i = th; i < npairs; i += nth) {
-                    const HostProfile &P = e->profiles[pp[i]];
-                    calibSequence(P.bg.data() + 4, (float)e->rates[5], e->bgfs1(), e->bgfs2(), seq_len, border,
-                                  (int)(i % trials), dna.data());
-                    decodeDna(dna.data(), len, e->codes.data() + codes0 + (size_t)i * len);
-                }
-            });
-        for (auto &t : pool) t.join();
-    }
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(hc ? hc : 1, (npairs + 63) / 64));
+    std::vector<std::thread> pool;
+    for (int th = 0; th < nth; th++)
+        pool.emplace_back([&, th]() {
+            std::vector<char> dna(len);
+            for (int64_t i = th; i < npairs; i += nth) {
+                const HostProfile &P = e->profiles[pp[i]];
+                calibSequence(P.bg.data() + 4, (float)e->rates[5], e->bgfs1(), e->bgfs2(), seq_len, border,
+                              (int)(i % trials), dna.data());
+                decodeDna(dna.data(), len, e->codes.data() + codes0 + (size_t)i * len);
+            }
+        });
+    for (auto &t : pool) t.join();
     e->seqDirty = true;
+}
+
+static void dropTrials(dm_engine *e, size_t codes0, size_t seqs0) {
+    e->codes.resize(codes0);
+    e->seqOff.resize(seqs0);
+    e->seqLen.resize(seqs0);
+    e->seqDirty = true;
+}
+
+int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border, int32_t trials,
+                 double *mid_scores, double *flt_scores, double *fits) {
+    if (!e || nprof < 1 || !prof || !fits) return DM_EINVAL;
+    int rc = checkCalibArgs(e, nprof, prof, seq_len, border, trials);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->device));
+    // The trial sequences go into the engine's store for the duration of the
+    // call and are removed afterwards.
+    const size_t codes0 = e->codes.size(), seqs0 = e->seqLen.size();
+    const int64_t npairs = (int64_t)nprof * trials;
+    std::vector<int32_t> pp, ss;
+    addTrials(e, nprof, prof, seq_len, border, trials, pp, ss);
     std::vector<PairScore> out(npairs);
     // the whole sweep (null model and DP launches of every chunk) is captured
     // as one hipGraph and replayed
-    int rc;
     {
         dm_plan P;
         rc = buildPlan(e, kScore, npairs, pp.data(), ss.data(), 1, P);
@@ -2097,10 +2121,7 @@ int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_l
         if (rc == DM_OK) rc = executePlan(e, P, nullptr, out.data());
         P.release();
     }
-    e->codes.resize(codes0);
-    e->seqOff.resize(seqs0);
-    e->seqLen.resize(seqs0);
-    e->seqDirty = true;
+    dropTrials(e, codes0, seqs0);
     if (rc) return rc;
     auto lnOf = [](float m, int ex) {  // scoreToLog, dummest-score.hh:201-206
         if (m == 0.0f) return -INFINITY * 1.0;
@@ -2117,6 +2138,32 @@ int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_l
         }
         gumbelFits(mid, (double)seq_len, fits + (size_t)k * 16);
         gumbelFits(flt, (double)seq_len, fits + (size_t)k * 16 + 8);
+    }
+    return DM_OK;
+}
+
+int dm_calibrate_anchors(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border,
+                         int32_t trials, int32_t *anchors, double *mid_scores) {
+    if (!e || nprof < 1 || !prof || !anchors) return DM_EINVAL;
+    int rc = checkCalibArgs(e, nprof, prof, seq_len, border, trials);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->device));
+    const size_t codes0 = e->codes.size(), seqs0 = e->seqLen.size();
+    const int64_t npairs = (int64_t)nprof * trials;
+    std::vector<int32_t> pp, ss;
+    addTrials(e, nprof, prof, seq_len, border, trials, pp, ss);
+    // a negative ratio reports the best anchor of each pair and no alignment
+    // (dummest-dp-forward.hh:536-543), from the same dumped matrices as hits
+    std::vector<double> ratio((size_t)npairs, -2.0);
+    std::vector<std::vector<Hit>> out;
+    rc = pairsHitsRun(e, npairs, pp.data(), ss.data(), ratio.data(), out);
+    dropTrials(e, codes0, seqs0);
+    if (rc) return rc;
+    for (int64_t i = 0; i < npairs; i++) {
+        const bool has = !out[i].empty();
+        anchors[2 * i] = has ? out[i][0].anchor1 : 0;
+        anchors[2 * i + 1] = has ? out[i][0].anchor2 : 0;
+        if (mid_scores) mid_scores[i] = has ? out[i][0].lnratio : -INFINITY;
     }
     return DM_OK;
 }

@@ -255,6 +255,18 @@ int dm_post_process(int32_t M, const float *values, const float *params, const f
 int dm_calibrate(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border, int32_t trials,
                  double *mid_scores, double *flt_scores, double *fits);
 
+/* The per-trial table that estimateK prints when the engine runs without
+ * sequences (dummest-calibration.hh:392-396, 511-519, the verbose branch of
+ * its trial loop): for the same trials as dm_calibrate, the anchor of each
+ * trial's best mid-anchored score as profile position and sequence position
+ * (anchors[2 (k trials + t)], [.. + 1]) and that score in nats (mid_scores,
+ * may be NULL), taken from the same dumped matrices as dm_pairs_hits with a
+ * negative minimum ratio (dummest-dp-forward.hh:536-543), so the anchor is the
+ * reference's: the first column holding the highest cell, and within it the
+ * first row. */
+int dm_calibrate_anchors(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t seq_len, int32_t border,
+                         int32_t trials, int32_t *anchors, double *mid_scores);
+
 /* Host-only pieces of dm_calibrate: the DNA letters (L + border) of trial
  * `trial` for a profile with background bg[29] (bg_probs, letters at 4..23)
  * under the given rates (dummest-calibration.hh:436-482), and estimateGumbel

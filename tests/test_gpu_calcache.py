@@ -52,7 +52,9 @@ def test_second_run_uses_the_cache(hmm, tmp_path):
     assert out1.count("# sum is") == 2 and out1.count("# Warning: using cached results") == 1
     # run 2: everything cached; otherwise the same text
     assert "# sum is" not in out2 and out2.count("# Warning: using cached results") == 3
-    assert re.sub(r"# sum is .*", "# Warning: using cached results", out1) == out2
+    # (a calibrated profile also prints its trial table, which a cached one has not)
+    assert re.sub(r"# sum is .*\n#trial.*\n#\tprofPos.*\n(?:\d+\t.*\n)*", "# Warning: using cached results\n",
+                  out1) == out2
     lines = out1.splitlines()
     cf = '# Cache file: "%s"' % (tmp_path / "cache.bin")
     # printed once, by the first profile's lookup, after its background line
@@ -93,8 +95,11 @@ def test_scores_file_always_calibrates(hmm, tmp_path):
     assert out.count("# sum is") == 3 and "using cached results" not in out
     lines = out.splitlines()
     cf = '# Cache file: "%s"' % (tmp_path / "cache.bin")
-    # with -S the first use of the cache is the store after calibrating
-    assert lines[lines.index(cf) - 1].startswith("# sum is")
+    # with -S the first use of the cache is the store after calibrating, which
+    # follows the first profile's trial table (40 trials after '# sum is' and
+    # the two header lines)
+    i = lines.index(cf)
+    assert lines[i - 1].startswith("40\t") and lines[i - 43].startswith("# sum is")
     rows = open(tmp_path / "s.tsv").read().splitlines()
     assert len(rows) == 1 + 3 * 40 * 4
 
@@ -138,3 +143,29 @@ def test_search_records_same_with_cached_calibration(tmp_path):
     assert recs1, "the case should produce hits"
     assert recs1 == recs2
     assert "# sum is" in out1 and "# sum is" not in out2
+
+
+def test_trial_table(hmm, tmp_path):
+    """Without sequences the engine prints each trial's best anchor and
+    mid-anchored score after '# sum is' (estimateK's verbose trial loop,
+    dummest-calibration.hh:392-396, 511-519), the end- and start-anchored
+    columns repeating the anchor with -inf: every line against the oracle's
+    report for the same trial of the same finalized profile."""
+    import oracle as O
+    out = run([str(hmm)], tmp_path)
+    lines = out.splitlines()
+    for k, name in enumerate(["profA", "profB"]):
+        M, values, params, bg, _ = D.parse_hmm_text(hmm.read_text(), index=k, mask_threshold=0.5)
+        prof = O.Profile(name, M, values, params, bg, [])
+        i = lines.index("# Profile name: " + name)
+        j = next(x for x in range(i, len(lines)) if lines[x].startswith("# sum is"))
+        assert lines[j + 1] == "#trial\tend-anchored\t\tstart-anchored\t\tmid-anchored"
+        assert lines[j + 2] == "#\tprofPos\tseqPos\tscore\tprofPos\tseqPos\tscore\tprofPos\tseqPos\tscore"
+        for t in range(40):
+            f = lines[j + 3 + t].split("\t")
+            codes = O.decode(O.calib_sequence(bg, 600, 20, t))
+            ln, m, e, a1, a2, _ = O.search(prof, codes, -2.0, need_align=False).hits[2]
+            assert f[0] == str(t + 1) and f[3] == "-inf" and f[6] == "-inf", f
+            assert f[1:3] == f[4:6] == f[7:9] == [str(a1), str(a2)], (name, t, f)
+            assert float(f[9]) == pytest.approx(ln / np.log(2), abs=2e-6), (name, t, f)
+        assert lines[j + 43].startswith("# Cache file") or lines[j + 43].startswith("#\tend-")

@@ -245,6 +245,30 @@ def test_calibration_matches_oracle(engine):
             assert fits[k]["filter"][key] == pytest.approx(ref, rel=1e-3), key
 
 
+def test_calibration_trial_anchors(engine):
+    """dm_calibrate_anchors: each trial's best anchor (profile and sequence
+    position) and score, the table estimateK prints without sequences
+    (dummest-calibration.hh:511-519), against the oracle's third similarity
+    of the same trial (dummest-dp-forward.hh:536-543, the end- and
+    start-anchored entries repeating its anchor with -inf): anchors identical,
+    scores equal (the post-gate matrices are the oracle's bit for bit)."""
+    print("calibration anchors", flush=True)
+    pa, prof_a, _, _ = load_case(engine, 503, 70, [], 0)
+    pb, prof_b, _, _ = load_case(engine, 504, 130, [], 0)
+    seq_len, border, trials = 260, 25, 20
+    anchors, mid = engine.calibrate_anchors([pa, pb], seq_len, border, trials)
+    for k, p in enumerate([prof_a, prof_b]):
+        for t in range(trials):
+            codes = O.decode(O.calib_sequence(p.bg, seq_len, border, t))
+            ref = O.search(p, codes, -2.0, need_align=False)
+            assert len(ref.hits) == 3
+            assert ref.hits[0][0] == -np.inf and ref.hits[1][0] == -np.inf
+            assert ref.hits[0][3:5] == ref.hits[2][3:5]
+            ln, m, e, a1, a2, _ = ref.hits[2]
+            assert (int(anchors[k, t, 0]), int(anchors[k, t, 1])) == (a1, a2), (k, t)
+            assert mid[k, t] == pytest.approx(ln, abs=1e-12), (k, t)
+
+
 def test_every_configuration(engine):
     """Each compiled configuration forced in turn: single-strip kernels with one
     or two waves per workgroup (two waves share the emission table of two units
