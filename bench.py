@@ -35,6 +35,7 @@ FLOPS_PER_CELL = 43          # fwd+bwd: backward 20 + forward 21 + anchor 2 (alg
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector with packed math / matrix)
 HBM_PEAK_GBS = 8000.0
 GRAD_BYTES_PER_CELL = 104    # gradient: 2 x 24 B of stored FP64 matrices written, 56 B read back
+                             # (FETCH_SIZE + WRITE_SIZE of its kernels measure 105.7 B per cell)
 
 
 def log(msg):
@@ -198,10 +199,17 @@ def main():
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     dp_ms_step = sum(v[0] for v in agg.values()) / args.steps
     kname = f"void dm::dp_kernel<{G}, {R}, 1, {'true' if ST else 'false'}>(dm::DPArgs)"
+    traffic = traffic_of(kname, c / launches, D.build_id())
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic_of(kname, c / launches, D.build_id()),
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "kernel": kname, "avg_launch_ms": round(avg_ms, 3),
                 "cells_per_launch": int(c / launches), "flops_per_cell": FLOPS_PER_CELL}
+    if traffic:
+        # the W1 round trip (stored by the backward, read by the forward) puts
+        # the launch's measured HBM rate beside the VALU fraction
+        gbs = traffic / (avg_ms * 1e-3) / 1e9
+        roofline["hbm_measured"] = {"GB_per_s": round(gbs, 1), "peak": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                    "bytes_per_cell": round(traffic / (c / launches), 2)}
 
     # plain per-pair form on a sample
     rng = np.random.default_rng(7)
