@@ -15,6 +15,7 @@
 #include <cmath>
 #include <fstream>
 #include <sstream>
+#include <thread>
 
 #include "dm_host.h"
 
@@ -207,9 +208,15 @@ bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string
 
 }  // namespace
 
-bool readHmmText(const std::string &text, int bgType, const double rates[6], std::vector<HostProfile> &out,
-                 std::string &msg, float maskThr, double filterSd, bool keepNonvarying) {
-    std::istringstream in(text);
+namespace {
+
+// The HMMER3 reader's state machine over text[b, e): profiles are appended to
+// out (finalized); *endState receives the state at the end (0 between
+// profiles). Returns false with msg on malformed data.
+bool readHmmRange(const std::string &text, size_t b, size_t e, int bgType, const double rates[6],
+                  std::vector<HostProfile> &out, std::string &msg, float maskThr, double filterSd, bool keepNonvarying,
+                  int *endState) {
+    std::istringstream in(text.substr(b, e - b));
     std::string line;
     int state = 0;
     HostProfile cur;
@@ -272,8 +279,61 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
             }
         }
     }
+    if (endState) *endState = state;
     if (state != 0) return bad("truncated profile data");
     return true;
+}
+
+}  // namespace
+
+// Profiles are independent records ending in a "//" line, so they are parsed
+// and finalized on a pool of host threads, one record at a time, into their
+// places in file order. A record that fails, or whose "//" does not end a
+// profile, sends the whole text through the serial reader instead, so that
+// malformed data gives exactly the serial reader's result and message.
+bool readHmmText(const std::string &text, int bgType, const double rates[6], std::vector<HostProfile> &out,
+                 std::string &msg, float maskThr, double filterSd, bool keepNonvarying) {
+    std::vector<size_t> ends;  // one past each "//" line
+    for (size_t pos = 0; pos < text.size();) {
+        size_t nl = text.find('\n', pos);
+        const size_t le = nl == std::string::npos ? text.size() : nl + 1;
+        size_t f = pos;
+        while (f < le && (text[f] == ' ' || text[f] == '\t')) f++;
+        if (f + 1 < le && text[f] == '/' && text[f + 1] == '/' &&
+            (f + 2 == le || std::isspace((unsigned char)text[f + 2])))
+            ends.push_back(le);
+        pos = le;
+    }
+    const size_t nrec = ends.size();
+    const unsigned hc = std::thread::hardware_concurrency();
+    const size_t nth = std::min<size_t>(std::max(1u, std::min(16u, hc ? hc : 1u)), nrec);
+    if (nrec >= 8 && nth > 1) {
+        std::vector<std::vector<HostProfile>> got(nrec);
+        std::vector<char> ok(nrec, 0);
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nth; t++)
+            pool.emplace_back([&, t]() {
+                std::string m;
+                for (size_t r = t; r < nrec; r += nth) {
+                    int st = -1;
+                    ok[r] = readHmmRange(text, r ? ends[r - 1] : 0, ends[r], bgType, rates, got[r], m, maskThr, filterSd,
+                                         keepNonvarying, &st) &&
+                            st == 0 && got[r].size() == 1;
+                }
+            });
+        for (auto &th : pool) th.join();
+        int tail = -1;
+        std::vector<HostProfile> rest;
+        std::string m;
+        const bool tailOk = readHmmRange(text, nrec ? ends[nrec - 1] : 0, text.size(), bgType, rates, rest, m, maskThr,
+                                         filterSd, keepNonvarying, &tail) &&
+                            rest.empty();
+        if (tailOk && std::all_of(ok.begin(), ok.end(), [](char c) { return c != 0; })) {
+            for (auto &g : got) out.push_back(std::move(g[0]));
+            return true;
+        }
+    }
+    return readHmmRange(text, 0, text.size(), bgType, rates, out, msg, maskThr, filterSd, keepNonvarying, nullptr);
 }
 
 bool finalizeRaw(HostProfile &p, int bgType, const double rates[6], std::string &msg, float maskThr, double filterSd,
@@ -288,9 +348,21 @@ bool readHmmFile(const char *path, int bgType, const double rates[6], std::vecto
         msg = std::string("can't open file: ") + path;
         return false;
     }
-    std::stringstream ss;
-    ss << f.rdbuf();
-    return readHmmText(ss.str(), bgType, rates, out, msg, maskThr, filterSd, keepNonvarying);
+    f.seekg(0, std::ios::end);
+    const std::streamoff size = f.tellg();
+    f.seekg(0, std::ios::beg);
+    std::string text;
+    if (size > 0) {
+        text.resize((size_t)size);
+        f.read(&text[0], size);
+        text.resize((size_t)f.gcount());
+    } else {  // not seekable: read the stream as it comes
+        f.clear();
+        std::stringstream ss;
+        ss << f.rdbuf();
+        text = ss.str();
+    }
+    return readHmmText(text, bgType, rates, out, msg, maskThr, filterSd, keepNonvarying);
 }
 
 // Codon table in TCAG order; non-ACGT letters make the codon '?'.
