@@ -43,6 +43,7 @@
 #include <sstream>
 #include <string>
 #include <thread>
+#include <numeric>
 #include <unordered_map>
 #include <vector>
 
@@ -329,6 +330,11 @@ static double wallNow() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 static double hitsWall = 0, hitsDevice = 0;
+// wall time per phase of the run (DM_TIMING)
+enum { kTProfiles, kTCalib, kTInput, kTPrepare, kTFilter, kTScores, kTOutput, kTPhases };
+static double phaseWall[kTPhases] = {};
+static const char *const phaseName[kTPhases] = {"profiles", "calibration", "input", "mask+decode+upload",
+                                                "filter", "scores", "records"};
 
 int main(int argc, char *argv[]) {
     const double runStart = wallNow();
@@ -496,6 +502,7 @@ int main(int argc, char *argv[]) {
     if (o.border > INT32_MAX - 64 - o.length) return die("sequence + border is too big");
 
     // ------------------------------------------------------------- profiles
+    double phaseStart = wallNow();
     std::vector<HostProfile> profiles;
     {
         std::string msg;
@@ -515,6 +522,7 @@ int main(int argc, char *argv[]) {
             return engineFail(E, "dm_add_profile", rc);
     }
 
+    phaseWall[kTProfiles] += wallNow() - phaseStart;
     std::cout << "# DUMMEST-AMD 1\n";
     std::cout << "# Bytes per floating-point number: 4\n";
     if (filterStdDev > 0)
@@ -578,6 +586,7 @@ int main(int argc, char *argv[]) {
     // verbose calibration prints each trial's best anchor (dummest-calibration.hh:511-519)
     std::vector<int32_t> trialAnchor;
     std::vector<double> trialMid;
+    phaseStart = wallNow();
     if (nc > 0) {
         if (o.trials < 2) return die("calibration needs at least 2 trials");
         rc = dm_calibrate(E, nc, calPids.data(), o.length, o.border, o.trials, o.scoresFile ? midS.data() : nullptr,
@@ -591,6 +600,7 @@ int main(int argc, char *argv[]) {
             if (rc) return engineFail(E, "dm_calibrate_anchors", rc);
         }
     }
+    phaseWall[kTCalib] += wallNow() - phaseStart;
     std::ofstream scores;
     if (o.scoresFile) {
         scores.open(o.scoresFile);
@@ -742,6 +752,7 @@ int main(int argc, char *argv[]) {
 
     auto flush = [&]() -> int {
         if (targets.empty()) return 0;
+        double t0 = wallNow();
         // -m 2: tantan on the chunk's contigs on the GPU; letters at or above
         // the threshold (a Float, as TANTAN_MASK_THRESHOLD) decode as non-ACGT
         if (o.mask & 2) {
@@ -792,6 +803,8 @@ int main(int argc, char *argv[]) {
             ss.push_back(first + reqTarget[k]);
             rq.push_back(k);
         }
+        phaseWall[kTPrepare] += wallNow() - t0;
+        t0 = wallNow();
         if (!o.maxMode && !pp.empty()) {
             std::vector<float> best(pp.size());
             r = dm_filter_scores(E, (int64_t)pp.size(), pp.data(), ss.data(), best.data());
@@ -815,6 +828,8 @@ int main(int argc, char *argv[]) {
             ss.swap(s2);
             rq.swap(q2);
         }
+        phaseWall[kTFilter] += wallNow() - t0;
+        t0 = wallNow();
         std::ostringstream os;
         if (!pp.empty()) {
             std::vector<float> mm(pp.size()), bm(pp.size());
@@ -823,6 +838,7 @@ int main(int argc, char *argv[]) {
                                  be.data());
             if (r) return engineFail(E, "dm_fwdbwd_scores", r);
             nFullPairs += (int64_t)pp.size();
+            phaseWall[kTScores] += wallNow() - t0;
             // gate: best anchored score >= minProbRatio (dummest-dp-forward.hh:263-271).
             // The score pass rounds differently from the reference (fused
             // multiply-adds, another summation order), so here a pair is
@@ -864,6 +880,7 @@ int main(int argc, char *argv[]) {
             if (r) return engineFail(E, "dm_pairs_hits", r);
             hitsWall += wallNow() - hitsStart;
             hitsDevice += dm_last_hits_ms(E) * 1e-3;
+            t0 = wallNow();
             int64_t so = 0;
             for (size_t gi = 0; gi < gk.size(); gi++) {
                 const size_t k = gk[gi];
@@ -925,17 +942,25 @@ int main(int argc, char *argv[]) {
         }
         const std::string out = os.str();
         std::fwrite(out.data(), 1, out.size(), stdout);
+        if (!pp.empty()) phaseWall[kTOutput] += wallNow() - t0;
         targets.clear();
         reqProfRatio.clear();
         reqTarget.clear();
         return 0;
     };
 
+    // profiles by name, in index order (a pipeline record's profile= picks them)
+    std::unordered_map<std::string, std::vector<int>> byName;
+    for (int j = 0; j < np; j++) byName[profiles[j].name].push_back(j);
+    std::vector<int> allProfiles(np);
+    std::iota(allProfiles.begin(), allProfiles.end(), 0);
+    const std::vector<int> noProfiles;
     {
         ContigReader rd(*in);
         SeqState s;
         Contig ct;
         std::vector<char> letters;
+        double inStart = wallNow();
         while (rd.next(s, ct, letters, err)) {
             if (ct.length == 0) continue;
             if (needTotal) minProbRatioTotal += ct.length;
@@ -949,16 +974,25 @@ int main(int argc, char *argv[]) {
                 T.nums[j] = (uint8_t)letters[j];
             }
             const int ti = (int)targets.size();
-            for (int j = 0; j < np; j++) {
-                if (!s.targetProfile.empty() && profiles[j].name != s.targetProfile) continue;
+            const std::vector<int> *req = &allProfiles;
+            if (!s.targetProfile.empty()) {
+                const auto it = byName.find(s.targetProfile);
+                req = it == byName.end() ? &noProfiles : &it->second;
+            }
+            for (const int j : *req) {
                 const float ratio =
                     o.evalue > 0 ? (float)std::pow(K[j] * (double)minProbRatioTotal / o.evalue, 1.0 / lambda[j]) : -1.f;
                 reqProfRatio.push_back({j, ratio});
                 reqTarget.push_back(ti);
             }
             targets.push_back(std::move(T));
-            if ((int)targets.size() >= batch && (rc = flush())) return rc;
+            if ((int)targets.size() >= batch) {
+                phaseWall[kTInput] += wallNow() - inStart;
+                if ((rc = flush())) return rc;
+                inStart = wallNow();
+            }
         }
+        phaseWall[kTInput] += wallNow() - inStart;
         if (!err.empty()) return die(err);
         if ((rc = flush())) return rc;
     }
@@ -970,6 +1004,9 @@ int main(int argc, char *argv[]) {
         const double total = wallNow() - runStart;
         std::fprintf(stderr, "timing: run %.3f s, post-gate hits %.3f s (%.1f%%; %.3f s on the device) for %lld gated pairs\n",
                      total, hitsWall, 100.0 * hitsWall / total, hitsDevice, (long long)nGated);
+        std::fprintf(stderr, "timing:");
+        for (int k = 0; k < kTPhases; k++) std::fprintf(stderr, " %s %.3f s", phaseName[k], phaseWall[k]);
+        std::fprintf(stderr, "\n");
     }
     return 0;
 }
