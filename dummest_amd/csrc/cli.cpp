@@ -44,6 +44,7 @@
 #include <string>
 #include <thread>
 #include <numeric>
+#include <string_view>
 #include <unordered_map>
 #include <vector>
 
@@ -753,28 +754,45 @@ int main(int argc, char *argv[]) {
     auto flush = [&]() -> int {
         if (targets.empty()) return 0;
         double t0 = wallNow();
+        // Records with the same letters (a pipeline lists a window once per
+        // candidate profile) get the same masks and codes: each distinct
+        // sequence is masked, decoded and uploaded once, and the records keep
+        // their own headers and coordinates.
+        std::vector<int> uniq, seqOf(targets.size());
+        {
+            std::unordered_map<std::string_view, int> seen;
+            seen.reserve(targets.size());
+            for (size_t k = 0; k < targets.size(); k++) {
+                const auto ins = seen.emplace(std::string_view(targets[k].letters), (int)uniq.size());
+                if (ins.second) uniq.push_back((int)k);
+                seqOf[k] = ins.first->second;
+            }
+        }
         // -m 2: tantan on the chunk's contigs on the GPU; letters at or above
         // the threshold (a Float, as TANTAN_MASK_THRESHOLD) decode as non-ACGT
         if (o.mask & 2) {
             std::string all;
             std::vector<int64_t> toff;
             std::vector<int32_t> tlen;
-            for (const Target &t : targets) {
+            for (const int u : uniq) {
                 toff.push_back((int64_t)all.size());
-                tlen.push_back((int32_t)t.letters.size());
-                all += t.letters;
+                tlen.push_back((int32_t)targets[u].letters.size());
+                all += targets[u].letters;
             }
             std::vector<float> probs(std::max<size_t>(1, all.size()));
-            int r = dm_tantan_dna(E, (int32_t)targets.size(), all.data(), toff.data(), tlen.data(), probs.data());
+            int r = dm_tantan_dna(E, (int32_t)uniq.size(), all.data(), toff.data(), tlen.data(), probs.data());
             if (r) return engineFail(E, "dm_tantan_dna", r);
             const float thr = (float)o.tantan;
-            for (size_t k = 0; k < targets.size(); k++) {
-                Target &t = targets[k];
+            for (size_t q = 0; q < uniq.size(); q++) {
+                Target &t = targets[uniq[q]];
                 t.masked.assign(t.letters.size(), 0);
-                for (size_t j = 0; j < t.letters.size(); j++) t.masked[j] = !(probs[toff[k] + j] < thr);
+                for (size_t j = 0; j < t.letters.size(); j++) t.masked[j] = !(probs[toff[q] + j] < thr);
             }
+            for (size_t k = 0; k < targets.size(); k++)
+                if (uniq[seqOf[k]] != (int)k) targets[k].masked = targets[uniq[seqOf[k]]].masked;
         }
-        for (Target &t : targets) {
+        for (const int u : uniq) {
+            Target &t = targets[u];
             std::string dec = t.letters;
             for (size_t j = 0; j < t.masked.size(); j++)
                 if (t.masked[j]) dec[j] = 'N';
@@ -784,7 +802,8 @@ int main(int argc, char *argv[]) {
         std::vector<uint8_t> codes;
         std::vector<int64_t> offs;
         std::vector<int32_t> lens;
-        for (const Target &t : targets) {
+        for (const int u : uniq) {
+            const Target &t = targets[u];
             offs.push_back((int64_t)codes.size());
             lens.push_back((int32_t)t.codes.size());
             codes.insert(codes.end(), t.codes.begin(), t.codes.end());
@@ -792,15 +811,16 @@ int main(int argc, char *argv[]) {
         }
         int r = dm_clear_sequences(E);
         int32_t first = 0;
-        if (!r) r = dm_add_sequences(E, (int32_t)targets.size(), codes.data(), offs.data(), lens.data(), &first);
+        if (!r) r = dm_add_sequences(E, (int32_t)uniq.size(), codes.data(), offs.data(), lens.data(), &first);
         if (r) return engineFail(E, "dm_add_sequences", r);
         // requests on sequences shorter than 3 cannot be scored (dummest-dp.hh:84)
         std::vector<int32_t> pp, ss;
         std::vector<size_t> rq;
         for (size_t k = 0; k < reqTarget.size(); k++) {
-            if (targets[reqTarget[k]].codes.size() < 3) continue;
+            const int u = seqOf[reqTarget[k]];
+            if (targets[uniq[u]].codes.size() < 3) continue;
             pp.push_back(pids[reqProfRatio[k].first]);
-            ss.push_back(first + reqTarget[k]);
+            ss.push_back(first + u);
             rq.push_back(k);
         }
         phaseWall[kTPrepare] += wallNow() - t0;
