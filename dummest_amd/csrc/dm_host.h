@@ -14,6 +14,10 @@
 
 namespace dm {
 
+// Worker threads for the library's host-side pools: the CPUs this process may
+// run on (its affinity mask), at most 16.
+int hostThreads();
+
 struct HostProfile {
     std::string name;
     int M = 0;

@@ -1757,7 +1757,7 @@ static int pairsHitsRun(dm_engine *e, int64_t npairs, const int32_t *prof, const
         // anchors ranked per pair (independent), on a pool of host threads
         std::vector<std::vector<Hit>> bestOnly(np);
         {
-            const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            const int nthr = hostThreads();
             auto rankRange = [&](int c0, int c1) {
                 std::vector<ES> colBest;
                 std::vector<int> colRow;
@@ -2075,8 +2075,7 @@ static void addTrials(dm_engine *e, int32_t nprof, const int32_t *prof, int32_t 
     }
     // the trials are independent (each seeds its own generator): host threads
     // generate and decode them into their slots
-    const unsigned hc = std::thread::hardware_concurrency();
-    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(hc ? hc : 1, (npairs + 63) / 64));
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(hostThreads(), (npairs + 63) / 64));
     std::vector<std::thread> pool;
     for (int th = 0; th < nth; th++)
         pool.emplace_back([&, th]() {

@@ -17,6 +17,8 @@
 #include <sstream>
 #include <thread>
 
+#include <sched.h>
+
 #include "dm_host.h"
 
 namespace dm {
@@ -208,6 +210,15 @@ bool finalizeImpl(HostProfile &p, int bgType, const double rates[6], std::string
 
 }  // namespace
 
+int hostThreads() {
+    int n = 0;
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(16, n));
+}
+
 namespace {
 
 // The HMMER3 reader's state machine over text[b, e): profiles are appended to
@@ -305,8 +316,7 @@ bool readHmmText(const std::string &text, int bgType, const double rates[6], std
         pos = le;
     }
     const size_t nrec = ends.size();
-    const unsigned hc = std::thread::hardware_concurrency();
-    const size_t nth = std::min<size_t>(std::max(1u, std::min(16u, hc ? hc : 1u)), nrec);
+    const size_t nth = std::min<size_t>((size_t)hostThreads(), nrec);
     if (nrec >= 8 && nth > 1) {
         std::vector<std::vector<HostProfile>> got(nrec);
         std::vector<char> ok(nrec, 0);
