@@ -666,6 +666,9 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
     HIPCHK(P.counter.ensure(sizeof(int) * std::max<size_t>(64, ncfg)));
     if (mode == kScore) {
         HIPCHK(P.wbuf.ensure(sizeof(float) * maxW));
+        // the score kernels skip the W1 stores of slots outside the matrix and
+        // read those slots with zero weight: they must hold finite values
+        HIPCHK(hipMemsetAsync(P.wbuf.p, 0, P.wbuf.cap, e->stream));
         HIPCHK(P.ebuf.ensure(sizeof(int) * maxE));
     }
     HIPCHK(P.bnd.ensure(sizeof(Bnd) * 3 * maxB));

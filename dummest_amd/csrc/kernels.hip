@@ -789,25 +789,52 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             float gq = bgl[rcur.codes & 255];  // g[c+1] of the next step
             // lanes whose rows all lie past row M store and load no W1
             const bool rowsLive = strip * ROWS + k * R <= M;
+            // Score mode: chunk j of a step's block is stored by runs of L_j
+            // lanes (L_j lanes of chunk j fill about one 128-byte line); a run
+            // is skipped when none of its lanes holds a cell of the matrix: all
+            // its columns lie outside [0, n) (the skew), or all its rows past M.
+            // Skipping whole runs keeps every write a whole line.
+            auto runLanes = [](int j) {
+                const int wj = (R - 4 * j < 4) ? R - 4 * j : 4;
+                const int l = (wj >= 3) ? 8 : (wj == 2 ? 16 : 32);
+                return l < G ? l : G;
+            };
+            int runLo[(R + 3) / 4], runSpan[(R + 3) / 4];  // run of chunk j: columns D - runLo - runSpan + 1 .. D - runLo
+#pragma unroll
+            for (int j = 0; j * 4 < R; j++) {
+                const int k0 = k & ~(runLanes(j) - 1);
+                runLo[j] = k0;
+                runSpan[j] = (active && strip * ROWS + k0 * R <= M) ? n + runLanes(j) - 1 : 0;
+            }
 
             // W1 of step tt (ring slot sl) and its exponent into the scratch.
-            // Score mode stores every lane's block of every step of the pair
-            // (anti-diagonals 0..Tb-1, inside the group's own scratch slot) with
-            // no test of rows or columns: the forward then reads only values
-            // this pair wrote, all finite, and the cells outside the matrix
-            // cannot raise the anchor maximum (columns outside the sequence meet
-            // a zero on one side, rows past M hold seed-only values that never
-            // exceed row M's at the same column). Dump mode stores the live
-            // cells of active pairs only (its buffers are the pairs' own).
-            // Score mode writes the exponents of a four-step iteration with one
-            // store (storeE4) when its W1 blocks are stored late, else one per step.
+            // Score mode stores the runs of lanes described above for every
+            // step of the pair (anti-diagonals 0..Tb-1, inside the group's own
+            // scratch slot). The forward loads every slot; the ones no run wrote
+            // hold finite values (the scratch is zeroed when it is allocated and
+            // only ever receives finite W1 values), and it gives them no weight
+            // wherever w0 is not zero (columns at or past n, lanes past row M;
+            // at columns before 0, w0 is zero). Stored cells outside the matrix
+            // cannot raise the anchor maximum either: columns outside the
+            // sequence meet a zero on one side, rows past M in a live lane hold
+            // seed-only values that never exceed row M's at the same column.
+            // Dump mode stores the live cells of active pairs only (its buffers
+            // are the pairs' own). Score mode writes the exponents of a four-step
+            // iteration with one store (storeE4) when its W1 blocks are stored
+            // late, else one per step.
             auto storeW1 = [&](int tt, int sl, bool withExp) {
                 const int D = Tb - 1 - tt;
                 if (MODE == kScore) {
                     if (D >= 0) {
                         float *dst = wsc + (int64_t)D * ROWS;
 #pragma unroll
-                        for (int q = 0; q < R; q++) dst[w1Off(k, q)] = H[sl][q];
+                        for (int j = 0; j * 4 < R; j++) {
+                            // some column of the run lies in [0, n) (rows checked in runSpan)
+                            if ((unsigned)(D - runLo[j]) < (unsigned)runSpan[j]) {
+#pragma unroll
+                                for (int q = 4 * j; q < 4 * j + 4 && q < R; q++) dst[w1Off(k, q)] = H[sl][q];
+                            }
+                        }
                         if (withExp) esc[D] = Eb;  // every lane of the group writes the same value
                     }
                 } else {
@@ -1076,6 +1103,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // Loads are unconditional (from a clamped, always-valid step) so that
             // no select waits for them; which of them count is decided at use.
             const int Tw = Tmine;
+            const int nW1 = (active && strip * ROWS + k * R <= M) ? n : 0;  // columns of this lane with W1
             auto loadW1 = [&](int tt, float *dst, int &e) {
                 const int ts = tt < Tw ? tt : (Tw > 0 ? Tw - 1 : 0);
                 if (MODE == kScore) {
@@ -1131,10 +1159,10 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 for (int q = 0; q < R; q++) eqN[q] = etk[((nx.codes >> 8) & 255) * ROWS + q];
                 const float gqN = bgl[(nx.codes >> 8) & 255];
                 // the stored block of this step scaled by 2^-100 (so that w0 W1
-                // cannot overflow), or zero past the pair's last step (the loads
-                // there repeat its last block)
+                // cannot overflow), or zero where the lane's column lies at or
+                // past n or all its rows past M (no W1 of the matrix there)
                 float W1v[R];
-                const float w1f = (MODE == kScore && t < Tmine) ? 7.888609052e-31f : 0.f;
+                const float w1f = (MODE == kScore && t - k < nW1) ? 7.888609052e-31f : 0.f;
 #pragma unroll
                 for (int q = 0; q < R; q++) W1v[q] = Wq[SL][q] * w1f;
                 const int EbNext = (t < Tw) ? ((MODE == kScore) ? e4cur.at<PH>() : Eq[SL]) : kNoStep;
