@@ -143,6 +143,24 @@ inline constexpr int dp_wg(int G, int R, bool strips, int mode) {
     return (mode == kDump || G * R <= 192 || G * R > 512) ? 1 : (G * R > 384 ? 4 : 2);
 }
 
+// The backward's column records for the last lane of each group pass through
+// an LDS ring, and so do, in the strip kernels outside dump mode, the strip
+// below's edge values that the last lane reads: blocks of kRingBlock steps,
+// each loaded with one entry per lane a block ahead, two blocks plus two
+// entries that repeat the first ones (so that a four-step iteration reads
+// without wrapping). Per entry: the (prefix seed, codes) pair and the suffix
+// seed, 12 bytes, and for the edge (value, Y, exponent) another 12.
+constexpr int kRingBlock = 8;
+constexpr int kRing = 2 * kRingBlock + 2;
+inline constexpr int dp_ring_bytes(bool strips, int mode) { return kRing * (12 + ((strips && mode != kDump) ? 12 : 0)); }
+
+// Dynamic LDS of a DP launch: the emission table (kNCode x G R floats), the
+// background table (32 floats) and every group's rings.
+inline constexpr size_t dp_smem(int G, int R, bool strips, int mode) {
+    return sizeof(float) * ((size_t)kNCode * G * R + 32) +
+           (size_t)dp_wg(G, R, strips, mode) * (64 / G) * dp_ring_bytes(strips, mode);
+}
+
 // tantan on DNA (sequence masking): bases 0..3 per letter, sequences at
 // offsets; probs in the same layout; scales: lengths/16 doubles per sequence at
 // scaleOff. Sixteen sequences per 64-lane wave, four lanes per sequence.

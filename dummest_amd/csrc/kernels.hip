@@ -569,13 +569,24 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
     // prefetch distance in steps (divides 4); the strip kernels keep 2, which
     // lets all but G = 32, R = 8 fit 256 registers (two waves per SIMD)
     constexpr int kPF = ((R < 8 && !STR) || (STR && G == 32 && R == 8)) ? 4 : 2;
-    extern __shared__ float etab[];  // [kNCode][ROWS], then the profile's background table (32)
+    extern __shared__ float etab[];  // [kNCode][ROWS], the profile's background table (32), the rings
     float *bgl = etab + kNCode * ROWS;
     const int lane = threadIdx.x & 63;
     const int wv = (WG > 1) ? (int)(threadIdx.x >> 6) : 0;  // wave of the workgroup
     const int wslot = (int)blockIdx.x * WG + wv;             // this wave's scratch slot
     const int grp = lane / G;
     const int k = lane - grp * G;
+    // this group's backward rings: kRing (prefix seed, codes) pairs, then
+    // (after every group's pairs) kRing suffix seeds; the strip kernels' edge
+    // values, Y values and exponents follow in the same way
+    constexpr int kRings = WG * (64 / G);
+    constexpr bool kGhostRing = STR && MODE != kDump;
+    const int ringId = wv * (64 / G) + grp;
+    uint2 *ringPC = reinterpret_cast<uint2 *>(bgl + 32) + ringId * kRing;
+    float *ringSS = reinterpret_cast<float *>(reinterpret_cast<uint2 *>(bgl + 32) + kRings * kRing) + ringId * kRing;
+    float *ringGV = ringSS + kRings * kRing;
+    float *ringGY = ringGV + kRings * kRing;
+    int *ringGE = reinterpret_cast<int *>(ringGY + kRings * kRing);
     __shared__ int sItem;
     for (;;) {
         int u;
@@ -763,9 +774,30 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 return recs[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
             };
             const bool injB = (k == G - 1);
-            Rec rq[kPF];  // the last lane's records of the next steps (slot = step mod kPF)
-#pragma unroll
-            for (int f = 1; f <= kPF; f++) rq[f % kPF] = brecAt(f, G - 1);
+            // The last lane's record of step tt sits in ring entry tt mod
+            // 2 kRingBlock (entries 2 kRingBlock and 2 kRingBlock + 1 repeat
+            // entries 0 and 1). Lane k < kRingBlock of the group writes entry k
+            // of a block; every lane reads the entry of its group. A block is
+            // written at the start of the iteration before it is first read and
+            // was loaded a block earlier, so the loop's memory waits no longer
+            // cover the W1 stores of the steps just before them.
+            auto ringPut = [&](int base, const Rec &r) {
+                if (k < kRingBlock) {
+                    ringPC[base + k] = uint2{__float_as_uint(r.ps), r.codes};
+                    ringSS[base + k] = r.ss;
+                    if (base == 0 && k < 2) {
+                        ringPC[2 * kRingBlock + k] = uint2{__float_as_uint(r.ps), r.codes};
+                        ringSS[2 * kRingBlock + k] = r.ss;
+                    }
+                }
+            };
+            auto ringGet = [&](int e) {
+                const uint2 pc = ringPC[e];
+                return Rec{__uint_as_float(pc.x), pc.y, ringSS[e]};
+            };
+            ringPut(0, brecAt(k & (kRingBlock - 1), G - 1));
+            Rec pend = brecAt(kRingBlock + (k & (kRingBlock - 1)), G - 1);  // the next block
+            Rec rnext = ringGet(1);  // the last lane's record of the next step
             Rec rcur = brecAt(0, k);
             // strip kernels: the strip below's top row (W1, Y, exponent), also ahead
             Bnd gq4[kPF];
@@ -780,8 +812,30 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 if (STR) return bIn[bidx(Tb - 1 - tt - (MODE == kDump ? k : G - 1), n)];
                 return Bnd{0.f, 0.f, 0};
             };
+            // Outside dump mode the group's edge value of step tt comes through
+            // the ghost ring, entry tt mod 2 kRingBlock, like the records.
+            auto gringPut = [&](int base, const Bnd &g) {
+                if (k < kRingBlock) {
+                    ringGV[base + k] = g.v;
+                    ringGY[base + k] = g.y;
+                    ringGE[base + k] = g.e;
+                    if (base == 0 && k < 2) {
+                        ringGV[2 * kRingBlock + k] = g.v;
+                        ringGY[2 * kRingBlock + k] = g.y;
+                        ringGE[2 * kRingBlock + k] = g.e;
+                    }
+                }
+            };
+            auto gringGet = [&](int e) { return Bnd{ringGV[e], ringGY[e], ringGE[e]}; };
+            Bnd gpend{0.f, 0.f, 0}, gnext{0.f, 0.f, 0};
+            if constexpr (kGhostRing) {
+                gringPut(0, ghostAt(k & (kRingBlock - 1)));
+                gpend = ghostAt(kRingBlock + (k & (kRingBlock - 1)));
+                gnext = gringGet(0);  // step 0
+            } else {
 #pragma unroll
-            for (int f = 0; f < kPF; f++) gq4[f] = ghostAt(f);
+                for (int f = 0; f < kPF; f++) gq4[f] = ghostAt(f);
+            }
             // emission values of the next step, read from LDS one step ahead
             float eq[R];
 #pragma unroll
@@ -870,17 +924,19 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 const float rcg = gq;
                 // the next step's record (the lane below's, or a new one at the last
                 // lane) and its emission values, read now so that they arrive in time
-                constexpr int SN = (PH + 1) % kPF;
                 Rec nx = rec_from_below<G>(rcur);
-                if (injB) nx = rq[SN];
-                rq[SN] = brecAt(t + 1 + kPF, G - 1);  // same address in every lane of the group
+                if (injB) nx = rnext;
+                rnext = ringGet(((t - PH) & (2 * kRingBlock - 1)) + PH + 2);  // step t + 2
                 float eqN[R];
 #pragma unroll
                 for (int q = 0; q < R; q++) eqN[q] = etk[(nx.codes & 255) * ROWS + q];
                 const float gqN = bgl[nx.codes & 255];
                 float gw = 0.f, gy = 0.f;
                 Bnd gcur{0.f, 0.f, 0};
-                if (STR) {
+                if constexpr (kGhostRing) {
+                    gcur = gnext;
+                    gnext = gringGet(((t - PH) & (2 * kRingBlock - 1)) + PH + 1);  // step t + 1
+                } else if (STR) {
                     gcur = gq4[PH % kPF];
                     gq4[PH % kPF] = ghostAt(t + kPF);
                 }
@@ -1004,6 +1060,15 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             };
             int t = 0;
             for (; t + 4 <= T; t += 4) {
+                // the ring block that starts at step t + 4, and the load of the one after it
+                if (((t + 4) & (kRingBlock - 1)) == 0) {
+                    ringPut((t + 4) & (2 * kRingBlock - 1), pend);
+                    pend = brecAt(t + 4 + kRingBlock + (k & (kRingBlock - 1)), G - 1);
+                    if constexpr (kGhostRing) {
+                        gringPut((t + 4) & (2 * kRingBlock - 1), gpend);
+                        gpend = ghostAt(t + 4 + kRingBlock + (k & (kRingBlock - 1)));
+                    }
+                }
                 // Score mode stores an iteration's four W1 blocks (still in the
                 // ring, in the units of the current exponent) when the next one
                 // starts: the loop's waits for prefetched loads also wait for the
@@ -1345,7 +1410,7 @@ static hipError_t launch_one(K kernel, size_t smem, int grid, int wg, hipStream_
 }
 
 hipError_t launch_dp(int G, int R, int mode, bool strips, const DPArgs &a, int grid, hipStream_t s) {
-    const size_t smem = sizeof(float) * (kNCode * G * R + 32);
+    const size_t smem = dp_smem(G, R, strips, mode);
     const int wg = dp_wg(G, R, strips, mode);
 #define DM_LAUNCH(g, r)                                                                                 \
     if (G == g && R == r && !strips) {                                                                  \
