@@ -587,6 +587,42 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
     float *ringGV = ringSS + kRings * kRing;
     float *ringGY = ringGV + kRings * kRing;
     int *ringGE = reinterpret_cast<int *>(ringGY + kRings * kRing);
+    // The injection lane's record of step tt of a pass (the last lane's in the
+    // backward, the first lane's in the forward) sits in ring entry tt mod
+    // 2 kRingBlock (entries 2 kRingBlock and 2 kRingBlock + 1 repeat entries 0
+    // and 1). Lane k < kRingBlock of the group writes entry k of a block; every
+    // lane reads the entry of its group. A block is written at the start of the
+    // iteration before it is first read and was loaded a block earlier, so the
+    // loops' memory waits no longer cover the stores and loads of the steps
+    // just before them. The passes run one after the other and share the ring.
+    auto ringPut = [&](int base, const Rec &r) {
+        if (k < kRingBlock) {
+            ringPC[base + k] = uint2{__float_as_uint(r.ps), r.codes};
+            ringSS[base + k] = r.ss;
+            if (base == 0 && k < 2) {
+                ringPC[2 * kRingBlock + k] = uint2{__float_as_uint(r.ps), r.codes};
+                ringSS[2 * kRingBlock + k] = r.ss;
+            }
+        }
+    };
+    auto ringGet = [&](int e) {
+        const uint2 pc = ringPC[e];
+        return Rec{__uint_as_float(pc.x), pc.y, ringSS[e]};
+    };
+    // the strip edge values of step tt (strip kernels outside dump mode), alike
+    auto gringPut = [&](int base, const Bnd &g) {
+        if (k < kRingBlock) {
+            ringGV[base + k] = g.v;
+            ringGY[base + k] = g.y;
+            ringGE[base + k] = g.e;
+            if (base == 0 && k < 2) {
+                ringGV[2 * kRingBlock + k] = g.v;
+                ringGY[2 * kRingBlock + k] = g.y;
+                ringGE[2 * kRingBlock + k] = g.e;
+            }
+        }
+    };
+    auto gringGet = [&](int e) { return Bnd{ringGV[e], ringGY[e], ringGE[e]}; };
     __shared__ int sItem;
     for (;;) {
         int u;
@@ -774,27 +810,6 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 return recs[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
             };
             const bool injB = (k == G - 1);
-            // The last lane's record of step tt sits in ring entry tt mod
-            // 2 kRingBlock (entries 2 kRingBlock and 2 kRingBlock + 1 repeat
-            // entries 0 and 1). Lane k < kRingBlock of the group writes entry k
-            // of a block; every lane reads the entry of its group. A block is
-            // written at the start of the iteration before it is first read and
-            // was loaded a block earlier, so the loop's memory waits no longer
-            // cover the W1 stores of the steps just before them.
-            auto ringPut = [&](int base, const Rec &r) {
-                if (k < kRingBlock) {
-                    ringPC[base + k] = uint2{__float_as_uint(r.ps), r.codes};
-                    ringSS[base + k] = r.ss;
-                    if (base == 0 && k < 2) {
-                        ringPC[2 * kRingBlock + k] = uint2{__float_as_uint(r.ps), r.codes};
-                        ringSS[2 * kRingBlock + k] = r.ss;
-                    }
-                }
-            };
-            auto ringGet = [&](int e) {
-                const uint2 pc = ringPC[e];
-                return Rec{__uint_as_float(pc.x), pc.y, ringSS[e]};
-            };
             ringPut(0, brecAt(k & (kRingBlock - 1), G - 1));
             Rec pend = brecAt(kRingBlock + (k & (kRingBlock - 1)), G - 1);  // the next block
             Rec rnext = ringGet(1);  // the last lane's record of the next step
@@ -814,19 +829,6 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             };
             // Outside dump mode the group's edge value of step tt comes through
             // the ghost ring, entry tt mod 2 kRingBlock, like the records.
-            auto gringPut = [&](int base, const Bnd &g) {
-                if (k < kRingBlock) {
-                    ringGV[base + k] = g.v;
-                    ringGY[base + k] = g.y;
-                    ringGE[base + k] = g.e;
-                    if (base == 0 && k < 2) {
-                        ringGV[2 * kRingBlock + k] = g.v;
-                        ringGY[2 * kRingBlock + k] = g.y;
-                        ringGE[2 * kRingBlock + k] = g.e;
-                    }
-                }
-            };
-            auto gringGet = [&](int e) { return Bnd{ringGV[e], ringGY[e], ringGE[e]}; };
             Bnd gpend{0.f, 0.f, 0}, gnext{0.f, 0.f, 0};
             if constexpr (kGhostRing) {
                 gringPut(0, ghostAt(k & (kRingBlock - 1)));
@@ -1162,7 +1164,6 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 return recs[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
             };
             const bool injF = (k == 0);
-            Rec fq[kPF];  // the first lane's records of the next steps (slot = step mod kPF)
             float Wq[kPF][R];
             int Eq[kPF];
             // Loads are unconditional (from a clamped, always-valid step) so that
@@ -1186,8 +1187,9 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             Int4u e4cur = loadE4(0), e4nxt = loadE4(4);
 #pragma unroll
             for (int f = 0; f < kPF; f++) loadW1(f, Wq[f], Eq[f]);
-#pragma unroll
-            for (int f = 1; f <= kPF; f++) fq[f % kPF] = frecAt(f, 0);
+            ringPut(0, frecAt(k & (kRingBlock - 1), 0));
+            Rec fpend = frecAt(kRingBlock + (k & (kRingBlock - 1)), 0);  // the next block
+            Rec fnext = ringGet(1);  // the first lane's record of the next step
             Rec fcur = frecAt(0, k);
 
             // strip kernels: the strip above's last row (carried sum, Y, exponent), also ahead
@@ -1198,8 +1200,15 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 if (STR) return bF[bidx(tt - (MODE == kDump ? k : 0), n)];
                 return Bnd{0.f, 0.f, 0};
             };
+            Bnd fgpend{0.f, 0.f, 0}, fgnext{0.f, 0.f, 0};
+            if constexpr (kGhostRing) {  // outside dump mode through the ghost ring
+                gringPut(0, fghostAt(k & (kRingBlock - 1)));
+                fgpend = fghostAt(kRingBlock + (k & (kRingBlock - 1)));
+                fgnext = gringGet(0);  // step 0
+            } else {
 #pragma unroll
-            for (int f = 0; f < kPF; f++) fq4[f] = fghostAt(f);
+                for (int f = 0; f < kPF; f++) fq4[f] = fghostAt(f);
+            }
             float eqf[R];  // emission values of the next step, one step ahead
 #pragma unroll
             for (int q = 0; q < R; q++) eqf[q] = etk[((fcur.codes >> 8) & 255) * ROWS + q];
@@ -1215,10 +1224,9 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 const float rcg = gqf;
                 // the next step's record (the lane above's, or a new one at the first
                 // lane) and its emission values, read now so that they arrive in time
-                constexpr int SN = (PH + 1) % kPF;
                 Rec nx = rec_from_above<G>(fcur);
-                if (injF) nx = fq[SN];
-                fq[SN] = frecAt(t + 1 + kPF, 0);  // same address in every lane of the group
+                if (injF) nx = fnext;
+                fnext = ringGet(((t - PH) & (2 * kRingBlock - 1)) + PH + 2);  // step t + 2
                 float eqN[R];
 #pragma unroll
                 for (int q = 0; q < R; q++) eqN[q] = etk[((nx.codes >> 8) & 255) * ROWS + q];
@@ -1245,7 +1253,10 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 loadW1(t + kPF, Wq[SL], Eq[SL]);
                 float gn = 0.f, gy = 0.f;
                 Bnd fcurB{0.f, 0.f, 0};
-                if (STR) {
+                if constexpr (kGhostRing) {
+                    fcurB = fgnext;
+                    fgnext = gringGet(((t - PH) & (2 * kRingBlock - 1)) + PH + 1);  // step t + 1
+                } else if (STR) {
                     fcurB = fq4[SL];
                     fq4[SL] = fghostAt(t + kPF);
                 }
@@ -1339,6 +1350,15 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             };
             int t = 0;
             for (; t + 4 <= T; t += 4) {
+                // the ring block that starts at step t + 4, and the load of the one after it
+                if (((t + 4) & (kRingBlock - 1)) == 0) {
+                    ringPut((t + 4) & (2 * kRingBlock - 1), fpend);
+                    fpend = frecAt(t + 4 + kRingBlock + (k & (kRingBlock - 1)), 0);
+                    if constexpr (kGhostRing) {
+                        gringPut((t + 4) & (2 * kRingBlock - 1), fgpend);
+                        fgpend = fghostAt(t + 4 + kRingBlock + (k & (kRingBlock - 1)));
+                    }
+                }
                 fstep(std::integral_constant<int, 0>{}, t);
                 fstep(std::integral_constant<int, 1>{}, t + 1);
                 fstep(std::integral_constant<int, 2>{}, t + 2);
