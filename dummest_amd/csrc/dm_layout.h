@@ -51,7 +51,7 @@ struct PairInfo {
     int64_t recOff;  // index of column 0 in the record arrays
     int64_t bufOff;  // float offset of this pair's W1 / w0 buffers (full and dump modes)
     float s1, s2, sC;  // per-pair background scales, packed floats (dummest-dp-backward.hh:244-246)
-    float pad2;
+    int outIdx;        // the pair's result slot (its index in the caller's pair list)
     int64_t nullOff;   // null-kernel scratch: entry j of this pair at nullOff + 64*j
 };
 
@@ -117,8 +117,8 @@ struct DPArgs {
     int64_t escGroup;     // score mode: ints of ebuf per group (strips of stripSteps + 8, see kernels.hip)
     struct Bnd *bnd;      // strip kernels: per-group boundary records (3 x bndCols)
     int64_t bndCols;      // >= maxN + 2
-    float *filterOut;     // filter mode: best W1*prefixSeed per pair
-    PairScore *scoreOut;  // score mode
+    float *filterOut;     // filter mode: best W1*prefixSeed per pair, at the pair's outIdx
+    PairScore *scoreOut;  // score mode, at the pair's outIdx
     int wantBwd;
 };
 

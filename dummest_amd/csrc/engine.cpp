@@ -429,6 +429,7 @@ static void buildChunk(const dm_engine *E, int mode, const std::vector<int64_t> 
                 pi.n = lens[pi.seq];
                 pi.seqOff = offs[pi.seq];
                 pi.recOff = C.records + kRPad;
+                pi.outIdx = (int)pidx;  // plans hold fewer than 2^31 pairs
                 C.records += recSpan(pi.n);
                 U.maxN = std::max(U.maxN, pi.n);
                 C.pairs.push_back(pi);
@@ -492,9 +493,7 @@ struct Launch {
 struct DevChunk {
     DevBuf pairs, units;
     int npairs = 0;
-    int64_t outOff = 0;  // first result slot of this chunk in the plan's host staging
     int64_t records = 0;
-    std::vector<int64_t> order;
     std::vector<Launch> launches;
 };
 
@@ -543,6 +542,7 @@ struct dm_plan {
 static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, const int32_t *seq, int wantBwd,
                      dm_plan &P) {
     if (npairs < 0 || (npairs > 0 && (!prof || !seq))) return fail(e, DM_EINVAL, "bad pair arrays");
+    if (npairs > INT32_MAX) return fail(e, DM_EINVAL, "a plan holds at most 2^31 - 1 pairs");
     int rc = syncProfiles(e);
     if (rc) return rc;
     rc = syncSequences(e);
@@ -606,14 +606,12 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         DevChunk D;
         D.npairs = (int)C.pairs.size();
         D.records = C.records;
-        D.outOff = (int64_t)b;
         std::vector<int64_t> cfgCells(ncfg, 0);
         for (size_t u = 0; u < C.units.size(); u++)
             for (int k = 0; k < C.units[u].count; k++) {
                 const PairInfo &pi = C.pairs[C.units[u].first + k];
                 cfgCells[C.unitCfg[u]] += (int64_t)(e->profiles[pi.prof].M + 1) * pi.n;
             }
-        D.order = std::move(C.order);
         HIPCHK(D.pairs.ensure(sizeof(PairInfo) * C.pairs.size()));
         HIPCHK(D.units.ensure(sizeof(Unit) * std::max<size_t>(1, C.units.size())));
         HIPCHK(hipMemcpy(D.pairs.p, C.pairs.data(), sizeof(PairInfo) * C.pairs.size(), hipMemcpyHostToDevice));
@@ -718,18 +716,21 @@ static int enqueueChunk(dm_engine *e, dm_plan &P, DevChunk &D, std::vector<std::
             evs->push_back({L.cfg, {b0, b1}});
         }
     }
-    const size_t bytes = (size_t)D.npairs * (P.mode == kFilter ? sizeof(float) : sizeof(PairScore));
-    char *stage = (char *)P.hostOut + D.outOff * (P.mode == kFilter ? sizeof(float) : sizeof(PairScore));
-    HIPCHK(hipMemcpyAsync(stage, P.out.p, bytes, hipMemcpyDeviceToHost, e->stream));
     return DM_OK;
 }
 
-static void scatterChunk(const dm_plan &P, const DevChunk &D, float *filterOut, PairScore *scoreOut) {
-    for (int k = 0; k < D.npairs; k++) {
-        const int64_t dst = D.order[k];
-        if (P.mode == kFilter) filterOut[dst] = ((const float *)P.hostOut)[D.outOff + k];
-        else scoreOut[dst] = ((const PairScore *)P.hostOut)[D.outOff + k];
-    }
+// After the last chunk: the results, which the kernels wrote at each pair's
+// index in the caller's list, into the pinned staging in one copy.
+static int enqueueResults(dm_engine *e, dm_plan &P) {
+    const size_t bytes = (size_t)P.npairs * (P.mode == kFilter ? sizeof(float) : sizeof(PairScore));
+    if (bytes) HIPCHK(hipMemcpyAsync(P.hostOut, P.out.p, bytes, hipMemcpyDeviceToHost, e->stream));
+    return DM_OK;
+}
+
+// The staged results of an execution, in the caller's pair order.
+static void copyResults(const dm_plan &P, float *filterOut, PairScore *scoreOut) {
+    if (P.mode == kFilter && filterOut) std::memcpy(filterOut, P.hostOut, sizeof(float) * (size_t)P.npairs);
+    if (P.mode != kFilter && scoreOut) std::memcpy(scoreOut, P.hostOut, sizeof(PairScore) * (size_t)P.npairs);
 }
 
 static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *scoreOut) {
@@ -746,7 +747,7 @@ static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *sc
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
         totalMs = ms;
-        for (const DevChunk &D : P.chunks) scatterChunk(P, D, filterOut, scoreOut);
+        copyResults(P, filterOut, scoreOut);
         e->lastMs = totalMs;
         e->lastCells = P.cells;
         return DM_OK;
@@ -757,6 +758,10 @@ static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *sc
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
     for (DevChunk &D : P.chunks) {
         int rc = enqueueChunk(e, P, D, &evs);
+        if (rc) return rc;
+    }
+    {
+        int rc = enqueueResults(e, P);
         if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -773,8 +778,8 @@ static int executePlan(dm_engine *e, dm_plan &P, float *filterOut, PairScore *sc
             e->cfgCells[L.cfg] += L.cells;
             e->cfgLaunches[L.cfg] += 1;
         }
-        scatterChunk(P, D, filterOut, scoreOut);
     }
+    copyResults(P, filterOut, scoreOut);
     e->lastMs = totalMs;
     e->lastCells = P.cells;
     return DM_OK;
@@ -837,6 +842,7 @@ static int capturePlan(dm_engine *e, dm_plan &P) {
         rc = enqueueChunk(e, P, D, nullptr);
         if (rc) break;
     }
+    if (rc == DM_OK) rc = enqueueResults(e, P);
     hipGraph_t g = nullptr;
     hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (rc) {
@@ -1350,6 +1356,7 @@ int dm_filter_scores_static(dm_engine *e, int32_t prof, int32_t nseq, const uint
     Chunk C;
     buildChunk(e, kFilter, sorted, cfgOf, pp.data(), ss.data(), 0, (size_t)nseq, C, lengths, offsets);
     const int np = (int)C.pairs.size();
+    for (int k = 0; k < np; k++) C.pairs[k].outIdx = k;  // results staged by queue position, scattered below
     // scratch layout, every part on a 256-byte boundary
     size_t at = 0;
     auto take = [&](size_t bytes) {
@@ -1976,9 +1983,9 @@ int dm_plan_execute(dm_engine *e, dm_plan *P, float *best, float *mid_m, int32_t
         return executePlan(e, *P, best, nullptr);
     }
     if (P->npairs > 0 && (!mid_m || !mid_e)) return DM_EINVAL;
-    std::vector<PairScore> out(P->npairs);
-    rc = executePlan(e, *P, nullptr, out.data());
+    rc = executePlan(e, *P, nullptr, nullptr);
     if (rc) return rc;
+    const PairScore *out = (const PairScore *)P->hostOut;
     for (int64_t i = 0; i < P->npairs; i++) {
         mid_m[i] = out[i].midM;
         mid_e[i] = out[i].midE;

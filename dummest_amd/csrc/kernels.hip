@@ -1102,7 +1102,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             float v = fbest;
 #pragma unroll
             for (int s = 1; s < G; s <<= 1) v = fmaxf(v, __shfl_xor(v, s, 64));
-            if (active && k == 0) a.filterOut[pi] = v;
+            if (active && k == 0) a.filterOut[PR.outIdx] = v;
             continue;
         }
         __syncthreads();  // W1 steps and exponents stored above are read below
@@ -1372,7 +1372,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
         if (MODE == kScore) {
             midMax.retire();
             const ES mid = group_max<G>(ES{midMax.sm, midMax.se});
-            if (active && k == 0) a.scoreOut[pi] = PairScore{mid.m, mid.e, bwdBest.m, bwdBest.e};
+            if (active && k == 0) a.scoreOut[PR.outIdx] = PairScore{mid.m, mid.e, bwdBest.m, bwdBest.e};
         }
     }
 }
