@@ -576,6 +576,8 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
     const int wslot = (int)blockIdx.x * WG + wv;             // this wave's scratch slot
     const int grp = lane / G;
     const int k = lane - grp * G;
+    // the backward start score is wanted (a scalar, so that its tests stay scalar)
+    const int wantB = __builtin_amdgcn_readfirstlane(a.wantBwd);
     // this group's backward rings: kRing (prefix seed, codes) pairs, then
     // (after every group's pairs) kRing suffix seeds; the strip kernels' edge
     // values, Y values and exponents follow in the same way
@@ -799,7 +801,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 for (int q = 0; q < R; q++) Yc[q] *= f;
                 Eb -= s;
                 rs = ldexpf(1.f, -Eb);
-                if (a.wantBwd) bwdMax.rebase(Eb);
+                if (wantB) bwdMax.rebase(Eb);
             };
 
             // Column records: lane k uses at step t the record lane k+1 used at
@@ -878,17 +880,21 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // are the pairs' own). Score mode writes the exponents of a four-step
             // iteration with one store (storeE4) when its W1 blocks are stored
             // late, else one per step.
+            // score mode: this lane's part of chunk j of anti-diagonal 0's block
+            float *chunk0[(R + 3) / 4];
+#pragma unroll
+            for (int j = 0; j * 4 < R; j++) chunk0[j] = wsc + w1Off(k, 4 * j);
             auto storeW1 = [&](int tt, int sl, bool withExp) {
                 const int D = Tb - 1 - tt;
                 if (MODE == kScore) {
                     if (D >= 0) {
-                        float *dst = wsc + (int64_t)D * ROWS;
 #pragma unroll
                         for (int j = 0; j * 4 < R; j++) {
                             // some column of the run lies in [0, n) (rows checked in runSpan)
                             if ((unsigned)(D - runLo[j]) < (unsigned)runSpan[j]) {
+                                float *dst = chunk0[j] + (int64_t)D * ROWS;
 #pragma unroll
-                                for (int q = 4 * j; q < 4 * j + 4 && q < R; q++) dst[w1Off(k, q)] = H[sl][q];
+                                for (int q = 4 * j; q < 4 * j + 4 && q < R; q++) dst[q - 4 * j] = H[sl][q];
                             }
                         }
                         if (withExp) esc[D] = Eb;  // every lane of the group writes the same value
@@ -1026,7 +1032,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
 #pragma unroll
                 for (int q = 0; q < R; q++) eq[q] = eqN[q];
                 gq = gqN;
-                if (MODE != kFilter && a.wantBwd) {
+                if (MODE != kFilter && wantB) {
                     // backward start score: max W1 * prefixSeed. The seed is
                     // shared by the lane's rows and non-negative, and a product
                     // of two floats is exact in double, so the max of the
@@ -1095,7 +1101,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             if (t < T) bstep(std::integral_constant<int, 1>{}, std::false_type{}, t++);
             if (t < T) bstep(std::integral_constant<int, 2>{}, std::false_type{}, t++);
             if (STR) __syncthreads();  // boundary written above is read by the next strip
-            if (a.wantBwd && MODE != kFilter) bwdMax.rebase(0);  // next strip starts at exponent 0
+            if (wantB && MODE != kFilter) bwdMax.rebase(0);  // next strip starts at exponent 0
         }
         if (MODE == kFilter) {
             // group max of the plain float best (no rescale in the filter)
@@ -1107,7 +1113,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
         }
         __syncthreads();  // W1 steps and exponents stored above are read below
         ES bwdBest = es_zero();
-        if (a.wantBwd) {
+        if (wantB) {
             bwdMax.retire();
             bwdBest = group_max<G>(ES{bwdMax.sm, bwdMax.se});
         }
@@ -1170,11 +1176,18 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // no select waits for them; which of them count is decided at use.
             const int Tw = Tmine;
             const int nW1 = (active && strip * ROWS + k * R <= M) ? n : 0;  // columns of this lane with W1
+            const float *fchunk0[(R + 3) / 4];  // this lane's part of chunk j of anti-diagonal 0
+#pragma unroll
+            for (int j = 0; j * 4 < R; j++) fchunk0[j] = wsc + w1Off(k, 4 * j);
             auto loadW1 = [&](int tt, float *dst, int &e) {
-                const int ts = tt < Tw ? tt : (Tw > 0 ? Tw - 1 : 0);
+                const int ts = min(tt, Tw - 1);  // Tw = n + G - 1 >= 1
                 if (MODE == kScore) {
 #pragma unroll
-                    for (int q = 0; q < R; q++) dst[q] = wsc[(int64_t)ts * ROWS + w1Off(k, q)];
+                    for (int j = 0; j * 4 < R; j++) {
+                        const float *src = fchunk0[j] + (int64_t)ts * ROWS;
+#pragma unroll
+                        for (int q = 4 * j; q < 4 * j + 4 && q < R; q++) dst[q] = src[q - 4 * j];
+                    }
                     e = 0;
                 } else {
                     e = esc[ts];
