@@ -89,17 +89,24 @@ def cpu_baseline(w, order, gpu_mid, budget_s=15.0, threads=None):
 def traffic_of(kernel, cells_per_launch, build):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (profiles/dp_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch,
-    scaled by the plain-copy reference), or None when that summary was taken
-    on another build, kernel or launch size."""
+    scaled by the plain-copy reference), with the source of the figure; None
+    when that summary was taken on another build or kernel. The autotuner's
+    choice of configuration per pair varies a little between runs, so a launch
+    with another number of cells gets the measured bytes per cell times its
+    own cells."""
     path = os.path.join(ROOT, "profiles", "dp_traffic.json")
     try:
         t = json.load(open(path))
     except (OSError, ValueError):
-        return None
-    if t.get("build_id") != build or t.get("kernel") != kernel or \
-            abs(t.get("cells_per_launch", 0) - cells_per_launch) > 0.01 * cells_per_launch:
-        return None
-    return t.get("hbm_bytes_per_launch")
+        return None, None
+    if t.get("build_id") != build or t.get("kernel") != kernel:
+        return None, None
+    measured = t.get("cells_per_launch", 0)
+    if abs(measured - cells_per_launch) <= 0.001 * cells_per_launch:
+        return t.get("hbm_bytes_per_launch"), "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch size"
+    bpc = t["hbm_bytes_per_launch"] / measured
+    return bpc * cells_per_launch, (f"rocprofv3 FETCH_SIZE + WRITE_SIZE: {bpc:.2f} B per cell measured on a "
+                                    f"launch of {measured:.4g} cells, times this launch's cells")
 
 
 def main():
@@ -205,9 +212,9 @@ def main():
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     dp_ms_step = sum(v[0] for v in agg.values()) / args.steps
     kname = f"void dm::dp_kernel<{G}, {R}, 1, {'true' if ST else 'false'}>(dm::DPArgs)"
-    traffic = traffic_of(kname, c / launches, D.build_id())
+    traffic, traffic_src = traffic_of(kname, c / launches, D.build_id())
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": kname, "avg_launch_ms": round(avg_ms, 3),
                 "cells_per_launch": int(c / launches), "flops_per_cell": FLOPS_PER_CELL}
     if traffic:
