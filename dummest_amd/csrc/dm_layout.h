@@ -143,16 +143,19 @@ inline constexpr int dp_wg(int G, int R, bool strips, int mode) {
     return (mode == kDump || G * R <= 192 || G * R > 512) ? 1 : (G * R > 384 ? 4 : 2);
 }
 
-// The backward's column records for the last lane of each group pass through
-// an LDS ring, and so do, in the strip kernels outside dump mode, the strip
-// below's edge values that the last lane reads: blocks of kRingBlock steps,
+// The column records for the injection lane of each group (score and dump
+// modes; the filter, which stores no W1, loads them per step) pass through an
+// LDS ring, and so do, in the score-mode strip kernels, the neighbouring
+// strip's edge values that lane reads: blocks of kRingBlock steps,
 // each loaded with one entry per lane a block ahead, two blocks plus two
 // entries that repeat the first ones (so that a four-step iteration reads
 // without wrapping). Per entry: the (prefix seed, codes) pair and the suffix
 // seed, 12 bytes, and for the edge (value, Y, exponent) another 12.
 constexpr int kRingBlock = 8;
 constexpr int kRing = 2 * kRingBlock + 2;
-inline constexpr int dp_ring_bytes(bool strips, int mode) { return kRing * (12 + ((strips && mode != kDump) ? 12 : 0)); }
+inline constexpr int dp_ring_bytes(bool strips, int mode) {
+    return mode == kFilter ? 0 : kRing * (12 + ((strips && mode == kScore) ? 12 : 0));
+}
 
 // Dynamic LDS of a DP launch: the emission table (kNCode x G R floats), the
 // background table (32 floats) and every group's rings.

@@ -582,7 +582,11 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
     // (after every group's pairs) kRing suffix seeds; the strip kernels' edge
     // values, Y values and exponents follow in the same way
     constexpr int kRings = WG * (64 / G);
-    constexpr bool kGhostRing = STR && MODE != kDump;
+    // The score-only filter stores no W1: its record loads never wait behind
+    // stores, and it keeps the per-step loads (the ring's LDS traffic costs it
+    // more than it saves).
+    constexpr bool kRecRing = MODE != kFilter;
+    constexpr bool kGhostRing = STR && MODE == kScore;
     const int ringId = wv * (64 / G) + grp;
     uint2 *ringPC = reinterpret_cast<uint2 *>(bgl + 32) + ringId * kRing;
     float *ringSS = reinterpret_cast<float *>(reinterpret_cast<uint2 *>(bgl + 32) + kRings * kRing) + ringId * kRing;
@@ -812,9 +816,16 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 return recs[c < -kRPad ? -kRPad : (c > n + kRPad - 1 ? n + kRPad - 1 : c)];
             };
             const bool injB = (k == G - 1);
-            ringPut(0, brecAt(k & (kRingBlock - 1), G - 1));
-            Rec pend = brecAt(kRingBlock + (k & (kRingBlock - 1)), G - 1);  // the next block
-            Rec rnext = ringGet(1);  // the last lane's record of the next step
+            Rec pend{}, rnext{};
+            Rec rq[kPF];  // filter mode: the last lane's records of the next steps (slot = step mod kPF)
+            if constexpr (kRecRing) {
+                ringPut(0, brecAt(k & (kRingBlock - 1), G - 1));
+                pend = brecAt(kRingBlock + (k & (kRingBlock - 1)), G - 1);  // the next block
+                rnext = ringGet(1);  // the last lane's record of the next step
+            } else {
+#pragma unroll
+                for (int f = 1; f <= kPF; f++) rq[f % kPF] = brecAt(f, G - 1);
+            }
             Rec rcur = brecAt(0, k);
             // strip kernels: the strip below's top row (W1, Y, exponent), also ahead
             Bnd gq4[kPF];
@@ -829,7 +840,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 if (STR) return bIn[bidx(Tb - 1 - tt - (MODE == kDump ? k : G - 1), n)];
                 return Bnd{0.f, 0.f, 0};
             };
-            // Outside dump mode the group's edge value of step tt comes through
+            // In score mode the group's edge value of step tt comes through
             // the ghost ring, entry tt mod 2 kRingBlock, like the records.
             Bnd gpend{0.f, 0.f, 0}, gnext{0.f, 0.f, 0};
             if constexpr (kGhostRing) {
@@ -933,8 +944,14 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 // the next step's record (the lane below's, or a new one at the last
                 // lane) and its emission values, read now so that they arrive in time
                 Rec nx = rec_from_below<G>(rcur);
-                if (injB) nx = rnext;
-                rnext = ringGet(((t - PH) & (2 * kRingBlock - 1)) + PH + 2);  // step t + 2
+                if constexpr (kRecRing) {
+                    if (injB) nx = rnext;
+                    rnext = ringGet(((t - PH) & (2 * kRingBlock - 1)) + PH + 2);  // step t + 2
+                } else {
+                    constexpr int SN = (PH + 1) % kPF;
+                    if (injB) nx = rq[SN];
+                    rq[SN] = brecAt(t + 1 + kPF, G - 1);  // same address in every lane of the group
+                }
                 float eqN[R];
 #pragma unroll
                 for (int q = 0; q < R; q++) eqN[q] = etk[(nx.codes & 255) * ROWS + q];
@@ -1069,7 +1086,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             int t = 0;
             for (; t + 4 <= T; t += 4) {
                 // the ring block that starts at step t + 4, and the load of the one after it
-                if (((t + 4) & (kRingBlock - 1)) == 0) {
+                if (kRecRing && ((t + 4) & (kRingBlock - 1)) == 0) {
                     ringPut((t + 4) & (2 * kRingBlock - 1), pend);
                     pend = brecAt(t + 4 + kRingBlock + (k & (kRingBlock - 1)), G - 1);
                     if constexpr (kGhostRing) {
@@ -1214,7 +1231,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 return Bnd{0.f, 0.f, 0};
             };
             Bnd fgpend{0.f, 0.f, 0}, fgnext{0.f, 0.f, 0};
-            if constexpr (kGhostRing) {  // outside dump mode through the ghost ring
+            if constexpr (kGhostRing) {  // score mode: through the ghost ring
                 gringPut(0, fghostAt(k & (kRingBlock - 1)));
                 fgpend = fghostAt(kRingBlock + (k & (kRingBlock - 1)));
                 fgnext = gringGet(0);  // step 0
