@@ -135,6 +135,7 @@ struct dm_engine {
     DevBuf dBnd;
     bool plain = false;
     int gridOpt = 0;
+    int64_t chunkPairs = kChunkPairs;  // pairs per plan chunk (option "chunk_pairs")
     int numCUs = 256;
     void *scratchDev = nullptr;    // this device's address of the static scratch array
     std::string scratchErr;        // why it could not be looked up
@@ -591,11 +592,11 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         P.scratchBudget = budget;
     }
     for (size_t b = 0, en = 0; b < (size_t)npairs; b = en) {
-        // chunk: up to kChunkPairs pairs whose records and null scratch fit kChunkBytes
+        // chunk: up to chunkPairs pairs whose records and null scratch fit kChunkBytes
         en = b + 1;
         if (!e->plain) {
             int64_t bytes = 0;
-            for (en = b; en < (size_t)npairs && en - b < (size_t)kChunkPairs; en++) {
+            for (en = b; en < (size_t)npairs && en - b < (size_t)e->chunkPairs; en++) {
                 const int64_t nn = e->seqLen[seq[sorted[en]]];
                 bytes += recSpan((int)nn) * (int64_t)sizeof(Rec) + (nn + 4) * (int64_t)sizeof(ES);
                 if (bytes > kChunkBytes && en > b) break;
@@ -2039,6 +2040,10 @@ int dm_set_option(dm_engine *e, const char *name, int64_t value) {
     if (!e || !name) return DM_EINVAL;
     if (!std::strcmp(name, "plain")) e->plain = value != 0;
     else if (!std::strcmp(name, "grid")) e->gridOpt = (int)value;
+    else if (!std::strcmp(name, "chunk_pairs")) {
+        if (value < 1 || value > kChunkPairs) return fail(e, DM_EINVAL, "chunk_pairs must lie in [1, 2^22]");
+        e->chunkPairs = value;
+    }
     else if (!std::strcmp(name, "grad_budget")) {
         if (value < 1) return fail(e, DM_EINVAL, "grad_budget must be positive");
         e->gradBudget = value;

@@ -372,7 +372,9 @@ int dm_config_table(const dm_engine *e, int32_t cap, int32_t *G, int32_t *R, int
  * "grid" = persistent grid size (0 = automatic); "config" = index into the
  * configuration table (dm_config_table) that every pair whose profile fits it
  * runs on (-1 = automatic choice); "bw_budget" = bytes of Baum-Welch matrices
- * per launch batch (default 8 GiB; dm_bw_prepare splits the sequences by it). */
+ * per launch batch (default 8 GiB; dm_bw_prepare splits the sequences by it);
+ * "chunk_pairs" = most pairs in one work-queue chunk of a plan (default 2^22; a
+ * plan's chunks run one after the other and share its device buffers). */
 int dm_set_option(dm_engine *e, const char *name, int64_t value);
 
 /* Device time of the DP kernels of the last scoring call (HIP events on the

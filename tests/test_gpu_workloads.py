@@ -22,7 +22,8 @@ against the oracle.
   descendant with 1-nt indels), through the engine executable with and
   without --max: the records against the oracle's.
 * the plain form (one launch per pair) and a captured plan: scores
-  bit-identical to the batched plan's.
+  bit-identical to the batched plan's; a plan cut into many chunks: results in
+  the caller's order, bit-identical to the single-chunk plan's.
 """
 import math
 import os
@@ -98,6 +99,7 @@ def test_uneven_reads_batched_plan(reads_case):
     assert len(stats) >= 4, "the mix should use several kernel configurations"
     assert any(s[2] for s in stats), "profiles past 512 rows should run on row-strip kernels"
     mid, bwd = oracle_scores(c)
+    c["oracle_scores"] = (mid, bwd)
     d_mid = np.array([abs(ln(mm[k], me[k]) - mid[k]) for k in range(npairs)])
     d_bwd = np.array([abs(ln(bm[k], be[k]) - bwd[k]) for k in range(npairs)])
     print(f"max |d mid| {d_mid.max():.2e}, max |d bwd| {d_bwd.max():.2e} nats", flush=True)
@@ -191,6 +193,44 @@ def test_plain_and_captured_plans(reads_case):
     for k in range(len(pick)):
         assert abs(ln(got[0][k], got[1][k]) - ln(ref[0][k], ref[1][k])) < TOL_NATS
         assert abs(ln(got[2][k], got[3][k]) - ln(ref[2][k], ref[3][k])) < TOL_NATS
+
+
+def test_chunked_plan_results_in_caller_order(reads_case):
+    """A plan cut into many work-queue chunks (chunk_pairs = 97): every chunk's
+    kernels write their pairs' results at the pairs' indices in the caller's
+    list, and one copy per execution returns them; the scores equal the
+    single-chunk plan's bit for bit (a pair's result does not depend on the
+    pairs it shares a unit with), over two executions of each plan (the
+    second reads W1 scratch the first left behind)."""
+    c = reads_case
+    e = c["e"]
+    rng = np.random.default_rng(11)
+    pick = rng.permutation(len(c["prof"]))[:1200]  # caller order unrelated to the queue order
+    prof, seq = c["prof"][pick], c["seq"][pick]
+    one = e.plan(prof, seq, "score", want_bwd=True)
+    ref = one.execute()
+    np.testing.assert_array_equal(ref[0], one.execute()[0])
+    one.close()
+    e.set_option("chunk_pairs", 97)
+    try:
+        many = e.plan(prof, seq, "score", want_bwd=True)
+        for _ in range(2):
+            got = many.execute()
+            for a, b in zip(got, ref):
+                np.testing.assert_array_equal(a, b)
+        many.close()
+        fplan = e.plan(prof, seq, "filter")
+        fgot = fplan.execute()
+        fplan.close()
+    finally:
+        e.set_option("chunk_pairs", 1 << 22)
+    fplan = e.plan(prof, seq, "filter")
+    fref = fplan.execute()
+    fplan.close()
+    np.testing.assert_array_equal(fgot, fref)
+    mid, _ = c.get("oracle_scores") or oracle_scores(c)
+    for j, k in enumerate(pick[:200]):
+        assert abs(ln(ref[0][j], ref[1][j]) - mid[k]) < TOL_NATS
 
 
 def test_windows_filter_plan():
