@@ -656,9 +656,16 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
         const int n = active ? PR.n : 0;
         const Rec *recs = a.rec + PR.recOff;
         const float s1 = PR.s1, s2 = PR.s2, sC = PR.sC;
-        const int T = U.maxN + G - 1;
-        const int Tmine = n + G - 1;
-        const int Tb = Tmine;  // anti-diagonal of the backward's first step
+        // Lanes of a strip whose rows all lie past row M never feed a live row
+        // (rows past M and row M itself have zero delete/enter coefficients), so
+        // score and filter modes start a strip's anti-diagonals at n + live - 1
+        // rather than n + G - 1, live = the lanes that hold a row <= M (dump mode
+        // keeps G: the host reads its matrices in that layout).
+        auto liveLanes = [&](int strip) {
+            if (MODE == kDump) return G;
+            const int l = (M + 1 - strip * ROWS + R - 1) / R;
+            return l < G ? l : G;
+        };
         const int64_t strideS = (int64_t)a.stripSteps * ROWS;  // floats per strip of W1 steps
         float *wsc0;
         int *esc0;
@@ -759,6 +766,9 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
 
         // ------------------------------------------------------------- backward
         for (int strip = S - 1; strip >= 0; strip--) {
+            const int T = U.maxN + liveLanes(strip) - 1;  // steps of this strip's pass
+            const int Tmine = n + liveLanes(strip) - 1;   // this pair's
+            const int Tb = Tmine;  // anti-diagonal of the backward's first step
             float a0sC[R], b0sC[R];
             loadStrip(strip, a0sC, b0sC, true);
             // rows of this lane that exist (r <= M): a prefix of its R rows; the
@@ -1139,6 +1149,8 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
         ExpMax midMax;
         midMax.init(kPre);
         for (int strip = 0; strip < S; strip++) {
+            const int T = U.maxN + liveLanes(strip) - 1;  // steps of this strip's pass
+            const int Tmine = n + liveLanes(strip) - 1;   // this pair's
             float a0[R], b0[R];
             loadStrip(strip, a0, b0, strip > 0);
             const float *wsc = wsc0 + (int64_t)strip * strideS;
