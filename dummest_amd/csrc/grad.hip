@@ -70,7 +70,7 @@ __device__ __forceinline__ RowParams row_params(const GradArgs &a, int row) {
 }
 
 constexpr int kExpLds = 256;      // 64-column blocks whose exponents the count pass stages in LDS
-constexpr int kRing = 256;        // columns of seeds and codes a pass keeps in LDS (the lanes of a step span 64)
+constexpr int kColRing = 256;        // columns of seeds and codes a pass keeps in LDS (the lanes of a step span 64)
 constexpr int kNoExp = -1000000;  // binary exponent given to zero
 constexpr int kRaise = 64;        // a block whose largest value passes 2^kRaise gets a higher exponent
 constexpr int kLower = 256;       // one whose values all stay below 2^-kLower gets a lower one (not below 0)
@@ -105,10 +105,10 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
     __shared__ double sBg[32];
-    // prefixSeed and code(j-2) of column j at slot j mod kRing: the lanes of a
+    // prefixSeed and code(j-2) of column j at slot j mod kColRing: the lanes of a
     // step span 64 columns, and every 64 steps the next 64 are written
-    __shared__ float sCol[kRing];
-    __shared__ uint8_t sCode[kRing];
+    __shared__ float sCol[kColRing];
+    __shared__ uint8_t sCode[kColRing];
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
@@ -175,11 +175,11 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
             double oC = lastC, oY = lastY, sw = 0.0, sy = 0.0, sz = 0.0;
             if (rowOk && j >= 0 && j < n) {
                 const double W0in = lane == 0 ? eA : inC, Yup = lane == 0 ? eB : inY;
-                const int code = sCode[j & (kRing - 1)];
+                const int code = sCode[j & (kColRing - 1)];
                 const double e = Er[code], g = sBg[code];
                 const double X = enterC * e * w3v;
                 const double z0 = g * sC * (P.a0 * w3v + P.b0 * z3);
-                const double w = W0in + (z0 + a1s1 * w1v + a2s2 * w2v + ldexp((double)sCol[j & (kRing - 1)], -El));
+                const double w = W0in + (z0 + a1s1 * w1v + a2s2 * w2v + ldexp((double)sCol[j & (kColRing - 1)], -El));
                 const double y = P.d0 * w + P.eps * Yup;
                 oC = X + y + d1s2 * w2v + d2s1 * w1v;  // row i+1's inflow at column j
                 oY = y;
@@ -213,8 +213,8 @@ __global__ __launch_bounds__(64) void grad_forward_kernel(GradArgs a) {
         auto boundary = [&](int t) {
             {
                 const int jw = t + lane;  // block t/64's columns (past n: unused slots)
-                sCol[jw & (kRing - 1)] = cur.ps;
-                sCode[jw & (kRing - 1)] = (uint8_t)((cur.codes >> 8) & 255);  // code(j-2)
+                sCol[jw & (kColRing - 1)] = cur.ps;
+                sCode[jw & (kColRing - 1)] = (uint8_t)((cur.codes >> 8) & 255);  // code(j-2)
                 cur = rec[min(t + 64 + lane, n - 1)];
             }
             if (t > 0) {
@@ -263,8 +263,8 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
 #pragma clang fp contract(off)
     __shared__ double sE[64 * kNCode];
     __shared__ double sBg[32];
-    __shared__ float sCol[kRing];  // suffixSeed and code(j+1) of column j at j mod kRing
-    __shared__ uint8_t sCode[kRing];
+    __shared__ float sCol[kColRing];  // suffixSeed and code(j+1) of column j at j mod kColRing
+    __shared__ uint8_t sCode[kColRing];
     const int p = a.pairBase + blockIdx.x, lane = threadIdx.x;
     const PairInfo PR = a.pairs[p];
     const int n = PR.n, M = a.prof->M, NB = (n + 63) >> 6;
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
             double oW = lastW, oY = lastY, oS = lastS, sw = 0.0, sy = 0.0, sz = 0.0;
             if (rowOk && j >= 0 && j < n) {
                 const double Wdn = lane == 0 ? eA : inW, Ynext = lane == 0 ? eB : inY, Sdn = lane == 0 ? eC : inS;
-                const int code = sCode[j & (kRing - 1)];
+                const int code = sCode[j & (kColRing - 1)];
                 const double e = Er[code], g = sBg[code];
                 double w = r3 * e * enterC + Ynext * P.d0;
                 w = w + r2 * d1s2;
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
                 w = w + zb3 * g * a0sC;
                 w = w + c2 * a2s2;
                 w = w + c1 * a1s1;
-                w = w + ldexp((double)sCol[j & (kRing - 1)], -El);
+                w = w + ldexp((double)sCol[j & (kColRing - 1)], -El);
                 const double y = __fma_rn(P.eps, Ynext, w);
                 const double zb = __fma_rn(b0sC, zb3 * g, w);
                 const double sum = Sdn + w;
@@ -390,8 +390,8 @@ __global__ __launch_bounds__(64) void grad_backward_kernel(GradArgs a) {
         auto boundary = [&](int t) {
             {
                 const int jw = n - 1 - t - lane;  // block t/64's columns, right to left (below 0: unused)
-                sCol[jw & (kRing - 1)] = cur.ss;
-                sCode[jw & (kRing - 1)] = (uint8_t)(cur.codes & 255);  // code(j+1)
+                sCol[jw & (kColRing - 1)] = cur.ss;
+                sCode[jw & (kColRing - 1)] = (uint8_t)(cur.codes & 255);  // code(j+1)
                 cur = rec[max(n - 1 - t - 64 - lane, 0)];
             }
             if (t > 0) {
