@@ -19,8 +19,12 @@
 //     GPU computes at once). Power-of-two scaling is exact in FP32 apart from
 //     flushing values more than ~2^226 below the step maximum.
 //   * The codon emission table of the unit's profile sits in LDS as
-//     [code][row]; per-column inputs (background codon probability, seeds,
-//     codon code) are 16-byte records read once per step per lane.
+//     [code][row]; per-column inputs (seeds and the codon codes) are 12-byte
+//     records that enter a group at one lane and move one lane per step; in
+//     score and dump mode the entering records come through a small LDS ring
+//     per group, refilled a block of steps ahead, so that the loops issue no
+//     per-step memory loads besides the stored W1 (the background codon
+//     probability comes from a 25-entry LDS table).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -578,8 +582,8 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
     const int k = lane - grp * G;
     // the backward start score is wanted (a scalar, so that its tests stay scalar)
     const int wantB = __builtin_amdgcn_readfirstlane(a.wantBwd);
-    // this group's backward rings: kRing (prefix seed, codes) pairs, then
-    // (after every group's pairs) kRing suffix seeds; the strip kernels' edge
+    // this group's rings: kRing (prefix seed, codes) pairs, then (after every
+    // group's pairs) kRing suffix seeds; the score-mode strip kernels' edge
     // values, Y values and exponents follow in the same way
     constexpr int kRings = WG * (64 / G);
     // The score-only filter stores no W1: its record loads never wait behind
@@ -615,7 +619,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
         const uint2 pc = ringPC[e];
         return Rec{__uint_as_float(pc.x), pc.y, ringSS[e]};
     };
-    // the strip edge values of step tt (strip kernels outside dump mode), alike
+    // the strip edge values of step tt (score-mode strip kernels), alike
     auto gringPut = [&](int base, const Bnd &g) {
         if (k < kRingBlock) {
             ringGV[base + k] = g.v;
@@ -1209,7 +1213,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
 #pragma unroll
             for (int j = 0; j * 4 < R; j++) fchunk0[j] = wsc + w1Off(k, 4 * j);
             auto loadW1 = [&](int tt, float *dst, int &e) {
-                const int ts = min(tt, Tw - 1);  // Tw = n + G - 1 >= 1
+                const int ts = min(tt, Tw - 1);  // Tw = n + live - 1 >= 1
                 if (MODE == kScore) {
 #pragma unroll
                     for (int j = 0; j * 4 < R; j++) {
