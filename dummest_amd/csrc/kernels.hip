@@ -1226,15 +1226,17 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                     e = esc[ts];
                 }
             };
-            // score mode: the exponents of the four-step block starting at tb,
-            // loaded a block ahead (entries past the pair's last step are unused;
-            // the guard ints after the strip keep the load in bounds)
+            // score mode: the exponents of the four-step block starting at tb
+            // (entries past the pair's last step are unused; the guard ints after
+            // the strip keep the load in bounds). An eight-step iteration keeps
+            // steps 0..3 in e4A and 4..7 in e4B, each loaded four steps before its
+            // first use and after its previous block's last, so that no register
+            // copy at the loop's end has to wait for a load.
             auto loadE4 = [&](int tb) { return *(const Int4u *)(esc + (tb < Tw ? tb : 0)); };
-            Int4u e4cur = loadE4(0), e4nxt = loadE4(4);
+            Int4u e4A = loadE4(0), e4B{0, 0, 0, 0};
 #pragma unroll
             for (int f = 0; f < kPF; f++) loadW1(f, Wq[f], Eq[f]);
-            ringPut(0, frecAt(k & (kRingBlock - 1), 0));
-            Rec fpend = frecAt(kRingBlock + (k & (kRingBlock - 1)), 0);  // the next block
+            ringPut(0, frecAt(k & (kRingBlock - 1), 0));  // steps 0..7; the loop writes the later blocks
             Rec fnext = ringGet(1);  // the first lane's record of the next step
             Rec fcur = frecAt(0, k);
 
@@ -1246,10 +1248,9 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 if (STR) return bF[bidx(tt - (MODE == kDump ? k : 0), n)];
                 return Bnd{0.f, 0.f, 0};
             };
-            Bnd fgpend{0.f, 0.f, 0}, fgnext{0.f, 0.f, 0};
+            Bnd fgnext{0.f, 0.f, 0};
             if constexpr (kGhostRing) {  // score mode: through the ghost ring
                 gringPut(0, fghostAt(k & (kRingBlock - 1)));
-                fgpend = fghostAt(kRingBlock + (k & (kRingBlock - 1)));
                 fgnext = gringGet(0);  // step 0
             } else {
 #pragma unroll
@@ -1262,7 +1263,8 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
 
             auto fstep = [&](auto PHc, int t) {
                 constexpr int PH = decltype(PHc)::value;
-                constexpr int S1 = (PH + 3) & 3, S2 = (PH + 2) & 3, S3 = (PH + 1) & 3, S4 = PH;
+                constexpr int P4 = PH & 3;  // step within a four-step exponent block
+                constexpr int S1 = (P4 + 3) & 3, S2 = (P4 + 2) & 3, S3 = (P4 + 1) & 3, S4 = P4;
                 constexpr int SL = PH % kPF;
                 const int D = t;
                 const int c = D - k;
@@ -1284,17 +1286,17 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 const float w1f = (MODE == kScore && t - k < nW1) ? 7.888609052e-31f : 0.f;
 #pragma unroll
                 for (int q = 0; q < R; q++) W1v[q] = Wq[SL][q] * w1f;
-                const int EbNext = (t < Tw) ? ((MODE == kScore) ? e4cur.at<PH>() : Eq[SL]) : kNoStep;
-                if (MODE == kScore && PH == 0) {
+                const Int4u &e4 = (PH < 4) ? e4A : e4B;  // this step's block
+                const int EbNext = (t < Tw) ? ((MODE == kScore) ? e4.at<P4>() : Eq[SL]) : kNoStep;
+                if (MODE == kScore && P4 == 0) {
                     // one test per four-step block: does any exponent of the block
                     // (of this pair's steps) differ from the current one
-                    const bool d = ((t < Tw) & (e4cur.v0 != EbPrev)) | ((t + 1 < Tw) & (e4cur.v1 != EbPrev)) |
-                                   ((t + 2 < Tw) & (e4cur.v2 != EbPrev)) | ((t + 3 < Tw) & (e4cur.v3 != EbPrev));
+                    const bool d = ((t < Tw) & (e4.v0 != EbPrev)) | ((t + 1 < Tw) & (e4.v1 != EbPrev)) |
+                                   ((t + 2 < Tw) & (e4.v2 != EbPrev)) | ((t + 3 < Tw) & (e4.v3 != EbPrev));
                     eChg = __any(d);
-                }
-                if (MODE == kScore && PH == 3) {
-                    e4cur = e4nxt;
-                    e4nxt = loadE4(t + 5);
+                    // the other register: the block four steps on
+                    if (PH < 4) e4B = loadE4(t + 4);
+                    else e4A = loadE4(t + 4);
                 }
                 loadW1(t + kPF, Wq[SL], Eq[SL]);
                 float gn = 0.f, gy = 0.f;
@@ -1389,30 +1391,36 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                     for (int s = 1; s < G; s <<= 1) gbig |= __shfl_xor((int)gbig, s, 64) != 0;
                     if (gbig) scaleF(-64);
                 }
-                if (PH == 3) {  // never below exponent 0: the seeds enter at 2^-Ef <= 1
+                if (P4 == 3) {  // never below exponent 0: the seeds enter at 2^-Ef <= 1
                     const float live = nnmax(nnmax(LVf[0], LVf[1]), nnmax(LVf[2], LVf[3]));
                     if (__builtin_expect(group_small<G>(live, grp), 0) && Ef >= 64) scaleF(64);
                 }
             };
             int t = 0;
-            for (; t + 4 <= T; t += 4) {
-                // the ring block that starts at step t + 4, and the load of the one after it
-                if (((t + 4) & (kRingBlock - 1)) == 0) {
-                    ringPut((t + 4) & (2 * kRingBlock - 1), fpend);
-                    fpend = frecAt(t + 4 + kRingBlock + (k & (kRingBlock - 1)), 0);
-                    if constexpr (kGhostRing) {
-                        gringPut((t + 4) & (2 * kRingBlock - 1), fgpend);
-                        fgpend = fghostAt(t + 4 + kRingBlock + (k & (kRingBlock - 1)));
-                    }
-                }
+            for (; t + 8 <= T; t += 8) {
+                // the ring block that starts at step t + 8: loaded now, written
+                // after step t + 5 (step t + 6 reads its first entry)
+                const Rec fnew = frecAt(t + 8 + (k & (kRingBlock - 1)), 0);
+                Bnd fgnew{0.f, 0.f, 0};
+                if constexpr (kGhostRing) fgnew = fghostAt(t + 8 + (k & (kRingBlock - 1)));
                 fstep(std::integral_constant<int, 0>{}, t);
                 fstep(std::integral_constant<int, 1>{}, t + 1);
                 fstep(std::integral_constant<int, 2>{}, t + 2);
                 fstep(std::integral_constant<int, 3>{}, t + 3);
+                fstep(std::integral_constant<int, 4>{}, t + 4);
+                fstep(std::integral_constant<int, 5>{}, t + 5);
+                ringPut((t + 8) & (2 * kRingBlock - 1), fnew);
+                if constexpr (kGhostRing) gringPut((t + 8) & (2 * kRingBlock - 1), fgnew);
+                fstep(std::integral_constant<int, 6>{}, t + 6);
+                fstep(std::integral_constant<int, 7>{}, t + 7);
             }
             if (t < T) fstep(std::integral_constant<int, 0>{}, t++);
             if (t < T) fstep(std::integral_constant<int, 1>{}, t++);
             if (t < T) fstep(std::integral_constant<int, 2>{}, t++);
+            if (t < T) fstep(std::integral_constant<int, 3>{}, t++);
+            if (t < T) fstep(std::integral_constant<int, 4>{}, t++);
+            if (t < T) fstep(std::integral_constant<int, 5>{}, t++);
+            if (t < T) fstep(std::integral_constant<int, 6>{}, t++);
             if (STR) __syncthreads();  // boundary written above is read by the next strip
         }
         if (MODE == kScore) {
