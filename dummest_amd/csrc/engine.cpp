@@ -563,6 +563,11 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         const int M = e->profiles[prof[i]].M, n = e->seqLen[seq[i]];
         if (n < 3) return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
         cfgOf[i] = chooseConfig(e, M, n, e->plain);
+        // the score kernels read a row strip of a wave's W1 blocks through one
+        // buffer resource: (64/G) groups x (n + G - 1) steps x G R floats < 2^31 bytes
+        if (mode == kScore && 256LL * e->cfgR[cfgOf[i]] * (n + e->cfgG[cfgOf[i]] - 1) > INT32_MAX)
+            return fail(e, DM_EUNSUP, "a sequence is too long for the score pass (at most 2^23 / R - G + 1 letters "
+                                      "for a configuration of G lanes and R rows per lane: about 1M at R = 8)");
         P.cells += (int64_t)(M + 1) * n;
         if (!used[prof[i]]) {
             used[prof[i]] = 1;
@@ -665,8 +670,9 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
     HIPCHK(P.counter.ensure(sizeof(int) * std::max<size_t>(64, ncfg)));
     if (mode == kScore) {
         HIPCHK(P.wbuf.ensure(sizeof(float) * maxW));
-        // the score kernels skip the W1 stores of slots outside the matrix and
-        // read those slots with zero weight: they must hold finite values
+        // the score kernels read only the W1 runs their backward stored in the
+        // same pass; the scratch is zeroed anyway so that no slot can ever hold
+        // a non-finite value
         HIPCHK(hipMemsetAsync(P.wbuf.p, 0, P.wbuf.cap, e->stream));
         HIPCHK(P.ebuf.ensure(sizeof(int) * maxE));
     }

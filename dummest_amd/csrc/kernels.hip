@@ -677,13 +677,14 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             wsc0 = a.wbuf + PR.bufOff;
             esc0 = a.ebuf + PR.bufOff / ROWS;
         } else {
-            const int64_t off = (int64_t)wslot * a.waveFloats + (int64_t)grp * (a.waveFloats / (64 / G));
-            wsc0 = a.wbuf + off;
+            // strip-major: strip s of group g at (s (64/G) + g) stripSteps ROWS
+            wsc0 = a.wbuf + (int64_t)wslot * a.waveFloats + (int64_t)grp * strideS;
             // per strip stripSteps + 8 ints: 4 guard ints on either side of
             // anti-diagonals 0..stripSteps-1 (exponents move four at a time)
             esc0 = a.ebuf + ((int64_t)wslot * (64 / G) + grp) * a.escGroup + kEscGuard;
         }
         const int64_t escStrip = (MODE == kDump) ? a.stripSteps : a.stripSteps + 2 * kEscGuard;
+        const int64_t wStrip = (MODE == kDump) ? strideS : strideS * (64 / G);  // W1 floats between strips
         // strip boundary buffers (per group), one (value, Y, exponent) record per
         // column, n + 2 entries each: two parities for the backward, one for the forward
         Bnd *bB = nullptr, *bF = nullptr;
@@ -702,6 +703,12 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
         auto w1Off = [&](int kk, int q) {
             const int j = q >> 2, wj = (R - 4 * j < 4) ? R - 4 * j : 4;
             return G * 4 * j + kk * wj + (q & 3);
+        };
+        // Lanes of a run of chunk j: L_j lanes of chunk j fill about one 128-byte line.
+        auto runLanes = [](int j) {
+            const int wj = (R - 4 * j < 4) ? R - 4 * j : 4;
+            const int l = (wj >= 3) ? 8 : (wj == 2 ? 16 : 32);
+            return l < G ? l : G;
         };
         float enterC[R], d0[R], d1s2[R], d2s1[R], a1s1[R], a2s2[R], eps[R];
         float fbest = 0.f;  // filter mode: max W1 * prefixSeed
@@ -783,7 +790,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 a0sC[q] *= sC;
                 b0sC[q] *= sC;
             }
-            float *wsc = wsc0 + (int64_t)strip * strideS;
+            float *wsc = wsc0 + (int64_t)strip * wStrip;
             int *esc = esc0 + (int64_t)strip * escStrip;
             const bool below = STR && strip < S - 1;  // a strip below feeds lane G-1
             const bool above = STR && strip > 0;      // lane 0 feeds the strip above
@@ -877,11 +884,6 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // is skipped when none of its lanes holds a cell of the matrix: all
             // its columns lie outside [0, n) (the skew), or all its rows past M.
             // Skipping whole runs keeps every write a whole line.
-            auto runLanes = [](int j) {
-                const int wj = (R - 4 * j < 4) ? R - 4 * j : 4;
-                const int l = (wj >= 3) ? 8 : (wj == 2 ? 16 : 32);
-                return l < G ? l : G;
-            };
             int runLo[(R + 3) / 4], runSpan[(R + 3) / 4];  // run of chunk j: columns D - runLo - runSpan + 1 .. D - runLo
 #pragma unroll
             for (int j = 0; j * 4 < R; j++) {
@@ -893,11 +895,11 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // W1 of step tt (ring slot sl) and its exponent into the scratch.
             // Score mode stores the runs of lanes described above for every
             // step of the pair (anti-diagonals 0..Tb-1, inside the group's own
-            // scratch slot). The forward loads every slot; the ones no run wrote
-            // hold finite values (the scratch is zeroed when it is allocated and
-            // only ever receives finite W1 values), and it gives them no weight
-            // wherever w0 is not zero (columns at or past n, lanes past row M;
-            // at columns before 0, w0 is zero). Stored cells outside the matrix
+            // scratch slot). The forward reads the same runs (the same test on
+            // the same anti-diagonal) and gets zeros for the others; within a
+            // run it gives no weight to the lanes whose column is at or past n
+            // or whose rows all lie past M (at columns before 0, w0 is zero).
+            // Stored cells outside the matrix
             // cannot raise the anchor maximum either: columns outside the
             // sequence meet a zero on one side, rows past M in a live lane hold
             // seed-only values that never exceed row M's at the same column.
@@ -1157,7 +1159,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             const int Tmine = n + liveLanes(strip) - 1;   // this pair's
             float a0[R], b0[R];
             loadStrip(strip, a0, b0, strip > 0);
-            const float *wsc = wsc0 + (int64_t)strip * strideS;
+            const float *wsc = wsc0 + (int64_t)strip * wStrip;
             const int *esc = esc0 + (int64_t)strip * escStrip;
             const bool above = STR && strip > 0;      // lane 0 reads the strip above
             const bool below = STR && strip < S - 1;  // last lane feeds the strip below
@@ -1205,21 +1207,54 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             const bool injF = (k == 0);
             float Wq[kPF][R];
             int Eq[kPF];
-            // Loads are unconditional (from a clamped, always-valid step) so that
-            // no select waits for them; which of them count is decided at use.
+            // Loads are unconditional (from a clamped step; runs the backward did
+            // not store read zeros, see below) so that no select waits for them;
+            // which of them count is decided at use.
             const int Tw = Tmine;
             const int nW1 = (active && strip * ROWS + k * R <= M) ? n : 0;  // columns of this lane with W1
-            const float *fchunk0[(R + 3) / 4];  // this lane's part of chunk j of anti-diagonal 0
+            // Score mode reads the wave's blocks of this strip through a buffer
+            // resource. A run of lanes whose store the backward skipped (no
+            // column in [0, n), or all rows past M) reads with an offset past
+            // the resource's end: the load returns zeros and fetches nothing.
+            const uint64_t wb = (uint64_t)(a.wbuf + (int64_t)wslot * a.waveFloats + (int64_t)strip * wStrip);
+            const uint32_t wlo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
+            const uint32_t whi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
+            // the host keeps a strip of the wave's blocks below 2^31 bytes (score mode)
+            const __amdgpu_buffer_rsrc_t w1r = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(((uint64_t)whi << 32) | wlo), 0, (int)(MODE == kScore ? wStrip * sizeof(float) : 0), 0x00020000);
+            uint32_t foff[(R + 3) / 4];  // byte offset of this lane's part of chunk j at anti-diagonal 0
+            int flo[(R + 3) / 4], fspan[(R + 3) / 4];  // the backward's runLo / runSpan
+            if constexpr (MODE == kScore) {
 #pragma unroll
-            for (int j = 0; j * 4 < R; j++) fchunk0[j] = wsc + w1Off(k, 4 * j);
+                for (int j = 0; j * 4 < R; j++) {
+                    foff[j] = (uint32_t)(((int64_t)grp * strideS + w1Off(k, 4 * j)) * sizeof(float));
+                    flo[j] = k & ~(runLanes(j) - 1);
+                    fspan[j] = (active && strip * ROWS + flo[j] * R <= M) ? n + runLanes(j) - 1 : 0;
+                }
+            }
             auto loadW1 = [&](int tt, float *dst, int &e) {
                 const int ts = min(tt, Tw - 1);  // Tw = n + live - 1 >= 1
-                if (MODE == kScore) {
+                if constexpr (MODE == kScore) {
+                    const uint32_t sb = (uint32_t)ts * (uint32_t)(ROWS * sizeof(float));
 #pragma unroll
                     for (int j = 0; j * 4 < R; j++) {
-                        const float *src = fchunk0[j] + (int64_t)ts * ROWS;
+                        const uint32_t off = ((unsigned)(ts - flo[j]) < (unsigned)fspan[j]) ? foff[j] + sb : 0x80000000u;
+                        const int wj = (R - 4 * j < 4) ? R - 4 * j : 4;
+                        if (wj == 4) {
+                            const auto v = __builtin_amdgcn_raw_buffer_load_b128(w1r, off, 0, 0);
 #pragma unroll
-                        for (int q = 4 * j; q < 4 * j + 4 && q < R; q++) dst[q] = src[q - 4 * j];
+                            for (int q = 0; q < 4; q++) dst[4 * j + q] = __uint_as_float(v[q]);
+                        } else if (wj == 3) {
+                            const auto v = __builtin_amdgcn_raw_buffer_load_b96(w1r, off, 0, 0);
+#pragma unroll
+                            for (int q = 0; q < 3; q++) dst[4 * j + q] = __uint_as_float(v[q]);
+                        } else if (wj == 2) {
+                            const auto v = __builtin_amdgcn_raw_buffer_load_b64(w1r, off, 0, 0);
+                            dst[4 * j] = __uint_as_float(v[0]);
+                            dst[4 * j + 1] = __uint_as_float(v[1]);
+                        } else {
+                            dst[4 * j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(w1r, off, 0, 0));
+                        }
                     }
                     e = 0;
                 } else {

@@ -199,7 +199,10 @@ double dm_last_hits_ms(const dm_engine *e);
 typedef struct dm_plan dm_plan;
 int dm_plan_create(dm_engine *e, int mode, int64_t npairs, const int32_t *prof, const int32_t *seq, int want_bwd,
                    dm_plan **out);
-/* mode 0 fills best[npairs]; mode 1 fills mid_m/mid_e (and bwd_m/bwd_e if want_bwd). */
+/* mode 0 fills best[npairs]; mode 1 fills mid_m/mid_e (and bwd_m/bwd_e if want_bwd).
+ * Mode 1 takes sequences of at most 2^23 / R - G + 1 letters, G lanes and R
+ * rows per lane being the configuration chosen for the pair (about a million
+ * letters at R = 8, the largest); longer ones return DM_EUNSUP. */
 int dm_plan_execute(dm_engine *e, dm_plan *plan, float *best, float *mid_m, int32_t *mid_e, float *bwd_m,
                     int32_t *bwd_e);
 void dm_plan_destroy(dm_engine *e, dm_plan *plan);

@@ -155,3 +155,29 @@ def test_hip_error_leaves_engine_usable(engine):
     for k in range(2):
         assert abs(ln(mm[k], me[k]) - ln(*rm[k])) < TOL_NATS
         assert abs(ln(bm[k], be[k]) - ln(*rb[k])) < TOL_NATS
+
+
+def test_score_plan_rejects_overlong_sequence(engine):
+    """The score pass reads a row strip of a wave's W1 blocks through one
+    buffer resource (below 2^31 bytes): a pair whose sequence exceeds
+    2^23 / R - G + 1 letters for every configuration (9M letters) is refused
+    with DM_EUNSUP when its plan is built, and the engine still scores the
+    next pairs as the oracle does."""
+    print("overlong score pass", flush=True)
+    text, prof, dnas = synthetic_case(67, 60, [400, 700], homolog_every=2)
+    M, v, p, b, _ = D.parse_hmm_text(text)
+    pid = engine.add_profile(M, v, p, b)
+    rng = np.random.default_rng(67)
+    longSeq = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, 9_000_000)].tobytes().decode()
+    lid = engine.add_sequences([D.decode(longSeq)])[0]
+    with pytest.raises(D.DummestError) as ex:
+        engine.plan([pid], [lid], mode="score")
+    assert ex.value.code == D.DM_EUNSUP
+    assert "too long" in str(ex.value)
+    codes = [D.decode(s) for s in dnas]
+    sids = engine.add_sequences(codes)
+    mm, me, bm, be = engine.fwdbwd_scores([pid] * 2, sids, want_bwd=True)
+    rm, rb = O.fwdbwd_batch(O.Profile("x", M, v, p, b, []), codes)
+    for k in range(2):
+        assert abs(ln(mm[k], me[k]) - ln(*rm[k])) < TOL_NATS
+        assert abs(ln(bm[k], be[k]) - ln(*rb[k])) < TOL_NATS
