@@ -545,6 +545,10 @@ constexpr float kStep = 5.421011e-20f;  // 2^-64
 constexpr int kEscGuard = 4;            // score mode: guard ints around a strip's exponents
 
 // Four exponents moved with one 16-byte access at a 4-byte-aligned address.
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+typedef unsigned v3u __attribute__((ext_vector_type(3)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
 struct __attribute__((aligned(4))) Int4u {
     int v0, v1, v2, v3;
     template <int I> __device__ __forceinline__ int at() const {
@@ -907,25 +911,42 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // are the pairs' own). Score mode writes the exponents of a four-step
             // iteration with one store (storeE4) when its W1 blocks are stored
             // late, else one per step.
-            // score mode: this lane's part of chunk j of anti-diagonal 0's block
-            float *chunk0[(R + 3) / 4];
+            // Score mode stores through a buffer resource over the wave's blocks
+            // of this strip: a skipped run's offset lies past its end, so the
+            // store writes nothing and needs no branch.
+            const uint64_t bwb = (uint64_t)(a.wbuf + (int64_t)wslot * a.waveFloats + (int64_t)strip * wStrip);
+            const uint32_t bwlo = __builtin_amdgcn_readfirstlane((uint32_t)bwb);
+            const uint32_t bwhi = __builtin_amdgcn_readfirstlane((uint32_t)(bwb >> 32));
+            const __amdgpu_buffer_rsrc_t w1s = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(((uint64_t)bwhi << 32) | bwlo), 0, (int)(MODE == kScore ? wStrip * sizeof(float) : 0), 0x00020000);
+            uint32_t boff[(R + 3) / 4];  // byte offset of this lane's part of chunk j at anti-diagonal 0
 #pragma unroll
-            for (int j = 0; j * 4 < R; j++) chunk0[j] = wsc + w1Off(k, 4 * j);
+            for (int j = 0; j * 4 < R; j++) boff[j] = (uint32_t)(((int64_t)grp * strideS + w1Off(k, 4 * j)) * sizeof(float));
             auto storeW1 = [&](int tt, int sl, bool withExp) {
                 const int D = Tb - 1 - tt;
                 if (MODE == kScore) {
-                    if (D >= 0) {
+                    const uint32_t sb = (uint32_t)D * (uint32_t)(ROWS * sizeof(float));
 #pragma unroll
-                        for (int j = 0; j * 4 < R; j++) {
-                            // some column of the run lies in [0, n) (rows checked in runSpan)
-                            if ((unsigned)(D - runLo[j]) < (unsigned)runSpan[j]) {
-                                float *dst = chunk0[j] + (int64_t)D * ROWS;
-#pragma unroll
-                                for (int q = 4 * j; q < 4 * j + 4 && q < R; q++) dst[q - 4 * j] = H[sl][q];
-                            }
+                    for (int j = 0; j * 4 < R; j++) {
+                        // some column of the run lies in [0, n) (rows checked in
+                        // runSpan; D < 0 fails the test too)
+                        const uint32_t off = ((unsigned)(D - runLo[j]) < (unsigned)runSpan[j]) ? boff[j] + sb : 0x80000000u;
+                        const int wj = (R - 4 * j < 4) ? R - 4 * j : 4;
+                        const float *h = &H[sl][4 * j];
+                        if (wj == 4) {
+                            const v4u v{__float_as_uint(h[0]), __float_as_uint(h[1]), __float_as_uint(h[2]), __float_as_uint(h[3])};
+                            __builtin_amdgcn_raw_buffer_store_b128(v, w1s, off, 0, 0);
+                        } else if (wj == 3) {
+                            const v3u v{__float_as_uint(h[0]), __float_as_uint(h[1]), __float_as_uint(h[2])};
+                            __builtin_amdgcn_raw_buffer_store_b96(v, w1s, off, 0, 0);
+                        } else if (wj == 2) {
+                            const v2u v{__float_as_uint(h[0]), __float_as_uint(h[1])};
+                            __builtin_amdgcn_raw_buffer_store_b64(v, w1s, off, 0, 0);
+                        } else {
+                            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(h[0]), w1s, off, 0, 0);
                         }
-                        if (withExp) esc[D] = Eb;  // every lane of the group writes the same value
                     }
+                    if (withExp && D >= 0) esc[D] = Eb;  // every lane of the group writes the same value
                 } else {
                     const int c = D - k;
                     if (D >= 0 && active) {
