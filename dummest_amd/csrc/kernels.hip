@@ -521,6 +521,7 @@ template <int G> __device__ __forceinline__ ES group_max(ES v) {
 constexpr float kBig = 1.2676506e30f;  // 2^100: rescale trigger
 constexpr float kSmall = 6.8719477e10f; // 2^36: the live state below this is raised by 2^64
 constexpr int kGhostTop = 100;          // strip ghost values are kept below 2^100 after conversion
+constexpr float kGhostTopV = 0x1p100f;  // 2^kGhostTop
 
 // True (in every lane of the group) when the whole group's live maximum is
 // below kSmall and not all zero.
@@ -1006,9 +1007,14 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                     // last lane: the strip below's top row from the boundary buffer.
                     // When that row sits far above this strip's scale, raise the
                     // exponent first (before any neighbour value is read).
+                    // The exponent is raised when max(v, y) 2^(e - Eb) >= 2^kGhostTop
+                    // on the last lane (frexp exponent + e - Eb > kGhostTop): tested
+                    // on the converted values, the exact amount worked out only then.
                     const float rw = gcur.v, ry = gcur.y;
                     const int ge = gcur.e;
-                    if (MODE != kFilter) {
+                    gw = ldexpf(rw, ge - Eb);
+                    gy = ldexpf(ry, ge - Eb);
+                    if (MODE != kFilter && __builtin_expect(__any(k == G - 1 && fmaxf(gw, gy) >= kGhostTopV), 0)) {
                         int need = 0;
                         if (k == G - 1 && (rw > 0.f || ry > 0.f)) {
                             int ex;
@@ -1016,10 +1022,10 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                             need = ex + ge - Eb - kGhostTop;
                         }
                         need = group_lane<G>(need, grp, G - 1);  // the group's last lane
-                        if (__builtin_expect(need > 0, 0)) scaleB(-((need + 63) & ~63));
+                        if (need > 0) scaleB(-((need + 63) & ~63));
+                        gw = ldexpf(rw, ge - Eb);
+                        gy = ldexpf(ry, ge - Eb);
                     }
-                    gw = ldexpf(rw, ge - Eb);
-                    gy = ldexpf(ry, ge - Eb);
                 }
                 // lane below (k+1): its row 0 at columns c+1, c+2, c+3 and its Y at c
                 float bw1 = from_below<G>(H[S2][0]);
@@ -1367,18 +1373,23 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                 if (STR && above) {
                     // lane 0: the strip above's last row from the boundary buffer;
                     // raise this strip's exponent first when that row is far above it
+                    // (tested as in the backward, on the first lane)
                     const float rn = fcurB.v, ry = fcurB.y;
                     const int ge = fcurB.e;
-                    int need = 0;
-                    if (k == 0 && (rn > 0.f || ry > 0.f)) {
-                        int ex;
-                        (void)frexpf(fmaxf(rn, ry), &ex);
-                        need = ex + ge - Ef - kGhostTop;
-                    }
-                    need = group_lane<G>(need, grp, 0);  // the group's first lane
-                    if (__builtin_expect(need > 0, 0)) scaleF(-((need + 63) & ~63));
                     gn = ldexpf(rn, ge - Ef);
                     gy = ldexpf(ry, ge - Ef);
+                    if (__builtin_expect(__any(k == 0 && fmaxf(gn, gy) >= kGhostTopV), 0)) {
+                        int need = 0;
+                        if (k == 0 && (rn > 0.f || ry > 0.f)) {
+                            int ex;
+                            (void)frexpf(fmaxf(rn, ry), &ex);
+                            need = ex + ge - Ef - kGhostTop;
+                        }
+                        need = group_lane<G>(need, grp, 0);  // the group's first lane
+                        if (need > 0) scaleF(-((need + 63) & ~63));
+                        gn = ldexpf(rn, ge - Ef);
+                        gy = ldexpf(ry, ge - Ef);
+                    }
                 }
                 float Nin = from_above<G>(lastN);
                 float Ya = from_above<G>(lastY);
