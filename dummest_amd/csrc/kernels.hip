@@ -531,6 +531,18 @@ template <int G> __device__ __forceinline__ bool group_small(float live, int grp
     const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G % 64)) - 1ull) << (grp * G));
     return ((bs & gm) == gm) && ((bn & gm) != 0ull);
 }
+// A buffer resource over bytes [0, size) from p, which must be the same for
+// every lane of the wave. Buffer accesses at an offset past size are dropped
+// (stores) or return zeros (loads) without touching memory; callers pass
+// kOutOfRange for the lanes that must not access.
+constexpr uint32_t kOutOfRange = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t wave_rsrc(const void *p, uint32_t size) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), 0, (int)size, 0x00020000);
+}
+
 // The value of lane `at` of this lane's group, in every lane of the group:
 // scalar reads of each group's lane (no LDS round trip).
 template <int G> __device__ __forceinline__ int group_lane(int v, int grp, int at) {
@@ -915,11 +927,8 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // Score mode stores through a buffer resource over the wave's blocks
             // of this strip: a skipped run's offset lies past its end, so the
             // store writes nothing and needs no branch.
-            const uint64_t bwb = (uint64_t)(a.wbuf + (int64_t)wslot * a.waveFloats + (int64_t)strip * wStrip);
-            const uint32_t bwlo = __builtin_amdgcn_readfirstlane((uint32_t)bwb);
-            const uint32_t bwhi = __builtin_amdgcn_readfirstlane((uint32_t)(bwb >> 32));
-            const __amdgpu_buffer_rsrc_t w1s = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(((uint64_t)bwhi << 32) | bwlo), 0, (int)(MODE == kScore ? wStrip * sizeof(float) : 0), 0x00020000);
+            const __amdgpu_buffer_rsrc_t w1s = wave_rsrc(a.wbuf + (int64_t)wslot * a.waveFloats + (int64_t)strip * wStrip,
+                                                         MODE == kScore ? (uint32_t)(wStrip * sizeof(float)) : 0u);
             uint32_t boff[(R + 3) / 4];  // byte offset of this lane's part of chunk j at anti-diagonal 0
 #pragma unroll
             for (int j = 0; j * 4 < R; j++) boff[j] = (uint32_t)(((int64_t)grp * strideS + w1Off(k, 4 * j)) * sizeof(float));
@@ -931,7 +940,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                     for (int j = 0; j * 4 < R; j++) {
                         // some column of the run lies in [0, n) (rows checked in
                         // runSpan; D < 0 fails the test too)
-                        const uint32_t off = ((unsigned)(D - runLo[j]) < (unsigned)runSpan[j]) ? boff[j] + sb : 0x80000000u;
+                        const uint32_t off = ((unsigned)(D - runLo[j]) < (unsigned)runSpan[j]) ? boff[j] + sb : kOutOfRange;
                         const int wj = (R - 4 * j < 4) ? R - 4 * j : 4;
                         const float *h = &H[sl][4 * j];
                         if (wj == 4) {
@@ -1243,12 +1252,9 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
             // resource. A run of lanes whose store the backward skipped (no
             // column in [0, n), or all rows past M) reads with an offset past
             // the resource's end: the load returns zeros and fetches nothing.
-            const uint64_t wb = (uint64_t)(a.wbuf + (int64_t)wslot * a.waveFloats + (int64_t)strip * wStrip);
-            const uint32_t wlo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
-            const uint32_t whi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
             // the host keeps a strip of the wave's blocks below 2^31 bytes (score mode)
-            const __amdgpu_buffer_rsrc_t w1r = __builtin_amdgcn_make_buffer_rsrc(
-                (void *)(((uint64_t)whi << 32) | wlo), 0, (int)(MODE == kScore ? wStrip * sizeof(float) : 0), 0x00020000);
+            const __amdgpu_buffer_rsrc_t w1r = wave_rsrc(a.wbuf + (int64_t)wslot * a.waveFloats + (int64_t)strip * wStrip,
+                                                         MODE == kScore ? (uint32_t)(wStrip * sizeof(float)) : 0u);
             uint32_t foff[(R + 3) / 4];  // byte offset of this lane's part of chunk j at anti-diagonal 0
             int flo[(R + 3) / 4], fspan[(R + 3) / 4];  // the backward's runLo / runSpan
             if constexpr (MODE == kScore) {
@@ -1265,7 +1271,7 @@ __attribute__((amdgpu_waves_per_eu(((!STR && (R == 7 || R == 8)) || (STR && R <=
                     const uint32_t sb = (uint32_t)ts * (uint32_t)(ROWS * sizeof(float));
 #pragma unroll
                     for (int j = 0; j * 4 < R; j++) {
-                        const uint32_t off = ((unsigned)(ts - flo[j]) < (unsigned)fspan[j]) ? foff[j] + sb : 0x80000000u;
+                        const uint32_t off = ((unsigned)(ts - flo[j]) < (unsigned)fspan[j]) ? foff[j] + sb : kOutOfRange;
                         const int wj = (R - 4 * j < 4) ? R - 4 * j : 4;
                         if (wj == 4) {
                             const auto v = __builtin_amdgcn_raw_buffer_load_b128(w1r, off, 0, 0);
