@@ -281,7 +281,8 @@ static double pairCost(const dm_engine *e, int c, int M, int n) {
     return steps * t;
 }
 
-static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
+// skip: configurations not to choose (none when null); -1 when nothing else fits.
+static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain, const std::vector<char> *skip = nullptr) {
     // a forced configuration (autotuning, the "config" option) applies where its rows fit
     if (e->forceCfg >= 0 && (e->cfgStrips[e->forceCfg] || e->cfgG[e->forceCfg] * e->cfgR[e->forceCfg] >= M + 1))
         return e->forceCfg;
@@ -290,6 +291,7 @@ static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
     const bool tuned = !e->cfgStepNs.empty();
     for (size_t c = 0; c < e->cfgG.size(); c++) {
         const int G = e->cfgG[c], R = e->cfgR[c];
+        if (skip && (*skip)[c]) continue;
         if (e->cfgStrips[c] && (!tuned || forcePlain)) continue;  // strips: only when measured (or needed)
         if (!e->cfgStrips[c] && G * R < M + 1) continue;
         if (forcePlain && G != 64) continue;
@@ -301,7 +303,7 @@ static int chooseConfig(const dm_engine *e, int M, int n, bool forcePlain) {
     }
     if (best < 0)  // profiles longer than one group: the cheapest row-strip kernel
         for (size_t c = 0; c < e->cfgG.size(); c++) {
-            if (!e->cfgStrips[c]) continue;
+            if (!e->cfgStrips[c] || (skip && (*skip)[c])) continue;
             const double cost = pairCost(e, (int)c, M, n);
             if (best < 0 || cost < bestCost * (1 - 1e-9)) {
                 best = (int)c;
@@ -563,17 +565,43 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
         const int M = e->profiles[prof[i]].M, n = e->seqLen[seq[i]];
         if (n < 3) return fail(e, DM_EINVAL, "sequences must have at least 3 letters (dummest-dp.hh:84)");
         cfgOf[i] = chooseConfig(e, M, n, e->plain);
-        // the score kernels read a row strip of a wave's W1 blocks through one
-        // buffer resource: (64/G) groups x (n + G - 1) steps x G R floats < 2^31 bytes
-        if (mode == kScore && 256LL * e->cfgR[cfgOf[i]] * (n + e->cfgG[cfgOf[i]] - 1) > INT32_MAX)
-            return fail(e, DM_EUNSUP, "a sequence is too long for the score pass (at most 2^23 / R - G + 1 letters "
-                                      "for a configuration of G lanes and R rows per lane: about 1M at R = 8)");
         P.cells += (int64_t)(M + 1) * n;
         if (!used[prof[i]]) {
             used[prof[i]] = 1;
             P.profM.push_back({prof[i], M});
         }
     }
+    // A configuration whose pairs would not fill one persistent grid (fewer
+    // waves than the device holds at once) leaves SIMDs idle for its whole
+    // launch: its pairs take the cheapest configuration that already has a
+    // full launch, where one fits. Decided over the whole pair list, so every
+    // chunk of the plan runs a pair with the same kernel.
+    if (!e->plain && e->forceCfg < 0 && !e->cfgStepNs.empty() && npairs > 0) {
+        const size_t nc = e->cfgG.size();
+        std::vector<int64_t> cnt(nc, 0);
+        for (int64_t i = 0; i < npairs; i++) cnt[cfgOf[i]]++;
+        const int64_t full = (int64_t)e->numCUs * 8;
+        std::vector<char> skip(nc, 0);
+        bool small = false, kept = false;
+        for (size_t c = 0; c < nc; c++) {
+            const int64_t waves = (cnt[c] * e->cfgG[c] + 63) / 64;
+            skip[c] = waves < full;
+            if (cnt[c] > 0) (skip[c] ? small : kept) = true;
+        }
+        if (small && kept)
+            for (int64_t i = 0; i < npairs; i++) {
+                if (!skip[cfgOf[i]]) continue;
+                const int c = chooseConfig(e, e->profiles[prof[i]].M, e->seqLen[seq[i]], false, &skip);
+                if (c >= 0) cfgOf[i] = c;
+            }
+    }
+    // the score kernels read a row strip of a wave's W1 blocks through one
+    // buffer resource: (64/G) groups x (n + G - 1) steps x G R floats < 2^31 bytes
+    if (mode == kScore)
+        for (int64_t i = 0; i < npairs; i++)
+            if (256LL * e->cfgR[cfgOf[i]] * (e->seqLen[seq[i]] + e->cfgG[cfgOf[i]] - 1) > INT32_MAX)
+                return fail(e, DM_EUNSUP, "a sequence is too long for the score pass (at most 2^23 / R - G + 1 "
+                                          "letters for a configuration of G lanes and R rows per lane: about 1M at R = 8)");
     std::vector<int64_t> sorted(npairs);
     std::iota(sorted.begin(), sorted.end(), 0);
     std::stable_sort(sorted.begin(), sorted.end(), [&](int64_t a, int64_t b) {
