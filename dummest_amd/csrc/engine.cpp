@@ -571,16 +571,16 @@ static int buildPlan(dm_engine *e, int mode, int64_t npairs, const int32_t *prof
             P.profM.push_back({prof[i], M});
         }
     }
-    // A configuration whose pairs would not fill one persistent grid (fewer
-    // waves than the device holds at once) leaves SIMDs idle for its whole
-    // launch: its pairs take the cheapest configuration that already has a
-    // full launch, where one fits. Decided over the whole pair list, so every
+    // A configuration whose pairs would fill fewer than four persistent grids
+    // (the waves the device holds at once) leaves SIMDs idle for much of its
+    // launch, the last round of units half empty on average: its pairs take
+    // the cheapest configuration that already has such a launch, where one fits. Decided over the whole pair list, so every
     // chunk of the plan runs a pair with the same kernel.
     if (!e->plain && e->forceCfg < 0 && !e->cfgStepNs.empty() && npairs > 0) {
         const size_t nc = e->cfgG.size();
         std::vector<int64_t> cnt(nc, 0);
         for (int64_t i = 0; i < npairs; i++) cnt[cfgOf[i]]++;
-        const int64_t full = (int64_t)e->numCUs * 8;
+        const int64_t full = (int64_t)e->numCUs * 8 * 4;
         std::vector<char> skip(nc, 0);
         bool small = false, kept = false;
         for (size_t c = 0; c < nc; c++) {
